@@ -1,0 +1,21 @@
+// The context behind the opaque rgpu_ctx handle of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+struct rgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // default-priority stream owned by the context
+    int cu_count = 0;
+    std::string arch;
+    std::mutex err_mu;
+    std::string last_error;
+};
+
+namespace rgpu {
+int fail(rgpu_ctx* ctx, int code, const std::string& msg);
+int hip_fail(rgpu_ctx* ctx, hipError_t e, const char* what);
+hipStream_t stream_of(rgpu_ctx* ctx, void* s);
+}  // namespace rgpu

@@ -1,0 +1,44 @@
+// Internal interface between the C ABI and the traversal kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rgpu_types.h"
+
+namespace rgpu {
+
+// 32-byte ray: origin, far clip, direction (rgpu_ray in the public header).
+struct alignas(16) TraceRay {
+    float ro[3];
+    float tmax;
+    float rd[3];
+    uint32_t pad;
+};
+// 16-byte hit: primitive id (-1 on a miss), t (the far clip on a miss), u, v.
+struct alignas(16) TraceHit {
+    int32_t prim;
+    float t;
+    float u;
+    float v;
+};
+
+enum { kVariantStackless = 0, kVariantThreaded = 1 };
+
+// Device-resident scene as the kernels see it.
+struct TraceScene {
+    const SLNode* sl_nodes = nullptr;
+    const MTNode* mt_nodes = nullptr;  // 6 tables of nnodes records
+    const PrimRec* prims = nullptr;
+    const float4* points = nullptr;    // original vertices (for geometric normals)
+    const uint32_t* indices = nullptr;  // 3 per triangle
+    const float4* spheres = nullptr;
+    uint32_t nnodes = 0;
+    int32_t sphere_begin = 0;
+};
+
+// counters: null for the production kernels; otherwise 5 device u64 totals
+// {branch visits, leaf visits, parent steps, primitive tests, threaded node visits}.
+hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays, TraceHit* hits, uint32_t n,
+                        uint32_t block, hipStream_t stream, unsigned long long* counters = nullptr);
+
+}  // namespace rgpu

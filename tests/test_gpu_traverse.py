@@ -1,0 +1,136 @@
+"""GPU parity of extension ray casting (both traversal variants) against the oracle.
+
+Bit-exact bar: primitive index, t, u and v of every ray equal the oracle's bits when
+the oracle runs the reference algorithm (kernels/extension_ray_cast_stackless.cl,
+common/threaded_bvh.hpp:527-569) on the reference layouts restated from the same tree.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from raccoongpurender_amd import _lib, scenes
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(tree, v, f, sph, rays, variant):
+    if variant == "stackless":
+        return O.intersect_stackless(O.RefStackless(tree), v, f, rays, sph)
+    return O.intersect_mtbvh(O.RefMTBVH(tree), v, f, rays, sph)
+
+
+def _ray_sets(v):
+    lo, hi = v.min(0), v.max(0)
+    return {
+        "incoherent": scenes.incoherent_rays(20000, lo, hi, seed=11),
+        "primary": scenes.primary_rays(96, 96),
+        "vertex": H.degenerate_rays(v, 4000, 3),
+        "clipped": H.with_tmax(scenes.incoherent_rays(4000, lo, hi, seed=12), np.float32(0.25)),
+        "zero_tmax": H.with_tmax(scenes.incoherent_rays(500, lo, hi, seed=13), np.float32(0.0)),
+    }
+
+
+def _meshes():
+    v, f = scenes.displaced_icosphere(level=5)
+    yield "icosphere5", v, f, None
+    v, f = H.triangle_soup(4000, 1)
+    yield "soup", v, f, None
+    v, f = H.grid_plane(32)
+    yield "grid", v, f, None
+    v, f = H.triangle_soup(300, 2)
+    rng = np.random.default_rng(4)
+    sph = np.concatenate([rng.uniform(-1, 1, (60, 3)), rng.uniform(0.02, 0.15, (60, 1))], 1).astype(np.float32)
+    yield "soup+spheres", v, f, sph
+
+
+@pytest.mark.parametrize("variant", ["stackless", "threaded"])
+def test_bit_exact_small_scenes(ctx, variant):
+    for name, v, f, sph in _meshes():
+        bvh = _lib.BVH(v, f, sph)
+        scene = _lib.Scene(ctx, bvh)
+        tree = bvh.export()
+        for rname, rays in _ray_sets(v).items():
+            print(f"case {name}/{rname}/{variant} n={len(rays)}", flush=True)
+            g = _lib.trace(ctx, scene, rays, variant)
+            o, _ = _oracle(tree, v, f, sph, rays, variant)
+            assert H.compare_hits(g, o) == dict(prim=0, t=0, uv=0), (name, rname, variant)
+        scene.close()
+
+
+def test_variants_agree_and_block_sizes_identical(ctx):
+    v, f = scenes.displaced_icosphere(level=5)
+    bvh = _lib.BVH(v, f)
+    scene = _lib.Scene(ctx, bvh)
+    rays = scenes.incoherent_rays(50000, v.min(0), v.max(0), seed=21)
+    base = _lib.trace(ctx, scene, rays, "stackless")
+    for block in (64, 128, 256):
+        for variant in ("stackless", "threaded"):
+            g = _lib.trace(ctx, scene, rays, variant, block=block)
+            ties, wrong = H.tie_excepted_mismatch(g, base)
+            assert wrong == 0
+            if variant == "stackless":
+                assert H.compare_hits(g, base) == dict(prim=0, t=0, uv=0)
+
+
+def test_empty_and_single_triangle(ctx):
+    v = np.zeros((0, 3), np.float32)
+    f = np.zeros((0, 3), np.uint32)
+    bvh = _lib.BVH(v, f)
+    scene = _lib.Scene(ctx, bvh)
+    rays = scenes.incoherent_rays(1000, [-1, -1, -1], [1, 1, 1], seed=1)
+    for variant in ("stackless", "threaded"):
+        g = _lib.trace(ctx, scene, rays, variant)
+        assert np.all(g["prim"] == -1) and np.array_equal(g["t"], rays["tmax"])
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    bvh = _lib.BVH(v, f)
+    scene = _lib.Scene(ctx, bvh)
+    rays = scenes.make_rays(np.array([[0.2, 0.2, 1.0], [2, 2, 1.0], [0.1, 0.1, -1.0]]),
+                            np.array([[0, 0, -1.0], [0, 0, -1.0], [0, 0, 1.0]]))
+    for variant in ("stackless", "threaded"):
+        g = _lib.trace(ctx, scene, rays, variant)
+        assert g["prim"].tolist() == [0, -1, 0]
+        assert g["t"][0] == 1.0 and g["t"][2] == 1.0
+
+
+def test_invalid_arguments_return_status(ctx):
+    v, f = scenes.displaced_icosphere(level=2)
+    scene = _lib.Scene(ctx, _lib.BVH(v, f))
+    rays = ctx.to_device(scenes.incoherent_rays(64, [-1] * 3, [1] * 3))
+    hits = ctx.alloc(64 * 16)
+    with pytest.raises(_lib.RgpuError) as e:
+        scene.extension_ray_cast(7, rays, hits, 64)
+    assert e.value.code == -1
+    with pytest.raises(_lib.RgpuError) as e:
+        scene.extension_ray_cast("stackless", rays, hits, 64, block=100)
+    assert e.value.code == -1
+
+
+@pytest.fixture(scope="module")
+def c2():
+    d = scenes.raycast_benchmark_scene(level=8)
+    d["bvh"] = _lib.BVH(d["points"], d["indices"])
+    d["tree"] = d["bvh"].export()
+    return d
+
+
+@pytest.mark.parametrize("variant", ["stackless", "threaded"])
+@pytest.mark.parametrize("rayset", ["coherent", "incoherent"])
+def test_bit_exact_raycast_benchmark_full_size(ctx, c2, variant, rayset):
+    """1,310,720 triangles, 4,194,304 rays per set: GPU == oracle bit for bit."""
+    v, f = c2["points"], c2["indices"]
+    assert len(f) == 1310720
+    scene = _lib.Scene(ctx, c2["bvh"])
+    rays = c2[rayset]
+    assert len(rays) == 4194304
+    print(f"case c2/{rayset}/{variant}", flush=True)
+    g = _lib.trace(ctx, scene, rays, variant)
+    o, cnt = _oracle(c2["tree"], v, f, None, rays, variant)
+    assert H.compare_hits(g, o) == dict(prim=0, t=0, uv=0)
+    # the oracle itself against brute force on a sample
+    idx = np.random.default_rng(0).choice(len(rays), 256, replace=False)
+    bf = O.intersect_bruteforce(v, f, rays[idx])
+    ties, wrong = H.tie_excepted_mismatch(o[idx], bf)
+    assert wrong == 0 and ties <= 2
+    scene.close()
