@@ -79,7 +79,21 @@ def main():
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level (8 = 1,310,720 triangles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", choices=["auto", "stackless", "threaded"], default="auto")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run the library's copy kernel on MiB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from raccoongpurender_amd import _lib
+
+        ctx = _lib.Context(0)
+        nb = args.copy_calibration << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        a.fill_u32(1)
+        for _ in range(3):
+            ctx.copy(b, a, nb)
+        ctx.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path shards by pixel/sample range with a replicated scene; see the design notes).",

@@ -96,6 +96,9 @@ int rgpu_download(rgpu_ctx* ctx, rgpu_stream stream, void* dst, const void* src,
 /* Fill `count` 32-bit words with `value` (OpenCLBuffer::fill, raccoon_ocl.hpp:427-434). */
 int rgpu_fill_u32(rgpu_ctx* ctx, rgpu_stream stream, void* dst, uint32_t value, size_t count);
 int rgpu_synchronize(rgpu_ctx* ctx, rgpu_stream stream);
+/* Device-to-device copy with the library's own copy kernel (copy_if_locked without the
+ * lock, kernels/mutex.cl:19-24); asynchronous on `stream`. */
+int rgpu_copy(rgpu_ctx* ctx, rgpu_stream stream, void* dst, const void* src, size_t bytes);
 
 /* ---------------------------------------------------------------- timing
  * Replaces profiling events (OpenCLEvent::wait, common/raccoon_ocl.hpp:172-189).

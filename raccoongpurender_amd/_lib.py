@@ -78,6 +78,7 @@ _sig("rgpu_upload", C.c_int, vp, vp, vp, vp, C.c_size_t)
 _sig("rgpu_download", C.c_int, vp, vp, vp, vp, C.c_size_t)
 _sig("rgpu_fill_u32", C.c_int, vp, vp, vp, u32, C.c_size_t)
 _sig("rgpu_synchronize", C.c_int, vp, vp)
+_sig("rgpu_copy", C.c_int, vp, vp, vp, vp, C.c_size_t)
 _sig("rgpu_time_extension_ray_cast", C.c_int, vp, vp, C.c_int, vp, vp, u32, u32, C.c_int, C.POINTER(C.c_double))
 _sig("rgpu_bvh_build", C.c_int, vp, u32, u32, vp, u32, vp, u32, C.POINTER(rgpu_bvh_options), C.POINTER(vp))
 _sig("rgpu_bvh_destroy", C.c_int, vp)
@@ -92,7 +93,7 @@ _sig("rgpu_extension_ray_cast_stats", C.c_int, vp, vp, C.c_int, vp, vp, u32, C.P
 # Every symbol the public header declares; tests check the library exports them all.
 EXPORTED = [
     "rgpu_create", "rgpu_destroy", "rgpu_last_error_string", "rgpu_device_count",
-    "rgpu_alloc", "rgpu_free", "rgpu_upload", "rgpu_download", "rgpu_fill_u32", "rgpu_synchronize",
+    "rgpu_alloc", "rgpu_free", "rgpu_upload", "rgpu_download", "rgpu_fill_u32", "rgpu_synchronize", "rgpu_copy",
     "rgpu_time_extension_ray_cast",
     "rgpu_bvh_build", "rgpu_bvh_destroy", "rgpu_bvh_get_stats", "rgpu_bvh_export",
     "rgpu_scene_create", "rgpu_scene_destroy", "rgpu_scene_bytes",
@@ -144,6 +145,9 @@ class Context:
 
     def synchronize(self):
         self.check(lib.rgpu_synchronize(self.h, None), "rgpu_synchronize")
+
+    def copy(self, dst, src, nbytes):
+        self.check(lib.rgpu_copy(self.h, None, dst.ptr, src.ptr, nbytes), "rgpu_copy")
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)

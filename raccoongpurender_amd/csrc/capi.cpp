@@ -55,6 +55,8 @@ int hip_fail(rgpu_ctx* ctx, hipError_t e, const char* what) {
 
 hipStream_t stream_of(rgpu_ctx* ctx, void* s) { return s ? (hipStream_t)s : ctx->stream; }
 
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 }  // namespace rgpu
 
 #define RGPU_HIP(ctx, call)                                  \
@@ -171,6 +173,13 @@ int rgpu_fill_u32(rgpu_ctx* ctx, rgpu_stream stream, void* dst, uint32_t value, 
     if (!count) return RGPU_OK;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)dst, (int)value, count, stream_of(ctx, stream)));
+    return RGPU_OK;
+}
+
+int rgpu_copy(rgpu_ctx* ctx, rgpu_stream stream, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return RGPU_ERR_INVALID_ARG;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, launch_copy(dst, src, bytes, stream_of(ctx, stream)));
     return RGPU_OK;
 }
 
