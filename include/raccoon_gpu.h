@@ -152,6 +152,127 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
 int rgpu_extension_ray_cast_stats(rgpu_ctx* ctx, const rgpu_scene* scene, int variant, const rgpu_ray* rays,
                                   rgpu_hit* hits, uint32_t count, uint64_t* counters5);
 
+/* ================================================================ wavefront path tracing */
+
+typedef struct rgpu_materials rgpu_materials;
+typedef struct rgpu_envmap rgpu_envmap;
+typedef struct rgpu_wavefront rgpu_wavefront;
+
+/* material types (kernels/types.cl:4-10) */
+#define RGPU_MAT_LAMBERTIAN 1
+#define RGPU_MAT_SPECULAR 2
+#define RGPU_MAT_DIELECTRIC 3
+#define RGPU_MAT_WARD 4
+#define RGPU_MAT_VOLUME 5 /* boundary of a homogeneous medium */
+
+/* Lambertian (kernels/types.cl:78-82). back_emission is stored but, as in the reference,
+ * emission is two-sided (kernels/lambertian.cl:113). */
+typedef struct rgpu_lambertian {
+    float Le[3];
+    float R[3];
+    int32_t back_emission;
+} rgpu_lambertian;
+/* Ward (kernels/types.cl:92-97) with separate roughness per tangent axis; alpha_x == alpha_y
+ * evaluates the reference's isotropic formulas. */
+typedef struct rgpu_ward {
+    float alpha_x, alpha_y;
+    float reflectance[3];
+    float edgetint[3];
+    float falloff;
+} rgpu_ward;
+typedef struct rgpu_dielectric {
+    float ior; /* the reference hard-codes 1.5 (kernels/delta_materials.cl:95-96) */
+} rgpu_dielectric;
+/* HomogeneousVolume (kernels/types.cl:99-102): C = extinction, R = scattering colour. */
+typedef struct rgpu_volume {
+    float C;
+    float R[3];
+} rgpu_volume;
+
+/* StandardCamera (kernels/camera.cl:4-22, common/wavefront_path_tracing.hpp:39-48) plus the
+ * thin-lens radius the reference hard-codes as 0.0045 (kernels/new_path.cl:61). */
+typedef struct rgpu_camera {
+    float eye[3], forward[3], up[3], right[3];
+    float imageplane_o[3], imageplane_r[3], imageplane_b[3];
+    uint32_t width, height;
+    float lens_radius;
+} rgpu_camera;
+
+#define RGPU_WF_ACCUM64 1u              /* accumulate pixel sums in double, 32-bit counts */
+#define RGPU_WF_NO_REFERENCE_QUIRKS 2u  /* skip the stale-volume strategy draws of new paths */
+
+typedef struct rgpu_wavefront_config {
+    uint32_t abi_version;   /* RGPU_ABI_VERSION */
+    uint32_t path_count;    /* resident path slots (kWavefrontPathCountGPU = 2^22 in the reference) */
+    uint64_t seed;          /* added to the splitmix64 seed of every slot (reference: 0) */
+    uint32_t rr_start;      /* Russian roulette after this many bounces; 0 = 25 (kernels/logic.cl:77) */
+    uint32_t max_depth;     /* terminate a path at this bounce; 0 = unlimited (reference) */
+    float radiance_clamp;   /* drop samples with a channel >= clamp; 0 = 1000 (kernels/logic.cl:126) */
+    float bxdf_select_p;    /* BSDF selection probability; 0 = 0.5 (kernels/mixture_density.cl:40) */
+    uint32_t flags;         /* RGPU_WF_* */
+    uint32_t variant;       /* RGPU_VARIANT_* traversal used by the extension ray cast */
+    uint32_t queue_blocks;  /* workgroups of the queue-driven kernels; 0 = 8 per CU */
+} rgpu_wavefront_config;
+
+/* Per-primitive materials: the primitive index is the material index (kernels/
+ * extension_ray_cast_stackless.cl:242), so `types`/`indices` have one entry per primitive,
+ * triangles first, then spheres. Replaces SceneManager::createMaterialBuffer
+ * (common/scene_manager.hpp:593-602) and its MaterialBuffer. */
+int rgpu_materials_create(rgpu_ctx* ctx, uint32_t nprims, const int32_t* types, const int32_t* indices,
+                          const rgpu_lambertian* lambertians, uint32_t nlambertian, const rgpu_ward* wards,
+                          uint32_t nward, const rgpu_dielectric* dielectrics, uint32_t ndielectric,
+                          const rgpu_volume* volumes, uint32_t nvolume, rgpu_materials** out);
+int rgpu_materials_destroy(rgpu_materials* m);
+
+/* Environment map and its 7 alias tables (1 uniform-weight + 6 six-axis). Replaces the
+ * ImageEnvmap construction in SceneManager::addPoint and createEnvmapBuffer
+ * (common/scene_manager.hpp:432-497, 604-650). rgb: width*height*channels floats, row 0 at
+ * the zenith. ctx may be NULL to build the host tables only (rgpu_envmap_export). */
+int rgpu_envmap_create(rgpu_ctx* ctx, const float* rgb, uint32_t channels, uint32_t width, uint32_t height,
+                       rgpu_envmap** out);
+int rgpu_envmap_destroy(rgpu_envmap* e);
+/* Host copy of table `table` (0 = uniform, 1..6 = +X,-X,+Y,-Y,+Z,-Z): pdf per texel, alias
+ * heights and aliases, fragments (4 floats: beg_y, end_y, beg_phi, end_phi), and whether the
+ * table fell back from all-zero weights. Outputs may be NULL. */
+int rgpu_envmap_export(const rgpu_envmap* e, int table, float* pdf, float* heights, int32_t* aliases,
+                       float* fragments, int32_t* degenerate);
+
+/* Pinhole/thin-lens camera from a look-at (the standardCamera image-plane construction of
+ * common/wavefront_path_tracing.hpp:103-118). focus_distance 0 = distance to target. */
+int rgpu_camera_look_at(const float* eye, const float* target, const float* up, float fov_y_deg,
+                        float focus_distance, uint32_t width, uint32_t height, float lens_radius,
+                        rgpu_camera* out);
+
+/* WavefrontLane (common/wavefront_path_tracing.hpp:339-1153): resident path state, stage
+ * queues and accumulation on one GPU. The scene, materials and envmap must outlive it. */
+int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_materials* materials,
+                          const rgpu_envmap* envmap, const rgpu_wavefront_config* cfg, rgpu_wavefront** out);
+int rgpu_wavefront_destroy(rgpu_wavefront* w);
+/* WavefrontLane::initialize(lane_index, num_device) (wavefront_path_tracing.hpp:532-570):
+ * RNG seeding, every slot a new path, pixel cursor at lane * (P / lanes), cleared
+ * accumulation. Also the restart path of re_start() (wavefront_path_tracing.hpp:592-604). */
+int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* camera, uint32_t lane_index,
+                              uint32_t num_lanes);
+/* `steps` calls of WavefrontLane::step() (wavefront_path_tracing.hpp:591-939), asynchronous. */
+int rgpu_wavefront_step(rgpu_wavefront* w, rgpu_stream stream, uint32_t steps);
+/* Runs `steps` steps between two events on the context stream; mean ms per step. */
+int rgpu_wavefront_time_steps(rgpu_wavefront* w, uint32_t steps, double* ms_per_step);
+/* Queue lengths after the last step (new path, lambertian, specular, dielectric, ward,
+ * volume boundary, volume inside) and the pixel cursor next_pixel_index. Synchronous. */
+int rgpu_wavefront_queue_counts(rgpu_wavefront* w, uint32_t* counts7, uint64_t* next_pixel);
+/* Per-pixel radiance sums (3 doubles) and sample counts. Synchronous. */
+int rgpu_wavefront_read_accumulation(rgpu_wavefront* w, double* rgb_sums, uint32_t* counts);
+/* Mean radiance -> RGBA8 through the half-float intermediate and gamma 2.2
+ * (kernels/accumlation.cl:6-25,68-79). Synchronous. */
+int rgpu_wavefront_tonemap(rgpu_wavefront* w, uint8_t* rgba8);
+/* Total accumulated samples (kernels/stat.cl). Synchronous. */
+int rgpu_wavefront_stat(rgpu_wavefront* w, uint64_t* total_samples);
+/* Per-slot state for testing: RNG states (4 u32), rays (ro, tmax, rd, pixel), paths (T, logic_i,
+ * L, pad), extension results (hit_surface, hit_volume, t, Ng, pad), shading (Le, T, pad),
+ * in-volume lists (7 ids + count), queue items (7 x path_count). Any may be NULL. */
+int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, float* paths, void* ext,
+                              float* shade, int32_t* vol, uint32_t* items);
+
 #ifdef __cplusplus
 }
 #endif

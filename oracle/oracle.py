@@ -199,3 +199,150 @@ def hit_normals(rays, hits, points, indices, spheres=None):
     lib.orc_hit_normals(len(hits), p(np.ascontiguousarray(rays)), p(np.ascontiguousarray(hits)), p(points),
                         points.shape[1], p(indices), ntris, p(spheres), p(out))
     return out
+
+
+# ====================================================================== wavefront step (restated)
+class orc_wf_desc(C.Structure):
+    _fields_ = [
+        ("n", u32), ("rr_start", u32), ("max_depth", u32), ("quirks", u32), ("variant", u32), ("nthreads", u32),
+        ("clamp", C.c_float), ("bxdf_p", C.c_float), ("seed", u64),
+        ("points", vp), ("stride", u32), ("indices", vp), ("sphere_begin", C.c_int32), ("spheres", vp),
+        ("sl_nodes", vp), ("sl_prims", vp), ("mt_nodes", vp), ("mt_links", vp), ("mt_nn", u32), ("mt_prims", vp),
+        ("nprims", u32), ("mat_types", vp), ("mat_indices", vp),
+        ("lambertians", vp), ("nl", u32), ("wards", vp), ("nw", u32), ("iors", vp), ("nd", u32),
+        ("volumes", vp), ("nv", u32),
+        ("env_rgb", vp), ("env_channels", u32), ("env_w", u32), ("env_h", u32),
+        ("camera", vp), ("width", u32), ("height", u32), ("lens_radius", C.c_float),
+    ]
+
+
+_sig("orc_wf_create", vp, C.POINTER(orc_wf_desc))
+_sig("orc_wf_destroy", None, vp)
+_sig("orc_wf_initialize", None, vp, u32, u32)
+_sig("orc_wf_step", None, vp, u32)
+_sig("orc_wf_queue_counts", None, vp, vp, C.POINTER(u64))
+_sig("orc_wf_read_accumulation", None, vp, vp, vp)
+_sig("orc_wf_read_state", None, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("orc_envmap_build", vp, vp, u32, u32, u32)
+_sig("orc_envmap_export", None, vp, C.c_int, vp, vp, vp, vp)
+_sig("orc_envmap_destroy", None, vp)
+_sig("orc_envmap_lookup", None, vp, u32, vp, C.c_int, vp)
+_sig("orc_envmap_sample", None, vp, vp, u32, vp, vp, vp)
+_sig("orc_envmap_pdf", None, vp, u32, vp, vp, vp)
+
+EXT_DTYPE = np.dtype([("hit_surface", np.int32), ("hit_volume", np.int32), ("tmin", np.float32), ("Ng", np.float32, 3),
+                      ("pad", np.float32, 2)])
+PATH_DTYPE = np.dtype([("T", np.float32, 3), ("logic_i", np.uint32), ("L", np.float32, 3), ("pad", np.uint32)])
+SHADE_DTYPE = np.dtype([("Le", np.float32, 3), ("T", np.float32, 3), ("pad", np.float32, 2)])
+VOL_DTYPE = np.dtype([("volumes", np.int32, 7), ("count", np.int32)])
+STATE_RAY_DTYPE = np.dtype([("ro", np.float32, 3), ("tmax", np.float32), ("rd", np.float32, 3), ("pixel", np.uint32)])
+
+
+class OracleEnvmap:
+    def __init__(self, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        self.rgb = rgb
+        self.height, self.width = rgb.shape[:2]
+        self.h = lib.orc_envmap_build(p(rgb), rgb.shape[2], self.width, self.height)
+
+    def export(self, table):
+        P = self.width * self.height
+        out = dict(pdf=np.zeros(P, np.float32), height=np.zeros(P, np.float32), alias=np.zeros(P, np.int32),
+                   fragments=np.zeros((P, 4), np.float32))
+        lib.orc_envmap_export(self.h, table, p(out["pdf"]), p(out["height"]), p(out["alias"]), p(out["fragments"]))
+        return out
+
+    def lookup(self, dirs, filtered):
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros((len(dirs), 3), np.float32)
+        lib.orc_envmap_lookup(self.h, len(dirs), p(dirs), 1 if filtered else 0, p(out))
+        return out
+
+    def sample(self, state, n, normal=None):
+        nrm = None if normal is None else np.ascontiguousarray(normal, np.float32)
+        dirs = np.zeros((n, 3), np.float32)
+        pdfs = np.zeros(n, np.float32)
+        lib.orc_envmap_sample(self.h, p(state), n, p(nrm), p(dirs), p(pdfs))
+        return dirs, pdfs
+
+    def pdf(self, dirs, normal=None):
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        nrm = None if normal is None else np.ascontiguousarray(normal, np.float32)
+        out = np.zeros(len(dirs), np.float32)
+        lib.orc_envmap_pdf(self.h, len(dirs), p(dirs), p(nrm), p(out))
+        return out
+
+    def __del__(self):
+        try:
+            lib.orc_envmap_destroy(self.h)
+        except Exception:
+            pass
+
+
+class OracleWavefront:
+    """The reference's WavefrontLane step restated on the CPU (stable queues)."""
+
+    def __init__(self, points, indices, tree, materials, env_rgb, camera, path_count, spheres=None, seed=0,
+                 rr_start=0, max_depth=0, clamp=0.0, bxdf_p=0.0, quirks=True, variant="stackless",
+                 nthreads=NTHREADS):
+        self.points, self.indices, self.ntris, self.spheres = _geom(points, indices, spheres)
+        self.keep = []
+        d = orc_wf_desc()
+        d.n, d.rr_start, d.max_depth, d.quirks = path_count, rr_start, max_depth, 1 if quirks else 0
+        d.variant = 1 if variant == "threaded" else 0
+        d.nthreads, d.clamp, d.bxdf_p, d.seed = nthreads, clamp, bxdf_p, seed
+        d.points, d.stride, d.indices = p(self.points), self.points.shape[1], p(self.indices)
+        d.sphere_begin, d.spheres = self.ntris, p(self.spheres)
+        if len(tree["left"]):
+            self.sl = RefStackless(tree)
+            self.mt = RefMTBVH(tree)
+            d.sl_nodes, d.sl_prims = p(self.sl.nodes), p(self.sl.prim_ids)
+            d.mt_nodes, d.mt_links, d.mt_nn, d.mt_prims = p(self.mt.nodes), p(self.mt.links), self.mt.nn, p(self.mt.prim_ids)
+        lam, war, ior, vol = materials.flat()
+        self.keep += [lam, war, ior, vol, materials.types, materials.indices]
+        d.nprims, d.mat_types, d.mat_indices = len(materials.types), p(materials.types), p(materials.indices)
+        d.lambertians, d.nl = p(lam), len(materials.lambertians)
+        d.wards, d.nw = p(war), len(materials.wards)
+        d.iors, d.nd = p(ior), len(materials.dielectrics)
+        d.volumes, d.nv = p(vol), len(materials.volumes)
+        env = np.ascontiguousarray(env_rgb, np.float32)
+        cam = camera.as_array()
+        self.keep += [env, cam]
+        d.env_rgb, d.env_channels, d.env_w, d.env_h = p(env), env.shape[2], env.shape[1], env.shape[0]
+        d.camera, d.width, d.height, d.lens_radius = p(cam), camera.width, camera.height, camera.lens_radius
+        self.desc = d
+        self.n = path_count
+        self.npixels = camera.width * camera.height
+        self.h = lib.orc_wf_create(C.byref(d))
+
+    def initialize(self, lane_index=0, num_lanes=1):
+        lib.orc_wf_initialize(self.h, lane_index, num_lanes)
+
+    def step(self, steps=1):
+        lib.orc_wf_step(self.h, steps)
+
+    def queue_counts(self):
+        c = np.zeros(7, np.uint32)
+        npx = u64(0)
+        lib.orc_wf_queue_counts(self.h, p(c), C.byref(npx))
+        return c, npx.value
+
+    def accumulation(self):
+        rgb = np.zeros((self.npixels, 3), np.float64)
+        cnt = np.zeros(self.npixels, np.uint32)
+        lib.orc_wf_read_accumulation(self.h, p(rgb), p(cnt))
+        return rgb, cnt
+
+    def state(self):
+        n = self.n
+        st = dict(rng=np.zeros((n, 4), np.uint32), rays=np.zeros(n, STATE_RAY_DTYPE), paths=np.zeros(n, PATH_DTYPE),
+                  ext=np.zeros(n, EXT_DTYPE), shade=np.zeros(n, SHADE_DTYPE), vol=np.zeros(n, VOL_DTYPE),
+                  items=np.zeros((7, n), np.uint32))
+        lib.orc_wf_read_state(self.h, *[p(st[k]) for k in ("rng", "rays", "paths", "ext", "shade", "vol", "items")])
+        return st
+
+    def __del__(self):
+        try:
+            lib.orc_wf_destroy(self.h)
+        except Exception:
+            pass

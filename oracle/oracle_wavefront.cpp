@@ -1,0 +1,1154 @@
+// ORACLE — test infrastructure only. CPU restatement of the reference's wavefront step
+// (common/wavefront_path_tracing.hpp:591-939) kernel by kernel, over the reference's AoS
+// slot records (kernels/types.cl), with stable (slot-ordered) queues. Used as the checker
+// by tests/ and __graft_entry__.smoke(); the product never links it.
+//
+// Floating point: -ffp-contract=off, IEEE denormals; sin/cos/log/exp/atan/atan2/pow are
+// evaluated in double and rounded once to float; "x * M_PI * 2.0f" is evaluated in double
+// as OpenCL does with its double M_PI on fp64 devices.
+//
+// Pinning: queue counts and RNG streams follow from the restated control flow; the
+// materials are pinned by closed-form furnace tests (tests/test_oracle_wavefront.py); the
+// envmap tables by the alias-table and pdf invariants.
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+extern "C" int orc_stackless_intersect(const void* nodes, const uint32_t* prim_ids, const float* points,
+                                       uint32_t stride, const uint32_t* indices, int32_t sphere_begin,
+                                       const float* spheres, uint32_t nrays, const void* rays, void* hits,
+                                       uint64_t* counters, int nthreads);
+extern "C" int orc_mtbvh_intersect(const void* nodes, const int32_t* links, uint32_t nn, const uint32_t* prim_ids,
+                                   const float* points, uint32_t stride, const uint32_t* indices, int32_t sphere_begin,
+                                   const float* spheres, uint32_t nrays, const void* rays, void* hits,
+                                   uint64_t* counters, int nthreads);
+extern "C" void orc_hit_normals(uint32_t nhits, const void* rays, const void* hits, const float* points,
+                                uint32_t stride, const uint32_t* indices, int32_t sphere_begin, const float* spheres,
+                                float* out_ng);
+
+namespace {
+
+const double kPiD = 3.14159265358979323846;
+const float kPiF = (float)kPiD;
+
+// ------------------------------------------------------------------ float3
+struct f3 {
+    float x, y, z;
+};
+inline f3 F(float x, float y, float z) { return f3{x, y, z}; }
+inline f3 operator+(f3 a, f3 b) { return F(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline f3 operator-(f3 a, f3 b) { return F(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline f3 operator-(f3 a) { return F(-a.x, -a.y, -a.z); }
+inline f3 operator*(f3 a, f3 b) { return F(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline f3 operator*(f3 a, float s) { return F(a.x * s, a.y * s, a.z * s); }
+inline f3 operator*(float s, f3 a) { return F(s * a.x, s * a.y, s * a.z); }
+inline f3 operator/(f3 a, float s) { return F(a.x / s, a.y / s, a.z / s); }
+inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline f3 cross(f3 a, f3 b) { return F(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline f3 normalize(f3 v) {
+    float inv = 1.0f / std::sqrt(dot(v, v));
+    return v * inv;
+}
+inline float fmn(float a, float b) { return std::fmin(a, b); }
+inline float fmx(float a, float b) { return std::fmax(a, b); }
+inline float sinf_(float x) { return (float)std::sin((double)x); }
+inline float cosf_(float x) { return (float)std::cos((double)x); }
+inline float logf_(float x) { return (float)std::log((double)x); }
+inline float expf_(float x) { return (float)std::exp((double)x); }
+inline float atanf_(float x) { return (float)std::atan((double)x); }
+inline float atan2f_(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+inline float powf_(float x, float y) { return (float)std::pow((double)x, (double)y); }
+inline float mix(float a, float b, float t) { return a + (b - a) * t; }
+inline float sqr(float x) { return x * x; }
+
+// ------------------------------------------------------------------ RNG (kernels/peseudo_random.cl)
+struct Rng {
+    uint32_t s[4];
+};
+inline uint32_t rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+inline uint32_t next(Rng& r) {
+    uint32_t* s = r.s;
+    const uint32_t res = rotl(s[0] * 5u, 7) * 9u;
+    const uint32_t t = s[1] << 9;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 11);
+    return res;
+}
+inline float uniform(Rng& r) {
+    uint32_t bits = (next(r) >> 9) | 0x3f800000u;
+    float f;
+    memcpy(&f, &bits, 4);
+    return f - 1.0f;
+}
+inline uint64_t uniform_integer(Rng& r) {
+    uint64_t a = next(r) >> 1;
+    uint64_t b = next(r) >> 1;
+    return (a << 31) | b;
+}
+inline uint64_t splitmix64(uint64_t* x) {
+    uint64_t z = (*x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// ------------------------------------------------------------------ reference records
+enum { kLambertian = 1, kSpecular = 2, kDierectric = 3, kWard = 4, kVolume = 5, kVolumeInside = 6 };
+enum { kBxdf = 0, kEnv = 1, kNone = 2 };
+
+struct WavefrontPath {
+    f3 T, L, ro, rd;
+    uint32_t logic_i, pixel_index;
+};
+struct InVolumeList {
+    int32_t volumes[7];
+    int32_t count;
+};
+struct ExtensionResult {
+    int32_t hit_surface_material;
+    int32_t hit_volume_material;
+    float tmin;
+    f3 Ng;
+};
+struct ShadingResult {
+    f3 Le, T;
+};
+struct IncidentSample {
+    f3 wi;
+    float selection_probs[2];
+    float pdfs[2];
+    uint32_t strategy;
+};
+struct Material {
+    int32_t type, index;
+};
+struct Lambertian {
+    float Le[3], R[3];
+    int32_t back;
+};
+struct Ward {
+    float ax, ay, refl[3], edge[3], falloff;
+};
+struct Volume {
+    float C, R[3];
+};
+struct Camera {
+    float eye[3], fwd[3], up[3], right[3], o[3], r[3], b[3];
+    uint32_t w, h;
+    float lens;
+};
+struct AliasB {
+    float height;
+    int32_t alias;
+};
+struct Frag {
+    float beg_y, end_y, beg_phi, end_phi;
+};
+struct Ray {
+    float ro[3];
+    float tmax;
+    float rd[3];
+    uint32_t pad;
+};
+struct Hit {
+    int32_t prim;
+    float t, u, v;
+};
+
+inline f3 v3(const float* p) { return F(p[0], p[1], p[2]); }
+
+// ------------------------------------------------------------------ envmap tables (restated)
+struct Kahan {
+    double sum = 0.0, c = 0.0;
+    void add(double x) {
+        double y = x - c;
+        double t = sum + y;
+        c = (t - sum) - y;
+        sum = t;
+    }
+    double get() const { return sum - c; }
+};
+
+// common/share/alias_method.hpp:47-114
+void alias_prepare(const std::vector<double>& w, std::vector<double>* probs, std::vector<AliasB>* out) {
+    int N = (int)w.size();
+    Kahan s;
+    for (double v : w) s.add(v);
+    double inv = 1.0 / s.get();
+    probs->resize(N);
+    std::vector<Kahan> h(N);
+    std::vector<int> alias(N, -1);
+    for (int i = 0; i < N; ++i) {
+        (*probs)[i] = w[i] * inv;
+        h[i].sum = (*probs)[i] * N;
+        h[i].c = 0.0;
+    }
+    std::vector<int> lower, upper;
+    for (int i = 0; i < N; ++i) {
+        if (h[i].get() < 1.0) lower.push_back(i);
+        else upper.push_back(i);
+    }
+    for (;;) {
+        if (lower.empty() || upper.empty()) break;
+        int li = lower.back();
+        lower.pop_back();
+        int ui = upper.back();
+        upper.pop_back();
+        double mov = 1.0 - h[li].get();
+        h[ui].add(-mov);
+        alias[li] = ui;
+        if (h[ui].get() < 1.0) lower.push_back(ui);
+        else upper.push_back(ui);
+    }
+    out->resize(N);
+    for (int i = 0; i < N; ++i) (*out)[i] = AliasB{(float)h[i].get(), alias[i]};
+}
+
+struct Envmap {
+    uint32_t w = 0, h = 0;
+    std::vector<float> rgba;
+    std::vector<Frag> frags;
+    std::vector<float> pdf[7];
+    std::vector<AliasB> alias[7];
+};
+
+// common/share/envmap.hpp:151-235 (ImageEnvmap) and 291-312 (six-axis weights); a table
+// whose weights sum to zero falls back to solid-angle x direction weights, then solid angle.
+void build_envmap(const float* rgb, uint32_t ch, uint32_t W, uint32_t H, Envmap* e) {
+    e->w = W;
+    e->h = H;
+    size_t P = (size_t)W * H;
+    e->rgba.assign(4 * P, 0.0f);
+    for (size_t i = 0; i < P; ++i)
+        for (int k = 0; k < 3; ++k) e->rgba[4 * i + k] = rgb[i * ch + k];
+    double phi_step = 2.0 * kPiD / W, theta_step = kPiD / H;
+    e->frags.resize(P);
+    std::vector<double> sr(H), th(H);
+    for (uint32_t y = 0; y < H; ++y) {
+        double lt = theta_step * y, ut = theta_step * (y + 1);
+        double by = std::cos(ut), ey = std::cos(lt);
+        sr[y] = (std::cos(lt) - std::cos(ut)) * (2.0 * kPiD) / W;
+        th[y] = (std::acos(by) + std::acos(ey)) * 0.5;
+        for (uint32_t x = 0; x < W; ++x) {
+            double up = 2.0 * kPiD - x * phi_step, lp = 2.0 * kPiD - (x + 1) * phi_step;
+            e->frags[(size_t)y * W + x] = Frag{(float)by, (float)ey, (float)lp, (float)up};
+        }
+    }
+    auto dirw = [](int t, double theta, double phi) {
+        double s = std::sin(theta);
+        double d[3] = {s * std::sin(phi), std::cos(theta), s * std::cos(phi)};
+        if (t == 0) return 1.0;
+        int a = (t - 1) / 2;
+        return std::max(((t - 1) & 1) ? -d[a] : d[a], 0.0);
+    };
+    for (int t = 0; t < 7; ++t) {
+        std::vector<double> w(P);
+        double tot = 0.0;
+        for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) {
+                size_t i = (size_t)y * W + x;
+                double phi = ((2.0 * kPiD - (x + 1) * phi_step) + (2.0 * kPiD - x * phi_step)) * 0.5;
+                const float* c = &e->rgba[4 * i];
+                float Y = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
+                w[i] = (double)Y * sr[y] * dirw(t, th[y], phi);
+                tot += w[i];
+            }
+        if (!(tot > 0.0)) {
+            tot = 0.0;
+            for (uint32_t y = 0; y < H; ++y)
+                for (uint32_t x = 0; x < W; ++x) {
+                    size_t i = (size_t)y * W + x;
+                    w[i] = sr[y] * dirw(t, th[y], 2.0 * kPiD - (x + 0.5) * phi_step);
+                    tot += w[i];
+                }
+            if (!(tot > 0.0))
+                for (uint32_t y = 0; y < H; ++y)
+                    for (uint32_t x = 0; x < W; ++x) w[(size_t)y * W + x] = sr[y];
+        }
+        std::vector<double> probs;
+        alias_prepare(w, &probs, &e->alias[t]);
+        e->pdf[t].resize(P);
+        for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) {
+                size_t i = (size_t)y * W + x;
+                e->pdf[t][i] = (float)(probs[i] * (1.0 / sr[y]));
+            }
+    }
+}
+
+// kernels/envmap_sampling.cl:25-34
+void cartesian_to_polar(f3 rd, float* theta, float* phi) {
+    float z = rd.y, x = rd.z, y = rd.x;
+    *theta = atan2f_(std::sqrt(x * x + y * y), z);
+    *phi = atan2f_(y, x);
+    if (!std::isfinite(*phi)) *phi = 0.0f;
+}
+// read_imagef, normalised coordinates, REPEAT addressing (OpenCL 1.2 section 8.2)
+f3 tex(const Envmap& e, int i, int j) {
+    const float* c = &e.rgba[4 * ((size_t)j * e.w + i)];
+    return F(c[0], c[1], c[2]);
+}
+f3 get_envmap_value(const Envmap& e, f3 rd, bool use_filter) {
+    float theta, phi;
+    cartesian_to_polar(rd, &theta, &phi);
+    const float pi = kPiF;
+    float u = 1.0f - phi / (2.0f * pi);
+    float v = theta / pi;
+    int W = (int)e.w, H = (int)e.h;
+    float uu = (u - std::floor(u)) * (float)W, vv = (v - std::floor(v)) * (float)H;
+    if (!use_filter) {
+        int i = (int)std::floor(uu), j = (int)std::floor(vv);
+        if (i > W - 1) i = i - W;
+        if (j > H - 1) j = j - H;
+        return tex(e, i, j);
+    }
+    float us = uu - 0.5f, vs = vv - 0.5f;
+    int i0 = (int)std::floor(us), j0 = (int)std::floor(vs);
+    int i1 = i0 + 1, j1 = j0 + 1;
+    float a = us - std::floor(us), b = vs - std::floor(vs);
+    if (i0 < 0) i0 += W;
+    if (j0 < 0) j0 += H;
+    if (i1 > W - 1) i1 -= W;
+    if (j1 > H - 1) j1 -= H;
+    f3 t00 = tex(e, i0, j0), t10 = tex(e, i1, j0), t01 = tex(e, i0, j1), t11 = tex(e, i1, j1);
+    float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return F(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x, w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+             w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z);
+}
+// kernels/envmap_sampling.cl:54-66
+uint32_t alias_method(Rng& st, const AliasB* b, uint32_t n) {
+    uint32_t idx = (uint32_t)(uniform_integer(st) % (uint64_t)n);
+    int alias = b[idx].alias;
+    if (alias < 0) return idx;
+    float height = b[idx].height;
+    float u = uniform(st);
+    if (u < height) return idx;
+    return (uint32_t)alias;
+}
+// kernels/envmap_sampling.cl:68-81
+int fract_int(int x, int m) {
+    int r = x % m;
+    return r < 0 ? r + m : r;
+}
+int envmap_index(const Envmap& e, f3 wi) {
+    float theta, phi;
+    cartesian_to_polar(wi, &theta, &phi);
+    const float pi = kPiF;
+    float u = 1.0f - phi / (2.0f * pi);
+    float v = theta / pi;
+    int ix = (int)std::floor(u * (float)e.w);
+    int iy = (int)std::floor(v * (float)e.h);
+    ix = fract_int(ix, (int)e.w);
+    iy = std::min(std::max(iy, 0), (int)e.h - 1);
+    return iy * (int)e.w + ix;
+}
+float envmap_pdf_sixAxis(const Envmap& e, f3 wi, f3 n) {
+    int xt = 0.0f < n.x ? 1 : 2, yt = 0.0f < n.y ? 3 : 4, zt = 0.0f < n.z ? 5 : 6;
+    f3 ap = n * n;
+    int index = envmap_index(e, wi);
+    float pdf = 0.0f;
+    pdf += ap.x * e.pdf[xt][index];
+    pdf += ap.y * e.pdf[yt][index];
+    pdf += ap.z * e.pdf[zt][index];
+    return pdf;
+}
+f3 sample_from_fragment(const Frag& f, Rng& st) {
+    float y = mix(f.beg_y, f.end_y, uniform(st));
+    float phi = mix(f.beg_phi, f.end_phi, uniform(st));
+    f3 p = F(sinf_(phi), y, cosf_(phi));
+    float r_xz = std::sqrt(fmx(1.0f - p.y * p.y, 0.0f));
+    p.x *= r_xz;
+    p.z *= r_xz;
+    return p;
+}
+
+// ------------------------------------------------------------------ BSDF helpers
+void get_orthonormal_basis(f3 z, f3* x, f3* y) {
+    const float sign = std::copysign(1.0f, z.z);
+    const float a = -1.0f / (sign + z.z);
+    const float b = z.x * z.y * a;
+    *x = F(1.0f + sign * z.x * z.x * a, sign * b, -sign * z.x);
+    *y = F(b, sign + z.y * z.y * a, -z.y);
+}
+f3 reflect(f3 d, f3 n) { return d - 2.0f * dot(d, n) * n; }
+float cubic(float x) { return x * x * x; }
+
+// Anisotropic Ward helpers (isotropic case = kernels/ward.cl formulas).
+float ward_exp(const Ward& w, f3 h, f3 z) {
+    if (w.ax == w.ay) {
+        float c2 = sqr(dot(h, z));
+        float t2 = (1.0f - c2) / c2;
+        return -t2 / sqr(w.ax);
+    }
+    f3 x, y;
+    get_orthonormal_basis(z, &x, &y);
+    float hx = dot(h, x) / w.ax, hy = dot(h, y) / w.ay, hz = dot(h, z);
+    return -(hx * hx + hy * hy) / (hz * hz);
+}
+f3 sample_ward(const Ward& w, f3 wo, f3 zaxis, float u0, float u1) {
+    const float pi = kPiF;
+    float phiH, thetaH;
+    if (w.ax == w.ay) {
+        phiH = u0 * 2.0f * pi;
+        thetaH = atanf_(w.ax * std::sqrt(-logf_(u1)));
+    } else {
+        float tp = 2.0f * pi * u0;
+        phiH = atan2f_(w.ay * sinf_(tp), w.ax * cosf_(tp));
+        float c = cosf_(phiH), s = sinf_(phiH);
+        float den = c * c / (w.ax * w.ax) + s * s / (w.ay * w.ay);
+        thetaH = atanf_(std::sqrt(-logf_(u1) / den));
+    }
+    float sinTheta = sinf_(thetaH);
+    f3 hl = F(sinTheta * cosf_(phiH), sinTheta * sinf_(phiH), cosf_(thetaH));
+    f3 x, y;
+    get_orthonormal_basis(zaxis, &x, &y);
+    f3 h = x * hl.x + y * hl.y + zaxis * hl.z;
+    return reflect(-wo, h);
+}
+float pdf_ward(const Ward& w, f3 wo, f3 wi, f3 z) {
+    if ((double)(dot(wo, z) * dot(wi, z)) < 0.0) return 0.0f;
+    const float pi = kPiF;
+    f3 h = normalize(wi + wo);
+    float a2 = w.ax == w.ay ? sqr(w.ax) : w.ax * w.ay;
+    float k0 = 1.0f / (4.0f * pi * a2 * dot(h, wi) * cubic(dot(h, z)));
+    float k1 = expf_(ward_exp(w, h, z));
+    return k0 * k1;
+}
+
+// ------------------------------------------------------------------ the lane
+struct Cfg {
+    uint32_t n, rr_start, max_depth, quirks, variant, nthreads;
+    float clamp, bxdf_p;
+    uint64_t seed;
+};
+
+struct Lane {
+    Cfg cfg;
+    // scene
+    const float* points;
+    uint32_t stride;
+    const uint32_t* indices;
+    int32_t sphere_begin;
+    const float* spheres;
+    const void* sl_nodes;
+    const uint32_t* sl_prims;
+    const void* mt_nodes;
+    const int32_t* mt_links;
+    uint32_t mt_nn;
+    const uint32_t* mt_prims;
+    std::vector<Material> mats;
+    std::vector<Lambertian> lamb;
+    std::vector<Ward> ward;
+    std::vector<float> ior;
+    std::vector<Volume> vols;
+    Envmap env;
+    Camera cam;
+    // slot state
+    std::vector<WavefrontPath> path;
+    std::vector<InVolumeList> vol;
+    std::vector<ExtensionResult> ext;
+    std::vector<ShadingResult> shade;
+    std::vector<IncidentSample> isample;
+    std::vector<Rng> rng;
+    std::vector<uint32_t> q[7];          // new_path + 6 material queues
+    std::vector<uint32_t> sq[4];         // sample_env, eval_env_pdf, sample_env_6axis, eval_env_6axis_pdf
+    uint64_t next_pixel = 0;
+    std::vector<double> accum;           // 3 per pixel
+    std::vector<uint32_t> accum_count;
+};
+
+void strategy_selection(Lane& L) {
+    for (int k = 0; k < 4; ++k) L.sq[k].clear();
+    for (uint32_t item = 0; item < L.cfg.n; ++item) {
+        const ExtensionResult& e = L.ext[item];
+        bool surface_hit = 0 <= e.hit_surface_material;
+        bool volume_hit = 0 <= e.hit_volume_material;
+        bool any_hit = surface_hit || volume_hit;
+        IncidentSample s = {};
+        if (any_hit) {
+            if (surface_hit) {
+                int t = L.mats[e.hit_surface_material].type;
+                if (t == kLambertian || t == kWard) {
+                    s.selection_probs[kBxdf] = L.cfg.bxdf_p;
+                    s.selection_probs[kEnv] = 1.0f - L.cfg.bxdf_p;
+                } else {
+                    s.selection_probs[kBxdf] = 1.0f;
+                    s.selection_probs[kEnv] = 0.0f;
+                }
+            } else if (volume_hit) {
+                s.selection_probs[kBxdf] = L.cfg.bxdf_p;
+                s.selection_probs[kEnv] = 1.0f - L.cfg.bxdf_p;
+            }
+            float u = uniform(L.rng[item]);
+            s.strategy = u <= s.selection_probs[kBxdf] ? kBxdf : kEnv;
+        } else {
+            s.strategy = kNone;
+        }
+        L.isample[item] = s;
+        const int idx_sample = volume_hit ? 0 : 2, idx_eval = volume_hit ? 1 : 3;
+        if (any_hit && s.selection_probs[kEnv] != 0.0f) L.sq[s.strategy == kEnv ? idx_sample : idx_eval].push_back(item);
+    }
+}
+
+float incident_pdf(const IncidentSample& s) {
+    float pdf = 0.0f;
+    for (int k = 0; k < 2; ++k) pdf += s.selection_probs[k] * s.pdfs[k];
+    return pdf;
+}
+
+void step(Lane& L) {
+    const Cfg& c = L.cfg;
+    const uint32_t P = L.cam.w * L.cam.h;
+    const uint32_t count_env = L.env.w * L.env.h;
+    // new_path (kernels/new_path.cl:17-83)
+    for (uint32_t gid = 0; gid < L.q[0].size(); ++gid) {
+        uint32_t pi = L.q[0][gid];
+        Rng st = L.rng[pi];
+        WavefrontPath& p = L.path[pi];
+        p.T = F(1, 1, 1);
+        p.L = F(0, 0, 0);
+        p.logic_i = 0;
+        uint64_t gpi = L.next_pixel + gid;
+        uint32_t pixel = (uint32_t)(gpi % (uint64_t)P);
+        p.pixel_index = pixel;
+        uint32_t x = pixel % L.cam.w, y = pixel / L.cam.w;
+        float jx = (float)x + uniform(st);
+        float jy = (float)y + uniform(st);
+        f3 on_plane = v3(L.cam.o) + v3(L.cam.r) * jx + v3(L.cam.b) * jy;
+        float u0 = uniform(st);
+        float u1 = uniform(st);
+        float dof = L.cam.lens;
+        float r = std::sqrt(u0);
+        float theta = (float)((double)u1 * kPiD * 2.0);
+        float lensx = dof * r * cosf_(theta);
+        float lensy = dof * r * sinf_(theta);
+        f3 lens = v3(L.cam.eye) + v3(L.cam.up) * lensx + v3(L.cam.right) * lensy;
+        p.ro = lens;
+        p.rd = normalize(on_plane - lens);
+        L.vol[pi].count = 0;
+        L.rng[pi] = st;
+        L.shade[pi].Le = F(0, 0, 0);
+        L.shade[pi].T = F(1, 1, 1);
+        L.ext[pi].hit_surface_material = -1;
+        L.ext[pi].tmin = FLT_MAX;
+        L.ext[pi].Ng = F(0, 0, 0);
+    }
+    // finalize_new_path
+    L.next_pixel += L.q[0].size();
+    L.q[0].clear();
+
+    strategy_selection(L);
+
+    // sample_envmap_6axis_stage (envmap_sampling.cl:169-217)
+    for (uint32_t index : L.sq[2]) {
+        Rng st = L.rng[index];
+        f3 wo = -L.path[index].rd;
+        f3 n = L.ext[index].Ng;
+        if (dot(n, wo) < 0.0f) n = -n;
+        int xt = 0.0f < n.x ? 1 : 2, yt = 0.0f < n.y ? 3 : 4, zt = 0.0f < n.z ? 5 : 6;
+        float u = uniform(st);
+        f3 ap = n * n;
+        int sel = u < ap.x ? xt : (u < ap.x + ap.y ? yt : zt);
+        uint32_t frag = alias_method(st, L.env.alias[sel].data(), count_env);
+        float pdf = 0.0f;
+        pdf += ap.x * L.env.pdf[xt][frag];
+        pdf += ap.y * L.env.pdf[yt][frag];
+        pdf += ap.z * L.env.pdf[zt][frag];
+        f3 wi = sample_from_fragment(L.env.frags[frag], st);
+        L.isample[index].wi = wi;
+        L.isample[index].pdfs[kEnv] = pdf;
+        L.rng[index] = st;
+    }
+    // sample_envmap_stage (envmap_sampling.cl:219-248)
+    for (uint32_t index : L.sq[0]) {
+        Rng st = L.rng[index];
+        uint32_t frag = alias_method(st, L.env.alias[0].data(), count_env);
+        float pdf = L.env.pdf[0][frag];
+        f3 wi = sample_from_fragment(L.env.frags[frag], st);
+        L.isample[index].wi = wi;
+        L.isample[index].pdfs[kEnv] = pdf;
+        L.rng[index] = st;
+    }
+    // sample_or_eval_lambertian_stage (lambertian.cl:34-71)
+    for (uint32_t item : L.q[kLambertian]) {
+        f3 Ng = L.ext[item].Ng;
+        float cosTheta;
+        if (L.isample[item].strategy == kBxdf) {
+            Rng st = L.rng[item];
+            float a = uniform(st);
+            float b = uniform(st);
+            L.rng[item] = st;
+            float r = std::sqrt(a);
+            float th = (float)((double)b * kPiD * 2.0);
+            f3 l = F(r * cosf_(th), r * sinf_(th), std::sqrt(fmx(1.0f - a, 0.0f)));
+            f3 x, y;
+            get_orthonormal_basis(Ng, &x, &y);
+            L.isample[item].wi = x * l.x + y * l.y + Ng * l.z;
+            cosTheta = l.z;
+        } else {
+            cosTheta = dot(Ng, L.isample[item].wi);
+        }
+        L.isample[item].pdfs[kBxdf] = fmx(cosTheta, 0.0f) / kPiF;
+    }
+    // sample_or_eval_ward_stage (ward.cl:76-117)
+    for (uint32_t item : L.q[kWard]) {
+        const Ward& w = L.ward[L.mats[L.ext[item].hit_surface_material].index];
+        f3 wo = -L.path[item].rd;
+        f3 Ng = L.ext[item].Ng;
+        if (dot(Ng, wo) < 0.0f) Ng = -Ng;
+        f3 wi;
+        if (L.isample[item].strategy == kBxdf) {
+            Rng st = L.rng[item];
+            float u0 = uniform(st);
+            float u1 = uniform(st);
+            L.rng[item] = st;
+            wi = sample_ward(w, wo, Ng, u0, u1);
+        } else {
+            wi = L.isample[item].wi;
+        }
+        L.isample[item].wi = wi;
+        L.isample[item].pdfs[kBxdf] = pdf_ward(w, wo, wi, Ng);
+    }
+    // sample_or_eval_homogeneous_volume_inside_stage (homogeneous_volume.cl:66-88)
+    for (uint32_t item : L.q[kVolumeInside]) {
+        if (L.isample[item].strategy == kBxdf) {
+            Rng st = L.rng[item];
+            float u0 = uniform(st);
+            float u1 = uniform(st);
+            L.rng[item] = st;
+            float phi = (float)((double)u0 * kPiD * 2.0);
+            float z = mix(-1.0f, 1.0f, u1);
+            float r = std::sqrt(fmx(1.0f - z * z, 0.0f));
+            L.isample[item].wi = F(r * cosf_(phi), r * sinf_(phi), z);
+        }
+        L.isample[item].pdfs[kBxdf] = (float)(1.0 / (kPiD * 4.0));
+    }
+    // evaluate_envmap_6axis_pdf_stage / evaluate_envmap_pdf_stage (envmap_sampling.cl:250-281)
+    for (uint32_t index : L.sq[3])
+        L.isample[index].pdfs[kEnv] = envmap_pdf_sixAxis(L.env, L.isample[index].wi, L.ext[index].Ng);
+    for (uint32_t index : L.sq[1]) L.isample[index].pdfs[kEnv] = L.env.pdf[0][envmap_index(L.env, L.isample[index].wi)];
+    // lambertian_stage (lambertian.cl:73-123)
+    for (uint32_t item : L.q[kLambertian]) {
+        const Lambertian& m = L.lamb[L.mats[L.ext[item].hit_surface_material].index];
+        f3 Ng = L.ext[item].Ng;
+        f3 ro = L.path[item].ro, rd = L.path[item].rd, wo = -rd;
+        if (dot(Ng, wo) < 0.0f) Ng = -Ng;
+        f3 wi = L.isample[item].wi;
+        float pdf = incident_pdf(L.isample[item]);
+        float cosTheta = dot(wi, Ng);
+        f3 T = F(0, 0, 0);
+        if (0.0 < cosTheta) T = v3(m.R) / kPiF * cosTheta / pdf;
+        L.shade[item].Le = v3(m.Le);
+        L.shade[item].T = T;
+        float tmin = L.ext[item].tmin;
+        L.path[item].ro = ro + rd * tmin + wi * 1.0e-5f + Ng * 1.0e-5f;
+        L.path[item].rd = wi;
+    }
+    // ward_stage (ward.cl:119-186)
+    for (uint32_t item : L.q[kWard]) {
+        const Ward& w = L.ward[L.mats[L.ext[item].hit_surface_material].index];
+        float tmin = L.ext[item].tmin;
+        f3 Ng = L.ext[item].Ng;
+        f3 ro = L.path[item].ro, rd = L.path[item].rd, wo = -rd;
+        if (dot(Ng, wo) < 0.0f) Ng = -Ng;
+        f3 wi = L.isample[item].wi;
+        float pdf = incident_pdf(L.isample[item]);
+        float cosTheta = dot(wi, Ng);
+        f3 T = F(0, 0, 0);
+        if (0.0f < cosTheta) {
+            f3 lv = wi + wo;
+            f3 h = normalize(lv);
+            float pi = kPiF;
+            float a2 = w.ax == w.ay ? sqr(w.ax) : w.ax * w.ay;
+            float mu = dot(h, wi);
+            float t = powf_(1.0f - mu, 1.0f / w.falloff);
+            f3 rho = F(mix(w.refl[0], w.edge[0], t), mix(w.refl[1], w.edge[1], t), mix(w.refl[2], w.edge[2], t));
+            f3 k0 = rho / (pi * a2);
+            float k1 = expf_(ward_exp(w, h, Ng));
+            float k2 = dot(lv, lv) / sqr(sqr(dot(lv, Ng)));
+            T = F(k0.x * k1 * k2 * cosTheta / pdf, k0.y * k1 * k2 * cosTheta / pdf, k0.z * k1 * k2 * cosTheta / pdf);
+        }
+        L.shade[item].Le = F(0, 0, 0);
+        L.shade[item].T = T;
+        L.path[item].ro = ro + rd * tmin + wi * 1.0e-5f + Ng * 1.0e-5f;
+        L.path[item].rd = wi;
+    }
+    // homogeneous_volume_stage (homogeneous_volume.cl:19-64)
+    auto vhash = [&](int mat) {
+        const Volume& v = L.vols[L.mats[mat].index];
+        uint32_t a, b, cc, d;
+        memcpy(&a, &v.C, 4);
+        memcpy(&b, &v.R[0], 4);
+        memcpy(&cc, &v.R[1], 4);
+        memcpy(&d, &v.R[2], 4);
+        return a ^ b ^ cc ^ d;
+    };
+    for (uint32_t item : L.q[kVolume]) {
+        int hsm = L.ext[item].hit_surface_material;
+        float tmin = L.ext[item].tmin;
+        f3 Ng = L.ext[item].Ng, ro = L.path[item].ro, rd = L.path[item].rd, wo = -rd;
+        bool backside = dot(Ng, wo) < 0.0f;
+        L.shade[item].Le = F(0, 0, 0);
+        L.shade[item].T = F(1, 1, 1);
+        InVolumeList v = L.vol[item];
+        if (backside) {
+            uint32_t h = vhash(hsm);
+            bool move = false;
+            for (int i = 0; i < v.count; ++i) {
+                if (move) v.volumes[i - 1] = v.volumes[i];
+                else if (vhash(v.volumes[i]) == h) move = true;
+            }
+            if (move) v.count--;
+        } else if (v.count < 7) {
+            v.volumes[v.count] = hsm;
+            v.count++;
+        }
+        L.vol[item] = v;
+        L.path[item].ro = ro + rd * (tmin + 1.0e-5f);
+    }
+    // homogeneous_volume_inside_stage (homogeneous_volume.cl:90-129)
+    for (uint32_t item : L.q[kVolumeInside]) {
+        const Volume& V = L.vols[L.mats[L.ext[item].hit_volume_material].index];
+        float tmin = L.ext[item].tmin;
+        f3 ro = L.path[item].ro, rd = L.path[item].rd;
+        f3 wi = L.isample[item].wi;
+        float pdf = incident_pdf(L.isample[item]);
+        L.shade[item].Le = F(0, 0, 0);
+        L.shade[item].T = v3(V.R) / (float)(4.0f * kPiD) / pdf;
+        L.path[item].ro = ro + rd * tmin;
+        L.path[item].rd = wi;
+    }
+    // delta_materials (delta_materials.cl:34-135): specular then dielectric per gid
+    size_t nd = std::max(L.q[kSpecular].size(), L.q[kDierectric].size());
+    for (size_t gid = 0; gid < nd; ++gid) {
+        if (gid < L.q[kSpecular].size()) {
+            uint32_t pi = L.q[kSpecular][gid];
+            float tmin = L.ext[pi].tmin;
+            f3 Ng = L.ext[pi].Ng, ro = L.path[pi].ro, rd = L.path[pi].rd, wo = -rd;
+            if (dot(Ng, wo) < 0.0f) Ng = -Ng;
+            f3 wi = reflect(rd, Ng);
+            L.shade[pi].Le = F(0, 0, 0);
+            L.shade[pi].T = F(1, 1, 1);
+            L.path[pi].ro = ro + rd * tmin + wi * 1.0e-5f;
+            L.path[pi].rd = wi;
+        }
+        if (gid < L.q[kDierectric].size()) {
+            uint32_t pi = L.q[kDierectric][gid];
+            float tmin = L.ext[pi].tmin;
+            f3 Ng = L.ext[pi].Ng, ro = L.path[pi].ro, rd = L.path[pi].rd, wo = -rd;
+            bool backside = dot(Ng, wo) < 0.0f;
+            float eta_i = 1.0f, eta_t = L.ior[L.mats[L.ext[pi].hit_surface_material].index];
+            if (backside) std::swap(eta_i, eta_t);
+            if (backside) Ng = -Ng;
+            float cc = std::fabs(dot(Ng, wo));
+            float g = std::sqrt(eta_t * eta_t / sqr(eta_i) - 1.0f + sqr(cc));
+            float fa = 0.5f * sqr(g - cc) / sqr(g + cc);
+            float fb = 1.0f + sqr(cc * (g + cc) - 1.0f) / sqr(cc * (g - cc) + 1.0f);
+            float f = fa * fb;
+            Rng st = L.rng[pi];
+            float u = uniform(st);
+            L.rng[pi] = st;
+            float comp = 1.0f;
+            f3 wi;
+            if (u < f) {
+                wi = reflect(rd, Ng);
+            } else {
+                float eta = eta_i / eta_t;
+                float NoD = dot(Ng, rd);
+                float k = 1.0f - eta * eta * (1.0f - NoD * NoD);
+                if (k <= 0.0f) {
+                    wi = reflect(rd, Ng);
+                } else {
+                    wi = eta * rd - (eta * NoD + std::sqrt(k)) * Ng;
+                    comp = sqr(eta_t / eta_i);
+                }
+            }
+            L.shade[pi].Le = F(0, 0, 0);
+            L.shade[pi].T = F(comp, comp, comp);
+            L.path[pi].ro = ro + rd * tmin + wi * 1.0e-5f;
+            L.path[pi].rd = wi;
+        }
+    }
+    // homogeneous_volume_through (homogeneous_volume_through.cl:7-45)
+    for (uint32_t item = 0; item < c.n; ++item) {
+        Rng st = L.rng[item];
+        L.ext[item].hit_surface_material = -1;
+        int hv = -1;
+        InVolumeList v = L.vol[item];
+        if (0 < v.count) {
+            uint64_t u = uniform_integer(st);
+            hv = v.volumes[u % (uint64_t)v.count];
+        }
+        L.ext[item].hit_volume_material = hv;
+        float tmin = FLT_MAX;
+        if (0 <= hv) {
+            const Volume& V = L.vols[L.mats[hv].index];
+            float u = uniform(st);
+            tmin = -1.0f / V.C * logf_(u);
+        }
+        L.ext[item].tmin = tmin;
+        L.rng[item] = st;
+    }
+    // extension_ray_cast (extension_ray_cast_stackless.cl:202-246), all slots as one batch
+    std::vector<Ray> rays(c.n);
+    std::vector<Hit> hits(c.n);
+    for (uint32_t i = 0; i < c.n; ++i) {
+        const WavefrontPath& p = L.path[i];
+        rays[i] = Ray{{p.ro.x, p.ro.y, p.ro.z}, L.ext[i].tmin, {p.rd.x, p.rd.y, p.rd.z}, 0};
+    }
+    if (L.sl_nodes || L.mt_nodes) {
+        if (c.variant == 1)
+            orc_mtbvh_intersect(L.mt_nodes, L.mt_links, L.mt_nn, L.mt_prims, L.points, L.stride, L.indices,
+                                L.sphere_begin, L.spheres, c.n, rays.data(), hits.data(), nullptr, (int)c.nthreads);
+        else
+            orc_stackless_intersect(L.sl_nodes, L.sl_prims, L.points, L.stride, L.indices, L.sphere_begin, L.spheres,
+                                    c.n, rays.data(), hits.data(), nullptr, (int)c.nthreads);
+        std::vector<float> ng(3 * (size_t)c.n);
+        orc_hit_normals(c.n, rays.data(), hits.data(), L.points, L.stride, L.indices, L.sphere_begin, L.spheres,
+                        ng.data());
+        for (uint32_t i = 0; i < c.n; ++i) {
+            if (hits[i].prim < 0) continue;
+            L.ext[i].hit_surface_material = hits[i].prim;
+            L.ext[i].tmin = hits[i].t;
+            L.ext[i].Ng = F(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2]);
+            L.ext[i].hit_volume_material = -1;
+        }
+    }
+    for (int k = 1; k < 7; ++k) L.q[k].clear();
+    // logic (logic.cl:117-280), stable queue order = slot order
+    for (uint32_t gid = 0; gid < c.n; ++gid) {
+        WavefrontPath& p = L.path[gid];
+        uint32_t logic_i = p.logic_i++;
+        f3 Lr = p.L + p.T * L.shade[gid].Le;
+        f3 T = p.T * L.shade[gid].T;
+        int hs = L.ext[gid].hit_surface_material, hv = L.ext[gid].hit_volume_material;
+        bool evalEnv, newPath;
+        if (hs < 0 && hv < 0) newPath = evalEnv = true;
+        else newPath = evalEnv = false;
+        float lum = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+        float cp = 1.0f;
+        if (c.rr_start < logic_i) cp = fmn(lum, 1.0f);
+        float u = uniform(L.rng[gid]);
+        if (u < cp) T = T / cp;
+        else newPath = true;
+        if (lum < 1.0e-5f) newPath = true;
+        if (c.max_depth != 0 && logic_i >= c.max_depth) newPath = true;
+        if (evalEnv) Lr = Lr + T * get_envmap_value(L.env, p.rd, logic_i == 0);
+        if (newPath) {
+            bool ok = std::isfinite(Lr.x) && std::isfinite(Lr.y) && std::isfinite(Lr.z) && Lr.x < c.clamp &&
+                      Lr.y < c.clamp && Lr.z < c.clamp;
+            if (ok) {
+                uint32_t px = p.pixel_index;
+                L.accum[3 * (size_t)px + 0] += Lr.x;
+                L.accum[3 * (size_t)px + 1] += Lr.y;
+                L.accum[3 * (size_t)px + 2] += Lr.z;
+                L.accum_count[px] += 1;
+            }
+        }
+        p.L = Lr;
+        p.T = T;
+        int qi;
+        if (newPath) qi = 0;
+        else if (0 <= hs) qi = L.mats[hs].type;
+        else qi = kVolumeInside;
+        L.q[qi].push_back(gid);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Reference-order CPU lane. Arrays passed in must outlive the handle.
+struct orc_wf_desc {
+    uint32_t n, rr_start, max_depth, quirks, variant, nthreads;
+    float clamp, bxdf_p;
+    uint64_t seed;
+    const float* points;
+    uint32_t stride;
+    const uint32_t* indices;
+    int32_t sphere_begin;
+    const float* spheres;
+    const void* sl_nodes;
+    const uint32_t* sl_prims;
+    const void* mt_nodes;
+    const int32_t* mt_links;
+    uint32_t mt_nn;
+    const uint32_t* mt_prims;
+    uint32_t nprims;
+    const int32_t* mat_types;
+    const int32_t* mat_indices;
+    const float* lambertians;  // 7 floats each: Le, R, back (as float)
+    uint32_t nl;
+    const float* wards;  // 9 floats each: ax, ay, refl, edge, falloff
+    uint32_t nw;
+    const float* iors;
+    uint32_t nd;
+    const float* volumes;  // 4 floats each: C, R
+    uint32_t nv;
+    const float* env_rgb;
+    uint32_t env_channels, env_w, env_h;
+    const float* camera;  // 21 floats: eye, forward, up, right, o, r, b
+    uint32_t width, height;
+    float lens_radius;
+};
+
+void* orc_wf_create(const orc_wf_desc* d) {
+    Lane* L = new Lane();
+    L->cfg = Cfg{d->n, d->rr_start ? d->rr_start : 25u, d->max_depth, d->quirks, d->variant,
+                 d->nthreads ? d->nthreads : 1u, d->clamp > 0 ? d->clamp : 1000.0f, d->bxdf_p > 0 ? d->bxdf_p : 0.5f,
+                 d->seed};
+    L->points = d->points;
+    L->stride = d->stride;
+    L->indices = d->indices;
+    L->sphere_begin = d->sphere_begin;
+    L->spheres = d->spheres;
+    L->sl_nodes = d->sl_nodes;
+    L->sl_prims = d->sl_prims;
+    L->mt_nodes = d->mt_nodes;
+    L->mt_links = d->mt_links;
+    L->mt_nn = d->mt_nn;
+    L->mt_prims = d->mt_prims;
+    L->mats.resize(d->nprims);
+    for (uint32_t i = 0; i < d->nprims; ++i) L->mats[i] = Material{d->mat_types[i], d->mat_indices[i]};
+    L->lamb.resize(d->nl);
+    for (uint32_t i = 0; i < d->nl; ++i) {
+        const float* s = d->lambertians + 7 * (size_t)i;
+        for (int k = 0; k < 3; ++k) {
+            L->lamb[i].Le[k] = s[k];
+            L->lamb[i].R[k] = s[3 + k];
+        }
+        L->lamb[i].back = (int32_t)s[6];
+    }
+    L->ward.resize(d->nw);
+    for (uint32_t i = 0; i < d->nw; ++i) {
+        const float* s = d->wards + 9 * (size_t)i;
+        Ward& w = L->ward[i];
+        w.ax = s[0];
+        w.ay = s[1];
+        for (int k = 0; k < 3; ++k) {
+            w.refl[k] = s[2 + k];
+            w.edge[k] = s[5 + k];
+        }
+        w.falloff = s[8];
+    }
+    L->ior.assign(d->iors, d->iors + d->nd);
+    L->vols.resize(d->nv);
+    for (uint32_t i = 0; i < d->nv; ++i) {
+        const float* s = d->volumes + 4 * (size_t)i;
+        L->vols[i].C = s[0];
+        for (int k = 0; k < 3; ++k) L->vols[i].R[k] = s[1 + k];
+    }
+    build_envmap(d->env_rgb, d->env_channels, d->env_w, d->env_h, &L->env);
+    Camera& c = L->cam;
+    float* dst[7] = {c.eye, c.fwd, c.up, c.right, c.o, c.r, c.b};
+    for (int a = 0; a < 7; ++a)
+        for (int k = 0; k < 3; ++k) dst[a][k] = d->camera[3 * a + k];
+    c.w = d->width;
+    c.h = d->height;
+    c.lens = d->lens_radius;
+    const uint32_t n = d->n;
+    L->path.assign(n, WavefrontPath{});
+    L->vol.assign(n, InVolumeList{});
+    L->ext.assign(n, ExtensionResult{-1, -1, FLT_MAX, F(0, 0, 0)});
+    L->shade.assign(n, ShadingResult{});
+    L->isample.assign(n, IncidentSample{});
+    L->rng.assign(n, Rng{});
+    L->accum.assign(3 * (size_t)c.w * c.h, 0.0);
+    L->accum_count.assign((size_t)c.w * c.h, 0);
+    return L;
+}
+
+void orc_wf_destroy(void* h) { delete (Lane*)h; }
+
+// WavefrontLane::initialize (common/wavefront_path_tracing.hpp:532-570)
+void orc_wf_initialize(void* h, uint32_t lane_index, uint32_t num_lanes) {
+    Lane* L = (Lane*)h;
+    const uint32_t n = L->cfg.n, P = L->cam.w * L->cam.h;
+    L->next_pixel = (uint64_t)lane_index * (P / num_lanes);
+    uint64_t seed_offset = L->cfg.seed + (uint64_t)lane_index * n;
+    for (uint32_t gid = 0; gid < n; ++gid) {
+        uint64_t x = seed_offset + gid;
+        uint64_t r0 = splitmix64(&x), r1 = splitmix64(&x);
+        Rng& s = L->rng[gid];
+        s.s[0] = (uint32_t)r0;
+        s.s[1] = (uint32_t)(r0 >> 32);
+        s.s[2] = (uint32_t)r1;
+        s.s[3] = (uint32_t)(r1 >> 32);
+        if ((s.s[0] | s.s[1] | s.s[2] | s.s[3]) == 0) s.s[0] = 1;
+    }
+    for (int k = 0; k < 7; ++k) L->q[k].clear();
+    for (uint32_t gid = 0; gid < n; ++gid) L->q[0].push_back(gid);
+    std::fill(L->accum.begin(), L->accum.end(), 0.0);
+    std::fill(L->accum_count.begin(), L->accum_count.end(), 0u);
+    for (uint32_t i = 0; i < n; ++i) {
+        L->ext[i] = ExtensionResult{-1, -1, FLT_MAX, F(0, 0, 0)};
+        L->vol[i] = InVolumeList{};
+    }
+}
+
+void orc_wf_step(void* h, uint32_t steps) {
+    Lane* L = (Lane*)h;
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (!L->cfg.quirks) {
+            // without the quirk, a new path does no strategy draw for a stale volume id
+            for (uint32_t pi : L->q[0]) L->ext[pi].hit_volume_material = -1;
+        }
+        step(*L);
+    }
+}
+
+void orc_wf_queue_counts(void* h, uint32_t* counts7, uint64_t* next_pixel) {
+    Lane* L = (Lane*)h;
+    for (int k = 0; k < 7; ++k) counts7[k] = (uint32_t)L->q[k].size();
+    if (next_pixel) *next_pixel = L->next_pixel;
+}
+
+void orc_wf_read_accumulation(void* h, double* rgb, uint32_t* counts) {
+    Lane* L = (Lane*)h;
+    size_t P = (size_t)L->cam.w * L->cam.h;
+    if (rgb) memcpy(rgb, L->accum.data(), 3 * P * 8);
+    if (counts) memcpy(counts, L->accum_count.data(), P * 4);
+}
+
+// Same record layouts as rgpu_wavefront_read_state.
+void orc_wf_read_state(void* h, uint32_t* rng, float* rays, float* paths, void* ext, float* shade, int32_t* vol,
+                       uint32_t* items) {
+    Lane* L = (Lane*)h;
+    const uint32_t n = L->cfg.n;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (rng) memcpy(rng + 4 * (size_t)i, L->rng[i].s, 16);
+        const WavefrontPath& p = L->path[i];
+        if (rays) {
+            float* r = rays + 8 * (size_t)i;
+            r[0] = p.ro.x;
+            r[1] = p.ro.y;
+            r[2] = p.ro.z;
+            r[3] = 0.0f;
+            r[4] = p.rd.x;
+            r[5] = p.rd.y;
+            r[6] = p.rd.z;
+            memcpy(&r[7], &p.pixel_index, 4);
+        }
+        if (paths) {
+            float* q = paths + 8 * (size_t)i;
+            q[0] = p.T.x;
+            q[1] = p.T.y;
+            q[2] = p.T.z;
+            memcpy(&q[3], &p.logic_i, 4);
+            q[4] = p.L.x;
+            q[5] = p.L.y;
+            q[6] = p.L.z;
+            q[7] = 0.0f;
+        }
+        if (ext) {
+            const ExtensionResult& e = L->ext[i];
+            int32_t* q = (int32_t*)ext + 8 * (size_t)i;
+            q[0] = e.hit_surface_material;
+            q[1] = e.hit_volume_material;
+            memcpy(&q[2], &e.tmin, 4);
+            memcpy(&q[3], &e.Ng.x, 4);
+            memcpy(&q[4], &e.Ng.y, 4);
+            memcpy(&q[5], &e.Ng.z, 4);
+            q[6] = q[7] = 0;
+        }
+        if (shade) {
+            const ShadingResult& s = L->shade[i];
+            float* q = shade + 8 * (size_t)i;
+            q[0] = s.Le.x;
+            q[1] = s.Le.y;
+            q[2] = s.Le.z;
+            q[3] = s.T.x;
+            q[4] = s.T.y;
+            q[5] = s.T.z;
+            q[6] = q[7] = 0.0f;
+        }
+        if (vol) {
+            memcpy(vol + 8 * (size_t)i, L->vol[i].volumes, 28);
+            vol[8 * (size_t)i + 7] = L->vol[i].count;
+        }
+    }
+    if (items)
+        for (int k = 0; k < 7; ++k)
+            for (size_t j = 0; j < L->q[k].size(); ++j) items[(size_t)k * n + j] = L->q[k][j];
+}
+
+// Envmap tables of the restated ImageEnvmap, for comparison with the product's.
+void* orc_envmap_build(const float* rgb, uint32_t channels, uint32_t w, uint32_t h) {
+    Envmap* e = new Envmap();
+    build_envmap(rgb, channels, w, h, e);
+    return e;
+}
+void orc_envmap_export(void* h, int table, float* pdf, float* heights, int32_t* aliases, float* frags) {
+    Envmap* e = (Envmap*)h;
+    size_t P = (size_t)e->w * e->h;
+    for (size_t i = 0; i < P; ++i) {
+        if (pdf) pdf[i] = e->pdf[table][i];
+        if (heights) heights[i] = e->alias[table][i].height;
+        if (aliases) aliases[i] = e->alias[table][i].alias;
+        if (frags) memcpy(frags + 4 * i, &e->frags[i], 16);
+    }
+}
+void orc_envmap_destroy(void* h) { delete (Envmap*)h; }
+// Envmap radiance lookups and samples for direct tests.
+void orc_envmap_lookup(void* h, uint32_t n, const float* dirs, int filter, float* out) {
+    Envmap* e = (Envmap*)h;
+    for (uint32_t i = 0; i < n; ++i) {
+        f3 r = get_envmap_value(*e, F(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), filter != 0);
+        out[3 * i] = r.x;
+        out[3 * i + 1] = r.y;
+        out[3 * i + 2] = r.z;
+    }
+}
+// Draw n samples from table `table` (0 uniform; 1..6 by the normal's six-axis mixture when
+// normal != NULL): directions and pdfs, consuming `state`.
+void orc_envmap_sample(void* h, uint32_t* state, uint32_t n, const float* normal, float* dirs, float* pdfs) {
+    Envmap* e = (Envmap*)h;
+    Rng st;
+    memcpy(st.s, state, 16);
+    uint32_t count = e->w * e->h;
+    for (uint32_t i = 0; i < n; ++i) {
+        f3 wi;
+        float pdf;
+        if (normal) {
+            f3 nn = F(normal[0], normal[1], normal[2]);
+            int xt = 0.0f < nn.x ? 1 : 2, yt = 0.0f < nn.y ? 3 : 4, zt = 0.0f < nn.z ? 5 : 6;
+            float u = uniform(st);
+            f3 ap = nn * nn;
+            int sel = u < ap.x ? xt : (u < ap.x + ap.y ? yt : zt);
+            uint32_t frag = alias_method(st, e->alias[sel].data(), count);
+            pdf = 0.0f;
+            pdf += ap.x * e->pdf[xt][frag];
+            pdf += ap.y * e->pdf[yt][frag];
+            pdf += ap.z * e->pdf[zt][frag];
+            wi = sample_from_fragment(e->frags[frag], st);
+        } else {
+            uint32_t frag = alias_method(st, e->alias[0].data(), count);
+            pdf = e->pdf[0][frag];
+            wi = sample_from_fragment(e->frags[frag], st);
+        }
+        dirs[3 * i] = wi.x;
+        dirs[3 * i + 1] = wi.y;
+        dirs[3 * i + 2] = wi.z;
+        pdfs[i] = pdf;
+    }
+    memcpy(state, st.s, 16);
+}
+void orc_envmap_pdf(void* h, uint32_t n, const float* dirs, const float* normal, float* pdfs) {
+    Envmap* e = (Envmap*)h;
+    for (uint32_t i = 0; i < n; ++i) {
+        f3 wi = F(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        pdfs[i] = normal ? envmap_pdf_sixAxis(*e, wi, F(normal[0], normal[1], normal[2]))
+                         : e->pdf[0][envmap_index(*e, wi)];
+    }
+}
+
+}  // extern "C"

@@ -303,3 +303,253 @@ def trace(ctx, scene, rays, variant="stackless", block=0):
     rd.free()
     hd.free()
     return out
+
+
+# ====================================================================== wavefront path tracing
+MAT_LAMBERTIAN, MAT_SPECULAR, MAT_DIELECTRIC, MAT_WARD, MAT_VOLUME = 1, 2, 3, 4, 5
+WF_ACCUM64 = 1
+WF_NO_REFERENCE_QUIRKS = 2
+QUEUE_NAMES = ("new_path", "lambertian", "specular", "dielectric", "ward", "volume_surface", "volume_inside")
+
+
+class rgpu_lambertian(C.Structure):
+    _fields_ = [("Le", C.c_float * 3), ("R", C.c_float * 3), ("back_emission", C.c_int32)]
+
+
+class rgpu_ward(C.Structure):
+    _fields_ = [("alpha_x", C.c_float), ("alpha_y", C.c_float), ("reflectance", C.c_float * 3),
+                ("edgetint", C.c_float * 3), ("falloff", C.c_float)]
+
+
+class rgpu_dielectric(C.Structure):
+    _fields_ = [("ior", C.c_float)]
+
+
+class rgpu_volume(C.Structure):
+    _fields_ = [("C", C.c_float), ("R", C.c_float * 3)]
+
+
+class rgpu_camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("forward", C.c_float * 3), ("up", C.c_float * 3), ("right", C.c_float * 3),
+                ("imageplane_o", C.c_float * 3), ("imageplane_r", C.c_float * 3), ("imageplane_b", C.c_float * 3),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("lens_radius", C.c_float)]
+
+    def as_array(self):
+        return np.array([*self.eye, *self.forward, *self.up, *self.right, *self.imageplane_o, *self.imageplane_r,
+                         *self.imageplane_b], np.float32)
+
+
+class rgpu_wavefront_config(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("path_count", C.c_uint32), ("seed", C.c_uint64),
+                ("rr_start", C.c_uint32), ("max_depth", C.c_uint32), ("radiance_clamp", C.c_float),
+                ("bxdf_select_p", C.c_float), ("flags", C.c_uint32), ("variant", C.c_uint32),
+                ("queue_blocks", C.c_uint32)]
+
+
+_sig("rgpu_materials_create", C.c_int, vp, u32, vp, vp, vp, u32, vp, u32, vp, u32, vp, u32, C.POINTER(vp))
+_sig("rgpu_materials_destroy", C.c_int, vp)
+_sig("rgpu_envmap_create", C.c_int, vp, vp, u32, u32, u32, C.POINTER(vp))
+_sig("rgpu_envmap_destroy", C.c_int, vp)
+_sig("rgpu_envmap_export", C.c_int, vp, C.c_int, vp, vp, vp, vp, vp)
+_sig("rgpu_camera_look_at", C.c_int, vp, vp, vp, C.c_float, C.c_float, u32, u32, C.c_float, C.POINTER(rgpu_camera))
+_sig("rgpu_wavefront_create", C.c_int, vp, vp, vp, vp, C.POINTER(rgpu_wavefront_config), C.POINTER(vp))
+_sig("rgpu_wavefront_destroy", C.c_int, vp)
+_sig("rgpu_wavefront_initialize", C.c_int, vp, C.POINTER(rgpu_camera), u32, u32)
+_sig("rgpu_wavefront_step", C.c_int, vp, vp, u32)
+_sig("rgpu_wavefront_time_steps", C.c_int, vp, u32, C.POINTER(C.c_double))
+_sig("rgpu_wavefront_queue_counts", C.c_int, vp, vp, C.POINTER(C.c_uint64))
+_sig("rgpu_wavefront_read_accumulation", C.c_int, vp, vp, vp)
+_sig("rgpu_wavefront_tonemap", C.c_int, vp, vp)
+_sig("rgpu_wavefront_stat", C.c_int, vp, C.POINTER(C.c_uint64))
+_sig("rgpu_wavefront_read_state", C.c_int, vp, vp, vp, vp, vp, vp, vp, vp)
+EXPORTED += [
+    "rgpu_materials_create", "rgpu_materials_destroy", "rgpu_envmap_create", "rgpu_envmap_destroy",
+    "rgpu_envmap_export", "rgpu_camera_look_at", "rgpu_wavefront_create", "rgpu_wavefront_destroy",
+    "rgpu_wavefront_initialize", "rgpu_wavefront_step", "rgpu_wavefront_time_steps", "rgpu_wavefront_queue_counts",
+    "rgpu_wavefront_read_accumulation", "rgpu_wavefront_tonemap", "rgpu_wavefront_stat", "rgpu_wavefront_read_state",
+]
+
+EXT_DTYPE = np.dtype([("hit_surface", np.int32), ("hit_volume", np.int32), ("tmin", np.float32), ("Ng", np.float32, 3),
+                      ("pad", np.float32, 2)])
+PATH_DTYPE = np.dtype([("T", np.float32, 3), ("logic_i", np.uint32), ("L", np.float32, 3), ("pad", np.uint32)])
+SHADE_DTYPE = np.dtype([("Le", np.float32, 3), ("T", np.float32, 3), ("pad", np.float32, 2)])
+VOL_DTYPE = np.dtype([("volumes", np.int32, 7), ("count", np.int32)])
+STATE_RAY_DTYPE = np.dtype([("ro", np.float32, 3), ("tmax", np.float32), ("rd", np.float32, 3), ("pixel", np.uint32)])
+
+
+def camera_look_at(eye, target, up, fov_y_deg, width, height, lens_radius=0.0, focus_distance=0.0):
+    cam = rgpu_camera()
+    a = [np.ascontiguousarray(x, np.float32) for x in (eye, target, up)]
+    rc = lib.rgpu_camera_look_at(ptr(a[0]), ptr(a[1]), ptr(a[2]), fov_y_deg, focus_distance, width, height,
+                                 lens_radius, C.byref(cam))
+    if rc != RGPU_OK:
+        raise RgpuError(rc, "rgpu_camera_look_at")
+    return cam
+
+
+class MaterialSet:
+    """Per-primitive material assignment plus the per-type parameter tables (host side)."""
+
+    def __init__(self, types, indices, lambertians=(), wards=(), dielectrics=(), volumes=()):
+        self.types = np.ascontiguousarray(types, np.int32)
+        self.indices = np.ascontiguousarray(indices, np.int32)
+        self.lambertians = list(lambertians)  # (Le3, R3)
+        self.wards = list(wards)              # (ax, ay, refl3, edge3, falloff)
+        self.dielectrics = list(dielectrics)  # ior
+        self.volumes = list(volumes)          # (C, R3)
+
+    def c_arrays(self):
+        L = (rgpu_lambertian * max(1, len(self.lambertians)))()
+        for i, (Le, R) in enumerate(self.lambertians):
+            L[i].Le[:] = list(map(float, Le))
+            L[i].R[:] = list(map(float, R))
+            L[i].back_emission = 0
+        W = (rgpu_ward * max(1, len(self.wards)))()
+        for i, (ax, ay, refl, edge, fo) in enumerate(self.wards):
+            W[i].alpha_x, W[i].alpha_y, W[i].falloff = ax, ay, fo
+            W[i].reflectance[:] = list(map(float, refl))
+            W[i].edgetint[:] = list(map(float, edge))
+        D = (rgpu_dielectric * max(1, len(self.dielectrics)))()
+        for i, ior in enumerate(self.dielectrics):
+            D[i].ior = ior
+        V = (rgpu_volume * max(1, len(self.volumes)))()
+        for i, (c, R) in enumerate(self.volumes):
+            V[i].C = c
+            V[i].R[:] = list(map(float, R))
+        return L, W, D, V
+
+    def flat(self):
+        """Float tables in the oracle's layout."""
+        lam = np.array([[*Le, *R, 0.0] for Le, R in self.lambertians] or [[0.0] * 7], np.float32)
+        war = np.array([[ax, ay, *r, *e, fo] for ax, ay, r, e, fo in self.wards] or [[0.0] * 9], np.float32)
+        ior = np.array(self.dielectrics or [1.5], np.float32)
+        vol = np.array([[c, *R] for c, R in self.volumes] or [[1.0, 0, 0, 0]], np.float32)
+        return lam, war, ior, vol
+
+
+class Materials:
+    def __init__(self, ctx, ms):
+        self.ctx = ctx
+        L, W, D, V = ms.c_arrays()
+        h = vp()
+        ctx.check(lib.rgpu_materials_create(ctx.h, len(ms.types), ptr(ms.types), ptr(ms.indices), L,
+                                            len(ms.lambertians), W, len(ms.wards), D, len(ms.dielectrics), V,
+                                            len(ms.volumes), C.byref(h)), "rgpu_materials_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.rgpu_materials_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Envmap:
+    """Environment map + alias tables. ctx=None builds the host tables only."""
+
+    def __init__(self, ctx, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        assert rgb.ndim == 3 and rgb.shape[2] in (3, 4)
+        self.ctx, self.rgb = ctx, rgb
+        self.height, self.width = rgb.shape[:2]
+        h = vp()
+        rc = lib.rgpu_envmap_create(ctx.h if ctx else None, ptr(rgb), rgb.shape[2], self.width, self.height,
+                                    C.byref(h))
+        if rc != RGPU_OK:
+            raise RgpuError(rc, "rgpu_envmap_create")
+        self.h = h
+
+    def export(self, table):
+        P = self.width * self.height
+        pdf = np.zeros(P, np.float32)
+        hts = np.zeros(P, np.float32)
+        al = np.zeros(P, np.int32)
+        fr = np.zeros((P, 4), np.float32)
+        deg = C.c_int32(0)
+        lib.rgpu_envmap_export(self.h, table, ptr(pdf), ptr(hts), ptr(al), ptr(fr), C.byref(deg))
+        return dict(pdf=pdf, height=hts, alias=al, fragments=fr, degenerate=bool(deg.value))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.rgpu_envmap_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Wavefront:
+    """One resident wavefront (WavefrontLane) on the GPU."""
+
+    def __init__(self, ctx, scene, materials, envmap, path_count, seed=0, rr_start=0, max_depth=0, clamp=0.0,
+                 bxdf_p=0.0, flags=0, variant="stackless", queue_blocks=0):
+        self.ctx = ctx
+        self.refs = (scene, materials, envmap)
+        self.n = path_count
+        cfg = rgpu_wavefront_config(RGPU_ABI_VERSION, path_count, seed, rr_start, max_depth, clamp, bxdf_p, flags,
+                                    VARIANTS[variant] if isinstance(variant, str) else variant, queue_blocks)
+        h = vp()
+        ctx.check(lib.rgpu_wavefront_create(ctx.h, scene.h, materials.h, envmap.h, C.byref(cfg), C.byref(h)),
+                  "rgpu_wavefront_create")
+        self.h = h
+        self.npixels = 0
+
+    def initialize(self, camera, lane_index=0, num_lanes=1):
+        self.ctx.check(lib.rgpu_wavefront_initialize(self.h, C.byref(camera), lane_index, num_lanes),
+                       "rgpu_wavefront_initialize")
+        self.npixels = camera.width * camera.height
+        self.camera = camera
+
+    def step(self, steps=1):
+        self.ctx.check(lib.rgpu_wavefront_step(self.h, None, steps), "rgpu_wavefront_step")
+
+    def time_steps(self, steps):
+        ms = C.c_double(0)
+        self.ctx.check(lib.rgpu_wavefront_time_steps(self.h, steps, C.byref(ms)), "rgpu_wavefront_time_steps")
+        return ms.value
+
+    def queue_counts(self):
+        c = np.zeros(7, np.uint32)
+        npx = C.c_uint64(0)
+        self.ctx.check(lib.rgpu_wavefront_queue_counts(self.h, ptr(c), C.byref(npx)), "rgpu_wavefront_queue_counts")
+        return c, npx.value
+
+    def accumulation(self):
+        rgb = np.zeros((self.npixels, 3), np.float64)
+        cnt = np.zeros(self.npixels, np.uint32)
+        self.ctx.check(lib.rgpu_wavefront_read_accumulation(self.h, ptr(rgb), ptr(cnt)),
+                       "rgpu_wavefront_read_accumulation")
+        return rgb, cnt
+
+    def tonemap(self):
+        out = np.zeros((self.npixels, 4), np.uint8)
+        self.ctx.check(lib.rgpu_wavefront_tonemap(self.h, ptr(out)), "rgpu_wavefront_tonemap")
+        return out
+
+    def stat(self):
+        t = C.c_uint64(0)
+        self.ctx.check(lib.rgpu_wavefront_stat(self.h, C.byref(t)), "rgpu_wavefront_stat")
+        return t.value
+
+    def state(self):
+        n = self.n
+        st = dict(rng=np.zeros((n, 4), np.uint32), rays=np.zeros(n, STATE_RAY_DTYPE), paths=np.zeros(n, PATH_DTYPE),
+                  ext=np.zeros(n, EXT_DTYPE), shade=np.zeros(n, SHADE_DTYPE), vol=np.zeros(n, VOL_DTYPE),
+                  items=np.zeros((7, n), np.uint32))
+        self.ctx.check(lib.rgpu_wavefront_read_state(self.h, *[ptr(st[k]) for k in
+                                                                ("rng", "rays", "paths", "ext", "shade", "vol",
+                                                                 "items")]), "rgpu_wavefront_read_state")
+        return st
+
+    def close(self):
+        if self.h:
+            lib.rgpu_wavefront_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libraccoongpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.cpp", "bvh_build.cpp", "traverse.hip", "util.hip"]
+SOURCES = ["capi.cpp", "bvh_build.cpp", "envmap_build.cpp", "traverse.hip", "util.hip", "wavefront.hip", "wavefront_capi.cpp"]
 
 # -ffp-contract=off keeps every float expression unfused so results match the CPU
 # oracle bit for bit; packed fp32 instructions are disabled for correctness.
@@ -38,7 +38,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         if src.endswith(".hip"):
             cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj]
         else:
-            cmd = [HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
+            cmd = [HIPCC, *COMMON, *DEVICE, "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

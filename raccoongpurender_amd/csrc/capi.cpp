@@ -15,23 +15,12 @@
 #include "bvh_build.h"
 #include "rgpu_traverse.h"
 #include "rgpu_ctx.h"
+#include "rgpu_scene.h"
 
 using namespace rgpu;
 
 struct rgpu_bvh {
     BVHTree tree;
-};
-
-struct rgpu_scene {
-    rgpu_ctx* ctx = nullptr;
-    TraceScene dev;
-    void* sl = nullptr;
-    void* mt = nullptr;
-    void* prims = nullptr;
-    void* points = nullptr;
-    void* indices = nullptr;
-    void* spheres = nullptr;
-    uint64_t bytes[4] = {0, 0, 0, 0};
 };
 
 namespace rgpu {
@@ -322,6 +311,7 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     s->dev.spheres = (const float4*)s->spheres;
     s->dev.nnodes = (uint32_t)bvh->tree.nodes.size();
     s->dev.sphere_begin = (int32_t)ntris;
+    s->nprims = ntris + nspheres;
     s->bytes[0] = sl.size() * sizeof(SLNode);
     s->bytes[1] = mt.size() * sizeof(MTNode);
     s->bytes[2] = pr.size() * sizeof(PrimRec);

@@ -1,0 +1,133 @@
+// Device layouts and launch interface of the wavefront path-tracing stages.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "envmap_build.h"
+#include "rgpu_traverse.h"
+
+namespace rgpu {
+
+// Material type ids (kernels/types.cl:4-10, common/material_type.hpp:14-19).
+enum : int32_t {
+    kMatNone = 0,
+    kMatLambertian = 1,
+    kMatSpecular = 2,
+    kMatDielectric = 3,
+    kMatWard = 4,
+    kMatVolume = 5,        // boundary of a homogeneous medium
+    kMatVolumeInside = 6,  // scattering event inside a medium (queue id only)
+};
+static const int kNumQueues = 7;  // new path + the six material queues of kernels/logic.cl:232-246
+
+struct Material {
+    int32_t type;
+    int32_t index;
+};
+struct Lambertian {  // kernels/types.cl:78-82
+    float Le[3];
+    float R[3];
+    int32_t back_emission;
+    int32_t pad;
+};
+struct Ward {  // kernels/types.cl:92-97, with separate alpha_x / alpha_y
+    float alpha_x, alpha_y;
+    float reflectance[3];
+    float edgetint[3];
+    float falloff;
+    float pad;
+};
+struct Dielectric {
+    float ior;  // the reference hard-codes 1.5 (kernels/delta_materials.cl:95-96)
+};
+struct Volume {  // kernels/types.cl:99-102
+    float C;     // extinction used for free flight
+    float R[3];  // single-scattering colour
+};
+
+// Per-slot state records (32 B each; the reference's AoS structs of kernels/types.cl).
+struct PathRec {  // WavefrontPath T, L, logic_i (ro/rd/pixel live in the TraceRay)
+    float T[3];
+    uint32_t logic_i;
+    float L[3];
+    uint32_t pad;
+};
+struct ExtRec {  // ExtensionResult
+    int32_t hit_surface;
+    int32_t hit_volume;
+    float tmin;
+    float ngx;
+    float ngy;
+    float ngz;
+    float pad0, pad1;
+};
+struct ShadeRec {  // ShadingResult
+    float Le[3];
+    float T[3];
+    float pad0, pad1;
+};
+struct VolList {  // InVolumeList
+    int32_t volumes[7];
+    int32_t count;
+};
+
+struct Camera {  // StandardCamera (kernels/camera.cl:4-22) + lens radius
+    float eye[3], forward[3], up[3], right[3];
+    float o[3], r[3], b[3];  // image plane origin and per-pixel steps
+    uint32_t width, height;
+    float lens_radius;
+};
+
+struct WFParams {
+    uint32_t n;               // resident path slots
+    uint32_t rr_start;        // Russian roulette after this many bounces (25)
+    uint32_t max_depth;       // 0 = unlimited
+    uint32_t accum64;         // accumulate in double
+    uint32_t quirks;          // reproduce the reference's stale-volume strategy draws
+    float clamp;              // per-sample radiance clamp (1000)
+    float bxdf_p;             // selection probability of BSDF sampling (0.5)
+};
+
+struct DevMaterials {
+    const Material* mat = nullptr;  // one per primitive
+    const Lambertian* lambertian = nullptr;
+    const Ward* ward = nullptr;
+    const Dielectric* dielectric = nullptr;
+    const Volume* volume = nullptr;
+};
+
+struct DevEnvmap {
+    const float4* texels = nullptr;  // width*height RGBA
+    const EnvFragment* fragments = nullptr;
+    const float* pdf = nullptr;           // 7 tables of width*height
+    const AliasBucket* alias = nullptr;   // 7 tables of width*height
+    uint32_t width = 0, height = 0;
+};
+
+struct DevState {
+    TraceRay* ray = nullptr;  // ro, tmax, rd, pixel index
+    PathRec* path = nullptr;
+    ExtRec* ext = nullptr;
+    ShadeRec* shade = nullptr;
+    VolList* vol = nullptr;
+    uint4* rng = nullptr;
+    uint8_t* qsel = nullptr;          // queue chosen by logic
+    uint32_t* items = nullptr;        // kNumQueues * n queue items
+    uint32_t* counts = nullptr;       // kNumQueues counts
+    uint32_t* block_counts = nullptr; // kNumQueues * nblocks
+    unsigned long long* next_pixel = nullptr;
+    float* accum = nullptr;           // 4 floats per pixel (r, g, b, count bits) or
+    double* accum64 = nullptr;        // 3 doubles per pixel + counts in accum_count
+    uint32_t* accum_count = nullptr;
+    uint32_t nblocks = 0;             // logic blocks (n / 256 rounded up)
+};
+
+hipError_t wf_initialize(const DevState& st, const WFParams& p, uint64_t seed_offset, uint64_t next_pixel,
+                         uint32_t npixels, hipStream_t s);
+hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
+                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, hipStream_t s);
+hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hipStream_t s);
+hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s);
+hipError_t wf_stat(const DevState& st, uint32_t npixels, bool accum64, unsigned long long* total, hipStream_t s);
+
+}  // namespace rgpu

@@ -1,0 +1,216 @@
+// Device-side traversal routines shared by the ray-cast kernels and the wavefront stages.
+// See traverse.hip for the reference behaviour they restate.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rgpu_types.h"
+
+namespace rgpu {
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 sub3(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
+    return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// One axis of the slab test: entry/exit parameters with the NaN substitution of
+// kernels/slab.cl:15-16 (t0 first, then t1 from the updated t0).
+__device__ __forceinline__ void slab_axis(float lo, float hi, float o, float inv, float* tn, float* tf) {
+    float t0 = (lo - o) * inv;
+    float t1 = (hi - o) * inv;
+    t0 = __builtin_isnan(t0) ? -t1 : t0;
+    t1 = __builtin_isnan(t1) ? -t0 : t1;
+    *tn = fminf(t0, t1);
+    *tf = fmaxf(t0, t1);
+}
+
+__device__ __forceinline__ bool slab_hit(const float* lo, const float* hi, F3 o, F3 inv, float farclip,
+                                         float* h0, float* h1) {
+    float nx, fx, ny, fy, nz, fz;
+    slab_axis(lo[0], hi[0], o.x, inv.x, &nx, &fx);
+    slab_axis(lo[1], hi[1], o.y, inv.y, &ny, &fy);
+    slab_axis(lo[2], hi[2], o.z, inv.z, &nz, &fz);
+    float rmin = fmaxf(fmaxf(nx, ny), nz);
+    float rmax = fminf(fminf(fx, fy), fz);
+    *h0 = fmaxf(0.0f, rmin);
+    *h1 = fminf(rmax, farclip);
+    return rmin <= rmax && 0.0f <= rmax && rmin <= farclip;
+}
+
+// Test one primitive record; updates (tmin, u, v, prim) when it is the closest so far.
+__device__ __forceinline__ void test_prim(const PrimRec* __restrict__ prims, uint32_t slot, int32_t sphere_begin,
+                                          F3 ro, F3 rd, float* tmin, float* uo, float* vo, int32_t* prim) {
+    const float4* rec = reinterpret_cast<const float4*>(prims + slot);
+    float4 q0 = rec[0];
+    float4 q2 = rec[2];
+    int32_t pid = (int32_t)__float_as_uint(q2.y);
+    if (pid < sphere_begin) {
+        float4 q1 = rec[1];
+        F3 v0 = f3(q0.x, q0.y, q0.z);
+        F3 e1 = f3(q0.w, q1.x, q1.y);
+        F3 e2 = f3(q1.z, q1.w, q2.x);
+        F3 pvec = cross3(rd, e2);
+        float det = dot3(e1, pvec);
+        if (fabsf(det) < 1.0e-8f) return;
+        float invDet = 1.0f / det;
+        F3 tvec = sub3(ro, v0);
+        float u = dot3(tvec, pvec) * invDet;
+        if (u < 0.0f || u > 1.0f) return;
+        F3 qvec = cross3(tvec, e1);
+        float v = dot3(rd, qvec) * invDet;
+        if (v < 0.0f || u + v > 1.0f) return;
+        float t = dot3(e2, qvec) * invDet;
+        if (t < 0.0f) return;
+        if (*tmin < t) return;
+        *tmin = t;
+        *uo = u;
+        *vo = v;
+        *prim = pid;
+    } else {
+        // kernels/extension_ray_cast_stackless.cl:52-76
+        F3 c = f3(q0.x, q0.y, q0.z);
+        float r = q0.w;
+        float r_sq = r * r;
+        F3 S = sub3(ro, c);
+        F3 crs = cross3(S, rd);
+        float d_sq = dot3(crs, crs);
+        float B = dot3(S, rd);
+        float D = r_sq - d_sq;
+        if (D < 0.0f) return;
+        float Ds = __builtin_sqrtf(D);
+        float t0 = -B - Ds;
+        float t1 = -B + Ds;
+        if (0.0f < t0 && t0 < *tmin) {
+            *tmin = t0;
+            *prim = pid;
+            return;
+        }
+        if (0.0f < t1 && t1 < *tmin) {
+            *tmin = t1;
+            *prim = pid;
+        }
+    }
+}
+
+// Per-ray traversal event counts (the instrumented kernels only).
+struct Cnt {
+    uint32_t branch = 0, leaf = 0, parent = 0, prim = 0, node = 0;
+};
+
+// Stackless traversal of one ray (kernels/extension_ray_cast_stackless.cl:81-200).
+template <bool kCount>
+__device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ nodes,
+                                                   const PrimRec* __restrict__ prims, int32_t sphere_begin,
+                                                   F3 ro, F3 rd, float* tmin_io, float* uo, float* vo,
+                                                   Cnt* cnt) {
+    const F3 inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    float tmin = *tmin_io;
+    float u = 0.0f, v = 0.0f;
+    int32_t prim = -1;
+    uint64_t bitstack = 0;
+    uint32_t node = 0;
+    const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
+    uint4 q3 = q3p[4 * node + 3];  // {a, b, parent, sibling}
+    for (;;) {
+        if ((q3.x & kLeafFlag) == 0) {
+            if (kCount) cnt->branch++;
+            const float4* rec = reinterpret_cast<const float4*>(nodes + node);
+            float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+            float lmin[3] = {q0.x, q0.y, q0.z}, lmax[3] = {q0.w, q1.x, q1.y};
+            float rmin[3] = {q1.z, q1.w, q2.x}, rmax[3] = {q2.y, q2.z, q2.w};
+            float Lh0, Lh1, Rh0, Rh1;
+            bool L = slab_hit(lmin, lmax, ro, inv, tmin, &Lh0, &Lh1);
+            bool R = slab_hit(rmin, rmax, ro, inv, tmin, &Rh0, &Rh1);
+            if (L || R) {
+                bitstack = bitstack << 1;
+                if (L && R) {
+                    bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
+                    node = nearL ? q3.x : q3.y;
+                    bitstack = bitstack | 1;
+                } else {
+                    node = L ? q3.x : q3.y;
+                }
+                q3 = q3p[4 * node + 3];
+                continue;
+            }
+        } else {
+            const uint32_t beg = q3.x & ~kLeafFlag, end = q3.y;
+            if (kCount) {
+                cnt->leaf++;
+                cnt->prim += end - beg;
+            }
+            for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, ro, rd, &tmin, &u, &v, &prim);
+        }
+        // Backtrack: climb while the top bit is 0, then step to the sibling.
+        bool done = false;
+        while ((bitstack & 1) == 0) {
+            if (bitstack == 0) {
+                done = true;
+                break;
+            }
+            node = q3.z;
+            if (kCount) cnt->parent++;
+            bitstack = bitstack >> 1;
+            if (bitstack != 0) q3 = q3p[4 * node + 3];
+        }
+        if (done) break;
+        node = q3.w;
+        bitstack = bitstack ^ 1;
+        q3 = q3p[4 * node + 3];
+    }
+    *tmin_io = tmin;
+    *uo = u;
+    *vo = v;
+    return prim;
+}
+
+// Ray-direction class of kernels/extension_ray_cast.cl:45-58.
+__device__ __forceinline__ uint32_t direction_class(F3 rd) {
+    float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
+    float maxYZ = fmaxf(ay, az);
+    float maxXZ = fmaxf(ax, az);
+    if (maxYZ < ax) return 0.0f < rd.x ? 0u : 1u;
+    if (maxXZ < ay) return 0.0f < rd.y ? 2u : 3u;
+    return 0.0f < rd.z ? 4u : 5u;
+}
+
+// Threaded traversal of one ray (common/threaded_bvh.hpp:527-569).
+template <bool kCount>
+__device__ __forceinline__ int32_t trace_threaded(const MTNode* __restrict__ tables, uint32_t nnodes,
+                                                  const PrimRec* __restrict__ prims, int32_t sphere_begin,
+                                                  F3 ro, F3 rd, float* tmin_io, float* uo, float* vo,
+                                                  Cnt* cnt) {
+    const MTNode* tab = tables + (size_t)direction_class(rd) * nnodes;
+    const F3 inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    float tmin = *tmin_io;
+    float u = 0.0f, v = 0.0f;
+    int32_t prim = -1;
+    uint32_t node = 0;
+    while (node != kInvalid) {
+        const float4* rec = reinterpret_cast<const float4*>(tab + node);
+        float4 q0 = rec[0], q1 = rec[1];
+        float lo[3] = {q0.x, q0.y, q0.z}, hi[3] = {q0.w, q1.x, q1.y};
+        uint32_t miss = __float_as_uint(q1.z), leaf = __float_as_uint(q1.w);
+        float h0, h1;
+        if (kCount) cnt->node++;
+        bool hit = slab_hit(lo, hi, ro, inv, tmin, &h0, &h1);
+        if (hit && leaf != 0) {
+            const uint32_t beg = leaf & 0x0FFFFFFFu;
+            const uint32_t end = beg + ((leaf >> 28) & 7u) + 1u;
+            if (kCount) cnt->prim += end - beg;
+            for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, ro, rd, &tmin, &u, &v, &prim);
+        }
+        node = (hit && leaf == 0) ? node + 1 : miss;
+    }
+    *tmin_io = tmin;
+    *uo = u;
+    *vo = v;
+    return prim;
+}
+
+}  // namespace rgpu

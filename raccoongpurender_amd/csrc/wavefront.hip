@@ -1,0 +1,941 @@
+// Wavefront path-tracing stages on gfx950.
+//
+// One bounce per step, with the reference's stage order and per-slot semantics
+// (common/wavefront_path_tracing.hpp:591-939, kernels/*.cl), reorganised for MI355X:
+//  - the strategy selection, envmap / BSDF sampling, pdf evaluation and shading that the
+//    reference runs as separate full-N launches are fused into one queue-driven kernel per
+//    material, which keeps the incident sample in registers. Every slot still draws its
+//    random numbers in the reference order: strategy (1), then the envmap sample or the
+//    BSDF sample, then the dielectric choice (1); a slot belongs to exactly one queue, so
+//    fusing per material preserves each slot's stream;
+//  - volume free-flight sampling (homogeneous_volume_through) is fused into the
+//    extension ray cast, which runs over every slot;
+//  - queues are built by wave ballots and a block prefix sum into per-block counts, a
+//    scan over blocks and a scatter: a stable, slot-ordered compaction, so the pixel that
+//    new_path assigns to a slot (next_pixel_index + queue position) is deterministic;
+//  - queue kernels run a device-sized grid that strides over the queue length read from
+//    device memory, instead of N threads that mostly exit.
+// Transcendentals are evaluated in double and rounded once to float, and the library is
+// built with -ffp-contract=off, so results agree with the CPU oracle to the last bit
+// except where accumulation order differs (pixel sums use float atomics).
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rgpu_wavefront.h"
+#include "trace_device.h"
+
+namespace rgpu {
+namespace {
+
+const float kPiF = 3.14159265358979323846f;
+const double kPiD = 3.14159265358979323846;
+const float kFltMax = 3.402823466e+38f;
+
+// ---------------------------------------------------------------- math
+__device__ __forceinline__ float fsin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float fcos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float flog(float x) { return (float)log((double)x); }
+__device__ __forceinline__ float fexp(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float fatan(float x) { return (float)atan((double)x); }
+__device__ __forceinline__ float fatan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ float fpow(float x, float y) { return (float)pow((double)x, (double)y); }
+// "x * M_PI * 2.0f" with OpenCL's double M_PI (kernels/lambertian.cl:11, new_path.cl:63)
+__device__ __forceinline__ float two_pi_times(float u) { return (float)((double)u * kPiD * 2.0); }
+
+__device__ __forceinline__ F3 add3(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 mul3(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ F3 mul3v(F3 a, F3 b) { return F3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ F3 neg3(F3 a) { return F3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ F3 normalize3(F3 v) {
+    float inv = 1.0f / sqrtf(dot3(v, v));
+    return mul3(v, inv);
+}
+__device__ __forceinline__ F3 load3(const float* p) { return F3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ bool finite3(F3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+// ---------------------------------------------------------------- RNG (kernels/peseudo_random.cl)
+__device__ __forceinline__ uint32_t rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+__device__ __forceinline__ uint32_t xo_next(uint4& s) {
+    const uint32_t r = rotl(s.x * 5u, 7) * 9u;
+    const uint32_t t = s.y << 9;
+    s.z ^= s.x;
+    s.w ^= s.y;
+    s.y ^= s.z;
+    s.x ^= s.w;
+    s.z ^= t;
+    s.w = rotl(s.w, 11);
+    return r;
+}
+__device__ __forceinline__ float xo_uniform(uint4& s) {
+    return __uint_as_float((xo_next(s) >> 9) | 0x3f800000u) - 1.0f;
+}
+__device__ __forceinline__ unsigned long long xo_uint62(uint4& s) {
+    unsigned long long a = xo_next(s) >> 1;
+    unsigned long long b = xo_next(s) >> 1;
+    return (a << 31) | b;
+}
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long* x) {
+    unsigned long long z = (*x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------- envmap (kernels/envmap_sampling.cl)
+__device__ __forceinline__ void cartesian_to_polar(F3 rd, float* theta, float* phi) {
+    float z = rd.y, x = rd.z, y = rd.x;
+    *theta = fatan2(sqrtf(x * x + y * y), z);
+    *phi = fatan2(y, x);
+    if (!isfinite(*phi)) *phi = 0.0f;
+}
+
+// read_imagef with normalised coordinates and CLK_ADDRESS_REPEAT, nearest or bilinear
+// (OpenCL 1.2 section 8.2 addressing/filtering formulas).
+__device__ __forceinline__ F3 texel(const DevEnvmap& e, int i, int j) {
+    float4 t = e.texels[(size_t)j * e.width + i];
+    return F3{t.x, t.y, t.z};
+}
+__device__ F3 envmap_value(const DevEnvmap& e, F3 rd, bool filter) {
+    float theta, phi;
+    cartesian_to_polar(rd, &theta, &phi);
+    const float pi = kPiF;
+    float u = 1.0f - phi / (2.0f * pi);
+    float v = theta / pi;
+    const int W = (int)e.width, H = (int)e.height;
+    float uu = (u - floorf(u)) * (float)W;
+    float vv = (v - floorf(v)) * (float)H;
+    if (!filter) {
+        int i = (int)floorf(uu);
+        int j = (int)floorf(vv);
+        if (i > W - 1) i -= W;
+        if (j > H - 1) j -= H;
+        return texel(e, i, j);
+    }
+    float us = uu - 0.5f, vs = vv - 0.5f;
+    int i0 = (int)floorf(us), j0 = (int)floorf(vs);
+    int i1 = i0 + 1, j1 = j0 + 1;
+    float a = us - floorf(us), b = vs - floorf(vs);
+    if (i0 < 0) i0 += W;
+    if (j0 < 0) j0 += H;
+    if (i1 > W - 1) i1 -= W;
+    if (j1 > H - 1) j1 -= H;
+    F3 t00 = texel(e, i0, j0), t10 = texel(e, i1, j0), t01 = texel(e, i0, j1), t11 = texel(e, i1, j1);
+    float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return F3{w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
+              w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+              w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z};
+}
+
+__device__ __forceinline__ int fract_int(int x, int m) {
+    int r = x % m;
+    return r < 0 ? r + m : r;
+}
+__device__ __forceinline__ int envmap_index(const DevEnvmap& e, F3 wi) {
+    float theta, phi;
+    cartesian_to_polar(wi, &theta, &phi);
+    const float pi = kPiF;
+    float u = 1.0f - phi / (2.0f * pi);
+    float v = theta / pi;
+    int ix = (int)floorf(u * (float)e.width);
+    int iy = (int)floorf(v * (float)e.height);
+    ix = fract_int(ix, (int)e.width);
+    iy = min(max(iy, 0), (int)e.height - 1);
+    return iy * (int)e.width + ix;
+}
+
+__device__ __forceinline__ uint32_t alias_sample(uint4& st, const AliasBucket* table, uint32_t count) {
+    uint32_t idx = (uint32_t)(xo_uint62(st) % (unsigned long long)count);
+    AliasBucket b = table[idx];
+    if (b.alias < 0) return idx;
+    float u = xo_uniform(st);
+    if (u < b.height) return idx;
+    return (uint32_t)b.alias;
+}
+
+__device__ __forceinline__ F3 sample_from_fragment(const EnvFragment& f, uint4& st) {
+    float y = f.beg_y + (f.end_y - f.beg_y) * xo_uniform(st);
+    float phi = f.beg_phi + (f.end_phi - f.beg_phi) * xo_uniform(st);
+    F3 p = F3{fsin(phi), y, fcos(phi)};
+    float r_xz = sqrtf(fmaxf(1.0f - p.y * p.y, 0.0f));
+    p.x *= r_xz;
+    p.z *= r_xz;
+    return p;
+}
+
+__device__ __forceinline__ void six_axis_tables(const DevEnvmap& e, F3 n, uint32_t* tx, uint32_t* ty, uint32_t* tz) {
+    *tx = 0.0f < n.x ? 1u : 2u;
+    *ty = 0.0f < n.y ? 3u : 4u;
+    *tz = 0.0f < n.z ? 5u : 6u;
+}
+
+// sample_envmap_6axis + sample_from_fragment (envmap_sampling.cl:126-157, 169-217)
+__device__ F3 sample_env_6axis(const DevEnvmap& e, F3 n, uint4& st, float* pdf) {
+    const uint32_t count = e.width * e.height;
+    uint32_t tx, ty, tz;
+    six_axis_tables(e, n, &tx, &ty, &tz);
+    float u = xo_uniform(st);
+    F3 ap = mul3v(n, n);
+    uint32_t sel = (u < ap.x) ? tx : ((u < ap.x + ap.y) ? ty : tz);
+    uint32_t idx = alias_sample(st, e.alias + (size_t)sel * count, count);
+    float p = 0.0f;
+    p += ap.x * e.pdf[(size_t)tx * count + idx];
+    p += ap.y * e.pdf[(size_t)ty * count + idx];
+    p += ap.z * e.pdf[(size_t)tz * count + idx];
+    *pdf = p;
+    return sample_from_fragment(e.fragments[idx], st);
+}
+// envmap_pdf_sixAxis (envmap_sampling.cl:100-124)
+__device__ float pdf_env_6axis(const DevEnvmap& e, F3 wi, F3 n) {
+    const uint32_t count = e.width * e.height;
+    uint32_t tx, ty, tz;
+    six_axis_tables(e, n, &tx, &ty, &tz);
+    F3 ap = mul3v(n, n);
+    int idx = envmap_index(e, wi);
+    float p = 0.0f;
+    p += ap.x * e.pdf[(size_t)tx * count + idx];
+    p += ap.y * e.pdf[(size_t)ty * count + idx];
+    p += ap.z * e.pdf[(size_t)tz * count + idx];
+    return p;
+}
+// sample_envmap_stage (envmap_sampling.cl:219-248): the uniform-weight table
+__device__ F3 sample_env_uniform(const DevEnvmap& e, uint4& st, float* pdf) {
+    const uint32_t count = e.width * e.height;
+    uint32_t idx = alias_sample(st, e.alias, count);
+    *pdf = e.pdf[idx];
+    return sample_from_fragment(e.fragments[idx], st);
+}
+__device__ __forceinline__ float pdf_env_uniform(const DevEnvmap& e, F3 wi) { return e.pdf[envmap_index(e, wi)]; }
+
+// ---------------------------------------------------------------- BSDF helpers
+// Duff et al. 2017 orthonormal basis, z up (kernels/lambertian.cl:24-32)
+__device__ __forceinline__ void onb(F3 z, F3* x, F3* y) {
+    const float sign = copysignf(1.0f, z.z);
+    const float a = -1.0f / (sign + z.z);
+    const float b = z.x * z.y * a;
+    *x = F3{1.0f + sign * z.x * z.x * a, sign * b, -sign * z.x};
+    *y = F3{b, sign + z.y * z.y * a, -z.y};
+}
+__device__ __forceinline__ F3 local_to_world(F3 x, F3 y, F3 z, F3 l) {
+    return add3(add3(mul3(x, l.x), mul3(y, l.y)), mul3(z, l.z));
+}
+__device__ __forceinline__ F3 reflect3(F3 d, F3 n) { return sub3(d, mul3(n, 2.0f * dot3(d, n))); }
+__device__ __forceinline__ F3 flip_toward(F3 n, F3 wo) { return dot3(n, wo) < 0.0f ? neg3(n) : n; }
+__device__ __forceinline__ float sqr(float x) { return x * x; }
+
+// Ward sampling / pdf / BRDF. alpha_x == alpha_y uses the reference's isotropic formulas
+// (kernels/ward.cl:40-74, 155-176) verbatim; otherwise the anisotropic generalisation of
+// the same (Walter 2005 half-vector sampling, Geisler-Moroder normalisation) in the ONB
+// of the shading normal.
+__device__ F3 sample_ward(const Ward& w, F3 wo, F3 z, float u0, float u1) {
+    F3 x, y;
+    float thetaH, phiH;
+    const float pi = kPiF;
+    if (w.alpha_x == w.alpha_y) {
+        phiH = u0 * 2.0f * pi;
+        thetaH = fatan(w.alpha_x * sqrtf(-flog(u1)));
+    } else {
+        float tp = 2.0f * pi * u0;
+        phiH = fatan2(w.alpha_y * fsin(tp), w.alpha_x * fcos(tp));
+        float c = fcos(phiH), s = fsin(phiH);
+        float denom = c * c / (w.alpha_x * w.alpha_x) + s * s / (w.alpha_y * w.alpha_y);
+        thetaH = fatan(sqrtf(-flog(u1) / denom));
+    }
+    float st = fsin(thetaH);
+    F3 hl = F3{st * fcos(phiH), st * fsin(phiH), fcos(thetaH)};
+    onb(z, &x, &y);
+    F3 h = local_to_world(x, y, z, hl);
+    return reflect3(neg3(wo), h);
+}
+__device__ float ward_exponent(const Ward& w, F3 h, F3 z) {
+    // tan^2(theta_h) weighted by the two roughnesses
+    if (w.alpha_x == w.alpha_y) {
+        float c2 = sqr(dot3(h, z));
+        float tan2 = (1.0f - c2) / c2;
+        return -tan2 / sqr(w.alpha_x);
+    }
+    F3 x, y;
+    onb(z, &x, &y);
+    float hx = dot3(h, x) / w.alpha_x, hy = dot3(h, y) / w.alpha_y, hz = dot3(h, z);
+    return -(hx * hx + hy * hy) / (hz * hz);
+}
+__device__ float pdf_ward(const Ward& w, F3 wo, F3 wi, F3 z) {
+    if ((double)(dot3(wo, z) * dot3(wi, z)) < 0.0) return 0.0f;
+    const float pi = kPiF;
+    F3 h = normalize3(add3(wi, wo));
+    float hz = dot3(h, z);
+    float axy = w.alpha_x == w.alpha_y ? sqr(w.alpha_x) : w.alpha_x * w.alpha_y;
+    float k0 = 1.0f / (4.0f * pi * axy * dot3(h, wi) * (hz * hz * hz));
+    float k1 = fexp(ward_exponent(w, h, z));
+    return k0 * k1;
+}
+
+// ---------------------------------------------------------------- per-slot records
+__device__ __forceinline__ void load_ray_rec(const TraceRay* r, uint32_t s, F3* ro, float* tmax, F3* rd, uint32_t* pix) {
+    const float4* q = reinterpret_cast<const float4*>(r + s);
+    float4 a = q[0], b = q[1];
+    *ro = F3{a.x, a.y, a.z};
+    *tmax = a.w;
+    *rd = F3{b.x, b.y, b.z};
+    *pix = __float_as_uint(b.w);
+}
+__device__ __forceinline__ void store_ray_rec(TraceRay* r, uint32_t s, F3 ro, float tmax, F3 rd, uint32_t pix) {
+    float4* q = reinterpret_cast<float4*>(r + s);
+    q[0] = make_float4(ro.x, ro.y, ro.z, tmax);
+    q[1] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(pix));
+}
+__device__ __forceinline__ void store_shade(ShadeRec* sh, uint32_t s, F3 Le, F3 T) {
+    float4* q = reinterpret_cast<float4*>(sh + s);
+    q[0] = make_float4(Le.x, Le.y, Le.z, T.x);
+    q[1] = make_float4(T.y, T.z, 0.0f, 0.0f);
+}
+
+// Strategy selection (kernels/mixture_density.cl:25-67) for a surface hit; returns true for BSDF.
+__device__ __forceinline__ bool select_bxdf(uint4& st, float sel_bxdf) {
+    float u = xo_uniform(st);
+    return u <= sel_bxdf;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void __launch_bounds__(256) k_init(DevState s, WFParams p, unsigned long long seed_offset,
+                                              unsigned long long next_pixel) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) {
+        *s.next_pixel = next_pixel;
+        for (int q = 0; q < kNumQueues; ++q) s.counts[q] = q == 0 ? p.n : 0u;
+    }
+    if (gid >= p.n) return;
+    // random_initialize (kernels/peseudo_random.cl:48-64)
+    unsigned long long x = seed_offset + gid;
+    unsigned long long r0 = splitmix64(&x);
+    unsigned long long r1 = splitmix64(&x);
+    uint4 st = make_uint4((uint32_t)r0, (uint32_t)(r0 >> 32), (uint32_t)r1, (uint32_t)(r1 >> 32));
+    if ((st.x | st.y | st.z | st.w) == 0) st.x = 1;
+    s.rng[gid] = st;
+    // initialize_all_as_new_path (kernels/new_path.cl:9-15)
+    s.items[gid] = gid;
+    // defined start state (the reference leaves these buffers uninitialised)
+    ExtRec e = {-1, -1, kFltMax, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    s.ext[gid] = e;
+    VolList v = {};
+    s.vol[gid] = v;
+    PathRec pr = {};
+    s.path[gid] = pr;
+    ShadeRec sh = {};
+    s.shade[gid] = sh;
+}
+
+__global__ void __launch_bounds__(256) k_clear_accum(DevState s, uint32_t npixels, uint32_t accum64) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npixels) return;
+    if (accum64) {
+        s.accum64[3 * (size_t)i + 0] = 0.0;
+        s.accum64[3 * (size_t)i + 1] = 0.0;
+        s.accum64[3 * (size_t)i + 2] = 0.0;
+        s.accum_count[i] = 0;
+    } else {
+        reinterpret_cast<float4*>(s.accum)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// new_path (kernels/new_path.cl:17-83) over the new-path queue, plus the strategy
+// selection and envmap draws the reference makes for a new path whose extension result
+// still carries a volume id from its previous life (kernels/mixture_density.cl:26-55,
+// kernels/envmap_sampling.cl:219-248).
+__global__ void __launch_bounds__(256) k_new_path(DevState s, WFParams p, Camera cam, DevEnvmap env) {
+    const uint32_t count = s.counts[0];
+    const unsigned long long base = *s.next_pixel;
+    const unsigned long long P = (unsigned long long)cam.width * cam.height;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = s.items[i];
+        uint4 st = s.rng[slot];
+        uint32_t pixel = (uint32_t)((base + i) % P);
+        uint32_t x = pixel % cam.width, y = pixel / cam.width;
+        F3 o = load3(cam.o), r = load3(cam.r), b = load3(cam.b);
+        float jx = (float)x + xo_uniform(st);
+        float jy = (float)y + xo_uniform(st);
+        F3 on_plane = add3(add3(o, mul3(r, jx)), mul3(b, jy));
+        float u0 = xo_uniform(st);
+        float u1 = xo_uniform(st);
+        float rr = sqrtf(u0);
+        float theta = two_pi_times(u1);
+        float lensx = cam.lens_radius * rr * fcos(theta);
+        float lensy = cam.lens_radius * rr * fsin(theta);
+        F3 lens = add3(add3(load3(cam.eye), mul3(load3(cam.up), lensx)), mul3(load3(cam.right), lensy));
+        F3 rd = normalize3(sub3(on_plane, lens));
+        store_ray_rec(s.ray, slot, lens, kFltMax, rd, pixel);
+        PathRec pr;
+        pr.T[0] = pr.T[1] = pr.T[2] = 1.0f;
+        pr.L[0] = pr.L[1] = pr.L[2] = 0.0f;
+        pr.logic_i = 0;
+        pr.pad = 0;
+        s.path[slot] = pr;
+        s.vol[slot].count = 0;
+        store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, F3{1.0f, 1.0f, 1.0f});
+        ExtRec e = s.ext[slot];
+        e.hit_surface = -1;
+        e.tmin = kFltMax;
+        e.ngx = e.ngy = e.ngz = 0.0f;
+        s.ext[slot] = e;
+        if (p.quirks && e.hit_volume >= 0) {
+            bool bxdf = select_bxdf(st, p.bxdf_p);
+            if (!bxdf) {
+                float pdf;
+                (void)sample_env_uniform(env, st, &pdf);
+            }
+        }
+        s.rng[slot] = st;
+    }
+}
+
+__device__ __forceinline__ void spawn(DevState& s, uint32_t slot, F3 ro, F3 rd, float t, F3 wi, F3 ng_offset,
+                                      bool offset_normal, uint32_t pixel) {
+    // ro + rd * tmin + wi * 1e-5 (+ Ng * 1e-5)  (kernels/lambertian.cl:117-119)
+    F3 o = add3(add3(ro, mul3(rd, t)), mul3(wi, 1.0e-5f));
+    if (offset_normal) o = add3(o, mul3(ng_offset, 1.0e-5f));
+    store_ray_rec(s.ray, slot, o, kFltMax, wi, pixel);
+}
+
+// Lambertian: strategy selection + envmap 6-axis sample / cosine sample + pdfs +
+// lambertian_stage (mixture_density.cl, envmap_sampling.cl:169-217,250-266, lambertian.cl:34-123).
+__global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+    const uint32_t count = s.counts[kMatLambertian];
+    const uint32_t* items = s.items + (size_t)kMatLambertian * p.n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = items[i];
+        uint4 st = s.rng[slot];
+        ExtRec e = s.ext[slot];
+        F3 ro, rd;
+        float tmax;
+        uint32_t pixel;
+        load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+        const Lambertian L = m.lambertian[m.mat[e.hit_surface].index];
+        const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
+        const F3 wo = neg3(rd);
+        const float sel_b = p.bxdf_p, sel_e = 1.0f - p.bxdf_p;
+        const bool bxdf = select_bxdf(st, sel_b);
+        F3 wi;
+        float pdf_e = 0.0f, pdf_b;
+        if (!bxdf) wi = sample_env_6axis(env, flip_toward(Ng, wo), st, &pdf_e);
+        float cosTheta;
+        if (bxdf) {
+            float u0 = xo_uniform(st);
+            float u1 = xo_uniform(st);
+            float r = sqrtf(u0);
+            float th = two_pi_times(u1);
+            F3 l = F3{r * fcos(th), r * fsin(th), sqrtf(fmaxf(1.0f - u0, 0.0f))};
+            F3 x, y;
+            onb(Ng, &x, &y);  // unflipped geometric normal (kernels/lambertian.cl:59-64)
+            wi = local_to_world(x, y, Ng, l);
+            cosTheta = l.z;
+        } else {
+            cosTheta = dot3(Ng, wi);
+        }
+        pdf_b = fmaxf(cosTheta, 0.0f) / kPiF;
+        if (bxdf) pdf_e = pdf_env_6axis(env, wi, Ng);  // unflipped (kernels/envmap_sampling.cl:263-265)
+        const F3 Ngf = flip_toward(Ng, wo);
+        float pdf = 0.0f;
+        pdf += sel_b * pdf_b;
+        pdf += sel_e * pdf_e;
+        float c = dot3(wi, Ngf);
+        F3 T = F3{0.0f, 0.0f, 0.0f};
+        if (0.0 < (double)c) {
+            T.x = L.R[0] / kPiF * c / pdf;
+            T.y = L.R[1] / kPiF * c / pdf;
+            T.z = L.R[2] / kPiF * c / pdf;
+        }
+        store_shade(s.shade, slot, load3(L.Le), T);
+        spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
+        s.rng[slot] = st;
+    }
+}
+
+// Ward (mixture_density.cl, envmap_sampling.cl, ward.cl:76-186)
+__global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+    const uint32_t count = s.counts[kMatWard];
+    const uint32_t* items = s.items + (size_t)kMatWard * p.n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = items[i];
+        uint4 st = s.rng[slot];
+        ExtRec e = s.ext[slot];
+        F3 ro, rd;
+        float tmax;
+        uint32_t pixel;
+        load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+        const Ward w = m.ward[m.mat[e.hit_surface].index];
+        const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
+        const F3 wo = neg3(rd);
+        const F3 Ngf = flip_toward(Ng, wo);
+        const float sel_b = p.bxdf_p, sel_e = 1.0f - p.bxdf_p;
+        const bool bxdf = select_bxdf(st, sel_b);
+        F3 wi;
+        float pdf_e = 0.0f;
+        if (!bxdf) wi = sample_env_6axis(env, Ngf, st, &pdf_e);
+        if (bxdf) {
+            float u0 = xo_uniform(st);
+            float u1 = xo_uniform(st);
+            wi = sample_ward(w, wo, Ngf, u0, u1);
+        }
+        float pdf_b = pdf_ward(w, wo, wi, Ngf);
+        if (bxdf) pdf_e = pdf_env_6axis(env, wi, Ng);
+        float pdf = 0.0f;
+        pdf += sel_b * pdf_b;
+        pdf += sel_e * pdf_e;
+        float c = dot3(wi, Ngf);
+        F3 T = F3{0.0f, 0.0f, 0.0f};
+        if (0.0f < c) {
+            const float pi = kPiF;
+            F3 lv = add3(wi, wo);
+            F3 h = normalize3(lv);
+            float mu = dot3(h, wi);
+            float k = fpow(1.0f - mu, 1.0f / w.falloff);
+            F3 rho = F3{w.reflectance[0] + (w.edgetint[0] - w.reflectance[0]) * k,
+                        w.reflectance[1] + (w.edgetint[1] - w.reflectance[1]) * k,
+                        w.reflectance[2] + (w.edgetint[2] - w.reflectance[2]) * k};
+            float axy = w.alpha_x == w.alpha_y ? sqr(w.alpha_x) : w.alpha_x * w.alpha_y;
+            F3 k0 = F3{rho.x / (pi * axy), rho.y / (pi * axy), rho.z / (pi * axy)};
+            float k1 = fexp(ward_exponent(w, h, Ngf));
+            float k2 = dot3(lv, lv) / sqr(sqr(dot3(lv, Ngf)));
+            T.x = k0.x * k1 * k2 * c / pdf;
+            T.y = k0.y * k1 * k2 * c / pdf;
+            T.z = k0.z * k1 * k2 * c / pdf;
+        }
+        store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, T);
+        spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
+        s.rng[slot] = st;
+    }
+}
+
+// Mirror (kernels/delta_materials.cl:48-77)
+__global__ void __launch_bounds__(256) k_shade_specular(DevState s, WFParams p) {
+    const uint32_t count = s.counts[kMatSpecular];
+    const uint32_t* items = s.items + (size_t)kMatSpecular * p.n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = items[i];
+        uint4 st = s.rng[slot];
+        ExtRec e = s.ext[slot];
+        F3 ro, rd;
+        float tmax;
+        uint32_t pixel;
+        load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+        (void)select_bxdf(st, 1.0f);  // strategy draw: selection (1, 0)
+        const F3 Ngf = flip_toward(F3{e.ngx, e.ngy, e.ngz}, neg3(rd));
+        F3 wi = reflect3(rd, Ngf);
+        store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, F3{1.0f, 1.0f, 1.0f});
+        spawn(s, slot, ro, rd, e.tmin, wi, Ngf, false, pixel);
+        s.rng[slot] = st;
+    }
+}
+
+// Dielectric (kernels/delta_materials.cl:79-134)
+__global__ void __launch_bounds__(256) k_shade_dielectric(DevState s, WFParams p, DevMaterials m) {
+    const uint32_t count = s.counts[kMatDielectric];
+    const uint32_t* items = s.items + (size_t)kMatDielectric * p.n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = items[i];
+        uint4 st = s.rng[slot];
+        ExtRec e = s.ext[slot];
+        F3 ro, rd;
+        float tmax;
+        uint32_t pixel;
+        load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+        (void)select_bxdf(st, 1.0f);
+        const Dielectric d = m.dielectric[m.mat[e.hit_surface].index];
+        F3 Ng = F3{e.ngx, e.ngy, e.ngz};
+        const F3 wo = neg3(rd);
+        const bool backside = dot3(Ng, wo) < 0.0f;
+        float eta_i = 1.0f, eta_t = d.ior;
+        if (backside) {
+            float t = eta_i;
+            eta_i = eta_t;
+            eta_t = t;
+            Ng = neg3(Ng);
+        }
+        // fresnel_dielectrics(|cos|, eta_t, eta_i)
+        float cth = fabsf(dot3(Ng, wo));
+        float g = sqrtf(eta_t * eta_t / sqr(eta_i) - 1.0f + sqr(cth));
+        float fa = 0.5f * sqr(g - cth) / sqr(g + cth);
+        float fb = 1.0f + sqr(cth * (g + cth) - 1.0f) / sqr(cth * (g - cth) + 1.0f);
+        float f = fa * fb;
+        float u = xo_uniform(st);
+        float comp = 1.0f;
+        F3 wi;
+        if (u < f) {
+            wi = reflect3(rd, Ng);
+        } else {
+            float eta = eta_i / eta_t;
+            float NoD = dot3(Ng, rd);
+            float k = 1.0f - eta * eta * (1.0f - NoD * NoD);
+            if (k <= 0.0f) {
+                wi = reflect3(rd, Ng);
+            } else {
+                wi = sub3(mul3(rd, eta), mul3(Ng, eta * NoD + sqrtf(k)));
+                comp = sqr(eta_t / eta_i);
+            }
+        }
+        store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, F3{comp, comp, comp});
+        spawn(s, slot, ro, rd, e.tmin, wi, Ng, false, pixel);
+        s.rng[slot] = st;
+    }
+}
+
+__device__ __forceinline__ uint32_t volume_hash(const Volume& v) {
+    return __float_as_uint(v.C) ^ __float_as_uint(v.R[0]) ^ __float_as_uint(v.R[1]) ^ __float_as_uint(v.R[2]);
+}
+
+// Medium boundary (kernels/homogeneous_volume.cl:19-64, InVolumeList ops types.cl:108-132)
+__global__ void __launch_bounds__(256) k_shade_volume_surface(DevState s, WFParams p, DevMaterials m) {
+    const uint32_t count = s.counts[kMatVolume];
+    const uint32_t* items = s.items + (size_t)kMatVolume * p.n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = items[i];
+        uint4 st = s.rng[slot];
+        ExtRec e = s.ext[slot];
+        F3 ro, rd;
+        float tmax;
+        uint32_t pixel;
+        load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+        (void)select_bxdf(st, 1.0f);
+        const bool backside = dot3(F3{e.ngx, e.ngy, e.ngz}, neg3(rd)) < 0.0f;
+        store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, F3{1.0f, 1.0f, 1.0f});
+        VolList v = s.vol[slot];
+        if (backside) {
+            uint32_t hsh = volume_hash(m.volume[m.mat[e.hit_surface].index]);
+            bool move = false;
+            for (int k = 0; k < v.count; ++k) {
+                if (move) v.volumes[k - 1] = v.volumes[k];
+                else if (volume_hash(m.volume[m.mat[v.volumes[k]].index]) == hsh) move = true;
+            }
+            if (move) v.count--;
+        } else if (v.count < 7) {
+            v.volumes[v.count] = e.hit_surface;
+            v.count++;
+        }
+        s.vol[slot] = v;
+        F3 o = add3(ro, mul3(rd, e.tmin + 1.0e-5f));
+        store_ray_rec(s.ray, slot, o, kFltMax, rd, pixel);
+        s.rng[slot] = st;
+    }
+}
+
+// Scattering inside a medium (mixture_density.cl:47-51, envmap_sampling.cl:219-248,268-281,
+// homogeneous_volume.cl:66-129)
+__global__ void __launch_bounds__(256) k_shade_volume_inside(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+    const uint32_t count = s.counts[kMatVolumeInside];
+    const uint32_t* items = s.items + (size_t)kMatVolumeInside * p.n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = items[i];
+        uint4 st = s.rng[slot];
+        ExtRec e = s.ext[slot];
+        F3 ro, rd;
+        float tmax;
+        uint32_t pixel;
+        load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+        const float sel_b = p.bxdf_p, sel_e = 1.0f - p.bxdf_p;
+        const bool bxdf = select_bxdf(st, sel_b);
+        F3 wi;
+        float pdf_e = 0.0f;
+        if (!bxdf) wi = sample_env_uniform(env, st, &pdf_e);
+        if (bxdf) {
+            float u0 = xo_uniform(st);
+            float u1 = xo_uniform(st);
+            float phi = two_pi_times(u0);
+            float z = -1.0f + (1.0f - -1.0f) * u1;
+            float r = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+            wi = F3{r * fcos(phi), r * fsin(phi), z};
+        }
+        const float pdf_b = (float)(1.0 / (kPiD * 4.0));
+        if (bxdf) pdf_e = pdf_env_uniform(env, wi);
+        float pdf = 0.0f;
+        pdf += sel_b * pdf_b;
+        pdf += sel_e * pdf_e;
+        const Volume v = m.volume[m.mat[e.hit_volume].index];
+        const float four_pi = (float)(4.0f * kPiD);
+        F3 T = F3{v.R[0] / four_pi / pdf, v.R[1] / four_pi / pdf, v.R[2] / four_pi / pdf};
+        store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, T);
+        store_ray_rec(s.ray, slot, add3(ro, mul3(rd, e.tmin)), kFltMax, wi, pixel);
+        s.rng[slot] = st;
+    }
+}
+
+// homogeneous_volume_through (kernels/homogeneous_volume_through.cl:7-45) fused with
+// extension_ray_cast (kernels/extension_ray_cast_stackless.cl:202-246) over every slot.
+template <int kVariant>
+__global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, TraceScene sc, DevMaterials m) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot == 0) *s.next_pixel += s.counts[0];  // finalize_new_path (kernels/new_path.cl:85-88)
+    if (slot >= p.n) return;
+    uint4 st = s.rng[slot];
+    VolList v = s.vol[slot];
+    int32_t hit_volume = -1;
+    if (0 < v.count) {
+        unsigned long long u = xo_uint62(st);
+        hit_volume = v.volumes[u % (unsigned long long)v.count];
+    }
+    float tmin = kFltMax;
+    if (0 <= hit_volume) {
+        const Volume vol = m.volume[m.mat[hit_volume].index];
+        float u = xo_uniform(st);
+        tmin = -1.0f / vol.C * flog(u);
+    }
+    s.rng[slot] = st;
+    F3 ro, rd;
+    float tmax;
+    uint32_t pixel;
+    load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+    ExtRec e = s.ext[slot];
+    e.hit_surface = -1;
+    e.hit_volume = hit_volume;
+    e.tmin = tmin;
+    if (sc.nnodes != 0) {
+        float t = tmin, u, vv;
+        Cnt cnt;
+        int32_t prim = kVariant == kVariantThreaded
+                           ? trace_threaded<false>(sc.mt_nodes, sc.nnodes, sc.prims, sc.sphere_begin, ro, rd, &t,
+                                                   &u, &vv, &cnt)
+                           : trace_stackless<false>(sc.sl_nodes, sc.prims, sc.sphere_begin, ro, rd, &t, &u, &vv, &cnt);
+        if (prim >= 0) {
+            F3 n;
+            if (prim < sc.sphere_begin) {
+                const uint32_t* ix = sc.indices + 3 * (size_t)prim;
+                float4 a = sc.points[ix[0]], b = sc.points[ix[1]], c = sc.points[ix[2]];
+                F3 v0 = F3{a.x, a.y, a.z}, v1 = F3{b.x, b.y, b.z}, v2 = F3{c.x, c.y, c.z};
+                n = cross3(sub3(v2, v1), sub3(v1, v0));  // clockwise winding
+            } else {
+                float4 sp = sc.spheres[prim - sc.sphere_begin];
+                n = sub3(add3(ro, mul3(rd, t)), F3{sp.x, sp.y, sp.z});
+            }
+            n = normalize3(n);
+            e.hit_surface = prim;
+            e.hit_volume = -1;
+            e.tmin = t;
+            e.ngx = n.x;
+            e.ngy = n.y;
+            e.ngz = n.z;
+        }
+    }
+    s.ext[slot] = e;
+}
+
+// logic (kernels/logic.cl:117-280): path update, Russian roulette, envmap on a miss,
+// accumulation of finished paths, and the per-block queue counts of a stable compaction.
+__global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+    __shared__ uint32_t wcount[kNumQueues][4];
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t q = 0xFFu;
+    if (slot < p.n) {
+        PathRec pr = s.path[slot];
+        const uint4 sh0 = reinterpret_cast<const uint4*>(s.shade + slot)[0];
+        const uint4 sh1 = reinterpret_cast<const uint4*>(s.shade + slot)[1];
+        const F3 Le = F3{__uint_as_float(sh0.x), __uint_as_float(sh0.y), __uint_as_float(sh0.z)};
+        const F3 Ts = F3{__uint_as_float(sh0.w), __uint_as_float(sh1.x), __uint_as_float(sh1.y)};
+        const ExtRec e = s.ext[slot];
+        uint4 st = s.rng[slot];
+        const uint32_t logic_i = pr.logic_i;
+        F3 L = add3(load3(pr.L), mul3v(load3(pr.T), Le));
+        F3 T = mul3v(load3(pr.T), Ts);
+        bool newPath = false, evalEnv = false;
+        if (e.hit_surface < 0 && e.hit_volume < 0) newPath = evalEnv = true;
+        float lumT = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+        float cp = 1.0f;
+        if (p.rr_start < logic_i) cp = fminf(lumT, 1.0f);
+        float u = xo_uniform(st);
+        if (u < cp) {
+            T.x = T.x / cp;
+            T.y = T.y / cp;
+            T.z = T.z / cp;
+        } else {
+            newPath = true;
+        }
+        if (lumT < 1.0e-5f) newPath = true;
+        if (p.max_depth != 0 && logic_i >= p.max_depth) newPath = true;
+        F3 ro, rd;
+        float tmax;
+        uint32_t pixel;
+        load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+        if (evalEnv) {
+            F3 em = envmap_value(env, rd, logic_i == 0);
+            L = add3(L, mul3v(T, em));
+        }
+        if (newPath && finite3(L) && L.x < p.clamp && L.y < p.clamp && L.z < p.clamp) {
+            if (p.accum64) {
+                unsafeAtomicAdd(s.accum64 + 3 * (size_t)pixel + 0, (double)L.x);
+                unsafeAtomicAdd(s.accum64 + 3 * (size_t)pixel + 1, (double)L.y);
+                unsafeAtomicAdd(s.accum64 + 3 * (size_t)pixel + 2, (double)L.z);
+                atomicAdd(s.accum_count + pixel, 1u);
+            } else {
+                float* a = s.accum + 4 * (size_t)pixel;
+                unsafeAtomicAdd(a + 0, L.x);
+                unsafeAtomicAdd(a + 1, L.y);
+                unsafeAtomicAdd(a + 2, L.z);
+                atomicAdd(reinterpret_cast<uint32_t*>(a + 3), 1u);
+            }
+        }
+        pr.L[0] = L.x;
+        pr.L[1] = L.y;
+        pr.L[2] = L.z;
+        pr.T[0] = T.x;
+        pr.T[1] = T.y;
+        pr.T[2] = T.z;
+        pr.logic_i = logic_i + 1;
+        s.path[slot] = pr;
+        s.rng[slot] = st;
+        if (newPath) q = 0;
+        else if (0 <= e.hit_surface) q = (uint32_t)m.mat[e.hit_surface].type;
+        else q = kMatVolumeInside;
+        s.qsel[slot] = (uint8_t)q;
+    }
+    for (int k = 0; k < kNumQueues; ++k) {
+        unsigned long long b = __ballot(q == (uint32_t)k);
+        if (lane == 0) wcount[k][wave] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (threadIdx.x < kNumQueues) {
+        const int k = threadIdx.x;
+        s.block_counts[(size_t)k * s.nblocks + blockIdx.x] = wcount[k][0] + wcount[k][1] + wcount[k][2] + wcount[k][3];
+    }
+}
+
+// Exclusive scan of the per-block counts of one queue (one workgroup per queue).
+__global__ void __launch_bounds__(1024) k_scan_queues(DevState s) {
+    __shared__ uint32_t part[1024];
+    const int q = blockIdx.x;
+    uint32_t* c = s.block_counts + (size_t)q * s.nblocks;
+    const uint32_t nb = s.nblocks;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t beg = threadIdx.x * per;
+    const uint32_t end = min(beg + per, nb);
+    uint32_t sum = 0;
+    for (uint32_t i = beg; i < end; ++i) sum += c[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;  // exclusive prefix of this thread's chunk
+    for (uint32_t i = beg; i < end; ++i) {
+        uint32_t v = c[i];
+        c[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) s.counts[q] = part[1023];
+}
+
+// Scatter slots into their queues in slot order.
+__global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p) {
+    __shared__ uint32_t wpre[kNumQueues][4];
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t q = slot < p.n ? (uint32_t)s.qsel[slot] : 0xFFu;
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t rank = 0;
+    for (int k = 0; k < kNumQueues; ++k) {
+        unsigned long long b = __ballot(q == (uint32_t)k);
+        if (lane == 0) wpre[k][wave] = (uint32_t)__popcll(b);
+        if (q == (uint32_t)k) rank = (uint32_t)__popcll(b & lt);
+    }
+    __syncthreads();
+    if (q < (uint32_t)kNumQueues) {
+        uint32_t off = s.block_counts[(size_t)q * s.nblocks + blockIdx.x];
+        for (uint32_t w = 0; w < wave; ++w) off += wpre[q][w];
+        s.items[(size_t)q * p.n + off + rank] = slot;
+    }
+}
+
+// Mean radiance -> RGBA8: accumlation_to_intermediate (half-float means, accumlation.cl:6-25)
+// followed by tonemap (pow(x, 1/2.2) * 256, accumlation.cl:68-79). Counts stay 32-bit.
+__global__ void __launch_bounds__(256) k_tonemap(DevState s, uint32_t npixels, uint32_t accum64, uchar4* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npixels) return;
+    float r, g, b, cnt;
+    if (accum64) {
+        cnt = (float)s.accum_count[i];
+        r = (float)s.accum64[3 * (size_t)i + 0];
+        g = (float)s.accum64[3 * (size_t)i + 1];
+        b = (float)s.accum64[3 * (size_t)i + 2];
+    } else {
+        float4 a = reinterpret_cast<const float4*>(s.accum)[i];
+        r = a.x;
+        g = a.y;
+        b = a.z;
+        cnt = (float)__float_as_uint(a.w);
+    }
+    float div = 1.0f / cnt;
+    if (!isfinite(div)) div = 0.0f;
+    float m[3] = {(float)(__half)(r * div), (float)(__half)(g * div), (float)(__half)(b * div)};
+    unsigned char c[3];
+    for (int k = 0; k < 3; ++k) {
+        int v = (int)(fpow(m[k], 1.0f / 2.2f) * 256.0f);
+        c[k] = (unsigned char)min(max(v, 0), 255);
+    }
+    out[i] = make_uchar4(c[0], c[1], c[2], 255);
+}
+
+// stat (kernels/stat.cl): total number of accumulated samples.
+__global__ void __launch_bounds__(256) k_stat(DevState s, uint32_t npixels, uint32_t accum64,
+                                              unsigned long long* total) {
+    __shared__ unsigned long long part[4];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = 0;
+    if (i < npixels) v = accum64 ? s.accum_count[i] : __float_as_uint(s.accum[4 * (size_t)i + 3]);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
+}
+
+}  // namespace
+
+hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s) {
+    dim3 grid((npixels + 255) / 256);
+    hipLaunchKernelGGL(k_tonemap, grid, dim3(256), 0, s, st, npixels, accum64 ? 1u : 0u, (uchar4*)rgba8);
+    return hipGetLastError();
+}
+
+hipError_t wf_stat(const DevState& st, uint32_t npixels, bool accum64, unsigned long long* total, hipStream_t s) {
+    dim3 grid((npixels + 255) / 256);
+    hipLaunchKernelGGL(k_stat, grid, dim3(256), 0, s, st, npixels, accum64 ? 1u : 0u, total);
+    return hipGetLastError();
+}
+
+hipError_t wf_initialize(const DevState& st, const WFParams& p, uint64_t seed_offset, uint64_t next_pixel,
+                         uint32_t npixels, hipStream_t s) {
+    (void)npixels;
+    dim3 grid((p.n + 255) / 256);
+    hipLaunchKernelGGL(k_init, grid, dim3(256), 0, s, st, p, (unsigned long long)seed_offset,
+                       (unsigned long long)next_pixel);
+    return hipGetLastError();
+}
+
+hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hipStream_t s) {
+    dim3 grid((npixels + 255) / 256);
+    hipLaunchKernelGGL(k_clear_accum, grid, dim3(256), 0, s, st, npixels, accum64 ? 1u : 0u);
+    return hipGetLastError();
+}
+
+hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
+                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, hipStream_t s) {
+    const dim3 qgrid(persistent_blocks > 0 ? persistent_blocks : 1024), blk(256);
+    const dim3 full((p.n + 255) / 256);
+    hipLaunchKernelGGL(k_new_path, qgrid, blk, 0, s, st, p, cam, env);
+    hipLaunchKernelGGL(k_shade_lambertian, qgrid, blk, 0, s, st, p, m, env);
+    hipLaunchKernelGGL(k_shade_specular, qgrid, blk, 0, s, st, p);
+    hipLaunchKernelGGL(k_shade_dielectric, qgrid, blk, 0, s, st, p, m);
+    hipLaunchKernelGGL(k_shade_ward, qgrid, blk, 0, s, st, p, m, env);
+    hipLaunchKernelGGL(k_shade_volume_surface, qgrid, blk, 0, s, st, p, m);
+    hipLaunchKernelGGL(k_shade_volume_inside, qgrid, blk, 0, s, st, p, m, env);
+    if (variant == kVariantThreaded)
+        hipLaunchKernelGGL(k_trace_step<kVariantThreaded>, full, blk, 0, s, st, p, sc, m);
+    else
+        hipLaunchKernelGGL(k_trace_step<kVariantStackless>, full, blk, 0, s, st, p, sc, m);
+    hipLaunchKernelGGL(k_logic, full, blk, 0, s, st, p, m, env);
+    hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues), dim3(1024), 0, s, st);
+    hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p);
+    return hipGetLastError();
+}
+
+}  // namespace rgpu

@@ -1,0 +1,506 @@
+// C ABI of the wavefront path tracer: materials, envmap, camera, and the
+// WavefrontLane-style lifecycle (setup / initialize / step / read back).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/raccoon_gpu.h"
+#include "envmap_build.h"
+#include "rgpu_ctx.h"
+#include "rgpu_scene.h"
+#include "rgpu_wavefront.h"
+
+using namespace rgpu;
+
+
+struct rgpu_materials {
+    rgpu_ctx* ctx = nullptr;
+    DevMaterials dev;
+    void* bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t nprims = 0;
+};
+
+struct rgpu_envmap {
+    rgpu_ctx* ctx = nullptr;
+    EnvmapTables host;
+    DevEnvmap dev;
+    void* bufs[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+struct rgpu_wavefront {
+    rgpu_ctx* ctx = nullptr;
+    const rgpu_scene* scene = nullptr;
+    const rgpu_materials* mats = nullptr;
+    const rgpu_envmap* env = nullptr;
+    WFParams params{};
+    uint64_t seed = 0;
+    int variant = 0;
+    int queue_blocks = 0;
+    Camera cam{};
+    bool initialized = false;
+    uint32_t npixels = 0;
+    DevState st;
+    std::vector<void*> owned;
+    unsigned long long* stat_total = nullptr;
+    uint8_t* tonemap_buf = nullptr;
+};
+
+#define RGPU_HIP(ctx, call)                                     \
+    do {                                                        \
+        hipError_t e_ = (call);                                 \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call);  \
+    } while (0)
+
+namespace {
+
+template <class T>
+int upload_vec(rgpu_ctx* ctx, const T* src, size_t n, void** dst) {
+    *dst = nullptr;
+    if (n == 0) {
+        // keep a valid pointer for kernels that index nothing
+        RGPU_HIP(ctx, hipMalloc(dst, 16));
+        return RGPU_OK;
+    }
+    RGPU_HIP(ctx, hipMalloc(dst, n * sizeof(T)));
+    RGPU_HIP(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return RGPU_OK;
+}
+
+int wf_alloc(rgpu_wavefront* w, size_t bytes, void** out) {
+    RGPU_HIP(w->ctx, hipMalloc(out, bytes < 16 ? 16 : bytes));
+    w->owned.push_back(*out);
+    return RGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rgpu_materials_create(rgpu_ctx* ctx, uint32_t nprims, const int32_t* types, const int32_t* indices,
+                          const rgpu_lambertian* lambertians, uint32_t nl, const rgpu_ward* wards, uint32_t nw,
+                          const rgpu_dielectric* dielectrics, uint32_t nd, const rgpu_volume* volumes, uint32_t nv,
+                          rgpu_materials** out) {
+    if (!ctx || !out || (nprims && (!types || !indices))) return RGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::vector<Material> mat(nprims);
+    for (uint32_t i = 0; i < nprims; ++i) {
+        int32_t t = types[i], k = indices[i];
+        uint32_t lim = t == kMatLambertian ? nl : t == kMatWard ? nw : t == kMatDielectric ? nd : t == kMatVolume ? nv : 1;
+        if (t < kMatLambertian || t > kMatVolume)
+            return fail(ctx, RGPU_ERR_INVALID_ARG, "material type must be 1..5 (lambertian..volume)");
+        if (t != kMatSpecular && (k < 0 || (uint32_t)k >= lim))
+            return fail(ctx, RGPU_ERR_INVALID_ARG, "material index out of range");
+        mat[i] = Material{t, k};
+    }
+    std::vector<Lambertian> L(nl);
+    for (uint32_t i = 0; i < nl; ++i) {
+        std::memcpy(L[i].Le, lambertians[i].Le, 12);
+        std::memcpy(L[i].R, lambertians[i].R, 12);
+        L[i].back_emission = lambertians[i].back_emission;
+        L[i].pad = 0;
+    }
+    std::vector<Ward> W(nw);
+    for (uint32_t i = 0; i < nw; ++i) {
+        W[i].alpha_x = wards[i].alpha_x;
+        W[i].alpha_y = wards[i].alpha_y;
+        std::memcpy(W[i].reflectance, wards[i].reflectance, 12);
+        std::memcpy(W[i].edgetint, wards[i].edgetint, 12);
+        W[i].falloff = wards[i].falloff;
+        W[i].pad = 0;
+        if (!(W[i].alpha_x > 0.0f) || !(W[i].alpha_y > 0.0f) || !(W[i].falloff > 0.0f))
+            return fail(ctx, RGPU_ERR_INVALID_ARG, "Ward roughness and falloff must be positive");
+    }
+    std::vector<Dielectric> D(nd);
+    for (uint32_t i = 0; i < nd; ++i) D[i].ior = dielectrics[i].ior;
+    std::vector<Volume> V(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        V[i].C = volumes[i].C;
+        std::memcpy(V[i].R, volumes[i].R, 12);
+        if (!(V[i].C > 0.0f)) return fail(ctx, RGPU_ERR_INVALID_ARG, "volume extinction must be positive");
+    }
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    rgpu_materials* m = new (std::nothrow) rgpu_materials();
+    if (!m) return RGPU_ERR_OUT_OF_MEMORY;
+    m->ctx = ctx;
+    m->nprims = nprims;
+    int rc = upload_vec(ctx, mat.data(), mat.size(), &m->bufs[0]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, L.data(), L.size(), &m->bufs[1]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, W.data(), W.size(), &m->bufs[2]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, D.data(), D.size(), &m->bufs[3]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, V.data(), V.size(), &m->bufs[4]);
+    if (rc != RGPU_OK) {
+        rgpu_materials_destroy(m);
+        return rc;
+    }
+    m->dev.mat = (const Material*)m->bufs[0];
+    m->dev.lambertian = (const Lambertian*)m->bufs[1];
+    m->dev.ward = (const Ward*)m->bufs[2];
+    m->dev.dielectric = (const Dielectric*)m->bufs[3];
+    m->dev.volume = (const Volume*)m->bufs[4];
+    *out = m;
+    return RGPU_OK;
+}
+
+int rgpu_materials_destroy(rgpu_materials* m) {
+    if (!m) return RGPU_ERR_INVALID_ARG;
+    (void)hipSetDevice(m->ctx->device);
+    for (void* p : m->bufs)
+        if (p) (void)hipFree(p);
+    delete m;
+    return RGPU_OK;
+}
+
+int rgpu_envmap_create(rgpu_ctx* ctx, const float* rgb, uint32_t channels, uint32_t width, uint32_t height,
+                       rgpu_envmap** out) {
+    if (!out || !rgb || width == 0 || height == 0 || (channels != 3 && channels != 4)) return RGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    if ((uint64_t)width * height >= (1ull << 31)) return fail(ctx, RGPU_ERR_INVALID_ARG, "envmap too large");
+    rgpu_envmap* e = new (std::nothrow) rgpu_envmap();
+    if (!e) return RGPU_ERR_OUT_OF_MEMORY;
+    e->ctx = ctx;
+    try {
+        build_envmap_tables(rgb, channels, width, height, &e->host);
+    } catch (const std::exception& ex) {
+        delete e;
+        return fail(ctx, RGPU_ERR_INVALID_ARG, ex.what());
+    }
+    if (!ctx) {  // host tables only
+        *out = e;
+        return RGPU_OK;
+    }
+    const size_t P = (size_t)width * height;
+    std::vector<float> pdf(7 * P);
+    std::vector<AliasBucket> alias(7 * P);
+    for (int t = 0; t < 7; ++t) {
+        std::memcpy(pdf.data() + t * P, e->host.pdf[t].data(), P * sizeof(float));
+        std::memcpy(alias.data() + t * P, e->host.alias[t].data(), P * sizeof(AliasBucket));
+    }
+    int rc = hipSetDevice(ctx->device) == hipSuccess ? RGPU_OK : RGPU_ERR_HIP;
+    if (rc == RGPU_OK) rc = upload_vec(ctx, e->host.texels.data(), e->host.texels.size(), &e->bufs[0]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, e->host.fragments.data(), P, &e->bufs[1]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, pdf.data(), pdf.size(), &e->bufs[2]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, alias.data(), alias.size(), &e->bufs[3]);
+    if (rc != RGPU_OK) {
+        rgpu_envmap_destroy(e);
+        return rc;
+    }
+    e->dev.texels = (const float4*)e->bufs[0];
+    e->dev.fragments = (const EnvFragment*)e->bufs[1];
+    e->dev.pdf = (const float*)e->bufs[2];
+    e->dev.alias = (const AliasBucket*)e->bufs[3];
+    e->dev.width = width;
+    e->dev.height = height;
+    *out = e;
+    return RGPU_OK;
+}
+
+int rgpu_envmap_destroy(rgpu_envmap* e) {
+    if (!e) return RGPU_ERR_INVALID_ARG;
+    if (e->ctx) (void)hipSetDevice(e->ctx->device);
+    for (void* p : e->bufs)
+        if (p) (void)hipFree(p);
+    delete e;
+    return RGPU_OK;
+}
+
+int rgpu_envmap_export(const rgpu_envmap* e, int table, float* pdf, float* heights, int32_t* aliases,
+                       float* fragments, int32_t* degenerate) {
+    if (!e || table < 0 || table > 6) return RGPU_ERR_INVALID_ARG;
+    const size_t P = (size_t)e->host.width * e->host.height;
+    for (size_t i = 0; i < P; ++i) {
+        if (pdf) pdf[i] = e->host.pdf[table][i];
+        if (heights) heights[i] = e->host.alias[table][i].height;
+        if (aliases) aliases[i] = e->host.alias[table][i].alias;
+        if (fragments) {
+            const EnvFragment& f = e->host.fragments[i];
+            fragments[4 * i + 0] = f.beg_y;
+            fragments[4 * i + 1] = f.end_y;
+            fragments[4 * i + 2] = f.beg_phi;
+            fragments[4 * i + 3] = f.end_phi;
+        }
+    }
+    if (degenerate) *degenerate = e->host.degenerate[table] ? 1 : 0;
+    return RGPU_OK;
+}
+
+int rgpu_camera_look_at(const float* eye, const float* target, const float* up, float fov_y_deg,
+                        float focus_distance, uint32_t width, uint32_t height, float lens_radius, rgpu_camera* out) {
+    if (!eye || !target || !up || !out || width == 0 || height == 0 || !(fov_y_deg > 0.0f) || !(fov_y_deg < 180.0f))
+        return RGPU_ERR_INVALID_ARG;
+    double f[3], r[3], u[3];
+    for (int k = 0; k < 3; ++k) f[k] = (double)target[k] - eye[k];
+    double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (!(fl > 0.0)) return RGPU_ERR_INVALID_ARG;
+    for (double& x : f) x /= fl;
+    r[0] = f[1] * up[2] - f[2] * up[1];
+    r[1] = f[2] * up[0] - f[0] * up[2];
+    r[2] = f[0] * up[1] - f[1] * up[0];
+    double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(rl > 0.0)) return RGPU_ERR_INVALID_ARG;
+    for (double& x : r) x /= rl;
+    u[0] = r[1] * f[2] - r[2] * f[1];
+    u[1] = r[2] * f[0] - r[0] * f[2];
+    u[2] = r[0] * f[1] - r[1] * f[0];
+    double dist = focus_distance > 0.0f ? focus_distance : fl;
+    double ph = 2.0 * dist * std::tan(fov_y_deg * 3.14159265358979323846 / 360.0);  // object-plane height
+    double pw = ph * width / height;
+    std::memset(out, 0, sizeof(*out));
+    for (int k = 0; k < 3; ++k) {
+        out->eye[k] = eye[k];
+        out->forward[k] = (float)f[k];
+        out->up[k] = (float)u[k];
+        out->right[k] = (float)r[k];
+        // image plane: top-left corner, then right / down steps per pixel (standardCamera,
+        // common/wavefront_path_tracing.hpp:103-118)
+        out->imageplane_o[k] = (float)(eye[k] + f[k] * dist - r[k] * pw * 0.5 + u[k] * ph * 0.5);
+        out->imageplane_r[k] = (float)(r[k] * pw / width);
+        out->imageplane_b[k] = (float)(-u[k] * ph / height);
+    }
+    out->width = width;
+    out->height = height;
+    out->lens_radius = lens_radius;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_destroy(rgpu_wavefront* w) {
+    if (!w) return RGPU_ERR_INVALID_ARG;
+    (void)hipSetDevice(w->ctx->device);
+    (void)hipStreamSynchronize(w->ctx->stream);
+    for (void* p : w->owned) (void)hipFree(p);
+    delete w;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_materials* mats, const rgpu_envmap* env,
+                          const rgpu_wavefront_config* cfg, rgpu_wavefront** out) {
+    if (!ctx || !scene || !mats || !env || !cfg || !out) return RGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!env->ctx) return fail(ctx, RGPU_ERR_INVALID_ARG, "envmap was built without a context (host tables only)");
+    if (cfg->abi_version != RGPU_ABI_VERSION) return fail(ctx, RGPU_ERR_INVALID_ARG, "abi_version mismatch");
+    if (cfg->path_count == 0 || cfg->path_count > (1u << 28))
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "path_count must be in 1..2^28");
+    if (cfg->variant != RGPU_VARIANT_STACKLESS && cfg->variant != RGPU_VARIANT_THREADED)
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
+    if (mats->nprims != scene->nprims)
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "materials must give exactly one entry per primitive");
+    rgpu_wavefront* w = new (std::nothrow) rgpu_wavefront();
+    if (!w) return RGPU_ERR_OUT_OF_MEMORY;
+    w->ctx = ctx;
+    w->scene = scene;
+    w->mats = mats;
+    w->env = env;
+    WFParams& p = w->params;
+    p.n = cfg->path_count;
+    p.rr_start = cfg->rr_start ? cfg->rr_start : 25u;
+    p.max_depth = cfg->max_depth;
+    p.accum64 = (cfg->flags & RGPU_WF_ACCUM64) ? 1u : 0u;
+    p.quirks = (cfg->flags & RGPU_WF_NO_REFERENCE_QUIRKS) ? 0u : 1u;
+    p.clamp = cfg->radiance_clamp > 0.0f ? cfg->radiance_clamp : 1000.0f;
+    p.bxdf_p = cfg->bxdf_select_p > 0.0f ? cfg->bxdf_select_p : 0.5f;
+    w->seed = cfg->seed;
+    w->variant = (int)cfg->variant;
+    w->queue_blocks = cfg->queue_blocks ? (int)cfg->queue_blocks : (ctx->cu_count > 0 ? ctx->cu_count * 8 : 2048);
+    const size_t n = p.n;
+    DevState& s = w->st;
+    s.nblocks = (uint32_t)((n + 255) / 256);
+    int rc = hipSetDevice(ctx->device) == hipSuccess ? RGPU_OK : RGPU_ERR_HIP;
+    void* ptr;
+    auto A = [&](size_t bytes) -> void* {
+        if (rc != RGPU_OK) return nullptr;
+        rc = wf_alloc(w, bytes, &ptr);
+        return rc == RGPU_OK ? ptr : nullptr;
+    };
+    s.ray = (TraceRay*)A(n * sizeof(TraceRay));
+    s.path = (PathRec*)A(n * sizeof(PathRec));
+    s.ext = (ExtRec*)A(n * sizeof(ExtRec));
+    s.shade = (ShadeRec*)A(n * sizeof(ShadeRec));
+    s.vol = (VolList*)A(n * sizeof(VolList));
+    s.rng = (uint4*)A(n * sizeof(uint4));
+    s.qsel = (uint8_t*)A(n);
+    s.items = (uint32_t*)A(n * kNumQueues * sizeof(uint32_t));
+    s.counts = (uint32_t*)A(kNumQueues * sizeof(uint32_t));
+    s.block_counts = (uint32_t*)A((size_t)s.nblocks * kNumQueues * sizeof(uint32_t));
+    s.next_pixel = (unsigned long long*)A(sizeof(unsigned long long));
+    w->stat_total = (unsigned long long*)A(sizeof(unsigned long long));
+    if (rc != RGPU_OK) {
+        rgpu_wavefront_destroy(w);
+        return rc;
+    }
+    *out = w;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* cam, uint32_t lane_index, uint32_t num_lanes) {
+    if (!w || !cam || num_lanes == 0 || lane_index >= num_lanes || cam->width == 0 || cam->height == 0)
+        return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t P = cam->width * cam->height;
+    if (P != w->npixels) {
+        // (re)allocate the accumulation buffers for this resolution
+        for (void* q : {(void*)w->st.accum, (void*)w->st.accum64, (void*)w->st.accum_count, (void*)w->tonemap_buf}) {
+            if (!q) continue;
+            for (auto it = w->owned.begin(); it != w->owned.end(); ++it)
+                if (*it == q) {
+                    (void)hipFree(q);
+                    w->owned.erase(it);
+                    break;
+                }
+        }
+        w->st.accum = nullptr;
+        w->st.accum64 = nullptr;
+        w->st.accum_count = nullptr;
+        void* ptr;
+        int rc;
+        if (w->params.accum64) {
+            if ((rc = wf_alloc(w, (size_t)P * 3 * sizeof(double), &ptr)) != RGPU_OK) return rc;
+            w->st.accum64 = (double*)ptr;
+            if ((rc = wf_alloc(w, (size_t)P * sizeof(uint32_t), &ptr)) != RGPU_OK) return rc;
+            w->st.accum_count = (uint32_t*)ptr;
+        } else {
+            if ((rc = wf_alloc(w, (size_t)P * 4 * sizeof(float), &ptr)) != RGPU_OK) return rc;
+            w->st.accum = (float*)ptr;
+        }
+        if ((rc = wf_alloc(w, (size_t)P * 4, &ptr)) != RGPU_OK) return rc;
+        w->tonemap_buf = (uint8_t*)ptr;
+        w->npixels = P;
+    }
+    Camera& c = w->cam;
+    for (int k = 0; k < 3; ++k) {
+        c.eye[k] = cam->eye[k];
+        c.forward[k] = cam->forward[k];
+        c.up[k] = cam->up[k];
+        c.right[k] = cam->right[k];
+        c.o[k] = cam->imageplane_o[k];
+        c.r[k] = cam->imageplane_r[k];
+        c.b[k] = cam->imageplane_b[k];
+    }
+    c.width = cam->width;
+    c.height = cam->height;
+    c.lens_radius = cam->lens_radius;
+    // WavefrontLane::initialize (common/wavefront_path_tracing.hpp:532-570): pixel offset
+    // lane * (P / lanes), RNG offset lane * N, every slot in the new-path queue.
+    const uint64_t next_pixel = (uint64_t)lane_index * (P / num_lanes);
+    const uint64_t seed_offset = w->seed + (uint64_t)lane_index * w->params.n;
+    RGPU_HIP(ctx, wf_initialize(w->st, w->params, seed_offset, next_pixel, P, ctx->stream));
+    RGPU_HIP(ctx, wf_clear_accum(w->st, P, w->params.accum64 != 0, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    w->initialized = true;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_step(rgpu_wavefront* w, rgpu_stream stream, uint32_t steps) {
+    if (!w) return RGPU_ERR_INVALID_ARG;
+    if (!w->initialized) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_initialize was not called");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = stream_of(ctx, stream);
+    for (uint32_t i = 0; i < steps; ++i)
+        RGPU_HIP(ctx, wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->variant,
+                              w->queue_blocks, s));
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_time_steps(rgpu_wavefront* w, uint32_t steps, double* ms_per_step) {
+    if (!w || !ms_per_step || steps == 0) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    hipEvent_t a, b;
+    RGPU_HIP(ctx, hipEventCreate(&a));
+    RGPU_HIP(ctx, hipEventCreate(&b));
+    (void)hipEventRecord(a, ctx->stream);
+    int rc = rgpu_wavefront_step(w, nullptr, steps);
+    (void)hipEventRecord(b, ctx->stream);
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc != RGPU_OK) return rc;
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventElapsedTime");
+    *ms_per_step = (double)ms / steps;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_queue_counts(rgpu_wavefront* w, uint32_t* counts7, uint64_t* next_pixel) {
+    if (!w || !counts7) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipMemcpyAsync(counts7, w->st.counts, kNumQueues * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (next_pixel)
+        RGPU_HIP(ctx, hipMemcpyAsync(next_pixel, w->st.next_pixel, 8, hipMemcpyDeviceToHost, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_read_accumulation(rgpu_wavefront* w, double* rgb, uint32_t* counts) {
+    if (!w || !w->initialized) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t P = w->npixels;
+    if (w->params.accum64) {
+        std::vector<double> a(3 * P);
+        std::vector<uint32_t> c(P);
+        RGPU_HIP(ctx, hipMemcpy(a.data(), w->st.accum64, 3 * P * 8, hipMemcpyDeviceToHost));
+        RGPU_HIP(ctx, hipMemcpy(c.data(), w->st.accum_count, P * 4, hipMemcpyDeviceToHost));
+        if (rgb) std::memcpy(rgb, a.data(), 3 * P * 8);
+        if (counts) std::memcpy(counts, c.data(), P * 4);
+    } else {
+        std::vector<float> a(4 * P);
+        RGPU_HIP(ctx, hipMemcpy(a.data(), w->st.accum, 4 * P * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < P; ++i) {
+            if (rgb)
+                for (int k = 0; k < 3; ++k) rgb[3 * i + k] = a[4 * i + k];
+            if (counts) std::memcpy(&counts[i], &a[4 * i + 3], 4);
+        }
+    }
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_tonemap(rgpu_wavefront* w, uint8_t* rgba8) {
+    if (!w || !w->initialized || !rgba8) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, wf_tonemap(w->st, w->npixels, w->params.accum64 != 0, w->tonemap_buf, ctx->stream));
+    RGPU_HIP(ctx, hipMemcpyAsync(rgba8, w->tonemap_buf, (size_t)w->npixels * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_stat(rgpu_wavefront* w, uint64_t* total_samples) {
+    if (!w || !w->initialized || !total_samples) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipMemsetAsync(w->stat_total, 0, 8, ctx->stream));
+    RGPU_HIP(ctx, wf_stat(w->st, w->npixels, w->params.accum64 != 0, w->stat_total, ctx->stream));
+    unsigned long long t = 0;
+    RGPU_HIP(ctx, hipMemcpyAsync(&t, w->stat_total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *total_samples = t;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, float* paths, void* ext, float* shade,
+                              int32_t* vol, uint32_t* items) {
+    if (!w || !w->initialized) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t n = w->params.n;
+    if (rng) RGPU_HIP(ctx, hipMemcpy(rng, w->st.rng, n * 16, hipMemcpyDeviceToHost));
+    if (rays) RGPU_HIP(ctx, hipMemcpy(rays, w->st.ray, n * 32, hipMemcpyDeviceToHost));
+    if (paths) RGPU_HIP(ctx, hipMemcpy(paths, w->st.path, n * 32, hipMemcpyDeviceToHost));
+    if (ext) RGPU_HIP(ctx, hipMemcpy(ext, w->st.ext, n * 32, hipMemcpyDeviceToHost));
+    if (shade) RGPU_HIP(ctx, hipMemcpy(shade, w->st.shade, n * 32, hipMemcpyDeviceToHost));
+    if (vol) RGPU_HIP(ctx, hipMemcpy(vol, w->st.vol, n * 32, hipMemcpyDeviceToHost));
+    if (items) RGPU_HIP(ctx, hipMemcpy(items, w->st.items, n * kNumQueues * 4, hipMemcpyDeviceToHost));
+    return RGPU_OK;
+}
+
+}  // extern "C"
