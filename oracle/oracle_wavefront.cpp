@@ -304,20 +304,26 @@ f3 get_envmap_value(const Envmap& e, f3 rd, bool use_filter) {
     float v = theta / pi;
     int W = (int)e.w, H = (int)e.h;
     float uu = (u - std::floor(u)) * (float)W, vv = (v - std::floor(v)) * (float)H;
+    // The clamps only matter for a non-finite direction (an out-of-range index otherwise).
+    auto cl = [](int v, int hi) { return std::min(std::max(v, 0), hi); };
     if (!use_filter) {
-        int i = (int)std::floor(uu), j = (int)std::floor(vv);
+        int i = std::isfinite(uu) ? (int)std::floor(uu) : 0, j = std::isfinite(vv) ? (int)std::floor(vv) : 0;
         if (i > W - 1) i = i - W;
         if (j > H - 1) j = j - H;
-        return tex(e, i, j);
+        return tex(e, cl(i, W - 1), cl(j, H - 1));
     }
     float us = uu - 0.5f, vs = vv - 0.5f;
-    int i0 = (int)std::floor(us), j0 = (int)std::floor(vs);
+    int i0 = std::isfinite(us) ? (int)std::floor(us) : 0, j0 = std::isfinite(vs) ? (int)std::floor(vs) : 0;
     int i1 = i0 + 1, j1 = j0 + 1;
     float a = us - std::floor(us), b = vs - std::floor(vs);
     if (i0 < 0) i0 += W;
     if (j0 < 0) j0 += H;
     if (i1 > W - 1) i1 -= W;
     if (j1 > H - 1) j1 -= H;
+    i0 = cl(i0, W - 1);
+    i1 = cl(i1, W - 1);
+    j0 = cl(j0, H - 1);
+    j1 = cl(j1, H - 1);
     f3 t00 = tex(e, i0, j0), t10 = tex(e, i1, j0), t01 = tex(e, i0, j1), t11 = tex(e, i1, j1);
     float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
     return F(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x, w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
@@ -344,8 +350,8 @@ int envmap_index(const Envmap& e, f3 wi) {
     const float pi = kPiF;
     float u = 1.0f - phi / (2.0f * pi);
     float v = theta / pi;
-    int ix = (int)std::floor(u * (float)e.w);
-    int iy = (int)std::floor(v * (float)e.h);
+    int ix = std::isfinite(u) ? (int)std::floor(u * (float)e.w) : 0;
+    int iy = std::isfinite(v) ? (int)std::floor(v * (float)e.h) : 0;
     ix = fract_int(ix, (int)e.w);
     iy = std::min(std::max(iy, 0), (int)e.h - 1);
     return iy * (int)e.w + ix;

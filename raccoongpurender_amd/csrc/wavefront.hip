@@ -105,12 +105,13 @@ __device__ F3 envmap_value(const DevEnvmap& e, F3 rd, bool filter) {
     const int W = (int)e.width, H = (int)e.height;
     float uu = (u - floorf(u)) * (float)W;
     float vv = (v - floorf(v)) * (float)H;
+    // The final clamps change nothing for a finite direction; they keep a non-finite one in bounds.
     if (!filter) {
         int i = (int)floorf(uu);
         int j = (int)floorf(vv);
         if (i > W - 1) i -= W;
         if (j > H - 1) j -= H;
-        return texel(e, i, j);
+        return texel(e, min(max(i, 0), W - 1), min(max(j, 0), H - 1));
     }
     float us = uu - 0.5f, vs = vv - 0.5f;
     int i0 = (int)floorf(us), j0 = (int)floorf(vs);
@@ -120,6 +121,10 @@ __device__ F3 envmap_value(const DevEnvmap& e, F3 rd, bool filter) {
     if (j0 < 0) j0 += H;
     if (i1 > W - 1) i1 -= W;
     if (j1 > H - 1) j1 -= H;
+    i0 = min(max(i0, 0), W - 1);
+    i1 = min(max(i1, 0), W - 1);
+    j0 = min(max(j0, 0), H - 1);
+    j1 = min(max(j1, 0), H - 1);
     F3 t00 = texel(e, i0, j0), t10 = texel(e, i1, j0), t01 = texel(e, i0, j1), t11 = texel(e, i1, j1);
     float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
     return F3{w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
