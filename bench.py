@@ -71,8 +71,36 @@ def cpu_baseline(tree, points, indices, rays, budget_s=10.0):
                       f"kernel on the reference 96-byte node layout, {threads} threads, {dt:.1f} s"}
 
 
+def render_bench(steps=32, warmup=16, paths=1 << 22):
+    """Full wavefront render at 1920x1080 (2,017,294 triangles, every material, a medium and a
+    2048x1024 sun envmap): samples/s and extension rays/s over `steps` bounces after `warmup`."""
+    from raccoongpurender_amd import _lib, render_workloads
+    from raccoongpurender_amd.render import build_lane
+
+    t0 = time.perf_counter()
+    sd = render_workloads.full_render_scene()
+    ctx = _lib.Context(0)
+    wf = build_lane(ctx, sd, paths)
+    setup = time.perf_counter() - t0
+    wf.step(warmup)
+    ctx.synchronize()
+    s0 = wf.stat()
+    ms = wf.time_steps(steps)
+    s1 = wf.stat()
+    sec = ms * steps * 1e-3
+    out = {"workload": "1920x1080, 2017294 tris, lambertian/ward/mirror/dielectric + medium + sun envmap",
+           "paths": paths, "steps": steps, "ms_per_step": round(ms, 3),
+           "samples_per_s": round((s1 - s0) / sec, 1), "ext_mrays_per_s": round(paths * steps / sec / 1e6, 1),
+           "setup_s": round(setup, 1)}
+    print(f"[bench] render: {ms:.3f} ms/step, {(s1 - s0) / sec / 1e6:.2f} Msamples/s, "
+          f"{paths * steps / sec / 1e6:.0f} Mrays/s", file=sys.stderr, flush=True)
+    wf.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--no-render", action="store_true", help="skip the full-render samples/s measurement")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -178,6 +206,9 @@ def main():
         },
     }
     out["extra"]["primary_mrays"] = round(max(results[(v, "coherent")]["mrays"] for v in variants), 2)
+    if not args.no_render:
+        scene.close()
+        out["extra"]["render"] = render_bench()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(bvh.export(), d["points"], d["indices"], d["incoherent"])
