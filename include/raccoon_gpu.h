@@ -273,6 +273,24 @@ int rgpu_wavefront_stat(rgpu_wavefront* w, uint64_t* total_samples);
 int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, float* paths, void* ext,
                               float* shade, int32_t* vol, uint32_t* items);
 
+/* hipGraph of the bounce loop: captures `steps` consecutive steps once; each launch replays
+ * them. Replaces the reference's per-step host launch list (wavefront_path_tracing.hpp:606-875).
+ * The graph holds the camera and buffers by value; initialize() and set_launch() drop it.
+ * Calls on one wavefront that overlap on different streams race. */
+int rgpu_wavefront_graph_capture(rgpu_wavefront* w, uint32_t steps);
+int rgpu_wavefront_graph_launch(rgpu_wavefront* w, rgpu_stream stream, uint32_t launches);
+
+/* Launch configuration: traversal variant, extension-ray-cast workgroup size (64/128/256) and
+ * grid of the queue-driven kernels. The reference leaves workgroup sizes to the driver
+ * (common/raccoon_ocl.hpp:906). */
+int rgpu_wavefront_set_launch(rgpu_wavefront* w, int variant, uint32_t trace_block, uint32_t queue_blocks);
+int rgpu_wavefront_get_launch(const rgpu_wavefront* w, int* variant, uint32_t* trace_block,
+                              uint32_t* queue_blocks);
+/* Autotuner: times every (variant, trace_block, queue_blocks) configuration for `steps` steps
+ * after `warmup`, keeps the fastest and leaves the wavefront initialised with `camera`. */
+int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32_t warmup, uint32_t steps,
+                            double* best_ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -362,6 +362,13 @@ _sig("rgpu_wavefront_read_accumulation", C.c_int, vp, vp, vp)
 _sig("rgpu_wavefront_tonemap", C.c_int, vp, vp)
 _sig("rgpu_wavefront_stat", C.c_int, vp, C.POINTER(C.c_uint64))
 _sig("rgpu_wavefront_read_state", C.c_int, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("rgpu_wavefront_graph_capture", C.c_int, vp, u32)
+_sig("rgpu_wavefront_graph_launch", C.c_int, vp, vp, u32)
+_sig("rgpu_wavefront_set_launch", C.c_int, vp, C.c_int, u32, u32)
+_sig("rgpu_wavefront_get_launch", C.c_int, vp, C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32))
+_sig("rgpu_wavefront_autotune", C.c_int, vp, C.POINTER(rgpu_camera), u32, u32, C.POINTER(C.c_double))
+EXPORTED += ["rgpu_wavefront_graph_capture", "rgpu_wavefront_graph_launch", "rgpu_wavefront_set_launch",
+             "rgpu_wavefront_get_launch", "rgpu_wavefront_autotune"]
 EXPORTED += [
     "rgpu_materials_create", "rgpu_materials_destroy", "rgpu_envmap_create", "rgpu_envmap_destroy",
     "rgpu_envmap_export", "rgpu_camera_look_at", "rgpu_wavefront_create", "rgpu_wavefront_destroy",
@@ -542,6 +549,30 @@ class Wavefront:
                                                                 ("rng", "rays", "paths", "ext", "shade", "vol",
                                                                  "items")]), "rgpu_wavefront_read_state")
         return st
+
+    def graph_capture(self, steps):
+        self.ctx.check(lib.rgpu_wavefront_graph_capture(self.h, steps), "rgpu_wavefront_graph_capture")
+
+    def graph_launch(self, launches=1):
+        self.ctx.check(lib.rgpu_wavefront_graph_launch(self.h, None, launches), "rgpu_wavefront_graph_launch")
+
+    def set_launch(self, variant, trace_block, queue_blocks):
+        v = VARIANTS[variant] if isinstance(variant, str) else variant
+        self.ctx.check(lib.rgpu_wavefront_set_launch(self.h, v, trace_block, queue_blocks),
+                       "rgpu_wavefront_set_launch")
+
+    def get_launch(self):
+        v, b, q = C.c_int(0), C.c_uint32(0), C.c_uint32(0)
+        self.ctx.check(lib.rgpu_wavefront_get_launch(self.h, C.byref(v), C.byref(b), C.byref(q)),
+                       "rgpu_wavefront_get_launch")
+        return {0: "stackless", 1: "threaded"}[v.value], b.value, q.value
+
+    def autotune(self, camera, warmup=4, steps=8):
+        ms = C.c_double(0)
+        self.ctx.check(lib.rgpu_wavefront_autotune(self.h, C.byref(camera), warmup, steps, C.byref(ms)),
+                       "rgpu_wavefront_autotune")
+        self.camera = camera
+        return ms.value
 
     def close(self):
         if self.h:

@@ -124,8 +124,11 @@ struct DevState {
 
 hipError_t wf_initialize(const DevState& st, const WFParams& p, uint64_t seed_offset, uint64_t next_pixel,
                          uint32_t npixels, hipStream_t s);
+// One bounce. persistent_blocks: grid of the queue-driven kernels; trace_block: workgroup
+// size of the extension ray cast (64, 128 or 256).
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
-                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, hipStream_t s);
+                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, int trace_block,
+                   hipStream_t s);
 hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hipStream_t s);
 hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s);
 hipError_t wf_stat(const DevState& st, uint32_t npixels, bool accum64, unsigned long long* total, hipStream_t s);

@@ -923,9 +923,12 @@ hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hi
 }
 
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
-                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, hipStream_t s) {
+                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, int trace_block,
+                   hipStream_t s) {
     const dim3 qgrid(persistent_blocks > 0 ? persistent_blocks : 1024), blk(256);
     const dim3 full((p.n + 255) / 256);
+    const uint32_t tb = (trace_block == 64 || trace_block == 128) ? (uint32_t)trace_block : 256u;
+    const dim3 tgrid((p.n + tb - 1) / tb), tblk(tb);
     hipLaunchKernelGGL(k_new_path, qgrid, blk, 0, s, st, p, cam, env);
     hipLaunchKernelGGL(k_shade_lambertian, qgrid, blk, 0, s, st, p, m, env);
     hipLaunchKernelGGL(k_shade_specular, qgrid, blk, 0, s, st, p);
@@ -934,9 +937,9 @@ hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, 
     hipLaunchKernelGGL(k_shade_volume_surface, qgrid, blk, 0, s, st, p, m);
     hipLaunchKernelGGL(k_shade_volume_inside, qgrid, blk, 0, s, st, p, m, env);
     if (variant == kVariantThreaded)
-        hipLaunchKernelGGL(k_trace_step<kVariantThreaded>, full, blk, 0, s, st, p, sc, m);
+        hipLaunchKernelGGL(k_trace_step<kVariantThreaded>, tgrid, tblk, 0, s, st, p, sc, m);
     else
-        hipLaunchKernelGGL(k_trace_step<kVariantStackless>, full, blk, 0, s, st, p, sc, m);
+        hipLaunchKernelGGL(k_trace_step<kVariantStackless>, tgrid, tblk, 0, s, st, p, sc, m);
     hipLaunchKernelGGL(k_logic, full, blk, 0, s, st, p, m, env);
     hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues), dim3(1024), 0, s, st);
     hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p);

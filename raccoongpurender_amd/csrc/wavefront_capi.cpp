@@ -47,7 +47,18 @@ struct rgpu_wavefront {
     std::vector<void*> owned;
     unsigned long long* stat_total = nullptr;
     uint8_t* tonemap_buf = nullptr;
+    int trace_block = 256;
+    hipGraphExec_t graph = nullptr;  // captured bounce loop
+    uint32_t graph_steps = 0;
 };
+
+namespace {
+void drop_graph(rgpu_wavefront* w) {
+    if (w->graph) (void)hipGraphExecDestroy(w->graph);
+    w->graph = nullptr;
+    w->graph_steps = 0;
+}
+}  // namespace
 
 #define RGPU_HIP(ctx, call)                                     \
     do {                                                        \
@@ -270,6 +281,7 @@ int rgpu_wavefront_destroy(rgpu_wavefront* w) {
     if (!w) return RGPU_ERR_INVALID_ARG;
     (void)hipSetDevice(w->ctx->device);
     (void)hipStreamSynchronize(w->ctx->stream);
+    drop_graph(w);
     for (void* p : w->owned) (void)hipFree(p);
     delete w;
     return RGPU_OK;
@@ -340,6 +352,7 @@ int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* cam, uint32_
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(w);  // a captured graph holds the old camera and buffers by value
     const uint32_t P = cam->width * cam->height;
     if (P != w->npixels) {
         // (re)allocate the accumulation buffers for this resolution
@@ -402,8 +415,101 @@ int rgpu_wavefront_step(rgpu_wavefront* w, rgpu_stream stream, uint32_t steps) {
     hipStream_t s = stream_of(ctx, stream);
     for (uint32_t i = 0; i < steps; ++i)
         RGPU_HIP(ctx, wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->variant,
-                              w->queue_blocks, s));
+                              w->queue_blocks, w->trace_block, s));
     return RGPU_OK;
+}
+
+int rgpu_wavefront_graph_capture(rgpu_wavefront* w, uint32_t steps) {
+    if (!w || steps == 0) return RGPU_ERR_INVALID_ARG;
+    if (!w->initialized) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_initialize was not called");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(w);
+    hipStream_t cs;
+    RGPU_HIP(ctx, hipStreamCreate(&cs));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    for (uint32_t i = 0; i < steps && e == hipSuccess; ++i)
+        e = wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->variant, w->queue_blocks,
+                    w->trace_block, cs);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        w->graph = nullptr;
+        return hip_fail(ctx, e, "rgpu_wavefront_graph_capture");
+    }
+    w->graph_steps = steps;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_graph_launch(rgpu_wavefront* w, rgpu_stream stream, uint32_t launches) {
+    if (!w) return RGPU_ERR_INVALID_ARG;
+    if (!w->graph) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "no captured graph (rgpu_wavefront_graph_capture)");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    for (uint32_t i = 0; i < launches; ++i) RGPU_HIP(ctx, hipGraphLaunch(w->graph, stream_of(ctx, stream)));
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_set_launch(rgpu_wavefront* w, int variant, uint32_t trace_block, uint32_t queue_blocks) {
+    if (!w) return RGPU_ERR_INVALID_ARG;
+    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
+    if (trace_block != 64 && trace_block != 128 && trace_block != 256)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "trace_block must be 64, 128 or 256");
+    if (queue_blocks == 0 || queue_blocks > 65536)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "queue_blocks must be in 1..65536");
+    drop_graph(w);
+    w->variant = variant;
+    w->trace_block = (int)trace_block;
+    w->queue_blocks = (int)queue_blocks;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_get_launch(const rgpu_wavefront* w, int* variant, uint32_t* trace_block, uint32_t* queue_blocks) {
+    if (!w) return RGPU_ERR_INVALID_ARG;
+    if (variant) *variant = w->variant;
+    if (trace_block) *trace_block = (uint32_t)w->trace_block;
+    if (queue_blocks) *queue_blocks = (uint32_t)w->queue_blocks;
+    return RGPU_OK;
+}
+
+// Times every launch configuration on the live wavefront (re-initialised with `camera`
+// before each trial) and keeps the fastest; the wavefront is left initialised with it.
+int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32_t warmup, uint32_t steps,
+                            double* best_ms_per_step) {
+    if (!w || !camera || steps == 0) return RGPU_ERR_INVALID_ARG;
+    const int cu = w->ctx->cu_count > 0 ? w->ctx->cu_count : 256;
+    const int variants[2] = {RGPU_VARIANT_STACKLESS, RGPU_VARIANT_THREADED};
+    const uint32_t blocks[3] = {64, 128, 256};
+    const uint32_t qmul[4] = {2, 4, 8, 16};
+    double best = 1e300;
+    int bv = w->variant;
+    uint32_t bb = (uint32_t)w->trace_block, bq = (uint32_t)w->queue_blocks;
+    for (int v : variants)
+        for (uint32_t b : blocks)
+            for (uint32_t q : qmul) {
+                int rc = rgpu_wavefront_set_launch(w, v, b, (uint32_t)cu * q);
+                if (rc == RGPU_OK) rc = rgpu_wavefront_initialize(w, camera, 0, 1);
+                if (rc == RGPU_OK && warmup) rc = rgpu_wavefront_step(w, nullptr, warmup);
+                double ms = 0.0;
+                if (rc == RGPU_OK) rc = rgpu_wavefront_time_steps(w, steps, &ms);
+                if (rc != RGPU_OK) return rc;
+                if (ms < best) {
+                    best = ms;
+                    bv = v;
+                    bb = b;
+                    bq = (uint32_t)cu * q;
+                }
+            }
+    int rc = rgpu_wavefront_set_launch(w, bv, bb, bq);
+    if (rc == RGPU_OK) rc = rgpu_wavefront_initialize(w, camera, 0, 1);
+    if (rc == RGPU_OK && best_ms_per_step) *best_ms_per_step = best;
+    return rc;
 }
 
 int rgpu_wavefront_time_steps(rgpu_wavefront* w, uint32_t steps, double* ms_per_step) {
