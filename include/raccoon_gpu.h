@@ -291,6 +291,41 @@ int rgpu_wavefront_get_launch(const rgpu_wavefront* w, int* variant, uint32_t* t
 int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32_t warmup, uint32_t steps,
                             double* best_ms_per_step);
 
+/* ================================================================ path guiding
+ * No counterpart in the reference (it has no path guiding). A small MLP maps a vertex (12-band
+ * frequency-encoded position in the scene box, normal, outgoing direction) to an 8-lobe von
+ * Mises-Fisher mixture; at lambertian and Ward vertices it is a third strategy of the
+ * reference's one-sample mixture pdf (kernels/types.cl:58-71, kernels/mixture_density.cl),
+ * chosen with probability select_p. It trains online (Adam) on samples the tracer records. */
+typedef struct rgpu_guide rgpu_guide;
+typedef struct rgpu_guide_config {
+    uint32_t abi_version;
+    float select_p;        /* guide selection probability in (0, 1), e.g. 0.5 */
+    uint32_t capacity;     /* training samples per pass; 0 = 2^18 */
+    uint32_t minibatch;    /* rows per Adam step; 0 = 16384 */
+    float learning_rate;   /* 0 = 1e-3 */
+    uint32_t seed;         /* weight initialisation */
+    float scene_lo[3];     /* scene box that normalises positions */
+    float scene_hi[3];
+} rgpu_guide_config;
+
+int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** out);
+int rgpu_guide_destroy(rgpu_guide* g);
+int rgpu_guide_param_count(void);
+/* Copy the fp32 weights out (get) and/or in (set; refreshes the bf16 copies). */
+int rgpu_guide_params(rgpu_guide* g, float* get, const float* set);
+/* Test hooks over host rows of 16 floats (position in [0,1]^3, normal, wo, wi, weight, pad):
+ * raw head outputs (64 per row), the gradient of the mean loss -weight*log q(wi), and training. */
+int rgpu_guide_forward(rgpu_guide* g, const float* samples, uint32_t rows, float* out);
+int rgpu_guide_gradients(rgpu_guide* g, const float* samples, uint32_t rows, float* grad);
+int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows);
+/* Attach (or detach with g = NULL) a guide: `sampling` enables guided sampling, `training`
+ * records vertices and collects training samples. */
+int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, int training);
+int rgpu_wavefront_guide_pending(rgpu_wavefront* w, uint32_t* pending);
+/* If at least min_samples are buffered: one pass of minibatch Adam steps, buffer emptied. */
+int rgpu_wavefront_guide_train(rgpu_wavefront* w, uint32_t min_samples, uint32_t* trained);
+
 #ifdef __cplusplus
 }
 #endif

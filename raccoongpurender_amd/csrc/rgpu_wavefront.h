@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "envmap_build.h"
+#include "guide_device.h"
 #include "rgpu_traverse.h"
 
 namespace rgpu {
@@ -127,8 +128,15 @@ hipError_t wf_initialize(const DevState& st, const WFParams& p, uint64_t seed_of
 // One bounce. persistent_blocks: grid of the queue-driven kernels; trace_block: workgroup
 // size of the extension ray cast (64, 128 or 256).
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
-                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, int trace_block,
-                   hipStream_t s);
+                   const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
+                   int trace_block, hipStream_t s);
+// Path-guiding network (guide.hip).
+hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, int blocks, hipStream_t s);
+hipError_t guide_train_batch(const DevGuide& g, uint32_t first, uint32_t rows, float lr, uint32_t step,
+                             hipStream_t s);
+hipError_t guide_refresh(const DevGuide& g, hipStream_t s);
+hipError_t guide_forward(const DevGuide& g, const GuideSample* in, uint32_t rows, float* out, hipStream_t s);
+hipError_t guide_gradients(const DevGuide& g, uint32_t rows, hipStream_t s);
 hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hipStream_t s);
 hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s);
 hipError_t wf_stat(const DevState& st, uint32_t npixels, bool accum64, unsigned long long* total, hipStream_t s);

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "guide_device.h"
 #include "rgpu_wavefront.h"
 #include "trace_device.h"
 
@@ -347,7 +348,7 @@ __global__ void __launch_bounds__(256) k_clear_accum(DevState s, uint32_t npixel
 // selection and envmap draws the reference makes for a new path whose extension result
 // still carries a volume id from its previous life (kernels/mixture_density.cl:26-55,
 // kernels/envmap_sampling.cl:219-248).
-__global__ void __launch_bounds__(256) k_new_path(DevState s, WFParams p, Camera cam, DevEnvmap env) {
+__global__ void __launch_bounds__(256) k_new_path(DevState s, WFParams p, Camera cam, DevEnvmap env, DevGuide g) {
     const uint32_t count = s.counts[0];
     const unsigned long long base = *s.next_pixel;
     const unsigned long long P = (unsigned long long)cam.width * cam.height;
@@ -376,6 +377,7 @@ __global__ void __launch_bounds__(256) k_new_path(DevState s, WFParams p, Camera
         pr.pad = 0;
         s.path[slot] = pr;
         s.vol[slot].count = 0;
+        if (g.train) g.rec_count[slot] = 0;
         store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, F3{1.0f, 1.0f, 1.0f});
         ExtRec e = s.ext[slot];
         e.hit_surface = -1;
@@ -401,9 +403,55 @@ __device__ __forceinline__ void spawn(DevState& s, uint32_t slot, F3 ro, F3 rd, 
     store_ray_rec(s.ray, slot, o, kFltMax, wi, pixel);
 }
 
+// Guided one-sample MIS: the guide is chosen with probability g.select_p, otherwise the
+// reference's BSDF / envmap choice is made with the same uniform rescaled.
+enum { kStratBxdf = 0, kStratEnv = 1, kStratGuide = 2 };
+__device__ __forceinline__ int select_strategy(uint4& st, float bxdf_p, bool guided, float guide_p) {
+    float u = xo_uniform(st);
+    if (guided) {
+        if (u < guide_p) return kStratGuide;
+        u = (u - guide_p) / (1.0f - guide_p);
+    }
+    return u <= bxdf_p ? kStratBxdf : kStratEnv;
+}
+
+__device__ __forceinline__ void load_lobes(const DevGuide& g, uint32_t slot, float* L) {
+    const float4* src = reinterpret_cast<const float4*>(g.lobes + (size_t)slot * kGSlotFloats);
+    for (int q = 0; q < kGSlotFloats / 4; ++q) {
+        float4 v = src[q];
+        L[4 * q] = v.x;
+        L[4 * q + 1] = v.y;
+        L[4 * q + 2] = v.z;
+        L[4 * q + 3] = v.w;
+    }
+}
+
+// Remember a guided vertex until its path ends (throughput before it, radiance after its emission).
+__device__ __forceinline__ void record_vertex(const DevGuide& g, uint32_t slot, F3 hit, F3 n, F3 wo, F3 wi,
+                                              const PathRec& pr, F3 Le) {
+    const uint32_t c = g.rec_count[slot];
+    if (c >= (uint32_t)kGRecords) return;
+    GuideRecord r;
+    r.p[0] = (hit.x - g.lo[0]) * g.inv_ext[0];
+    r.p[1] = (hit.y - g.lo[1]) * g.inv_ext[1];
+    r.p[2] = (hit.z - g.lo[2]) * g.inv_ext[2];
+    r.n[0] = n.x; r.n[1] = n.y; r.n[2] = n.z;
+    r.wo[0] = wo.x; r.wo[1] = wo.y; r.wo[2] = wo.z;
+    r.wi[0] = wi.x; r.wi[1] = wi.y; r.wi[2] = wi.z;
+    for (int k = 0; k < 3; ++k) {
+        r.T[k] = pr.T[k];
+        r.L[k] = pr.L[k] + pr.T[k] * (k == 0 ? Le.x : k == 1 ? Le.y : Le.z);
+    }
+    r.pad[0] = r.pad[1] = 0.0f;
+    g.records[(size_t)slot * kGRecords + c] = r;
+    g.rec_count[slot] = c + 1;
+}
+
 // Lambertian: strategy selection + envmap 6-axis sample / cosine sample + pdfs +
 // lambertian_stage (mixture_density.cl, envmap_sampling.cl:169-217,250-266, lambertian.cl:34-123).
-__global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+template <bool kGuided>
+__global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p, DevMaterials m, DevEnvmap env,
+                                                          DevGuide g) {
     const uint32_t count = s.counts[kMatLambertian];
     const uint32_t* items = s.items + (size_t)kMatLambertian * p.n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -417,11 +465,22 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         const Lambertian L = m.lambertian[m.mat[e.hit_surface].index];
         const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
         const F3 wo = neg3(rd);
-        const float sel_b = p.bxdf_p, sel_e = 1.0f - p.bxdf_p;
-        const bool bxdf = select_bxdf(st, sel_b);
+        const float gp = kGuided ? g.select_p : 0.0f;
+        const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
+        const int strat = kGuided ? select_strategy(st, p.bxdf_p, true, gp)
+                                  : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
+        const bool bxdf = strat == kStratBxdf;
+        float lobes[kGuided ? kGSlotFloats : 1];
+        if (kGuided) load_lobes(g, slot, lobes);
         F3 wi;
         float pdf_e = 0.0f, pdf_b;
-        if (!bxdf) wi = sample_env_6axis(env, flip_toward(Ng, wo), st, &pdf_e);
+        if (strat == kStratEnv) wi = sample_env_6axis(env, flip_toward(Ng, wo), st, &pdf_e);
+        if (kGuided && strat == kStratGuide) {
+            float u0 = xo_uniform(st), u1 = xo_uniform(st), u2 = xo_uniform(st);
+            float w[3];
+            guide_sample(lobes, u0, u1, u2, w);
+            wi = F3{w[0], w[1], w[2]};
+        }
         float cosTheta;
         if (bxdf) {
             float u0 = xo_uniform(st);
@@ -437,11 +496,12 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
             cosTheta = dot3(Ng, wi);
         }
         pdf_b = fmaxf(cosTheta, 0.0f) / kPiF;
-        if (bxdf) pdf_e = pdf_env_6axis(env, wi, Ng);  // unflipped (kernels/envmap_sampling.cl:263-265)
+        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, Ng);  // unflipped (kernels/envmap_sampling.cl:263-265)
         const F3 Ngf = flip_toward(Ng, wo);
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;
+        if (kGuided) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
         float c = dot3(wi, Ngf);
         F3 T = F3{0.0f, 0.0f, 0.0f};
         if (0.0 < (double)c) {
@@ -450,13 +510,15 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
             T.z = L.R[2] / kPiF * c / pdf;
         }
         store_shade(s.shade, slot, load3(L.Le), T);
+        if (kGuided && g.train) record_vertex(g, slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot], load3(L.Le));
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
     }
 }
 
 // Ward (mixture_density.cl, envmap_sampling.cl, ward.cl:76-186)
-__global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+template <bool kGuided>
+__global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevMaterials m, DevEnvmap env, DevGuide g) {
     const uint32_t count = s.counts[kMatWard];
     const uint32_t* items = s.items + (size_t)kMatWard * p.n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -471,21 +533,33 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevM
         const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
         const F3 wo = neg3(rd);
         const F3 Ngf = flip_toward(Ng, wo);
-        const float sel_b = p.bxdf_p, sel_e = 1.0f - p.bxdf_p;
-        const bool bxdf = select_bxdf(st, sel_b);
+        const float gp = kGuided ? g.select_p : 0.0f;
+        const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
+        const int strat = kGuided ? select_strategy(st, p.bxdf_p, true, gp)
+                                  : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
+        const bool bxdf = strat == kStratBxdf;
+        float lobes[kGuided ? kGSlotFloats : 1];
+        if (kGuided) load_lobes(g, slot, lobes);
         F3 wi;
         float pdf_e = 0.0f;
-        if (!bxdf) wi = sample_env_6axis(env, Ngf, st, &pdf_e);
+        if (strat == kStratEnv) wi = sample_env_6axis(env, Ngf, st, &pdf_e);
+        if (kGuided && strat == kStratGuide) {
+            float u0 = xo_uniform(st), u1 = xo_uniform(st), u2 = xo_uniform(st);
+            float d[3];
+            guide_sample(lobes, u0, u1, u2, d);
+            wi = F3{d[0], d[1], d[2]};
+        }
         if (bxdf) {
             float u0 = xo_uniform(st);
             float u1 = xo_uniform(st);
             wi = sample_ward(w, wo, Ngf, u0, u1);
         }
         float pdf_b = pdf_ward(w, wo, wi, Ngf);
-        if (bxdf) pdf_e = pdf_env_6axis(env, wi, Ng);
+        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, Ng);
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;
+        if (kGuided) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
         float c = dot3(wi, Ngf);
         F3 T = F3{0.0f, 0.0f, 0.0f};
         if (0.0f < c) {
@@ -506,6 +580,8 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevM
             T.z = k0.z * k1 * k2 * c / pdf;
         }
         store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, T);
+        if (kGuided && g.train)
+            record_vertex(g, slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot], F3{0.0f, 0.0f, 0.0f});
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
     }
@@ -724,7 +800,7 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, Trac
 
 // logic (kernels/logic.cl:117-280): path update, Russian roulette, envmap on a miss,
 // accumulation of finished paths, and the per-block queue counts of a stable compaction.
-__global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+__global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMaterials m, DevEnvmap env, DevGuide g) {
     __shared__ uint32_t wcount[kNumQueues][4];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -763,7 +839,34 @@ __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMateri
             F3 em = envmap_value(env, rd, logic_i == 0);
             L = add3(L, mul3v(T, em));
         }
-        if (newPath && finite3(L) && L.x < p.clamp && L.y < p.clamp && L.z < p.clamp) {
+        const bool accept = newPath && finite3(L) && L.x < p.clamp && L.y < p.clamp && L.z < p.clamp;
+        if (g.train && newPath) {
+            // a finished path turns its recorded guided vertices into training samples:
+            // weight = luminance((L_final - L_after) / T_before) = F / p of the sampled direction
+            const uint32_t nrec = accept ? g.rec_count[slot] : 0u;
+            for (uint32_t j = 0; j < nrec; ++j) {
+                const GuideRecord r = g.records[(size_t)slot * kGRecords + j];
+                float c[3];
+                const float Lf[3] = {L.x, L.y, L.z};
+                for (int k = 0; k < 3; ++k) c[k] = r.T[k] > 0.0f ? (Lf[k] - r.L[k]) / r.T[k] : 0.0f;
+                const float wgt = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
+                if (!(wgt > 0.0f) || !isfinite(wgt)) continue;
+                const uint32_t idx = atomicAdd(g.sample_count, 1u);
+                if (idx >= g.capacity) break;
+                GuideSample gs;
+                for (int k = 0; k < 3; ++k) {
+                    gs.p[k] = r.p[k];
+                    gs.n[k] = r.n[k];
+                    gs.wo[k] = r.wo[k];
+                    gs.wi[k] = r.wi[k];
+                }
+                gs.weight = wgt;
+                gs.pad[0] = gs.pad[1] = gs.pad[2] = 0.0f;
+                g.samples[idx] = gs;
+            }
+            g.rec_count[slot] = 0;
+        }
+        if (accept) {
             if (p.accum64) {
                 unsafeAtomicAdd(s.accum64 + 3 * (size_t)pixel + 0, (double)L.x);
                 unsafeAtomicAdd(s.accum64 + 3 * (size_t)pixel + 1, (double)L.y);
@@ -922,25 +1025,34 @@ hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hi
     return hipGetLastError();
 }
 
+hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, int blocks, hipStream_t s);
+
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
-                   const DevEnvmap& env, const Camera& cam, int variant, int persistent_blocks, int trace_block,
-                   hipStream_t s) {
+                   const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
+                   int trace_block, hipStream_t s) {
     const dim3 qgrid(persistent_blocks > 0 ? persistent_blocks : 1024), blk(256);
     const dim3 full((p.n + 255) / 256);
     const uint32_t tb = (trace_block == 64 || trace_block == 128) ? (uint32_t)trace_block : 256u;
     const dim3 tgrid((p.n + tb - 1) / tb), tblk(tb);
-    hipLaunchKernelGGL(k_new_path, qgrid, blk, 0, s, st, p, cam, env);
-    hipLaunchKernelGGL(k_shade_lambertian, qgrid, blk, 0, s, st, p, m, env);
+    hipLaunchKernelGGL(k_new_path, qgrid, blk, 0, s, st, p, cam, env, g);
+    if (g.enabled) {
+        hipError_t e = guide_infer(st, p, g, persistent_blocks > 0 ? 2 * persistent_blocks : 2048, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_shade_lambertian<true>, qgrid, blk, 0, s, st, p, m, env, g);
+        hipLaunchKernelGGL(k_shade_ward<true>, qgrid, blk, 0, s, st, p, m, env, g);
+    } else {
+        hipLaunchKernelGGL(k_shade_lambertian<false>, qgrid, blk, 0, s, st, p, m, env, g);
+        hipLaunchKernelGGL(k_shade_ward<false>, qgrid, blk, 0, s, st, p, m, env, g);
+    }
     hipLaunchKernelGGL(k_shade_specular, qgrid, blk, 0, s, st, p);
     hipLaunchKernelGGL(k_shade_dielectric, qgrid, blk, 0, s, st, p, m);
-    hipLaunchKernelGGL(k_shade_ward, qgrid, blk, 0, s, st, p, m, env);
     hipLaunchKernelGGL(k_shade_volume_surface, qgrid, blk, 0, s, st, p, m);
     hipLaunchKernelGGL(k_shade_volume_inside, qgrid, blk, 0, s, st, p, m, env);
     if (variant == kVariantThreaded)
         hipLaunchKernelGGL(k_trace_step<kVariantThreaded>, tgrid, tblk, 0, s, st, p, sc, m);
     else
         hipLaunchKernelGGL(k_trace_step<kVariantStackless>, tgrid, tblk, 0, s, st, p, sc, m);
-    hipLaunchKernelGGL(k_logic, full, blk, 0, s, st, p, m, env);
+    hipLaunchKernelGGL(k_logic, full, blk, 0, s, st, p, m, env, g);
     hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues), dim3(1024), 0, s, st);
     hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p);
     return hipGetLastError();

@@ -2,6 +2,7 @@
 // WavefrontLane-style lifecycle (setup / initialize / step / read back).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -50,6 +51,18 @@ struct rgpu_wavefront {
     int trace_block = 256;
     hipGraphExec_t graph = nullptr;  // captured bounce loop
     uint32_t graph_steps = 0;
+    DevGuide guide;                  // network + per-slot guiding buffers (guide.enabled == 0: plain)
+    struct rgpu_guide* guide_obj = nullptr;
+};
+
+struct rgpu_guide {
+    rgpu_ctx* ctx = nullptr;
+    DevGuide dev;                   // network, Adam state and training buffer (per-slot fields unused)
+    std::vector<void*> owned;
+    uint32_t step = 0;              // Adam steps taken
+    float lr = 1e-3f;
+    uint32_t minibatch = 16384;
+    float select_p = 0.5f;
 };
 
 namespace {
@@ -414,8 +427,8 @@ int rgpu_wavefront_step(rgpu_wavefront* w, rgpu_stream stream, uint32_t steps) {
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = stream_of(ctx, stream);
     for (uint32_t i = 0; i < steps; ++i)
-        RGPU_HIP(ctx, wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->variant,
-                              w->queue_blocks, w->trace_block, s));
+        RGPU_HIP(ctx, wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->guide,
+                              w->variant, w->queue_blocks, w->trace_block, s));
     return RGPU_OK;
 }
 
@@ -430,8 +443,8 @@ int rgpu_wavefront_graph_capture(rgpu_wavefront* w, uint32_t steps) {
     RGPU_HIP(ctx, hipStreamCreate(&cs));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     for (uint32_t i = 0; i < steps && e == hipSuccess; ++i)
-        e = wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->variant, w->queue_blocks,
-                    w->trace_block, cs);
+        e = wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->guide, w->variant,
+                    w->queue_blocks, w->trace_block, cs);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e == hipSuccess) e = e2;
@@ -606,6 +619,223 @@ int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, flo
     if (shade) RGPU_HIP(ctx, hipMemcpy(shade, w->st.shade, n * 32, hipMemcpyDeviceToHost));
     if (vol) RGPU_HIP(ctx, hipMemcpy(vol, w->st.vol, n * 32, hipMemcpyDeviceToHost));
     if (items) RGPU_HIP(ctx, hipMemcpy(items, w->st.items, n * kNumQueues * 4, hipMemcpyDeviceToHost));
+    return RGPU_OK;
+}
+
+// ------------------------------------------------------------------ path guiding
+static int guide_alloc(rgpu_guide* g, size_t bytes, void** out) {
+    RGPU_HIP(g->ctx, hipMalloc(out, bytes < 16 ? 16 : bytes));
+    g->owned.push_back(*out);
+    return RGPU_OK;
+}
+
+int rgpu_guide_destroy(rgpu_guide* g) {
+    if (!g) return RGPU_ERR_INVALID_ARG;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    for (void* p : g->owned) (void)hipFree(p);
+    delete g;
+    return RGPU_OK;
+}
+
+int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** out) {
+    if (!ctx || !cfg || !out) return RGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (cfg->abi_version != RGPU_ABI_VERSION) return fail(ctx, RGPU_ERR_INVALID_ARG, "abi_version mismatch");
+    if (!(cfg->select_p > 0.0f && cfg->select_p < 1.0f))
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "guide selection probability must be in (0, 1)");
+    rgpu_guide* g = new (std::nothrow) rgpu_guide();
+    if (!g) return RGPU_ERR_OUT_OF_MEMORY;
+    g->ctx = ctx;
+    g->lr = cfg->learning_rate > 0.0f ? cfg->learning_rate : 1e-3f;
+    g->minibatch = cfg->minibatch ? cfg->minibatch : 16384u;
+    g->select_p = cfg->select_p;
+    DevGuide& d = g->dev;
+    d.capacity = cfg->capacity ? cfg->capacity : (1u << 18);
+    d.select_p = cfg->select_p;
+    for (int k = 0; k < 3; ++k) {
+        float ext = cfg->scene_hi[k] - cfg->scene_lo[k];
+        d.lo[k] = cfg->scene_lo[k];
+        d.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 1.0f;
+    }
+    int rc = hipSetDevice(ctx->device) == hipSuccess ? RGPU_OK : RGPU_ERR_HIP;
+    void* p = nullptr;
+    auto A = [&](size_t bytes) -> void* {
+        if (rc != RGPU_OK) return nullptr;
+        rc = guide_alloc(g, bytes, &p);
+        return rc == RGPU_OK ? p : nullptr;
+    };
+    d.params = (float*)A(kGParams * 4);
+    d.w_bf = (uint16_t*)A(kGParams * 2);
+    d.wt_bf = (uint16_t*)A(kGParams * 2);
+    d.grad = (float*)A(kGParams * 4);
+    d.adam_m = (float*)A(kGParams * 4);
+    d.adam_v = (float*)A(kGParams * 4);
+    d.samples = (GuideSample*)A((size_t)d.capacity * sizeof(GuideSample));
+    d.sample_count = (uint32_t*)A(4);
+    if (rc != RGPU_OK) {
+        rgpu_guide_destroy(g);
+        return rc;
+    }
+    // He-uniform hidden layers; a small output layer so the initial mixture is broad and even.
+    std::vector<float> w(kGParams, 0.0f);
+    uint64_t x = cfg->seed ^ 0x5DEECE66Dull;
+    auto uni = [&]() {
+        uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        return (double)(z >> 11) / 9007199254740992.0 * 2.0 - 1.0;
+    };
+    const int offs[4] = {kGoW0, kGoW1, kGoW2, kGoW3}, nin[4] = {kGIn, kGHid, kGHid, kGHid},
+              nout[4] = {kGHid, kGHid, kGHid, kGOut};
+    for (int l = 0; l < 4; ++l) {
+        double a = std::sqrt(6.0 / (l == 0 ? 78 : nin[l])) * (l == 3 ? 0.1 : 1.0);
+        for (int o = 0; o < nout[l]; ++o)
+            for (int i = 0; i < nin[l]; ++i) w[offs[l] + o * nin[l] + i] = (l == 0 && i >= 78) ? 0.0f : (float)(a * uni());
+    }
+    std::vector<float> zeros(kGParams, 0.0f);
+    rc = RGPU_OK;
+    if (hipMemcpy(d.params, w.data(), kGParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.adam_m, zeros.data(), kGParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.adam_v, zeros.data(), kGParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(d.sample_count, 0, 4) != hipSuccess || guide_refresh(d, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = RGPU_ERR_HIP;
+    if (rc != RGPU_OK) {
+        rgpu_guide_destroy(g);
+        return fail(ctx, rc, "guide initialisation failed");
+    }
+    *out = g;
+    return RGPU_OK;
+}
+
+int rgpu_guide_params(rgpu_guide* g, float* get, const float* set) {
+    if (!g) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = g->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (set) {
+        RGPU_HIP(ctx, hipMemcpy(g->dev.params, set, kGParams * 4, hipMemcpyHostToDevice));
+        RGPU_HIP(ctx, guide_refresh(g->dev, ctx->stream));
+        RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (get) RGPU_HIP(ctx, hipMemcpy(get, g->dev.params, kGParams * 4, hipMemcpyDeviceToHost));
+    return RGPU_OK;
+}
+
+int rgpu_guide_param_count(void) { return kGParams; }
+
+// Test hooks: `samples` are rows of 16 floats (p01, n, wo, wi, weight, pad).
+int rgpu_guide_forward(rgpu_guide* g, const float* samples, uint32_t rows, float* out) {
+    if (!g || (rows && (!samples || !out))) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = g->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    void *din = nullptr, *dout = nullptr;
+    RGPU_HIP(ctx, hipMalloc(&din, (size_t)rows * 64 + 16));
+    hipError_t e = hipMalloc(&dout, (size_t)rows * kGOut * 4 + 16);
+    if (e == hipSuccess) e = hipMemcpy(din, samples, (size_t)rows * 64, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = guide_forward(g->dev, (const GuideSample*)din, rows, (float*)dout, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)rows * kGOut * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rgpu_guide_forward");
+    return RGPU_OK;
+}
+
+// Gradient of the mean guiding loss over `rows` samples (they replace the training buffer).
+int rgpu_guide_gradients(rgpu_guide* g, const float* samples, uint32_t rows, float* grad) {
+    if (!g || !samples || !grad || rows == 0 || rows > g->dev.capacity) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = g->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipMemcpy(g->dev.samples, samples, (size_t)rows * 64, hipMemcpyHostToDevice));
+    RGPU_HIP(ctx, hipMemset(g->dev.grad, 0, kGParams * 4));
+    RGPU_HIP(ctx, guide_gradients(g->dev, rows, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RGPU_HIP(ctx, hipMemcpy(grad, g->dev.grad, kGParams * 4, hipMemcpyDeviceToHost));
+    RGPU_HIP(ctx, hipMemset(g->dev.grad, 0, kGParams * 4));
+    return RGPU_OK;
+}
+
+// Train on host-provided samples: minibatches of the configured size, one Adam step each.
+int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows) {
+    if (!g || !samples || rows == 0 || rows > g->dev.capacity) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = g->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipMemcpy(g->dev.samples, samples, (size_t)rows * 64, hipMemcpyHostToDevice));
+    for (uint32_t first = 0; first < rows; first += g->minibatch) {
+        uint32_t n = std::min(g->minibatch, rows - first);
+        RGPU_HIP(ctx, guide_train_batch(g->dev, first, n, g->lr, ++g->step, ctx->stream));
+    }
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, int training) {
+    if (!w) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(w);
+    if (!g) {
+        w->guide = DevGuide();
+        w->guide_obj = nullptr;
+        return RGPU_OK;
+    }
+    DevGuide d = g->dev;
+    const size_t n = w->params.n;
+    if (!w->guide.lobes) {
+        void* p;
+        int rc;
+        if ((rc = wf_alloc(w, n * kGSlotFloats * 4, &p)) != RGPU_OK) return rc;
+        d.lobes = (float*)p;
+        if ((rc = wf_alloc(w, n * sizeof(GuideRecord) * kGRecords, &p)) != RGPU_OK) return rc;
+        d.records = (GuideRecord*)p;
+        if ((rc = wf_alloc(w, n * 4, &p)) != RGPU_OK) return rc;
+        d.rec_count = (uint32_t*)p;
+        RGPU_HIP(ctx, hipMemset(d.rec_count, 0, n * 4));
+    } else {
+        d.lobes = w->guide.lobes;
+        d.records = w->guide.records;
+        d.rec_count = w->guide.rec_count;
+    }
+    d.enabled = sampling ? 1u : 0u;
+    d.train = training ? 1u : 0u;
+    w->guide = d;
+    w->guide_obj = g;
+    return RGPU_OK;
+}
+
+// Number of training samples the attached guide has collected (may exceed its capacity;
+// samples beyond it were dropped).
+int rgpu_wavefront_guide_pending(rgpu_wavefront* w, uint32_t* pending) {
+    if (!w || !pending || !w->guide_obj) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipMemcpyAsync(pending, w->guide.sample_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RGPU_OK;
+}
+
+// One training pass if at least `min_samples` samples are buffered: minibatch Adam steps over
+// the buffer, then the buffer is emptied. *trained = samples used (0 if none).
+int rgpu_wavefront_guide_train(rgpu_wavefront* w, uint32_t min_samples, uint32_t* trained) {
+    if (!w || !w->guide_obj) return RGPU_ERR_INVALID_ARG;
+    rgpu_guide* g = w->guide_obj;
+    rgpu_ctx* ctx = w->ctx;
+    uint32_t pending = 0;
+    int rc = rgpu_wavefront_guide_pending(w, &pending);
+    if (rc != RGPU_OK) return rc;
+    uint32_t rows = std::min(pending, g->dev.capacity);
+    if (trained) *trained = 0;
+    if (rows == 0 || rows < min_samples) return RGPU_OK;
+    for (uint32_t first = 0; first < rows; first += g->minibatch) {
+        uint32_t n = std::min(g->minibatch, rows - first);
+        RGPU_HIP(ctx, guide_train_batch(g->dev, first, n, g->lr, ++g->step, ctx->stream));
+    }
+    RGPU_HIP(ctx, hipMemsetAsync(g->dev.sample_count, 0, 4, ctx->stream));
+    if (trained) *trained = rows;
     return RGPU_OK;
 }
 

@@ -1,0 +1,70 @@
+"""Equal-time comparison of guided and plain sampling against a high-sample reference.
+
+relMSE = mean over pixels of (Y - Y_ref)^2 / (Y_ref^2 + 0.01), Y = luminance of the mean
+radiance. All three renders run without the radiance clamp so none of them is biased.
+"""
+import time
+
+import numpy as np
+
+from . import _lib
+from .guide import Guide, attach, train_pass
+from .render import build_lane
+
+NO_CLAMP = 1.0e30
+
+
+def luminance_image(wf):
+    rgb, cnt = wf.accumulation()
+    m = rgb / np.maximum(cnt, 1)[:, None]
+    return 0.2126 * m[:, 0] + 0.7152 * m[:, 1] + 0.0722 * m[:, 2], cnt
+
+
+def rel_mse(y, ref):
+    return float(np.mean((y - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def render_reference(ctx, sd, spp, paths, seed=777):
+    wf = build_lane(ctx, sd, paths, clamp=NO_CLAMP, flags=_lib.WF_ACCUM64, seed=seed)
+    P = sd["camera"]["width"] * sd["camera"]["height"]
+    t0 = time.perf_counter()
+    while True:
+        wf.step(16)
+        if wf.stat() >= spp * P:
+            break
+    y, cnt = luminance_image(wf)
+    return y, time.perf_counter() - t0, float(cnt.mean())
+
+
+def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=1 << 18, select_p=0.5, lr=1e-3):
+    wf = build_lane(ctx, sd, paths, clamp=NO_CLAMP, seed=seed)
+    g = None
+    if guided:
+        lo = sd["points"].min(0) - 1e-3
+        hi = sd["points"].max(0) + 1e-3
+        g = Guide(ctx, lo, hi, select_p=select_p, capacity=capacity, lr=lr, seed=seed)
+        attach(wf, g, sampling=True, training=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    passes = 0
+    while time.perf_counter() - t0 < seconds:
+        wf.step(train_every)
+        if guided:
+            passes += train_pass(wf, capacity) > 0
+        else:
+            ctx.synchronize()
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    y, cnt = luminance_image(wf)
+    return y, el, float(cnt.mean()), passes
+
+
+def equal_time_relmse(ctx, sd, seconds=2.0, ref_spp=16384, paths=1 << 20, seed=5):
+    ref, t_ref, spp_ref = render_reference(ctx, sd, ref_spp, paths)
+    plain, t_p, spp_p, _ = render_timed(ctx, sd, seconds, paths, False, seed)
+    guided, t_g, spp_g, passes = render_timed(ctx, sd, seconds, paths, True, seed)
+    out = {"relmse_plain": rel_mse(plain, ref), "relmse_guided": rel_mse(guided, ref),
+           "seconds": seconds, "plain_spp": round(spp_p, 1), "guided_spp": round(spp_g, 1),
+           "guided_training_passes": passes, "reference_spp": round(spp_ref, 1), "reference_seconds": round(t_ref, 1)}
+    out["ratio_plain_over_guided"] = out["relmse_plain"] / max(out["relmse_guided"], 1e-30)
+    return out

@@ -1,0 +1,119 @@
+"""The FP64 model of the path-guiding network (oracle/oracle_mlp.cpp), pinned by finite
+differences, the vMF-mixture normalisation integral, sampling histograms and Adam's closed
+form."""
+import math
+
+import numpy as np
+
+import oracle_mlp as M
+
+
+def _rand_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    return rng.normal(scale=scale, size=M.PARAM_COUNT) / np.sqrt(8.0)
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def test_encoding_values():
+    x = M.encode(np.array([[0.0, 0.5, 0.25]]), np.array([[0, 0, 1.0]]), np.array([[1.0, 0, 0]]))[0]
+    for k in range(12):
+        assert abs(x[6 * k + 0]) < 1e-12 and abs(x[6 * k + 1] - 1.0) < 1e-12  # p = 0
+        assert abs(x[6 * k + 2] - math.sin(2 ** k * math.pi * 0.5)) < 1e-12
+        assert abs(x[6 * k + 5] - math.cos(2 ** k * math.pi * 0.25)) < 1e-12
+    assert list(x[72:78]) == [0, 0, 1, 1, 0, 0] and not x[78:].any()
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    params = _rand_params(1)
+    n = 4
+    x = M.encode(rng.random((n, 3)), _unit(rng, n), _unit(rng, n))
+    wi = _unit(rng, n)
+    wt = rng.uniform(0.5, 2.0, n)
+    loss, g = M.loss_grad(params, x, wi, wt)
+    idx = rng.choice(M.PARAM_COUNT, 300, replace=False)
+    h = 1e-6
+    ok = 0
+    for i in idx:
+        p1, p2 = params.copy(), params.copy()
+        p1[i] += h
+        p2[i] -= h
+        fd = (M.loss_grad(p1, x, wi, wt)[0] - M.loss_grad(p2, x, wi, wt)[0]) / (2 * h)
+        ok += abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g[i]))
+    assert ok >= 297  # a parameter on a ReLU kink may disagree
+    assert np.abs(g).max() > 1e-4
+
+
+def _fib_sphere(n):
+    i = np.arange(n) + 0.5
+    z = 1 - 2 * i / n
+    r = np.sqrt(1 - z * z)
+    ph = math.pi * (1 + 5 ** 0.5) * i
+    return np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+
+
+def _head(rng, max_logk=3.0):
+    out = rng.normal(size=(1, M.N_OUT))
+    out[0, 3:40:5] = rng.uniform(-3, max_logk, 8)
+    return out
+
+
+def test_vmf_mixture_integrates_to_one():
+    rng = np.random.default_rng(2)
+    d = _fib_sphere(400000)
+    for _ in range(4):
+        out = _head(rng)
+        q = M.pdf(np.repeat(out, len(d), 0), d)
+        assert abs(q.mean() * 4 * math.pi - 1.0) < 2e-3
+
+
+def test_vmf_mixture_sampling_follows_pdf():
+    rng = np.random.default_rng(3)
+    out = _head(rng, max_logk=2.5)
+    n = 200000
+    w = M.sample(out[0], rng.random((n, 3)))
+    assert np.allclose(np.linalg.norm(w, axis=1), 1.0, atol=1e-9)
+    # equal-area bins in (z, phi)
+    nz, nph = 16, 32
+    bz = np.clip(((w[:, 2] + 1) / 2 * nz).astype(int), 0, nz - 1)
+    bp = ((np.arctan2(w[:, 1], w[:, 0]) + math.pi) / (2 * math.pi) * nph).astype(int) % nph
+    counts = np.bincount(bz * nph + bp, minlength=nz * nph)
+    d = _fib_sphere(2000000)
+    q = M.pdf(np.repeat(out, len(d), 0), d)
+    dz = np.clip(((d[:, 2] + 1) / 2 * nz).astype(int), 0, nz - 1)
+    dp = ((np.arctan2(d[:, 1], d[:, 0]) + math.pi) / (2 * math.pi) * nph).astype(int) % nph
+    mass = np.bincount(dz * nph + dp, weights=q, minlength=nz * nph) * 4 * math.pi / len(d)
+    expect = mass * n
+    m = expect > 10
+    chi2 = ((counts[m] - expect[m]) ** 2 / expect[m]).sum()
+    dof = m.sum() - 1
+    assert chi2 < dof + 6 * math.sqrt(2 * dof), (chi2, dof)
+
+
+def test_adam_first_step_closed_form():
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=50)
+    g = rng.normal(size=50)
+    w0 = w.copy()
+    m, v = np.zeros(50), np.zeros(50)
+    M.adam(w, m, v, g, 1e-3, 1)
+    # bias-corrected first step: m_hat = g, v_hat = g^2
+    assert np.allclose(w, w0 - 1e-3 * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-12)
+    M.adam(w, m, v, g, 1e-3, 2)
+    mh = (0.9 * 0.1 * g + 0.1 * g) / (1 - 0.9 ** 2)
+    vh = (0.999 * 0.001 * g * g + 0.001 * g * g) / (1 - 0.999 ** 2)
+    assert np.allclose(w, w0 - 1e-3 * g / (np.abs(g) + 1e-8) - 1e-3 * mh / (np.sqrt(vh) + 1e-8), atol=1e-12)
+
+
+def test_emulated_rounding_is_close_to_exact():
+    rng = np.random.default_rng(5)
+    params = _rand_params(6)
+    x = M.encode(rng.random((16, 3)), _unit(rng, 16), _unit(rng, 16))
+    a = M.forward(params, x)
+    b = M.forward(params, x, emulate=True)
+    assert np.abs(a - b).max() < 0.05 * (1 + np.abs(a).max())
+    assert M.bf16(1.0 + 2 ** -9) == 1.0 and M.bf16(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7  # ties to even
