@@ -200,6 +200,13 @@ typedef struct rgpu_camera {
 
 #define RGPU_WF_ACCUM64 1u              /* accumulate pixel sums in double, 32-bit counts */
 #define RGPU_WF_NO_REFERENCE_QUIRKS 2u  /* skip the stale-volume strategy draws of new paths */
+/* Sample the lambertian cosine lobe and evaluate envmap pdfs about the normal flipped toward
+ * the outgoing direction. The reference uses the unflipped normal there
+ * (kernels/lambertian.cl:59-64, kernels/envmap_sampling.cl:263-265) while it shades and samples
+ * the envmap with the flipped one: on a back face its cosine-lobe samples point into the
+ * surface and return nothing, and a Ward vertex prices envmap directions on the wrong
+ * hemisphere. */
+#define RGPU_WF_CONSISTENT_NORMALS 4u
 
 typedef struct rgpu_wavefront_config {
     uint32_t abi_version;   /* RGPU_ABI_VERSION */

@@ -212,7 +212,7 @@ class orc_wf_desc(C.Structure):
         ("lambertians", vp), ("nl", u32), ("wards", vp), ("nw", u32), ("iors", vp), ("nd", u32),
         ("volumes", vp), ("nv", u32),
         ("env_rgb", vp), ("env_channels", u32), ("env_w", u32), ("env_h", u32),
-        ("camera", vp), ("width", u32), ("height", u32), ("lens_radius", C.c_float),
+        ("camera", vp), ("width", u32), ("height", u32), ("lens_radius", C.c_float), ("consistent", u32),
     ]
 
 
@@ -284,7 +284,7 @@ class OracleWavefront:
 
     def __init__(self, points, indices, tree, materials, env_rgb, camera, path_count, spheres=None, seed=0,
                  rr_start=0, max_depth=0, clamp=0.0, bxdf_p=0.0, quirks=True, variant="stackless",
-                 nthreads=NTHREADS):
+                 nthreads=NTHREADS, consistent=False):
         self.points, self.indices, self.ntris, self.spheres = _geom(points, indices, spheres)
         self.keep = []
         d = orc_wf_desc()
@@ -310,6 +310,7 @@ class OracleWavefront:
         self.keep += [env, cam]
         d.env_rgb, d.env_channels, d.env_w, d.env_h = p(env), env.shape[2], env.shape[1], env.shape[0]
         d.camera, d.width, d.height, d.lens_radius = p(cam), camera.width, camera.height, camera.lens_radius
+        d.consistent = 1 if consistent else 0
         self.desc = d
         self.n = path_count
         self.npixels = camera.width * camera.height

@@ -434,6 +434,7 @@ struct Cfg {
     uint32_t n, rr_start, max_depth, quirks, variant, nthreads;
     float clamp, bxdf_p;
     uint64_t seed;
+    uint32_t consistent = 0;  // flip Ng toward wo for the cosine lobe and the 6-axis envmap pdf too
 };
 
 struct Lane {
@@ -583,9 +584,10 @@ void step(Lane& L) {
         L.isample[index].pdfs[kEnv] = pdf;
         L.rng[index] = st;
     }
-    // sample_or_eval_lambertian_stage (lambertian.cl:34-71)
+    // sample_or_eval_lambertian_stage (lambertian.cl:34-71): Ng is used unflipped there
     for (uint32_t item : L.q[kLambertian]) {
         f3 Ng = L.ext[item].Ng;
+        if (L.cfg.consistent && dot(Ng, -L.path[item].rd) < 0.0f) Ng = -Ng;
         float cosTheta;
         if (L.isample[item].strategy == kBxdf) {
             Rng st = L.rng[item];
@@ -638,8 +640,11 @@ void step(Lane& L) {
         L.isample[item].pdfs[kBxdf] = (float)(1.0 / (kPiD * 4.0));
     }
     // evaluate_envmap_6axis_pdf_stage / evaluate_envmap_pdf_stage (envmap_sampling.cl:250-281)
-    for (uint32_t index : L.sq[3])
-        L.isample[index].pdfs[kEnv] = envmap_pdf_sixAxis(L.env, L.isample[index].wi, L.ext[index].Ng);
+    for (uint32_t index : L.sq[3]) {
+        f3 n = L.ext[index].Ng;  // unflipped in envmap_sampling.cl:263-265
+        if (L.cfg.consistent && dot(n, -L.path[index].rd) < 0.0f) n = -n;
+        L.isample[index].pdfs[kEnv] = envmap_pdf_sixAxis(L.env, L.isample[index].wi, n);
+    }
     for (uint32_t index : L.sq[1]) L.isample[index].pdfs[kEnv] = L.env.pdf[0][envmap_index(L.env, L.isample[index].wi)];
     // lambertian_stage (lambertian.cl:73-123)
     for (uint32_t item : L.q[kLambertian]) {
@@ -905,6 +910,7 @@ struct orc_wf_desc {
     const float* camera;  // 21 floats: eye, forward, up, right, o, r, b
     uint32_t width, height;
     float lens_radius;
+    uint32_t consistent;
 };
 
 void* orc_wf_create(const orc_wf_desc* d) {
@@ -912,6 +918,7 @@ void* orc_wf_create(const orc_wf_desc* d) {
     L->cfg = Cfg{d->n, d->rr_start ? d->rr_start : 25u, d->max_depth, d->quirks, d->variant,
                  d->nthreads ? d->nthreads : 1u, d->clamp > 0 ? d->clamp : 1000.0f, d->bxdf_p > 0 ? d->bxdf_p : 0.5f,
                  d->seed};
+    L->cfg.consistent = d->consistent;
     L->points = d->points;
     L->stride = d->stride;
     L->indices = d->indices;

@@ -5,6 +5,7 @@ there is no fallback path: a missing library raises at import of this module.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -126,14 +127,28 @@ class Context:
         if rc != RGPU_OK:
             raise RgpuError(rc, "rgpu_create")
         self.h = h
+        self._children = []  # weak references, creation order
 
     def check(self, rc, what):
         if rc != RGPU_OK:
             detail = lib.rgpu_last_error_string(self.h)
             raise RgpuError(rc, what, detail.decode() if detail else "")
 
+    def adopt(self, obj):
+        """Track an object whose C handle refers to this context: close() releases it first,
+        because every rgpu_*_destroy of a device object reads its context."""
+        if len(self._children) >= 64 and len(self._children) % 64 == 0:
+            self._children = [r for r in self._children if r() is not None]
+        self._children.append(weakref.ref(obj))
+        return obj
+
     def close(self):
         if self.h:
+            for r in reversed(self._children):
+                o = r()
+                if o is not None:
+                    o.close()
+            self._children = []
             lib.rgpu_destroy(self.h)
             self.h = None
 
@@ -166,6 +181,7 @@ class DeviceBuffer:
         h = vp()
         ctx.check(lib.rgpu_alloc(ctx.h, self.nbytes, C.byref(h)), "rgpu_alloc")
         self.ptr = h
+        ctx.adopt(self)
 
     def upload(self, arr):
         arr = np.ascontiguousarray(arr)
@@ -183,14 +199,15 @@ class DeviceBuffer:
         self.ctx.check(lib.rgpu_fill_u32(self.ctx.h, None, self.ptr, value, count), "rgpu_fill_u32")
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             lib.rgpu_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
-            if self.ctx.h:
-                self.free()
+            self.free()
         except Exception:
             pass
 
@@ -249,6 +266,7 @@ class Scene:
         ctx.check(lib.rgpu_scene_create(ctx.h, bvh.h, ptr(p), p.shape[1], p.shape[0], ptr(bvh.indices), bvh.ntris,
                                         ptr(bvh.spheres), bvh.nspheres, C.byref(h)), "rgpu_scene_create")
         self.h = h
+        ctx.adopt(self)
 
     def bytes(self):
         out = (C.c_uint64 * 4)()
@@ -275,9 +293,9 @@ class Scene:
         return ms.value
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib.rgpu_scene_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -309,6 +327,7 @@ def trace(ctx, scene, rays, variant="stackless", block=0):
 MAT_LAMBERTIAN, MAT_SPECULAR, MAT_DIELECTRIC, MAT_WARD, MAT_VOLUME = 1, 2, 3, 4, 5
 WF_ACCUM64 = 1
 WF_NO_REFERENCE_QUIRKS = 2
+WF_CONSISTENT_NORMALS = 4
 QUEUE_NAMES = ("new_path", "lambertian", "specular", "dielectric", "ward", "volume_surface", "volume_inside")
 
 
@@ -443,12 +462,16 @@ class Materials:
                                             len(ms.lambertians), W, len(ms.wards), D, len(ms.dielectrics), V,
                                             len(ms.volumes), C.byref(h)), "rgpu_materials_create")
         self.h = h
+        ctx.adopt(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib.rgpu_materials_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:
-            if self.h:
-                lib.rgpu_materials_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 
@@ -467,6 +490,8 @@ class Envmap:
         if rc != RGPU_OK:
             raise RgpuError(rc, "rgpu_envmap_create")
         self.h = h
+        if ctx is not None:
+            ctx.adopt(self)
 
     def export(self, table):
         P = self.width * self.height
@@ -478,11 +503,14 @@ class Envmap:
         lib.rgpu_envmap_export(self.h, table, ptr(pdf), ptr(hts), ptr(al), ptr(fr), C.byref(deg))
         return dict(pdf=pdf, height=hts, alias=al, fragments=fr, degenerate=bool(deg.value))
 
+    def close(self):
+        if self.h and (self.ctx is None or self.ctx.h):
+            lib.rgpu_envmap_destroy(self.h)
+        self.h = None
+
     def __del__(self):
         try:
-            if self.h:
-                lib.rgpu_envmap_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 
@@ -502,6 +530,7 @@ class Wavefront:
                   "rgpu_wavefront_create")
         self.h = h
         self.npixels = 0
+        ctx.adopt(self)
 
     def initialize(self, camera, lane_index=0, num_lanes=1):
         self.ctx.check(lib.rgpu_wavefront_initialize(self.h, C.byref(camera), lane_index, num_lanes),
@@ -575,9 +604,9 @@ class Wavefront:
         return ms.value
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib.rgpu_wavefront_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

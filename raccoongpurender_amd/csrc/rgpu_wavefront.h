@@ -87,6 +87,7 @@ struct WFParams {
     uint32_t quirks;          // reproduce the reference's stale-volume strategy draws
     float clamp;              // per-sample radiance clamp (1000)
     float bxdf_p;             // selection probability of BSDF sampling (0.5)
+    uint32_t consistent;      // flip the normal toward wo in BSDF sampling and envmap pdfs too
 };
 
 struct DevMaterials {

@@ -481,6 +481,10 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
             guide_sample(lobes, u0, u1, u2, w);
             wi = F3{w[0], w[1], w[2]};
         }
+        // The reference samples and evaluates the cosine lobe and the envmap pdf with the
+        // unflipped normal (kernels/lambertian.cl:59-64, envmap_sampling.cl:263-265); the
+        // consistent-normals option flips it toward wo like the shading does.
+        const F3 Ns = p.consistent ? flip_toward(Ng, wo) : Ng;
         float cosTheta;
         if (bxdf) {
             float u0 = xo_uniform(st);
@@ -489,14 +493,14 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
             float th = two_pi_times(u1);
             F3 l = F3{r * fcos(th), r * fsin(th), sqrtf(fmaxf(1.0f - u0, 0.0f))};
             F3 x, y;
-            onb(Ng, &x, &y);  // unflipped geometric normal (kernels/lambertian.cl:59-64)
-            wi = local_to_world(x, y, Ng, l);
+            onb(Ns, &x, &y);
+            wi = local_to_world(x, y, Ns, l);
             cosTheta = l.z;
         } else {
-            cosTheta = dot3(Ng, wi);
+            cosTheta = dot3(Ns, wi);
         }
         pdf_b = fmaxf(cosTheta, 0.0f) / kPiF;
-        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, Ng);  // unflipped (kernels/envmap_sampling.cl:263-265)
+        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, Ns);
         const F3 Ngf = flip_toward(Ng, wo);
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
@@ -555,7 +559,7 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevM
             wi = sample_ward(w, wo, Ngf, u0, u1);
         }
         float pdf_b = pdf_ward(w, wo, wi, Ngf);
-        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, Ng);
+        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, p.consistent ? Ngf : Ng);
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;

@@ -324,6 +324,7 @@ int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_mat
     p.max_depth = cfg->max_depth;
     p.accum64 = (cfg->flags & RGPU_WF_ACCUM64) ? 1u : 0u;
     p.quirks = (cfg->flags & RGPU_WF_NO_REFERENCE_QUIRKS) ? 0u : 1u;
+    p.consistent = (cfg->flags & RGPU_WF_CONSISTENT_NORMALS) ? 1u : 0u;
     p.clamp = cfg->radiance_clamp > 0.0f ? cfg->radiance_clamp : 1000.0f;
     p.bxdf_p = cfg->bxdf_select_p > 0.0f ? cfg->bxdf_select_p : 0.5f;
     w->seed = cfg->seed;

@@ -46,6 +46,7 @@ class Guide:
         h = vp()
         ctx.check(_lib.lib.rgpu_guide_create(ctx.h, C.byref(cfg), C.byref(h)), "rgpu_guide_create")
         self.h = h
+        ctx.adopt(self)
 
     def params(self):
         out = np.zeros(PARAM_COUNT, np.float32)
@@ -76,9 +77,9 @@ class Guide:
                        "rgpu_guide_train_samples")
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             _lib.lib.rgpu_guide_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -174,15 +174,24 @@ def material_test_scene(width=64, height=48, env_size=(64, 32), sun_width_deg=6.
     return dict(points=v, indices=f, materials=ms, env=env, camera=cam)
 
 
-def furnace_scene(kind, width=16, height=16, value=0.5):
-    """A single convex object under a constant environment (closed-form furnace tests)."""
+def furnace_scene(kind, width=16, height=16, value=0.5, fill=False, flip=False):
+    """A single convex object under a constant environment (closed-form furnace tests).
+
+    fill=True narrows the camera so that every primary ray hits the object: every path then
+    has the same number of bounces, and a lambertian sphere of reflectance R reads R * value
+    in every pixel. flip=True reverses the sphere's winding so that the camera sees back
+    faces only (geometric normals point inward)."""
     mb = MeshBuilder()
     iv, iff = scenes.icosphere(3)
-    iff = iff[:, [0, 2, 1]]
-    lam, diel, vols = [], [], []
+    if not flip:
+        iff = iff[:, [0, 2, 1]]
+    lam, wards, diel, vols = [], [], [], []
     if kind == "lambertian":
         lam = [((0, 0, 0), (0.5, 0.5, 0.5))]
         mb.add(iv, iff, MAT_LAMBERTIAN, 0)
+    elif kind == "ward":
+        wards = [(0.12, 0.3, (0.7, 0.6, 0.5), (1.0, 1.0, 1.0), 0.5)]
+        mb.add(iv, iff, MAT_WARD, 0)
     elif kind == "mirror":
         mb.add(iv, iff, MAT_SPECULAR, 0)
     elif kind == "dielectric":
@@ -195,7 +204,8 @@ def furnace_scene(kind, width=16, height=16, value=0.5):
     else:
         raise ValueError(kind)
     v, f, types, idx = mb.build()
-    ms = MaterialSet(types, idx, lambertians=lam, dielectrics=diel, volumes=vols)
-    cam = dict(eye=(0.0, 0.0, 4.5), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=30.0, width=width,
-               height=height, lens_radius=0.0)
+    ms = MaterialSet(types, idx, lambertians=lam, wards=wards, dielectrics=diel, volumes=vols)
+    # fill: 10 degree field of view from distance 3 stays inside the sphere's 19 degree silhouette
+    cam = dict(eye=(0.0, 0.0, 3.0 if fill else 4.5), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+               fov=10.0 if fill else 30.0, width=width, height=height, lens_radius=0.0)
     return dict(points=v, indices=f, materials=ms, env=constant_envmap(value), camera=cam)

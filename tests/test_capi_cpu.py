@@ -6,6 +6,7 @@ import re
 import numpy as np
 
 from raccoongpurender_amd import _lib
+from raccoongpurender_amd import guide as _guide  # noqa: F401  (registers the rgpu_guide_* signatures)
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "raccoon_gpu.h")
 

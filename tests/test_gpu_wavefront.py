@@ -57,6 +57,11 @@ def test_all_materials_without_reference_quirks(ctx):
     _compare(ctx, sd, 32 * 24 * 3, 10, quirks=False)
 
 
+def test_all_materials_with_consistent_normals(ctx):
+    sd = R.material_test_scene(32, 24, volume=True)
+    _compare(ctx, sd, 32 * 24 * 3, 10, consistent=True)
+
+
 def test_options_depth_clamp_selection(ctx):
     sd = R.material_test_scene(32, 24)
     _compare(ctx, sd, 32 * 24 * 2, 8, max_depth=3, clamp=50.0, bxdf_p=0.3, rr_start=2, seed=1234)

@@ -36,6 +36,26 @@ def test_furnace_lambertian_convex_sphere():
     assert np.allclose(corner, 0.5, atol=1e-6)
 
 
+def test_consistent_normals_make_back_faces_exact():
+    # The camera sees only back faces of a lambertian sphere that fills the frame. With the
+    # reference's unflipped normal the cosine lobe points into the surface: those samples
+    # return nothing, and since the lobe's pdf is evaluated about the same normal (zero for
+    # every outward direction) the estimate stays unbiased, only noisier. With consistent
+    # normals both strategies contribute and the noise drops several times.
+    sd = R.furnace_scene("lambertian", 16, 16, value=0.5, fill=True, flip=True)
+    out = {}
+    for consistent in (False, True):
+        lane = W.oracle_lane(sd, 16 * 16 * 16, consistent=consistent, seed=3)
+        lane.step(24)
+        m, cnt = W.mean_image(lane)
+        assert cnt.min() > 0
+        y = m[:, 0]
+        out[consistent] = (y.mean(), y.std(ddof=1) / np.sqrt(y.size))
+    for mean, se in out.values():
+        assert abs(mean - 0.25) < 5 * se
+    assert out[False][1] > 3 * out[True][1]
+
+
 def test_queue_invariants_and_determinism():
     sd = R.material_test_scene(32, 24)
     a = W.oracle_lane(sd, 32 * 24 * 2)

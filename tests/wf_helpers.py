@@ -25,6 +25,8 @@ def gpu_lane(ctx, sd, n, **kw):
     env = _lib.Envmap(ctx, sd["env"])
     quirks = kw.pop("quirks", True)
     flags = kw.pop("flags", 0) | (0 if quirks else _lib.WF_NO_REFERENCE_QUIRKS)
+    if kw.pop("consistent", False):
+        flags |= _lib.WF_CONSISTENT_NORMALS
     wf = _lib.Wavefront(ctx, scene, mats, env, n, flags=flags, **kw)
     wf.initialize(camera(sd["camera"]))
     wf._keep = (bvh, scene, mats, env)
