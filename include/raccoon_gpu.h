@@ -314,6 +314,7 @@ typedef struct rgpu_guide_config {
     uint32_t seed;         /* weight initialisation */
     float scene_lo[3];     /* scene box that normalises positions */
     float scene_hi[3];
+    uint32_t max_depth;    /* guide the first max_depth vertices of a path; 0 = every vertex */
 } rgpu_guide_config;
 
 int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** out);
@@ -322,7 +323,8 @@ int rgpu_guide_param_count(void);
 /* Copy the fp32 weights out (get) and/or in (set; refreshes the bf16 copies). */
 int rgpu_guide_params(rgpu_guide* g, float* get, const float* set);
 /* Test hooks over host rows of 16 floats (position in [0,1]^3, normal, wo, wi, weight, pad):
- * raw head outputs (64 per row), the gradient of the mean loss -weight*log q(wi), and training. */
+ * raw head outputs (64 per row), the gradient of the mean loss -weight*log q(wi) (at most
+ * minibatch rows), and training (minibatch Adam steps over at most capacity rows). */
 int rgpu_guide_forward(rgpu_guide* g, const float* samples, uint32_t rows, float* out);
 int rgpu_guide_gradients(rgpu_guide* g, const float* samples, uint32_t rows, float* grad);
 int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows);

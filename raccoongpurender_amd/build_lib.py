@@ -21,9 +21,12 @@ COMMON = [
 ]
 DEVICE = [
     "--offload-arch=gfx950",
-    "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# Correctly rounded fp32 division and square root wherever results are compared with the
+# oracle bit for bit; the network kernels (guide.hip) are checked to a tolerance instead.
+EXACT = ["-fhip-fp32-correctly-rounded-divide-sqrt"]
+TOLERANT = {"guide.hip"}
 
 
 def build(verbose: bool = False, jobs: int = 8) -> str:
@@ -35,10 +38,11 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
+        flags = [*COMMON, *DEVICE, *([] if src in TOLERANT else EXACT)]
         if src.endswith(".hip"):
-            cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj]
+            cmd = [HIPCC, "-x", "hip", *flags, "-c", path, "-o", obj]
         else:
-            cmd = [HIPCC, *COMMON, *DEVICE, "-c", path, "-o", obj]
+            cmd = [HIPCC, *flags, "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

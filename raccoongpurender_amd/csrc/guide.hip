@@ -1,16 +1,22 @@
 // Path-guiding network kernels on gfx950 matrix cores (bf16 operands, fp32 accumulation).
 //
-// A workgroup of two wave64s processes a tile of 32 rows (one row = one path vertex or one
-// training sample); each wave owns 16 rows. Every layer is a product of the tile's bf16
-// activations held in LDS with the bf16 weights read from global memory (they stay in L1/L2:
-// 37 KB per copy), computed 16x16x32 at a time by v_mfma_f32_16x16x32_bf16:
+// Rows (path vertices or training samples) go through the network in tiles of 32, one
+// workgroup of two wave64s per tile, each wave owning 16 rows. Activations live in LDS as
+// bf16; the bf16 weights (37 KB, row-major [out][in], plus a transposed copy for
+// back-propagation) are read from global memory, where they stay cache-resident, so that a
+// tile needs only 29-38 KB of LDS and four or five workgroups share a CU. Products run
+// 16x16x32 at a time on v_mfma_f32_16x16x32_bf16:
 //   A operand: lane l holds 8 consecutive k of row (l mod 16),
 //   B operand: lane l holds the same 8 k of column (l mod 16),
 //   result:    lane l holds rows 4*(l/16)..+3 of column (l mod 16).
-// Back-propagation computes input deltas with the transposed weight copy and weight
-// gradients over the tile's 32 rows, accumulated across workgroups with float atomics.
+// Training is split in two: k_guide_fwdbwd runs forward, loss gradient and the input deltas
+// of every tile and writes the layer inputs and output deltas to a bf16 buffer (column-major
+// per tile); k_guide_dw then forms the weight gradients dW = dZ^T In with the rows as the
+// reduction dimension, one partial sum per split of the tiles, which k_adam adds up.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "guide_device.h"
 #include "rgpu_wavefront.h"
@@ -71,15 +77,18 @@ __device__ void fwd_layer(const uint16_t* X, int px, const uint16_t* W, const fl
     }
 }
 
-// Encoded input features of one row (written as bf16 into X).
+// Encoded input features of one row (written as bf16 into X): for band k and axis a,
+// sin and cos of 2^k pi p_a. 2^k p_a is exact in fp32 and so is its reduction modulo 2,
+// which leaves sinpi/cospi a small argument.
 __device__ void encode_row(uint16_t* X, const float* p01, const float* n, const float* wo, int part) {
     // part 0..3: each of four lanes writes 24 of the 96 features
     for (int f = part * 24; f < part * 24 + 24; ++f) {
         float v;
         if (f < 72) {
-            int k = f / 6, a = (f % 6) / 2;
-            double arg = (double)(1u << k) * 3.14159265358979323846 * (double)p01[a];
-            v = (f & 1) ? (float)cos(arg) : (float)sin(arg);
+            const int k = f / 6, a = (f % 6) / 2;
+            float x = ldexpf(p01[a], k);
+            x = x - 2.0f * floorf(0.5f * x);
+            v = (f & 1) ? cospif(x) : sinpif(x);
         } else if (f < 75) {
             v = n[f - 72];
         } else if (f < 78) {
@@ -91,51 +100,66 @@ __device__ void encode_row(uint16_t* X, const float* p01, const float* n, const 
     }
 }
 
-// Raw head outputs of one row -> 8 lobes (mu normalised, kappa, log weight).
-__device__ void head_lobes(const float* out, float* lobes) {
-    float mx = -3.0e38f;
-    for (int k = 0; k < kGLobes; ++k) mx = fmaxf(mx, out[5 * k + 4]);
-    float z = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) z += expf(out[5 * k + 4] - mx);
-    const float lz = logf(z);
-    for (int k = 0; k < kGLobes; ++k) {
-        const float* o = out + 5 * k;
-        float m = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
-        lobes[5 * k + 0] = o[0] / m;
-        lobes[5 * k + 1] = o[1] / m;
-        lobes[5 * k + 2] = o[2] / m;
-        lobes[5 * k + 3] = expf(fminf(fmaxf(o[3], kGLogKappaMin), kGLogKappaMax));
-        lobes[5 * k + 4] = o[4] - mx - lz;
-    }
+// Lobe k of one row from its raw outputs o[0..4] and the row's softmax constants.
+__device__ __forceinline__ void lobe_params(const float* o, float mx, float lz, float* L) {
+    const float m = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
+    const float kap = expf(fminf(fmaxf(o[3], kGLogKappaMin), kGLogKappaMax));
+    const float lw = o[4] - mx - lz;
+    L[0] = o[0] / m;
+    L[1] = o[1] / m;
+    L[2] = o[2] / m;
+    L[3] = kap;
+    L[4] = expf(lw);
+    L[5] = lw + guide_log_norm(kap);
 }
 
-// d(-weight * log q(wi)) / d(raw outputs); zero for the 24 unused outputs.
-__device__ void head_grad(const float* out, const float* wi, float weight, float* d) {
-    float L[kGSlotFloats];
-    head_lobes(out, L);
-    float a[kGLobes], mx = -3.0e38f;
-    float c[kGLobes];
-    for (int k = 0; k < kGLobes; ++k) {
-        c[k] = L[5 * k] * wi[0] + L[5 * k + 1] * wi[1] + L[5 * k + 2] * wi[2];
-        a[k] = L[5 * k + 4] + guide_log_vmf(L[5 * k + 3], c[k]);
-        mx = fmaxf(mx, a[k]);
+// The mixture head runs on four lanes per row (lanes 4r..4r+3 of a workgroup hold row r),
+// two lobes per lane; softmax and log-sum-exp combine over the four lanes.
+__device__ __forceinline__ float quad_max(float v) {
+    v = fmaxf(v, __shfl_xor(v, 1));
+    return fmaxf(v, __shfl_xor(v, 2));
+}
+__device__ __forceinline__ float quad_sum(float v) {
+    v += __shfl_xor(v, 1);
+    return v + __shfl_xor(v, 2);
+}
+
+// Raw head outputs of one row -> this lane's two lobes (mu, kappa, weight, log(weight C(kappa))).
+__device__ void head_lobes_part(const float* out, int part, float* L) {
+    const float* o0 = out + 5 * (2 * part);
+    const float* o1 = o0 + 5;
+    const float mx = quad_max(fmaxf(o0[4], o1[4]));
+    const float lz = logf(quad_sum(expf(o0[4] - mx) + expf(o1[4] - mx)));
+    lobe_params(o0, mx, lz, L);
+    lobe_params(o1, mx, lz, L + kGLobeFloats);
+}
+
+// d(-weight * log q(wi)) / d(raw outputs) for this lane's two lobes: d[j] is the derivative
+// by output 10 part + j (the 24 unused outputs have none).
+__device__ void head_grad_part(const float* out, int part, const float* wi, float weight, float* d) {
+    float L[2 * kGLobeFloats], a[2], cm1[2];
+    head_lobes_part(out, part, L);
+    for (int q = 0; q < 2; ++q) {
+        const float* Lk = L + kGLobeFloats * q;
+        const float dx = wi[0] - Lk[0], dy = wi[1] - Lk[1], dz = wi[2] - Lk[2];
+        cm1[q] = -0.5f * (dx * dx + dy * dy + dz * dz);  // mu . wi - 1
+        a[q] = Lk[5] + Lk[3] * cm1[q];
     }
-    float s = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) s += expf(a[k] - mx);
-    const float lse = mx + logf(s);
-    for (int i = 0; i < kGOut; ++i) d[i] = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) {
-        const float r = expf(a[k] - lse);
-        const float pi_k = expf(L[5 * k + 4]);
-        d[5 * k + 4] = weight * (pi_k - r);
-        const float kap = L[5 * k + 3];
+    const float mx = quad_max(fmaxf(a[0], a[1]));
+    const float lse = mx + logf(quad_sum(expf(a[0] - mx) + expf(a[1] - mx)));
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const float* Lk = L + kGLobeFloats * q;
+        const float* o = out + 5 * (2 * part + q);
+        const float r = expf(a[q] - lse);  // responsibility of the lobe for wi
+        d[5 * q + 4] = weight * (Lk[4] - r);
+        const float kap = Lk[3];
         const float e = expf(-2.0f * kap);
-        const float dlogf = 1.0f / kap - 2.0f * e / (1.0f - e) + (c[k] - 1.0f);
-        const float raw = out[5 * k + 3];
-        d[5 * k + 3] = (raw > kGLogKappaMin && raw < kGLogKappaMax) ? -weight * r * dlogf * kap : 0.0f;
-        const float* o = out + 5 * k;
+        const float dlogf = 1.0f / kap - 2.0f * e / (1.0f - e) + cm1[q];
+        d[5 * q + 3] = (o[3] > kGLogKappaMin && o[3] < kGLogKappaMax) ? -weight * r * dlogf * kap : 0.0f;
         const float m = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
-        for (int q = 0; q < 3; ++q) d[5 * k + q] = -weight * r * kap * (wi[q] - c[k] * L[5 * k + q]) / m;
+        const float c = cm1[q] + 1.0f;
+        for (int j = 0; j < 3; ++j) d[5 * q + j] = -weight * r * kap * (wi[j] - c * Lk[j]) / m;
     }
 }
 
@@ -192,15 +216,19 @@ __global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, Dev
         __syncthreads();
         tile_forward(t, g, r0, lane);
         __syncthreads();
-        if (threadIdx.x < kTile) {
-            const uint32_t i = base + threadIdx.x;
+        {
+            const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
+            const uint32_t i = base + row;
+            float L[2 * kGLobeFloats];
+            head_lobes_part(t.F + row * kPF, part, L);  // all four lanes of a row take part
             if (i < rows) {
                 const uint32_t slot = i < c1 ? s.items[(size_t)kMatLambertian * p.n + i]
                                              : s.items[(size_t)kMatWard * p.n + (i - c1)];
-                float L[kGSlotFloats];
-                head_lobes(t.F + threadIdx.x * kPF, L);
-                float* dst = g.lobes + (size_t)slot * kGSlotFloats;
-                for (int q = 0; q < kGSlotFloats; ++q) dst[q] = L[q];
+                float4* dst =
+                    reinterpret_cast<float4*>(g.lobes + (size_t)slot * kGSlotFloats + 2 * kGLobeFloats * part);
+                dst[0] = make_float4(L[0], L[1], L[2], L[3]);
+                dst[1] = make_float4(L[4], L[5], L[6], L[7]);
+                dst[2] = make_float4(L[8], L[9], L[10], L[11]);
             }
         }
         __syncthreads();
@@ -208,7 +236,7 @@ __global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, Dev
 }
 
 // Back-propagation through one layer for this wave's 16 rows:
-// dIn = (dZ W) masked by In > 0, written as bf16 into dIn (skip with dIn == nullptr).
+// dIn = (dZ W) masked by In > 0, written as bf16 into dIn. WT: [in][64] transposed weights.
 template <int K>
 __device__ void bwd_dx(const uint16_t* dZ, const uint16_t* WT, const uint16_t* In, int pin, uint16_t* dIn, int r0,
                        int lane) {
@@ -228,32 +256,22 @@ __device__ void bwd_dx(const uint16_t* dZ, const uint16_t* WT, const uint16_t* I
     }
 }
 
-// Weight and bias gradients of one layer over the tile's 32 rows: dW[o][i] += sum_r dZ[r][o] In[r][i].
-template <int K>
-__device__ void bwd_dw(const uint16_t* dZ, const uint16_t* In, int pin, float* gW, float* gB, int wave, int lane) {
-    const int ntiles = 4 * (K / 16);
-    for (int tix = wave; tix < ntiles; tix += 2) {
-        const int ot = tix / (K / 16), it = tix % (K / 16);
-        bf16x8 a = ld8s(dZ + ((lane >> 4) * 8) * kPH + ot * 16 + (lane & 15), kPH);
-        bf16x8 b = ld8s(In + ((lane >> 4) * 8) * pin + it * 16 + (lane & 15), pin);
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = mma(a, b, acc);
-        const int i = it * 16 + (lane & 15);
-        for (int r = 0; r < 4; ++r) {
-            const int o = ot * 16 + (lane >> 4) * 4 + r;
-            unsafeAtomicAdd(gW + o * K + i, acc[r]);
-        }
-    }
-    const int tid = wave * 64 + lane;
-    if (tid < kGHid) {
-        float sb = 0.0f;
-        for (int r = 0; r < kTile; ++r) sb += bf2f(dZ[r * kPH + tid]);
-        unsafeAtomicAdd(gB + tid, sb);
+// Per-tile block of the training buffer: 32 rows of each matrix below, column-major
+// (element (r, c) of a matrix at its column offset * 32 + c * 32 + r).
+constexpr int kAcX = 0, kAcH1 = kAcX + kGIn, kAcH2 = kAcH1 + kGHid, kAcH3 = kAcH2 + kGHid;
+constexpr int kAcD0 = kAcH3 + kGHid, kAcD1 = kAcD0 + kGHid, kAcD2 = kAcD1 + kGHid, kAcD3 = kAcD2 + kGHid;
+constexpr int kAcCols = kAcD3 + kGOut;  // 544 columns, 34,816 bytes per tile
+
+__device__ __forceinline__ void store_cm(const uint16_t* lds, int pitch, int cols, uint16_t* dst) {
+    for (int idx = threadIdx.x; idx < cols * kTile; idx += blockDim.x) {
+        const int c = idx / kTile, r = idx % kTile;
+        dst[c * kTile + r] = lds[r * pitch + c];
     }
 }
 
-// Forward + loss gradient + back-propagation for `rows` training samples starting at `first`.
-__global__ void __launch_bounds__(128) k_guide_train(DevGuide g, uint32_t first, uint32_t rows, float inv_batch) {
+// Forward, loss gradient and input deltas of one tile of training rows; layer inputs and
+// output deltas go to the tile's block of g.act.
+__global__ void __launch_bounds__(128) k_guide_fwdbwd(DevGuide g, uint32_t first, uint32_t rows, float inv_batch) {
     __shared__ TileLds t;
     __shared__ uint16_t dA[kTile * kPH];
     __shared__ uint16_t dB[kTile * kPH];
@@ -261,6 +279,7 @@ __global__ void __launch_bounds__(128) k_guide_train(DevGuide g, uint32_t first,
     const int r0 = wave * 16;
     const uint32_t base = blockIdx.x * kTile;
     if (base >= rows) return;
+    uint16_t* blk = g.act + (size_t)blockIdx.x * kAcCols * kTile;
     {
         const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
         float p01[3] = {0, 0, 0}, n[3] = {0, 0, 0}, wo[3] = {0, 0, 0};
@@ -277,29 +296,95 @@ __global__ void __launch_bounds__(128) k_guide_train(DevGuide g, uint32_t first,
     __syncthreads();
     tile_forward(t, g, r0, lane);
     __syncthreads();
-    if (threadIdx.x < kTile) {
-        const int row = threadIdx.x;
-        float d[kGOut];
-        if (base + row < rows) {
+    {
+        const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
+        const bool live = base + row < rows;
+        float wi[3] = {0.0f, 0.0f, 1.0f}, wgt = 0.0f;  // padding rows contribute nothing
+        if (live) {
             const GuideSample& sm = g.samples[first + base + row];
-            head_grad(t.F + row * kPF, sm.wi, sm.weight * inv_batch, d);
-        } else {
-            for (int i = 0; i < kGOut; ++i) d[i] = 0.0f;
+            wi[0] = sm.wi[0];
+            wi[1] = sm.wi[1];
+            wi[2] = sm.wi[2];
+            wgt = sm.weight * inv_batch;
         }
-        for (int i = 0; i < kGOut; ++i) dA[row * kPH + i] = f2bf(d[i]);
+        float d[10];
+        head_grad_part(t.F + row * kPF, part, wi, wgt, d);
+        uint16_t* drow = dA + row * kPH;
+#pragma unroll
+        for (int i = 0; i < 10; ++i) drow[10 * part + i] = f2bf(d[i]);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) drow[5 * kGLobes + 6 * part + i] = 0;
     }
     __syncthreads();
-    // layer 3: dW3, then dZ2 = (dY3 W3) * (H3 > 0)
-    bwd_dw<kGHid>(dA, t.H3, kPH, g.grad + kGoW3, g.grad + kGoB3, wave, lane);
-    bwd_dx<kGHid>(dA, g.wt_bf + kGoW3, t.H3, kPH, dB, r0, lane);
+    store_cm(t.X, kPX, kGIn, blk + kAcX * kTile);
+    store_cm(t.H1, kPH, kGHid, blk + kAcH1 * kTile);
+    store_cm(t.H2, kPH, kGHid, blk + kAcH2 * kTile);
+    store_cm(t.H3, kPH, kGHid, blk + kAcH3 * kTile);
+    store_cm(dA, kPH, kGOut, blk + kAcD3 * kTile);
+    bwd_dx<kGHid>(dA, g.wt_bf + kGoW3, t.H3, kPH, dB, r0, lane);  // dZ2 = (dZ3 W3) * (H3 > 0)
     __syncthreads();
-    bwd_dw<kGHid>(dB, t.H2, kPH, g.grad + kGoW2, g.grad + kGoB2, wave, lane);
+    store_cm(dB, kPH, kGHid, blk + kAcD2 * kTile);
     bwd_dx<kGHid>(dB, g.wt_bf + kGoW2, t.H2, kPH, dA, r0, lane);
     __syncthreads();
-    bwd_dw<kGHid>(dA, t.H1, kPH, g.grad + kGoW1, g.grad + kGoB1, wave, lane);
+    store_cm(dA, kPH, kGHid, blk + kAcD1 * kTile);
     bwd_dx<kGHid>(dA, g.wt_bf + kGoW1, t.H1, kPH, dB, r0, lane);
     __syncthreads();
-    bwd_dw<kGIn>(dB, t.X, kPX, g.grad + kGoW0, g.grad + kGoB0, wave, lane);
+    store_cm(dB, kPH, kGHid, blk + kAcD0 * kTile);
+}
+
+// Weight and bias gradients of one layer over a split of the tiles:
+// dW[o][i] = sum_r dZ[r][o] In[r][i] with the 32 rows of a tile as the MFMA's k, written as
+// this split's partial sum. blockIdx.x = split * 4 + layer; four waves share the output tiles.
+__global__ void __launch_bounds__(256) k_guide_dw(DevGuide g, uint32_t ntiles, uint32_t splits) {
+    const int layer = blockIdx.x & 3, split = blockIdx.x >> 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int K = layer == 0 ? kGIn : kGHid;
+    const int in_col = layer == 0 ? kAcX : layer == 1 ? kAcH1 : layer == 2 ? kAcH2 : kAcH3;
+    const int dz_col = layer == 0 ? kAcD0 : layer == 1 ? kAcD1 : layer == 2 ? kAcD2 : kAcD3;
+    const int w_off = layer == 0 ? kGoW0 : layer == 1 ? kGoW1 : layer == 2 ? kGoW2 : kGoW3;
+    const int b_off = layer == 0 ? kGoB0 : layer == 1 ? kGoB1 : layer == 2 ? kGoB2 : kGoB3;
+    const uint32_t t0 = (uint32_t)((uint64_t)ntiles * split / splits);
+    const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (split + 1) / splits);
+    const int ntile_out = 4 * (K / 16);  // 16x16 output tiles of this layer: 16 or 24
+    __shared__ float bpart[4][kGHid];
+    f32x4 acc[6];
+    for (int j = 0; j < 6; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;  // column (threadIdx & 63) of dZ, rows 8 (threadIdx / 64) .. + 7 of each tile
+#pragma unroll 2
+    for (uint32_t tt = t0; tt < t1; ++tt) {
+        const uint16_t* blk = g.act + (size_t)tt * kAcCols * kTile;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int tix = wave + 4 * j;
+            if (tix < ntile_out) {
+                const int ot = tix / (K / 16), it = tix % (K / 16);
+                const bf16x8 a = ld8(blk + (dz_col + ot * 16 + (lane & 15)) * kTile + (lane >> 4) * 8);
+                const bf16x8 b = ld8(blk + (in_col + it * 16 + (lane & 15)) * kTile + (lane >> 4) * 8);
+                acc[j] = mma(a, b, acc[j]);
+            }
+        }
+        const uint4 v = *reinterpret_cast<const uint4*>(blk + (dz_col + lane) * kTile + wave * 8);
+        bsum += (bf2f((uint16_t)v.x) + bf2f((uint16_t)(v.x >> 16))) + (bf2f((uint16_t)v.y) + bf2f((uint16_t)(v.y >> 16))) +
+                (bf2f((uint16_t)v.z) + bf2f((uint16_t)(v.z >> 16))) + (bf2f((uint16_t)v.w) + bf2f((uint16_t)(v.w >> 16)));
+    }
+    bpart[wave][lane] = bsum;
+    __syncthreads();
+    float* dst = g.partials + (size_t)split * kGParams;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int tix = wave + 4 * j;
+        if (tix < ntile_out) {
+            const int ot = tix / (K / 16), it = tix % (K / 16);
+            const int i = it * 16 + (lane & 15);
+            for (int r = 0; r < 4; ++r) {
+                const int o = ot * 16 + (lane >> 4) * 4 + r;
+                dst[w_off + o * K + i] = acc[j][r];
+            }
+        }
+    }
+    if (threadIdx.x < kGHid)
+        dst[b_off + threadIdx.x] = (bpart[0][threadIdx.x] + bpart[1][threadIdx.x]) +
+                                   (bpart[2][threadIdx.x] + bpart[3][threadIdx.x]);
 }
 
 // Forward only over given encoded-input samples (tests): raw head outputs per row.
@@ -337,15 +422,30 @@ __device__ __forceinline__ bool is_weight(int i, int* layer_off, int* nin, int* 
     return false;
 }
 
-// Adam (Kingma & Ba 2015) on the fp32 master weights, then refresh the bf16 copies and clear
-// the gradient. lr == 0 only refreshes the copies.
-__global__ void __launch_bounds__(256) k_adam(DevGuide g, float lr, float b1, float b2, float eps, float c1,
-                                              float c2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kGParams) return;
+// Sum of the `groups` partial gradients, then Adam (Kingma & Ba 2015) on the fp32 master
+// weights and a refresh of the bf16 copies. lr == 0: the summed gradient is stored in g.grad
+// instead (groups == 0 only refreshes the copies). A workgroup covers 64 parameters; its four
+// waves sum interleaved quarters of the partials.
+__global__ void __launch_bounds__(256) k_adam(DevGuide g, uint32_t groups, float lr, float b1, float b2, float eps,
+                                              float c1, float c2) {
+    __shared__ float part[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < kGParams) {
+        uint32_t w = (uint32_t)wave;
+        for (; w + 28 < groups; w += 32) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] += g.partials[(size_t)(w + 4 * u) * kGParams + i];
+        }
+        for (; w < groups; w += 4) acc[0] += g.partials[(size_t)w * kGParams + i];
+    }
+    part[wave][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (wave != 0 || i >= kGParams) return;
+    const float gr = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
     float w = g.params[i];
     if (lr != 0.0f) {
-        const float gr = g.grad[i];
         float m = b1 * g.adam_m[i] + (1.0f - b1) * gr;
         float v = b2 * g.adam_v[i] + (1.0f - b2) * gr * gr;
         g.adam_m[i] = m;
@@ -353,8 +453,9 @@ __global__ void __launch_bounds__(256) k_adam(DevGuide g, float lr, float b1, fl
         const float mh = m / c1, vh = v / c2;
         w -= lr * mh / (sqrtf(vh) + eps);
         g.params[i] = w;
+    } else if (groups != 0) {
+        g.grad[i] = gr;
     }
-    g.grad[i] = 0.0f;
     int off, nin, nout;
     if (is_weight(i, &off, &nin, &nout)) {
         const int o = (i - off) / nin, in = (i - off) % nin;
@@ -366,25 +467,35 @@ __global__ void __launch_bounds__(256) k_adam(DevGuide g, float lr, float b1, fl
 
 }  // namespace
 
-hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_infer, dim3(blocks > 0 ? blocks : 2048), dim3(128), 0, s, st, p, g);
+hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_infer, dim3(g.infer_blocks ? g.infer_blocks : 2048u), dim3(128), 0, s, st, p, g);
     return hipGetLastError();
+}
+
+// Forward/backward of `rows` samples from `first`, then the weight gradients; returns the
+// number of partial sums written.
+static uint32_t launch_gradients(const DevGuide& g, uint32_t first, uint32_t rows, hipStream_t s) {
+    const uint32_t ntiles = (rows + kTile - 1) / kTile;
+    const uint32_t splits = std::min<uint32_t>(kGTrainGroups, ntiles);
+    hipLaunchKernelGGL(k_guide_fwdbwd, dim3(ntiles), dim3(128), 0, s, g, first, rows, 1.0f / (float)rows);
+    hipLaunchKernelGGL(k_guide_dw, dim3(4 * splits), dim3(256), 0, s, g, ntiles, splits);
+    return splits;
 }
 
 hipError_t guide_train_batch(const DevGuide& g, uint32_t first, uint32_t rows, float lr, uint32_t step,
                              hipStream_t s) {
     if (rows == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guide_train, dim3((rows + kTile - 1) / kTile), dim3(128), 0, s, g, first, rows,
-                       1.0f / (float)rows);
+    if (rows > g.act_rows) return hipErrorInvalidValue;
+    const uint32_t groups = launch_gradients(g, first, rows, s);
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-    hipLaunchKernelGGL(k_adam, dim3((kGParams + 255) / 256), dim3(256), 0, s, g, lr, b1, b2, 1e-8f, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3((kGParams + 63) / 64), dim3(256), 0, s, g, groups, lr, b1, b2, 1e-8f, c1, c2);
     return hipGetLastError();
 }
 
 hipError_t guide_refresh(const DevGuide& g, hipStream_t s) {
-    hipLaunchKernelGGL(k_adam, dim3((kGParams + 255) / 256), dim3(256), 0, s, g, 0.0f, 0.9f, 0.999f, 1e-8f, 1.0f,
-                       1.0f);
+    hipLaunchKernelGGL(k_adam, dim3((kGParams + 63) / 64), dim3(256), 0, s, g, 0u, 0.0f, 0.9f, 0.999f, 1e-8f,
+                       1.0f, 1.0f);
     return hipGetLastError();
 }
 
@@ -396,8 +507,10 @@ hipError_t guide_forward(const DevGuide& g, const GuideSample* in, uint32_t rows
 
 hipError_t guide_gradients(const DevGuide& g, uint32_t rows, hipStream_t s) {
     if (rows == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guide_train, dim3((rows + kTile - 1) / kTile), dim3(128), 0, s, g, 0u, rows,
-                       1.0f / (float)rows);
+    if (rows > g.act_rows) return hipErrorInvalidValue;
+    const uint32_t groups = launch_gradients(g, 0u, rows, s);
+    hipLaunchKernelGGL(k_adam, dim3((kGParams + 63) / 64), dim3(256), 0, s, g, groups, 0.0f, 0.9f, 0.999f, 1e-8f,
+                       1.0f, 1.0f);
     return hipGetLastError();
 }
 

@@ -3,8 +3,8 @@
 // Network: 96 encoded inputs (12-band sin/cos of the position normalised to the scene box,
 // normal, outgoing direction, zero padding) -> 3 ReLU layers of width 64 -> 64 outputs, of
 // which 40 parameterise 8 von Mises-Fisher lobes (unnormalised mean direction, log
-// concentration, mixture logit). Weights are kept in fp32 and mirrored in bfloat16 for the
-// matrix cores (row-major [out][in] for forward passes, transposed for back-propagation).
+// concentration, mixture logit). Weights are kept in fp32 and mirrored in bfloat16 (row-major
+// [out][in]) for the matrix cores.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -15,9 +15,14 @@ constexpr int kGIn = 96, kGHid = 64, kGOut = 64, kGLobes = 8, kGBands = 12;
 constexpr int kGoW0 = 0, kGoB0 = kGoW0 + kGHid * kGIn, kGoW1 = kGoB0 + kGHid, kGoB1 = kGoW1 + kGHid * kGHid,
               kGoW2 = kGoB1 + kGHid, kGoB2 = kGoW2 + kGHid * kGHid, kGoW3 = kGoB2 + kGHid,
               kGoB3 = kGoW3 + kGOut * kGHid, kGParams = kGoB3 + kGOut;
+constexpr int kGTrainGroups = 128;  // partial gradient sums of one training step (tile splits)
+constexpr int kGActBytesPerRow = 544 * 2;  // layer inputs and output deltas of one training row
 constexpr float kGLogKappaMin = -3.0f, kGLogKappaMax = 7.5f;
-constexpr int kGLobeFloats = 5;                  // mu.xyz, kappa, log weight
-constexpr int kGSlotFloats = kGLobes * kGLobeFloats;  // 40 floats of lobe parameters per slot
+// Stored per lobe: mu.xyz (unit), kappa, mixture weight, log(weight * C(kappa)) with
+// C(kappa) = kappa / (2 pi (1 - exp(-2 kappa))) the vMF normalisation, so that evaluating
+// the mixture needs one exp per lobe.
+constexpr int kGLobeFloats = 6;
+constexpr int kGSlotFloats = kGLobes * kGLobeFloats;  // 48 floats of lobe parameters per slot
 
 // One training sample: position (normalised to the scene box), normal, outgoing and
 // incident directions, and the weight F/p of the direction (luminance of the incident
@@ -27,72 +32,85 @@ struct GuideSample {
 };
 static_assert(sizeof(GuideSample) == 64, "guide sample is 64 bytes");
 
-// Per-vertex record kept until the path ends (p, n, wo, wi, throughput before the vertex,
-// radiance after the vertex's own emission).
-struct GuideRecord {
-    float p[3], n[3], wo[3], wi[3], T[3], L[3], pad[2];
+// Per-vertex record kept until the path ends: the sample's geometry (p, n, wo, wi), and in
+// separate arrays the throughput before the vertex and the radiance after its own emission.
+// Record j of slot s sits at j * nslots + s, so that a wave touches consecutive records.
+struct GuideVertex {
+    float p[3], n[3], wo[3], wi[3];
 };
-static_assert(sizeof(GuideRecord) == 80, "guide record is 80 bytes");
+static_assert(sizeof(GuideVertex) == 48, "guide vertex is 48 bytes");
 constexpr int kGRecords = 4;  // guided vertices recorded per path
 
 struct DevGuide {
     float* params = nullptr;          // fp32 master weights
     uint16_t* w_bf = nullptr;         // bf16 weights [out][in] at the parameter offsets
     uint16_t* wt_bf = nullptr;        // bf16 transposed weights [in][out]
-    float* grad = nullptr;
+    uint16_t* act = nullptr;          // training buffer of layer inputs and deltas (bf16)
+    uint32_t act_rows = 0;            // rows it holds (the minibatch)
+    float* grad = nullptr;            // summed gradient (test hook only)
+    float* partials = nullptr;        // kGTrainGroups per-workgroup gradient sums
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     float* lobes = nullptr;           // per slot kGSlotFloats
-    GuideRecord* records = nullptr;   // per slot kGRecords
+    GuideVertex* rec_geo = nullptr;   // kGRecords * nslots
+    float4* rec_T = nullptr;          // kGRecords * nslots (xyz used)
+    float4* rec_L = nullptr;          // kGRecords * nslots (xyz used)
     uint32_t* rec_count = nullptr;    // per slot
+    float4* emit_w = nullptr;         // per slot: weights of the samples a finished path emits
+    uint32_t nslots = 0;
+    uint32_t infer_blocks = 0;        // persistent inference grid
     GuideSample* samples = nullptr;   // training buffer
     uint32_t* sample_count = nullptr; // appended samples (may exceed capacity; extra are dropped)
     uint32_t capacity = 0;
     float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};  // scene box normalisation
     float select_p = 0.5f;            // one-sample MIS selection probability of the guide
+    uint32_t max_depth = 0;           // guided vertices per path (0 = all)
     uint32_t enabled = 0;             // guided sampling on
     uint32_t train = 0;               // record training samples
 };
 
 // ---------------------------------------------------------------- vMF mixture (device)
-__device__ __forceinline__ float guide_log_vmf(float kappa, float c) {
-    return (float)(log((double)kappa) - log(2.0 * 3.14159265358979323846) - log1p(-exp(-2.0 * (double)kappa))) +
-           kappa * (c - 1.0f);
+// log C(kappa) for kappa in [e^-3, e^7.5]
+__device__ __forceinline__ float guide_log_norm(float kappa) {
+    return logf(kappa) - 1.83787706641f - log1pf(-expf(-2.0f * kappa));
 }
 
-// q(w) of the lobes stored for one slot
+// q(w) of the lobes stored for one slot. kappa (c - 1) is formed as -kappa |w - mu|^2 / 2,
+// which keeps its precision for sharp lobes where c is close to 1.
 __device__ __forceinline__ float guide_pdf(const float* lobes, float wx, float wy, float wz) {
-    float a[kGLobes], mx = -3.0e38f;
+    float s = 0.0f;
+#pragma unroll
     for (int k = 0; k < kGLobes; ++k) {
         const float* L = lobes + kGLobeFloats * k;
-        float c = L[0] * wx + L[1] * wy + L[2] * wz;
-        a[k] = L[4] + guide_log_vmf(L[3], c);
-        mx = fmaxf(mx, a[k]);
+        const float dx = wx - L[0], dy = wy - L[1], dz = wz - L[2];
+        s += expf(L[5] - 0.5f * L[3] * (dx * dx + dy * dy + dz * dz));
     }
-    float s = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) s += expf(a[k] - mx);
-    return expf(mx) * s;
+    return s;
 }
 
 // Sample a direction: lobe by weight (u0), then Jakob's inversion of the vMF (u1, u2).
+// The selected lobe is gathered with compile-time indices so that a lobe table held in
+// registers is never indexed dynamically.
 __device__ __forceinline__ void guide_sample(const float* lobes, float u0, float u1, float u2, float* w) {
+    constexpr int kLast = kGLobeFloats * (kGLobes - 1);
+    float z0 = lobes[kLast], z1 = lobes[kLast + 1], z2 = lobes[kLast + 2], kap = lobes[kLast + 3];
     float c = 0.0f;
-    int k = kGLobes - 1;
-    for (int j = 0; j < kGLobes; ++j) {
-        c += expf(lobes[kGLobeFloats * j + 4]);
-        if (u0 < c) {
-            k = j;
-            break;
+    bool found = false;
+#pragma unroll
+    for (int j = 0; j < kGLobes - 1; ++j) {
+        c += lobes[kGLobeFloats * j + 4];
+        if (!found && u0 < c) {
+            z0 = lobes[kGLobeFloats * j];
+            z1 = lobes[kGLobeFloats * j + 1];
+            z2 = lobes[kGLobeFloats * j + 2];
+            kap = lobes[kGLobeFloats * j + 3];
+            found = true;
         }
     }
-    const float* L = lobes + kGLobeFloats * k;
-    const float kap = L[3];
-    float ct = 1.0f + (float)(log((double)u1 + (1.0 - (double)u1) * exp(-2.0 * (double)kap)) / (double)kap);
+    float ct = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap;
     ct = fminf(fmaxf(ct, -1.0f), 1.0f);
-    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-    float ph = 2.0f * 3.14159265f * u2;
-    float cp = (float)cos((double)ph), sp = (float)sin((double)ph);
-    const float z0 = L[0], z1 = L[1], z2 = L[2];
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    const float cp = cospif(2.0f * u2), sp = sinpif(2.0f * u2);
     const float sign = copysignf(1.0f, z2);
     const float a = -1.0f / (sign + z2);
     const float b = z0 * z1 * a;

@@ -132,7 +132,7 @@ hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, 
                    const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
                    int trace_block, hipStream_t s);
 // Path-guiding network (guide.hip).
-hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, int blocks, hipStream_t s);
+hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, hipStream_t s);
 hipError_t guide_train_batch(const DevGuide& g, uint32_t first, uint32_t rows, float lr, uint32_t step,
                              hipStream_t s);
 hipError_t guide_refresh(const DevGuide& g, hipStream_t s);

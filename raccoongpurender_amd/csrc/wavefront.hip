@@ -406,6 +406,10 @@ __device__ __forceinline__ void spawn(DevState& s, uint32_t slot, F3 ro, F3 rd, 
 // Guided one-sample MIS: the guide is chosen with probability g.select_p, otherwise the
 // reference's BSDF / envmap choice is made with the same uniform rescaled.
 enum { kStratBxdf = 0, kStratEnv = 1, kStratGuide = 2 };
+// logic_i counts the path's vertices from the camera (1 at the first hit).
+__device__ __forceinline__ bool guided_vertex(const DevGuide& g, uint32_t logic_i) {
+    return g.max_depth == 0 || logic_i <= g.max_depth;
+}
 __device__ __forceinline__ int select_strategy(uint4& st, float bxdf_p, bool guided, float guide_p) {
     float u = xo_uniform(st);
     if (guided) {
@@ -415,35 +419,22 @@ __device__ __forceinline__ int select_strategy(uint4& st, float bxdf_p, bool gui
     return u <= bxdf_p ? kStratBxdf : kStratEnv;
 }
 
-__device__ __forceinline__ void load_lobes(const DevGuide& g, uint32_t slot, float* L) {
-    const float4* src = reinterpret_cast<const float4*>(g.lobes + (size_t)slot * kGSlotFloats);
-    for (int q = 0; q < kGSlotFloats / 4; ++q) {
-        float4 v = src[q];
-        L[4 * q] = v.x;
-        L[4 * q + 1] = v.y;
-        L[4 * q + 2] = v.z;
-        L[4 * q + 3] = v.w;
-    }
-}
-
 // Remember a guided vertex until its path ends (throughput before it, radiance after its emission).
 __device__ __forceinline__ void record_vertex(const DevGuide& g, uint32_t slot, F3 hit, F3 n, F3 wo, F3 wi,
                                               const PathRec& pr, F3 Le) {
     const uint32_t c = g.rec_count[slot];
     if (c >= (uint32_t)kGRecords) return;
-    GuideRecord r;
+    GuideVertex r;
     r.p[0] = (hit.x - g.lo[0]) * g.inv_ext[0];
     r.p[1] = (hit.y - g.lo[1]) * g.inv_ext[1];
     r.p[2] = (hit.z - g.lo[2]) * g.inv_ext[2];
     r.n[0] = n.x; r.n[1] = n.y; r.n[2] = n.z;
     r.wo[0] = wo.x; r.wo[1] = wo.y; r.wo[2] = wo.z;
     r.wi[0] = wi.x; r.wi[1] = wi.y; r.wi[2] = wi.z;
-    for (int k = 0; k < 3; ++k) {
-        r.T[k] = pr.T[k];
-        r.L[k] = pr.L[k] + pr.T[k] * (k == 0 ? Le.x : k == 1 ? Le.y : Le.z);
-    }
-    r.pad[0] = r.pad[1] = 0.0f;
-    g.records[(size_t)slot * kGRecords + c] = r;
+    const size_t at = (size_t)c * g.nslots + slot;
+    g.rec_geo[at] = r;
+    g.rec_T[at] = make_float4(pr.T[0], pr.T[1], pr.T[2], 0.0f);
+    g.rec_L[at] = make_float4(pr.L[0] + pr.T[0] * Le.x, pr.L[1] + pr.T[1] * Le.y, pr.L[2] + pr.T[2] * Le.z, 0.0f);
     g.rec_count[slot] = c + 1;
 }
 
@@ -465,13 +456,13 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         const Lambertian L = m.lambertian[m.mat[e.hit_surface].index];
         const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
         const F3 wo = neg3(rd);
-        const float gp = kGuided ? g.select_p : 0.0f;
+        const bool guide_here = kGuided && guided_vertex(g, s.path[slot].logic_i);
+        const float gp = guide_here ? g.select_p : 0.0f;
         const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
-        const int strat = kGuided ? select_strategy(st, p.bxdf_p, true, gp)
-                                  : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
+        const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
+                                     : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
         const bool bxdf = strat == kStratBxdf;
-        float lobes[kGuided ? kGSlotFloats : 1];
-        if (kGuided) load_lobes(g, slot, lobes);
+        const float* lobes = kGuided ? g.lobes + (size_t)slot * kGSlotFloats : nullptr;
         F3 wi;
         float pdf_e = 0.0f, pdf_b;
         if (strat == kStratEnv) wi = sample_env_6axis(env, flip_toward(Ng, wo), st, &pdf_e);
@@ -505,7 +496,7 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;
-        if (kGuided) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
+        if (guide_here) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
         float c = dot3(wi, Ngf);
         F3 T = F3{0.0f, 0.0f, 0.0f};
         if (0.0 < (double)c) {
@@ -514,7 +505,7 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
             T.z = L.R[2] / kPiF * c / pdf;
         }
         store_shade(s.shade, slot, load3(L.Le), T);
-        if (kGuided && g.train) record_vertex(g, slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot], load3(L.Le));
+        if (guide_here && g.train) record_vertex(g, slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot], load3(L.Le));
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
     }
@@ -537,13 +528,13 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevM
         const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
         const F3 wo = neg3(rd);
         const F3 Ngf = flip_toward(Ng, wo);
-        const float gp = kGuided ? g.select_p : 0.0f;
+        const bool guide_here = kGuided && guided_vertex(g, s.path[slot].logic_i);
+        const float gp = guide_here ? g.select_p : 0.0f;
         const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
-        const int strat = kGuided ? select_strategy(st, p.bxdf_p, true, gp)
-                                  : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
+        const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
+                                     : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
         const bool bxdf = strat == kStratBxdf;
-        float lobes[kGuided ? kGSlotFloats : 1];
-        if (kGuided) load_lobes(g, slot, lobes);
+        const float* lobes = kGuided ? g.lobes + (size_t)slot * kGSlotFloats : nullptr;
         F3 wi;
         float pdf_e = 0.0f;
         if (strat == kStratEnv) wi = sample_env_6axis(env, Ngf, st, &pdf_e);
@@ -563,7 +554,7 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevM
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;
-        if (kGuided) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
+        if (guide_here) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
         float c = dot3(wi, Ngf);
         F3 T = F3{0.0f, 0.0f, 0.0f};
         if (0.0f < c) {
@@ -584,7 +575,7 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevM
             T.z = k0.z * k1 * k2 * c / pdf;
         }
         store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, T);
-        if (kGuided && g.train)
+        if (guide_here && g.train)
             record_vertex(g, slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot], F3{0.0f, 0.0f, 0.0f});
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
@@ -806,9 +797,12 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, Trac
 // accumulation of finished paths, and the per-block queue counts of a stable compaction.
 __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMaterials m, DevEnvmap env, DevGuide g) {
     __shared__ uint32_t wcount[kNumQueues][4];
+    __shared__ uint32_t wsamples[4];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t q = 0xFFu;
+    uint32_t rec_mask = 0;  // training samples this slot emits (recorded vertices of a finished path)
+    float rec_w[kGRecords];
     if (slot < p.n) {
         PathRec pr = s.path[slot];
         const uint4 sh0 = reinterpret_cast<const uint4*>(s.shade + slot)[0];
@@ -848,25 +842,20 @@ __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMateri
             // a finished path turns its recorded guided vertices into training samples:
             // weight = luminance((L_final - L_after) / T_before) = F / p of the sampled direction
             const uint32_t nrec = accept ? g.rec_count[slot] : 0u;
-            for (uint32_t j = 0; j < nrec; ++j) {
-                const GuideRecord r = g.records[(size_t)slot * kGRecords + j];
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)kGRecords; ++j) {
+                if (j >= nrec) break;
+                const float4 rT = g.rec_T[(size_t)j * g.nslots + slot];
+                const float4 rL = g.rec_L[(size_t)j * g.nslots + slot];
+                const float Tr[3] = {rT.x, rT.y, rT.z}, Lr[3] = {rL.x, rL.y, rL.z};
                 float c[3];
                 const float Lf[3] = {L.x, L.y, L.z};
-                for (int k = 0; k < 3; ++k) c[k] = r.T[k] > 0.0f ? (Lf[k] - r.L[k]) / r.T[k] : 0.0f;
+                for (int k = 0; k < 3; ++k) c[k] = Tr[k] > 0.0f ? (Lf[k] - Lr[k]) / Tr[k] : 0.0f;
                 const float wgt = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
-                if (!(wgt > 0.0f) || !isfinite(wgt)) continue;
-                const uint32_t idx = atomicAdd(g.sample_count, 1u);
-                if (idx >= g.capacity) break;
-                GuideSample gs;
-                for (int k = 0; k < 3; ++k) {
-                    gs.p[k] = r.p[k];
-                    gs.n[k] = r.n[k];
-                    gs.wo[k] = r.wo[k];
-                    gs.wi[k] = r.wi[k];
+                if (wgt > 0.0f && isfinite(wgt)) {
+                    rec_w[j] = wgt;
+                    rec_mask |= 1u << j;
                 }
-                gs.weight = wgt;
-                gs.pad[0] = gs.pad[1] = gs.pad[2] = 0.0f;
-                g.samples[idx] = gs;
             }
             g.rec_count[slot] = 0;
         }
@@ -898,6 +887,19 @@ __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMateri
         else q = kMatVolumeInside;
         s.qsel[slot] = (uint8_t)q;
     }
+    if (g.train) {
+        // training samples leave through the queue scan: the per-block count goes to row
+        // kNumQueues of block_counts, the slot's sample mask to qsel bits 3..6 and the
+        // weights to emit_w; k_scatter_queues writes the samples in slot order
+        if (rec_mask != 0) {
+            g.emit_w[slot] = make_float4(rec_w[0], rec_w[1], rec_w[2], rec_w[3]);
+            s.qsel[slot] = (uint8_t)(q | (rec_mask << 3));
+        }
+        const uint32_t cnt = (uint32_t)__popc(rec_mask);
+        uint32_t total = 0;
+        for (int b = 0; b < 3; ++b) total += (uint32_t)__popcll(__ballot((cnt >> b) & 1u)) << b;
+        if (lane == 0) wsamples[wave] = total;
+    }
     for (int k = 0; k < kNumQueues; ++k) {
         unsigned long long b = __ballot(q == (uint32_t)k);
         if (lane == 0) wcount[k][wave] = (uint32_t)__popcll(b);
@@ -907,12 +909,18 @@ __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMateri
         const int k = threadIdx.x;
         s.block_counts[(size_t)k * s.nblocks + blockIdx.x] = wcount[k][0] + wcount[k][1] + wcount[k][2] + wcount[k][3];
     }
+    if (g.train && threadIdx.x == kNumQueues)
+        s.block_counts[(size_t)kNumQueues * s.nblocks + blockIdx.x] =
+            wsamples[0] + wsamples[1] + wsamples[2] + wsamples[3];
 }
 
-// Exclusive scan of the per-block counts of one queue (one workgroup per queue).
-__global__ void __launch_bounds__(1024) k_scan_queues(DevState s) {
+// Exclusive scan of the per-block counts of one queue (one workgroup per queue). Workgroup
+// kNumQueues scans the training-sample counts on top of the samples already buffered and
+// advances the buffer cursor (saturating at the buffer's capacity).
+__global__ void __launch_bounds__(1024) k_scan_queues(DevState s, uint32_t* sample_count, uint32_t sample_cap) {
     __shared__ uint32_t part[1024];
     const int q = blockIdx.x;
+    if (q == kNumQueues && !sample_count) return;
     uint32_t* c = s.block_counts + (size_t)q * s.nblocks;
     const uint32_t nb = s.nblocks;
     const uint32_t per = (nb + 1023) / 1024;
@@ -928,21 +936,30 @@ __global__ void __launch_bounds__(1024) k_scan_queues(DevState s) {
         part[threadIdx.x] += v;
         __syncthreads();
     }
-    uint32_t run = part[threadIdx.x] - sum;  // exclusive prefix of this thread's chunk
+    const uint32_t base = q == kNumQueues ? *sample_count : 0u;
+    uint32_t run = base + part[threadIdx.x] - sum;  // exclusive prefix of this thread's chunk
     for (uint32_t i = beg; i < end; ++i) {
         uint32_t v = c[i];
         c[i] = run;
         run += v;
     }
-    if (threadIdx.x == 1023) s.counts[q] = part[1023];
+    __syncthreads();
+    if (threadIdx.x == 1023) {
+        if (q == kNumQueues) *sample_count = min(base + part[1023], sample_cap);
+        else s.counts[q] = part[1023];
+    }
 }
 
-// Scatter slots into their queues in slot order.
-__global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p) {
+// Scatter slots into their queues in slot order; with training on, also write the finished
+// paths' training samples (slot order) into the guide's buffer.
+__global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p, DevGuide g) {
     __shared__ uint32_t wpre[kNumQueues][4];
+    __shared__ uint32_t wsam[4];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t q = slot < p.n ? (uint32_t)s.qsel[slot] : 0xFFu;
+    const uint32_t qs = slot < p.n ? (uint32_t)s.qsel[slot] : 0xFFu;
+    const uint32_t q = qs == 0xFFu ? 0xFFu : (qs & 7u);
+    const uint32_t mask = (qs == 0xFFu || !g.train) ? 0u : (qs >> 3);
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t rank = 0;
     for (int k = 0; k < kNumQueues; ++k) {
@@ -950,11 +967,42 @@ __global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p) 
         if (lane == 0) wpre[k][wave] = (uint32_t)__popcll(b);
         if (q == (uint32_t)k) rank = (uint32_t)__popcll(b & lt);
     }
+    uint32_t srank = 0;
+    if (g.train) {
+        const uint32_t cnt = (uint32_t)__popc(mask);
+        uint32_t total = 0;
+        for (int b = 0; b < 3; ++b) {
+            const unsigned long long bb = __ballot((cnt >> b) & 1u);
+            srank += (uint32_t)__popcll(bb & lt) << b;
+            total += (uint32_t)__popcll(bb) << b;
+        }
+        if (lane == 0) wsam[wave] = total;
+    }
     __syncthreads();
     if (q < (uint32_t)kNumQueues) {
         uint32_t off = s.block_counts[(size_t)q * s.nblocks + blockIdx.x];
         for (uint32_t w = 0; w < wave; ++w) off += wpre[q][w];
         s.items[(size_t)q * p.n + off + rank] = slot;
+    }
+    if (mask != 0) {
+        uint32_t idx = s.block_counts[(size_t)kNumQueues * s.nblocks + blockIdx.x] + srank;
+        for (uint32_t w = 0; w < wave; ++w) idx += wsam[w];
+        const float4 wts = g.emit_w[slot];
+        const float wj[kGRecords] = {wts.x, wts.y, wts.z, wts.w};
+#pragma unroll
+        for (int j = 0; j < kGRecords; ++j) {
+            if (!((mask >> j) & 1u)) continue;
+            if (idx < g.capacity) {
+                // 48-byte vertex -> 64-byte sample (p n wo wi | weight), 16-byte moves
+                const float4* r = reinterpret_cast<const float4*>(g.rec_geo + (size_t)j * g.nslots + slot);
+                float4* o = reinterpret_cast<float4*>(g.samples + idx);
+                o[0] = r[0];
+                o[1] = r[1];
+                o[2] = r[2];
+                o[3] = make_float4(wj[j], 0.0f, 0.0f, 0.0f);
+            }
+            ++idx;
+        }
     }
 }
 
@@ -1040,7 +1088,7 @@ hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, 
     const dim3 tgrid((p.n + tb - 1) / tb), tblk(tb);
     hipLaunchKernelGGL(k_new_path, qgrid, blk, 0, s, st, p, cam, env, g);
     if (g.enabled) {
-        hipError_t e = guide_infer(st, p, g, persistent_blocks > 0 ? 2 * persistent_blocks : 2048, s);
+        hipError_t e = guide_infer(st, p, g, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_shade_lambertian<true>, qgrid, blk, 0, s, st, p, m, env, g);
         hipLaunchKernelGGL(k_shade_ward<true>, qgrid, blk, 0, s, st, p, m, env, g);
@@ -1057,8 +1105,9 @@ hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, 
     else
         hipLaunchKernelGGL(k_trace_step<kVariantStackless>, tgrid, tblk, 0, s, st, p, sc, m);
     hipLaunchKernelGGL(k_logic, full, blk, 0, s, st, p, m, env, g);
-    hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues), dim3(1024), 0, s, st);
-    hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p);
+    hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues + 1), dim3(1024), 0, s, st,
+                       g.train ? g.sample_count : (uint32_t*)nullptr, g.capacity);
+    hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p, g);
     return hipGetLastError();
 }
 

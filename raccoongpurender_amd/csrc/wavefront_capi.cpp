@@ -349,7 +349,7 @@ int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_mat
     s.qsel = (uint8_t*)A(n);
     s.items = (uint32_t*)A(n * kNumQueues * sizeof(uint32_t));
     s.counts = (uint32_t*)A(kNumQueues * sizeof(uint32_t));
-    s.block_counts = (uint32_t*)A((size_t)s.nblocks * kNumQueues * sizeof(uint32_t));
+    s.block_counts = (uint32_t*)A((size_t)s.nblocks * (kNumQueues + 1) * sizeof(uint32_t));
     s.next_pixel = (unsigned long long*)A(sizeof(unsigned long long));
     w->stat_total = (unsigned long long*)A(sizeof(unsigned long long));
     if (rc != RGPU_OK) {
@@ -654,6 +654,7 @@ int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** 
     DevGuide& d = g->dev;
     d.capacity = cfg->capacity ? cfg->capacity : (1u << 18);
     d.select_p = cfg->select_p;
+    d.max_depth = cfg->max_depth;
     for (int k = 0; k < 3; ++k) {
         float ext = cfg->scene_hi[k] - cfg->scene_lo[k];
         d.lo[k] = cfg->scene_lo[k];
@@ -670,6 +671,10 @@ int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** 
     d.w_bf = (uint16_t*)A(kGParams * 2);
     d.wt_bf = (uint16_t*)A(kGParams * 2);
     d.grad = (float*)A(kGParams * 4);
+    d.partials = (float*)A((size_t)kGTrainGroups * kGParams * 4);
+    g->minibatch = std::min(g->minibatch, d.capacity);
+    d.act_rows = (g->minibatch + 31u) / 32u * 32u;
+    d.act = (uint16_t*)A((size_t)d.act_rows * kGActBytesPerRow);
     d.adam_m = (float*)A(kGParams * 4);
     d.adam_v = (float*)A(kGParams * 4);
     d.samples = (GuideSample*)A((size_t)d.capacity * sizeof(GuideSample));
@@ -747,7 +752,7 @@ int rgpu_guide_forward(rgpu_guide* g, const float* samples, uint32_t rows, float
 
 // Gradient of the mean guiding loss over `rows` samples (they replace the training buffer).
 int rgpu_guide_gradients(rgpu_guide* g, const float* samples, uint32_t rows, float* grad) {
-    if (!g || !samples || !grad || rows == 0 || rows > g->dev.capacity) return RGPU_ERR_INVALID_ARG;
+    if (!g || !samples || !grad || rows == 0 || rows > g->dev.act_rows) return RGPU_ERR_INVALID_ARG;
     rgpu_ctx* ctx = g->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipMemcpy(g->dev.samples, samples, (size_t)rows * 64, hipMemcpyHostToDevice));
@@ -791,16 +796,28 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
         int rc;
         if ((rc = wf_alloc(w, n * kGSlotFloats * 4, &p)) != RGPU_OK) return rc;
         d.lobes = (float*)p;
-        if ((rc = wf_alloc(w, n * sizeof(GuideRecord) * kGRecords, &p)) != RGPU_OK) return rc;
-        d.records = (GuideRecord*)p;
+        if ((rc = wf_alloc(w, n * sizeof(GuideVertex) * kGRecords, &p)) != RGPU_OK) return rc;
+        d.rec_geo = (GuideVertex*)p;
+        if ((rc = wf_alloc(w, n * sizeof(float4) * kGRecords, &p)) != RGPU_OK) return rc;
+        d.rec_T = (float4*)p;
+        if ((rc = wf_alloc(w, n * sizeof(float4) * kGRecords, &p)) != RGPU_OK) return rc;
+        d.rec_L = (float4*)p;
+        if ((rc = wf_alloc(w, n * sizeof(float4), &p)) != RGPU_OK) return rc;
+        d.emit_w = (float4*)p;
         if ((rc = wf_alloc(w, n * 4, &p)) != RGPU_OK) return rc;
         d.rec_count = (uint32_t*)p;
         RGPU_HIP(ctx, hipMemset(d.rec_count, 0, n * 4));
     } else {
         d.lobes = w->guide.lobes;
-        d.records = w->guide.records;
+        d.rec_geo = w->guide.rec_geo;
+        d.rec_T = w->guide.rec_T;
+        d.rec_L = w->guide.rec_L;
+        d.emit_w = w->guide.emit_w;
         d.rec_count = w->guide.rec_count;
     }
+    d.nslots = (uint32_t)n;
+    // four 29 KB-LDS inference workgroups per CU, each looping over tiles
+    d.infer_blocks = (uint32_t)(ctx->cu_count > 0 ? 4 * ctx->cu_count : 1024);
     d.enabled = sampling ? 1u : 0u;
     d.train = training ? 1u : 0u;
     w->guide = d;

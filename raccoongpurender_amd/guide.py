@@ -17,7 +17,7 @@ LOBES = 8
 class rgpu_guide_config(C.Structure):
     _fields_ = [("abi_version", u32), ("select_p", C.c_float), ("capacity", u32), ("minibatch", u32),
                 ("learning_rate", C.c_float), ("seed", u32), ("scene_lo", C.c_float * 3),
-                ("scene_hi", C.c_float * 3)]
+                ("scene_hi", C.c_float * 3), ("max_depth", u32)]
 
 
 _sig = _lib._sig
@@ -39,10 +39,10 @@ PARAM_COUNT = _lib.lib.rgpu_guide_param_count()
 
 class Guide:
     def __init__(self, ctx, scene_lo=(0, 0, 0), scene_hi=(1, 1, 1), select_p=0.5, capacity=1 << 18,
-                 minibatch=16384, lr=1e-3, seed=1):
+                 minibatch=16384, lr=1e-3, seed=1, max_depth=0):
         self.ctx = ctx
         cfg = rgpu_guide_config(RGPU_ABI_VERSION, select_p, capacity, minibatch, lr, seed,
-                                (C.c_float * 3)(*scene_lo), (C.c_float * 3)(*scene_hi))
+                                (C.c_float * 3)(*scene_lo), (C.c_float * 3)(*scene_hi), max_depth)
         h = vp()
         ctx.check(_lib.lib.rgpu_guide_create(ctx.h, C.byref(cfg), C.byref(h)), "rgpu_guide_create")
         self.h = h
