@@ -98,9 +98,26 @@ def render_bench(steps=32, warmup=16, paths=1 << 22):
     return out
 
 
+def guiding_bench(width=480, height=270, seconds=4.0):
+    """Equal-time relMSE of guided against plain sampling on the full-render scene at a reduced
+    resolution, scored against a 16k-spp plain render (all unclamped)."""
+    from raccoongpurender_amd import _lib, guiding_study, render_workloads
+
+    sd = render_workloads.full_render_scene(width, height)
+    ctx = _lib.Context(0)
+    out = guiding_study.equal_time_relmse(ctx, sd, seconds=seconds)
+    out["workload"] = f"{width}x{height}, 2017294 tris, same scene as the render line; relMSE vs 16k-spp plain"
+    print(f"[bench] guiding: relMSE plain {out['relmse_plain']:.4f} guided {out['relmse_guided']:.4f} "
+          f"(trimmed 1%: {out['relmse_trim1pct_plain']:.4f} / {out['relmse_trim1pct_guided']:.4f})",
+          file=sys.stderr, flush=True)
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--no-render", action="store_true", help="skip the full-render samples/s measurement")
+    ap.add_argument("--no-guiding", action="store_true", help="skip the equal-time guided/plain relMSE study")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -209,6 +226,8 @@ def main():
     if not args.no_render:
         scene.close()
         out["extra"]["render"] = render_bench()
+    if not args.no_guiding:
+        out["extra"]["guiding"] = guiding_bench()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(bvh.export(), d["points"], d["indices"], d["incoherent"])

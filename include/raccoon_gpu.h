@@ -298,6 +298,41 @@ int rgpu_wavefront_get_launch(const rgpu_wavefront* w, int* variant, uint32_t* t
 int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32_t warmup, uint32_t steps,
                             double* best_ms_per_step);
 
+/* ================================================================ batches of render jobs
+ * Several render jobs (own scene, materials, envmap, camera, resolution, samples per pixel and
+ * seed) in one persistent wavefront, generalising the reference's single pixel cursor
+ * next_pixel_index (kernels/new_path.cl:41-43) to one cursor per job. Job j owns
+ * ceil(path_count / 256) blocks of 256 consecutive slots; its queue items stay one slot-ordered
+ * run in every stage queue, so each job's slots evolve exactly as a single render of that job
+ * with the same path_count (rounded up to 256), seed, scene and camera. A job with spp > 0
+ * starts width * height * spp samples (every pixel spp times), then parks its slots. */
+typedef struct rgpu_batch_job {
+    const rgpu_scene* scene;
+    const rgpu_materials* materials;
+    const rgpu_envmap* envmap;
+    rgpu_camera camera;
+    uint32_t path_count;    /* slots of this job; rounded up to a multiple of 256 */
+    uint32_t spp;           /* samples per pixel to start; 0 = unlimited */
+    uint64_t seed;          /* RNG seed, as rgpu_wavefront_config.seed of a single render */
+} rgpu_batch_job;
+
+/* Creates and initialises the batch (cfg->path_count and cfg->seed are ignored; RGPU_WF_ACCUM64
+ * is not supported). step, time_steps, queue_counts (totals), read_state, graph_capture/launch,
+ * set_launch and autotune (camera may be NULL) apply to a batch; initialize, the single-render
+ * readers and guiding do not. The scenes, materials and envmaps must outlive it. */
+int rgpu_wavefront_create_batch(rgpu_ctx* ctx, const rgpu_batch_job* jobs, uint32_t njobs,
+                                const rgpu_wavefront_config* cfg, rgpu_wavefront** out);
+int rgpu_wavefront_job_count(const rgpu_wavefront* w, uint32_t* njobs);
+/* A job's queue lengths after the last step and the samples it has started. Synchronous. */
+int rgpu_wavefront_job_queue_counts(rgpu_wavefront* w, uint32_t job, uint32_t* counts7, uint64_t* samples_started);
+/* A job's per-pixel radiance sums (3 floats) and sample counts; either may be NULL. Synchronous. */
+int rgpu_wavefront_job_accumulation(rgpu_wavefront* w, uint32_t job, float* rgb_sums, uint32_t* counts);
+int rgpu_wavefront_job_stat(rgpu_wavefront* w, uint32_t job, uint64_t* total_samples);
+int rgpu_wavefront_job_tonemap(rgpu_wavefront* w, uint32_t job, uint8_t* rgba8);
+/* Steps (check_every at a time) until every job has finished its samples or max_steps have
+ * run; every job needs spp > 0. */
+int rgpu_wavefront_run_batch(rgpu_wavefront* w, uint32_t max_steps, uint32_t check_every, uint32_t* steps_done);
+
 /* ================================================================ path guiding
  * No counterpart in the reference (it has no path guiding). A small MLP maps a vertex (12-band
  * frequency-encoded position in the scene box, normal, outgoing direction) to an 8-lobe von

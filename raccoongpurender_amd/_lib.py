@@ -388,6 +388,24 @@ _sig("rgpu_wavefront_get_launch", C.c_int, vp, C.POINTER(C.c_int), C.POINTER(C.c
 _sig("rgpu_wavefront_autotune", C.c_int, vp, C.POINTER(rgpu_camera), u32, u32, C.POINTER(C.c_double))
 EXPORTED += ["rgpu_wavefront_graph_capture", "rgpu_wavefront_graph_launch", "rgpu_wavefront_set_launch",
              "rgpu_wavefront_get_launch", "rgpu_wavefront_autotune"]
+
+
+class rgpu_batch_job(C.Structure):
+    _fields_ = [("scene", vp), ("materials", vp), ("envmap", vp), ("camera", rgpu_camera), ("path_count", u32),
+                ("spp", u32), ("seed", C.c_uint64)]
+
+
+_sig("rgpu_wavefront_create_batch", C.c_int, vp, C.POINTER(rgpu_batch_job), u32, C.POINTER(rgpu_wavefront_config),
+     C.POINTER(vp))
+_sig("rgpu_wavefront_job_count", C.c_int, vp, C.POINTER(u32))
+_sig("rgpu_wavefront_job_queue_counts", C.c_int, vp, u32, vp, C.POINTER(C.c_uint64))
+_sig("rgpu_wavefront_job_accumulation", C.c_int, vp, u32, vp, vp)
+_sig("rgpu_wavefront_job_stat", C.c_int, vp, u32, C.POINTER(C.c_uint64))
+_sig("rgpu_wavefront_job_tonemap", C.c_int, vp, u32, vp)
+_sig("rgpu_wavefront_run_batch", C.c_int, vp, u32, u32, C.POINTER(u32))
+EXPORTED += ["rgpu_wavefront_create_batch", "rgpu_wavefront_job_count", "rgpu_wavefront_job_queue_counts",
+             "rgpu_wavefront_job_accumulation", "rgpu_wavefront_job_stat", "rgpu_wavefront_job_tonemap",
+             "rgpu_wavefront_run_batch"]
 EXPORTED += [
     "rgpu_materials_create", "rgpu_materials_destroy", "rgpu_envmap_create", "rgpu_envmap_destroy",
     "rgpu_envmap_export", "rgpu_camera_look_at", "rgpu_wavefront_create", "rgpu_wavefront_destroy",
@@ -598,8 +616,8 @@ class Wavefront:
 
     def autotune(self, camera, warmup=4, steps=8):
         ms = C.c_double(0)
-        self.ctx.check(lib.rgpu_wavefront_autotune(self.h, C.byref(camera), warmup, steps, C.byref(ms)),
-                       "rgpu_wavefront_autotune")
+        cam = C.byref(camera) if camera is not None else None
+        self.ctx.check(lib.rgpu_wavefront_autotune(self.h, cam, warmup, steps, C.byref(ms)), "rgpu_wavefront_autotune")
         self.camera = camera
         return ms.value
 
@@ -613,3 +631,72 @@ class Wavefront:
             self.close()
         except Exception:
             pass
+
+
+class BatchJob:
+    """One render job of a batch: device scene, materials, envmap, camera, slots, spp, seed."""
+
+    def __init__(self, scene, materials, envmap, camera, path_count, spp=0, seed=0):
+        self.scene, self.materials, self.envmap, self.camera = scene, materials, envmap, camera
+        self.path_count, self.spp, self.seed = int(path_count), int(spp), int(seed)
+        self.slots = (self.path_count + 255) // 256 * 256
+        self.npixels = camera.width * camera.height
+
+
+class Batch(Wavefront):
+    """Several render jobs in one persistent wavefront (rgpu_wavefront_create_batch)."""
+
+    def __init__(self, ctx, jobs, rr_start=0, max_depth=0, clamp=0.0, bxdf_p=0.0, flags=0, variant="stackless",
+                 queue_blocks=0):
+        self.ctx = ctx
+        self.jobs = list(jobs)
+        self.refs = tuple((j.scene, j.materials, j.envmap) for j in self.jobs)
+        self.n = sum(j.slots for j in self.jobs)
+        self.first_slot = np.cumsum([0] + [j.slots for j in self.jobs])[:-1]
+        arr = (rgpu_batch_job * len(self.jobs))()
+        for k, j in enumerate(self.jobs):
+            arr[k] = rgpu_batch_job(j.scene.h, j.materials.h, j.envmap.h, j.camera, j.path_count, j.spp, j.seed)
+        cfg = rgpu_wavefront_config(RGPU_ABI_VERSION, 0, 0, rr_start, max_depth, clamp, bxdf_p, flags,
+                                    VARIANTS[variant] if isinstance(variant, str) else variant, queue_blocks)
+        h = vp()
+        ctx.check(lib.rgpu_wavefront_create_batch(ctx.h, arr, len(self.jobs), C.byref(cfg), C.byref(h)),
+                  "rgpu_wavefront_create_batch")
+        self.h = h
+        self.npixels = 0
+        ctx.adopt(self)
+
+    def job_queue_counts(self, job):
+        c = np.zeros(7, np.uint32)
+        started = C.c_uint64(0)
+        self.ctx.check(lib.rgpu_wavefront_job_queue_counts(self.h, job, ptr(c), C.byref(started)),
+                       "rgpu_wavefront_job_queue_counts")
+        return c, started.value
+
+    def job_accumulation(self, job):
+        P = self.jobs[job].npixels
+        rgb = np.zeros((P, 3), np.float32)
+        cnt = np.zeros(P, np.uint32)
+        self.ctx.check(lib.rgpu_wavefront_job_accumulation(self.h, job, ptr(rgb), ptr(cnt)),
+                       "rgpu_wavefront_job_accumulation")
+        return rgb, cnt
+
+    def job_stat(self, job):
+        t = C.c_uint64(0)
+        self.ctx.check(lib.rgpu_wavefront_job_stat(self.h, job, C.byref(t)), "rgpu_wavefront_job_stat")
+        return t.value
+
+    def job_tonemap(self, job):
+        out = np.zeros((self.jobs[job].npixels, 4), np.uint8)
+        self.ctx.check(lib.rgpu_wavefront_job_tonemap(self.h, job, ptr(out)), "rgpu_wavefront_job_tonemap")
+        return out
+
+    def job_slots(self, job):
+        a = int(self.first_slot[job])
+        return slice(a, a + self.jobs[job].slots)
+
+    def run(self, max_steps=100000, check_every=16):
+        done = u32(0)
+        self.ctx.check(lib.rgpu_wavefront_run_batch(self.h, max_steps, check_every, C.byref(done)),
+                       "rgpu_wavefront_run_batch")
+        return done.value
+

@@ -124,13 +124,42 @@ struct DevState {
     uint32_t nblocks = 0;             // logic blocks (n / 256 rounded up)
 };
 
+// ---------------------------------------------------------------- batches of render jobs
+// A batch runs several render jobs in one wavefront: job j owns the slots of 256-slot blocks
+// [first_block, first_block + nblocks), so its queue items form one contiguous, slot-ordered
+// run in every queue and the per-block queue offsets locate that run. Each job has its own
+// scene, materials, envmap, camera, accumulation buffer, sample cursor and sample budget.
+constexpr uint32_t kJobBlock = 256;          // slot alignment of a job (the logic block)
+constexpr uint32_t kDeadPixel = 0xFFFFFFFFu;  // pixel of a slot whose job has started all its samples
+
+struct JobDesc {
+    TraceScene sc;
+    DevMaterials m;
+    DevEnvmap env;
+    Camera cam;
+    float* accum = nullptr;               // 4 floats per pixel of this job
+    unsigned long long next_pixel = 0;    // samples started (new_path cursor)
+    unsigned long long budget = 0;        // samples to start; 0 = unlimited
+    unsigned long long seed = 0;          // RNG seed of the job's first slot
+    uint32_t first_block = 0;
+    uint32_t nblocks = 0;
+};
+
+struct DevJobs {
+    JobDesc* jobs = nullptr;               // device table
+    const uint16_t* job_of_block = nullptr;
+    uint32_t njobs = 0;
+};
+
 hipError_t wf_initialize(const DevState& st, const WFParams& p, uint64_t seed_offset, uint64_t next_pixel,
                          uint32_t npixels, hipStream_t s);
+hipError_t wf_initialize_jobs(const DevState& st, const WFParams& p, const DevJobs& jobs, hipStream_t s);
 // One bounce. persistent_blocks: grid of the queue-driven kernels; trace_block: workgroup
-// size of the extension ray cast (64, 128 or 256).
+// size of the extension ray cast (64, 128 or 256). jobs: null for a single render (scene,
+// materials, envmap and camera as given), else the batch's job table (those are ignored).
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
                    const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
-                   int trace_block, hipStream_t s);
+                   int trace_block, hipStream_t s, const DevJobs* jobs = nullptr);
 // Path-guiding network (guide.hip).
 hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, hipStream_t s);
 hipError_t guide_train_batch(const DevGuide& g, uint32_t first, uint32_t rows, float lr, uint32_t step,

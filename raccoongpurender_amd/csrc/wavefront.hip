@@ -290,6 +290,9 @@ __device__ __forceinline__ void store_ray_rec(TraceRay* r, uint32_t s, F3 ro, fl
     q[0] = make_float4(ro.x, ro.y, ro.z, tmax);
     q[1] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(pix));
 }
+__device__ __forceinline__ uint32_t ray_pixel(const TraceRay* r, uint32_t s) {
+    return reinterpret_cast<const uint32_t*>(r + s)[7];
+}
 __device__ __forceinline__ void store_shade(ShadeRec* sh, uint32_t s, F3 Le, F3 T) {
     float4* q = reinterpret_cast<float4*>(sh + s);
     q[0] = make_float4(Le.x, Le.y, Le.z, T.x);
@@ -302,17 +305,57 @@ __device__ __forceinline__ bool select_bxdf(uint4& st, float sel_bxdf) {
     return u <= sel_bxdf;
 }
 
+// ---------------------------------------------------------------- where a slot's inputs come from
+// OneJob: a single render, its scene, materials, envmap and camera passed as kernel arguments
+// (the kernels compile to straight uses of those). ManyJobs: a batch, the slot's job found
+// through its 256-slot block and its inputs read from the job table.
+struct OneJob {
+    TraceScene sc;
+    DevMaterials m;
+    DevEnvmap env;
+    Camera cam;
+    static constexpr bool kBatch = false;
+    __device__ __forceinline__ uint32_t job(uint32_t) const { return 0; }
+    __device__ __forceinline__ TraceScene scene(uint32_t) const { return sc; }
+    __device__ __forceinline__ DevMaterials mats(uint32_t) const { return m; }
+    __device__ __forceinline__ DevEnvmap envmap(uint32_t) const { return env; }
+    __device__ __forceinline__ const Camera& camera(uint32_t) const { return cam; }
+    __device__ __forceinline__ float* accum(uint32_t, const DevState& s) const { return s.accum; }
+};
+
+struct ManyJobs {
+    DevJobs d;
+    static constexpr bool kBatch = true;
+    __device__ __forceinline__ uint32_t job(uint32_t slot) const { return d.job_of_block[slot / kJobBlock]; }
+    __device__ __forceinline__ TraceScene scene(uint32_t j) const { return d.jobs[j].sc; }
+    __device__ __forceinline__ DevMaterials mats(uint32_t j) const { return d.jobs[j].m; }
+    __device__ __forceinline__ DevEnvmap envmap(uint32_t j) const { return d.jobs[j].env; }
+    __device__ __forceinline__ const Camera& camera(uint32_t j) const { return d.jobs[j].cam; }
+    __device__ __forceinline__ float* accum(uint32_t j, const DevState&) const { return d.jobs[j].accum; }
+};
+
 // ---------------------------------------------------------------- kernels
-__global__ void __launch_bounds__(256) k_init(DevState s, WFParams p, unsigned long long seed_offset,
+// Start state of every slot. A single render seeds slot i with seed_offset + i; a batch
+// seeds the k-th slot of job j with that job's seed + k, as a single render of that job.
+template <class J>
+__global__ void __launch_bounds__(256) k_init(DevState s, WFParams p, J jobs, unsigned long long seed_offset,
                                               unsigned long long next_pixel) {
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
-        *s.next_pixel = next_pixel;
+        *s.next_pixel = next_pixel;  // a batch keeps its cursors in the job table (this stays 0)
         for (int q = 0; q < kNumQueues; ++q) s.counts[q] = q == 0 ? p.n : 0u;
     }
+    // every slot starts in the new-path queue: its per-block exclusive offsets
+    if (gid < s.nblocks) s.block_counts[gid] = gid * 256u;
     if (gid >= p.n) return;
     // random_initialize (kernels/peseudo_random.cl:48-64)
-    unsigned long long x = seed_offset + gid;
+    unsigned long long x;
+    if constexpr (J::kBatch) {
+        const JobDesc& jd = jobs.d.jobs[jobs.job(gid)];
+        x = jd.seed + (gid - jd.first_block * kJobBlock);
+    } else {
+        x = seed_offset + gid;
+    }
     unsigned long long r0 = splitmix64(&x);
     unsigned long long r1 = splitmix64(&x);
     uint4 st = make_uint4((uint32_t)r0, (uint32_t)(r0 >> 32), (uint32_t)r1, (uint32_t)(r1 >> 32));
@@ -348,14 +391,30 @@ __global__ void __launch_bounds__(256) k_clear_accum(DevState s, uint32_t npixel
 // selection and envmap draws the reference makes for a new path whose extension result
 // still carries a volume id from its previous life (kernels/mixture_density.cl:26-55,
 // kernels/envmap_sampling.cl:219-248).
-__global__ void __launch_bounds__(256) k_new_path(DevState s, WFParams p, Camera cam, DevEnvmap env, DevGuide g) {
+// In a batch the sample index is the job's cursor plus the item's rank among the job's new
+// paths (its queue position less the job's first position, the scanned offset of its first
+// block); a job past its sample budget parks the slot (pixel kDeadPixel, in no queue).
+template <class J>
+__global__ void __launch_bounds__(256) k_new_path(DevState s, WFParams p, J jobs, DevGuide g) {
     const uint32_t count = s.counts[0];
-    const unsigned long long base = *s.next_pixel;
-    const unsigned long long P = (unsigned long long)cam.width * cam.height;
+    const unsigned long long base0 = J::kBatch ? 0ull : *s.next_pixel;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t slot = s.items[i];
+        const uint32_t jb = jobs.job(slot);
+        const Camera& cam = jobs.camera(jb);
+        const DevEnvmap env = jobs.envmap(jb);
+        unsigned long long sample = base0 + i;
+        if constexpr (J::kBatch) {
+            const JobDesc& jd = jobs.d.jobs[jb];
+            sample = jd.next_pixel + (i - s.block_counts[jd.first_block]);
+            if (jd.budget != 0 && sample >= jd.budget) {
+                store_ray_rec(s.ray, slot, F3{0.0f, 0.0f, 0.0f}, -1.0f, F3{0.0f, 0.0f, 1.0f}, kDeadPixel);
+                continue;
+            }
+        }
+        const unsigned long long P = (unsigned long long)cam.width * cam.height;
         uint4 st = s.rng[slot];
-        uint32_t pixel = (uint32_t)((base + i) % P);
+        uint32_t pixel = (uint32_t)(sample % P);
         uint32_t x = pixel % cam.width, y = pixel / cam.width;
         F3 o = load3(cam.o), r = load3(cam.r), b = load3(cam.b);
         float jx = (float)x + xo_uniform(st);
@@ -440,13 +499,15 @@ __device__ __forceinline__ void record_vertex(const DevGuide& g, uint32_t slot, 
 
 // Lambertian: strategy selection + envmap 6-axis sample / cosine sample + pdfs +
 // lambertian_stage (mixture_density.cl, envmap_sampling.cl:169-217,250-266, lambertian.cl:34-123).
-template <bool kGuided>
-__global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p, DevMaterials m, DevEnvmap env,
-                                                          DevGuide g) {
+template <bool kGuided, class J>
+__global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p, J jobs, DevGuide g) {
     const uint32_t count = s.counts[kMatLambertian];
     const uint32_t* items = s.items + (size_t)kMatLambertian * p.n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t slot = items[i];
+        const uint32_t jb = jobs.job(slot);
+        const DevMaterials m = jobs.mats(jb);
+        const DevEnvmap env = jobs.envmap(jb);
         uint4 st = s.rng[slot];
         ExtRec e = s.ext[slot];
         F3 ro, rd;
@@ -512,12 +573,15 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
 }
 
 // Ward (mixture_density.cl, envmap_sampling.cl, ward.cl:76-186)
-template <bool kGuided>
-__global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, DevMaterials m, DevEnvmap env, DevGuide g) {
+template <bool kGuided, class J>
+__global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jobs, DevGuide g) {
     const uint32_t count = s.counts[kMatWard];
     const uint32_t* items = s.items + (size_t)kMatWard * p.n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t slot = items[i];
+        const uint32_t jb = jobs.job(slot);
+        const DevMaterials m = jobs.mats(jb);
+        const DevEnvmap env = jobs.envmap(jb);
         uint4 st = s.rng[slot];
         ExtRec e = s.ext[slot];
         F3 ro, rd;
@@ -604,11 +668,13 @@ __global__ void __launch_bounds__(256) k_shade_specular(DevState s, WFParams p) 
 }
 
 // Dielectric (kernels/delta_materials.cl:79-134)
-__global__ void __launch_bounds__(256) k_shade_dielectric(DevState s, WFParams p, DevMaterials m) {
+template <class J>
+__global__ void __launch_bounds__(256) k_shade_dielectric(DevState s, WFParams p, J jobs) {
     const uint32_t count = s.counts[kMatDielectric];
     const uint32_t* items = s.items + (size_t)kMatDielectric * p.n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t slot = items[i];
+        const DevMaterials m = jobs.mats(jobs.job(slot));
         uint4 st = s.rng[slot];
         ExtRec e = s.ext[slot];
         F3 ro, rd;
@@ -660,11 +726,13 @@ __device__ __forceinline__ uint32_t volume_hash(const Volume& v) {
 }
 
 // Medium boundary (kernels/homogeneous_volume.cl:19-64, InVolumeList ops types.cl:108-132)
-__global__ void __launch_bounds__(256) k_shade_volume_surface(DevState s, WFParams p, DevMaterials m) {
+template <class J>
+__global__ void __launch_bounds__(256) k_shade_volume_surface(DevState s, WFParams p, J jobs) {
     const uint32_t count = s.counts[kMatVolume];
     const uint32_t* items = s.items + (size_t)kMatVolume * p.n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t slot = items[i];
+        const DevMaterials m = jobs.mats(jobs.job(slot));
         uint4 st = s.rng[slot];
         ExtRec e = s.ext[slot];
         F3 ro, rd;
@@ -696,11 +764,15 @@ __global__ void __launch_bounds__(256) k_shade_volume_surface(DevState s, WFPara
 
 // Scattering inside a medium (mixture_density.cl:47-51, envmap_sampling.cl:219-248,268-281,
 // homogeneous_volume.cl:66-129)
-__global__ void __launch_bounds__(256) k_shade_volume_inside(DevState s, WFParams p, DevMaterials m, DevEnvmap env) {
+template <class J>
+__global__ void __launch_bounds__(256) k_shade_volume_inside(DevState s, WFParams p, J jobs) {
     const uint32_t count = s.counts[kMatVolumeInside];
     const uint32_t* items = s.items + (size_t)kMatVolumeInside * p.n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t slot = items[i];
+        const uint32_t jb = jobs.job(slot);
+        const DevMaterials m = jobs.mats(jb);
+        const DevEnvmap env = jobs.envmap(jb);
         uint4 st = s.rng[slot];
         ExtRec e = s.ext[slot];
         F3 ro, rd;
@@ -736,11 +808,24 @@ __global__ void __launch_bounds__(256) k_shade_volume_inside(DevState s, WFParam
 
 // homogeneous_volume_through (kernels/homogeneous_volume_through.cl:7-45) fused with
 // extension_ray_cast (kernels/extension_ray_cast_stackless.cl:202-246) over every slot.
-template <int kVariant>
-__global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, TraceScene sc, DevMaterials m) {
+template <int kVariant, class J>
+__global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jobs) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot == 0) *s.next_pixel += s.counts[0];  // finalize_new_path (kernels/new_path.cl:85-88)
+    // finalize_new_path (kernels/new_path.cl:85-88): advance the sample cursor(s) by the
+    // new paths started this bounce (a job's share: the span of its run in queue 0)
+    if constexpr (!J::kBatch) {
+        if (slot == 0) *s.next_pixel += s.counts[0];
+    } else if (slot < jobs.d.njobs) {
+        JobDesc& jd = jobs.d.jobs[slot];
+        const uint32_t lo = s.block_counts[jd.first_block];
+        const uint32_t hi = slot + 1 < jobs.d.njobs ? s.block_counts[jobs.d.jobs[slot + 1].first_block] : s.counts[0];
+        jd.next_pixel += hi - lo;
+    }
     if (slot >= p.n) return;
+    if (J::kBatch && ray_pixel(s.ray, slot) == kDeadPixel) return;
+    const uint32_t jb = jobs.job(slot);
+    const TraceScene sc = jobs.scene(jb);
+    const DevMaterials m = jobs.mats(jb);
     uint4 st = s.rng[slot];
     VolList v = s.vol[slot];
     int32_t hit_volume = -1;
@@ -795,7 +880,8 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, Trac
 
 // logic (kernels/logic.cl:117-280): path update, Russian roulette, envmap on a miss,
 // accumulation of finished paths, and the per-block queue counts of a stable compaction.
-__global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMaterials m, DevEnvmap env, DevGuide g) {
+template <class J>
+__global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, J jobs, DevGuide g) {
     __shared__ uint32_t wcount[kNumQueues][4];
     __shared__ uint32_t wsamples[4];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -803,7 +889,15 @@ __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMateri
     uint32_t q = 0xFFu;
     uint32_t rec_mask = 0;  // training samples this slot emits (recorded vertices of a finished path)
     float rec_w[kGRecords];
-    if (slot < p.n) {
+    bool live = slot < p.n;
+    if (J::kBatch && live && ray_pixel(s.ray, slot) == kDeadPixel) {
+        live = false;
+        s.qsel[slot] = 0xFFu;  // parked: in no queue
+    }
+    if (live) {
+        const uint32_t jb = jobs.job(slot);
+        const DevMaterials m = jobs.mats(jb);
+        const DevEnvmap env = jobs.envmap(jb);
         PathRec pr = s.path[slot];
         const uint4 sh0 = reinterpret_cast<const uint4*>(s.shade + slot)[0];
         const uint4 sh1 = reinterpret_cast<const uint4*>(s.shade + slot)[1];
@@ -866,7 +960,7 @@ __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, DevMateri
                 unsafeAtomicAdd(s.accum64 + 3 * (size_t)pixel + 2, (double)L.z);
                 atomicAdd(s.accum_count + pixel, 1u);
             } else {
-                float* a = s.accum + 4 * (size_t)pixel;
+                float* a = jobs.accum(jb, s) + 4 * (size_t)pixel;
                 unsafeAtomicAdd(a + 0, L.x);
                 unsafeAtomicAdd(a + 1, L.y);
                 unsafeAtomicAdd(a + 2, L.z);
@@ -1066,8 +1160,14 @@ hipError_t wf_initialize(const DevState& st, const WFParams& p, uint64_t seed_of
                          uint32_t npixels, hipStream_t s) {
     (void)npixels;
     dim3 grid((p.n + 255) / 256);
-    hipLaunchKernelGGL(k_init, grid, dim3(256), 0, s, st, p, (unsigned long long)seed_offset,
+    hipLaunchKernelGGL(k_init<OneJob>, grid, dim3(256), 0, s, st, p, OneJob{}, (unsigned long long)seed_offset,
                        (unsigned long long)next_pixel);
+    return hipGetLastError();
+}
+
+hipError_t wf_initialize_jobs(const DevState& st, const WFParams& p, const DevJobs& jobs, hipStream_t s) {
+    dim3 grid((p.n + 255) / 256);
+    hipLaunchKernelGGL(k_init<ManyJobs>, grid, dim3(256), 0, s, st, p, ManyJobs{jobs}, 0ull, 0ull);
     return hipGetLastError();
 }
 
@@ -1077,38 +1177,46 @@ hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hi
     return hipGetLastError();
 }
 
-hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, int blocks, hipStream_t s);
 
-hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
-                   const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
-                   int trace_block, hipStream_t s) {
+namespace {
+template <class J>
+hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const DevGuide& g, int variant,
+                     int persistent_blocks, int trace_block, hipStream_t s) {
     const dim3 qgrid(persistent_blocks > 0 ? persistent_blocks : 1024), blk(256);
     const dim3 full((p.n + 255) / 256);
     const uint32_t tb = (trace_block == 64 || trace_block == 128) ? (uint32_t)trace_block : 256u;
     const dim3 tgrid((p.n + tb - 1) / tb), tblk(tb);
-    hipLaunchKernelGGL(k_new_path, qgrid, blk, 0, s, st, p, cam, env, g);
+    hipLaunchKernelGGL(k_new_path<J>, qgrid, blk, 0, s, st, p, jobs, g);
     if (g.enabled) {
         hipError_t e = guide_infer(st, p, g, s);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_shade_lambertian<true>, qgrid, blk, 0, s, st, p, m, env, g);
-        hipLaunchKernelGGL(k_shade_ward<true>, qgrid, blk, 0, s, st, p, m, env, g);
+        hipLaunchKernelGGL((k_shade_lambertian<true, J>), qgrid, blk, 0, s, st, p, jobs, g);
+        hipLaunchKernelGGL((k_shade_ward<true, J>), qgrid, blk, 0, s, st, p, jobs, g);
     } else {
-        hipLaunchKernelGGL(k_shade_lambertian<false>, qgrid, blk, 0, s, st, p, m, env, g);
-        hipLaunchKernelGGL(k_shade_ward<false>, qgrid, blk, 0, s, st, p, m, env, g);
+        hipLaunchKernelGGL((k_shade_lambertian<false, J>), qgrid, blk, 0, s, st, p, jobs, g);
+        hipLaunchKernelGGL((k_shade_ward<false, J>), qgrid, blk, 0, s, st, p, jobs, g);
     }
     hipLaunchKernelGGL(k_shade_specular, qgrid, blk, 0, s, st, p);
-    hipLaunchKernelGGL(k_shade_dielectric, qgrid, blk, 0, s, st, p, m);
-    hipLaunchKernelGGL(k_shade_volume_surface, qgrid, blk, 0, s, st, p, m);
-    hipLaunchKernelGGL(k_shade_volume_inside, qgrid, blk, 0, s, st, p, m, env);
+    hipLaunchKernelGGL(k_shade_dielectric<J>, qgrid, blk, 0, s, st, p, jobs);
+    hipLaunchKernelGGL(k_shade_volume_surface<J>, qgrid, blk, 0, s, st, p, jobs);
+    hipLaunchKernelGGL(k_shade_volume_inside<J>, qgrid, blk, 0, s, st, p, jobs);
     if (variant == kVariantThreaded)
-        hipLaunchKernelGGL(k_trace_step<kVariantThreaded>, tgrid, tblk, 0, s, st, p, sc, m);
+        hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs);
     else
-        hipLaunchKernelGGL(k_trace_step<kVariantStackless>, tgrid, tblk, 0, s, st, p, sc, m);
-    hipLaunchKernelGGL(k_logic, full, blk, 0, s, st, p, m, env, g);
+        hipLaunchKernelGGL((k_trace_step<kVariantStackless, J>), tgrid, tblk, 0, s, st, p, jobs);
+    hipLaunchKernelGGL(k_logic<J>, full, blk, 0, s, st, p, jobs, g);
     hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues + 1), dim3(1024), 0, s, st,
                        g.train ? g.sample_count : (uint32_t*)nullptr, g.capacity);
     hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p, g);
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
+                   const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
+                   int trace_block, hipStream_t s, const DevJobs* jobs) {
+    if (jobs) return step_impl(st, p, ManyJobs{*jobs}, g, variant, persistent_blocks, trace_block, s);
+    return step_impl(st, p, OneJob{sc, m, env, cam}, g, variant, persistent_blocks, trace_block, s);
 }
 
 }  // namespace rgpu

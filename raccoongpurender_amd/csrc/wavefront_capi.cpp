@@ -53,6 +53,19 @@ struct rgpu_wavefront {
     uint32_t graph_steps = 0;
     DevGuide guide;                  // network + per-slot guiding buffers (guide.enabled == 0: plain)
     struct rgpu_guide* guide_obj = nullptr;
+    // batch of render jobs (rgpu_wavefront_create_batch); empty for a single render
+    struct Job {
+        const rgpu_scene* scene;
+        const rgpu_materials* mats;
+        const rgpu_envmap* env;
+        Camera cam;
+        float* accum;
+        uint32_t npixels, spp, first_block, nblocks;
+        uint64_t seed;
+    };
+    std::vector<Job> jobs;
+    DevJobs djobs;
+    bool is_batch() const { return !jobs.empty(); }
 };
 
 struct rgpu_guide {
@@ -98,6 +111,115 @@ int wf_alloc(rgpu_wavefront* w, size_t bytes, void** out) {
     RGPU_HIP(w->ctx, hipMalloc(out, bytes < 16 ? 16 : bytes));
     w->owned.push_back(*out);
     return RGPU_OK;
+}
+
+hipError_t step_once(rgpu_wavefront* w, hipStream_t s) {
+    if (w->is_batch())
+        return wf_step(w->st, w->params, TraceScene{}, DevMaterials{}, DevEnvmap{}, Camera{}, w->guide, w->variant,
+                       w->queue_blocks, w->trace_block, s, &w->djobs);
+    return wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->guide, w->variant,
+                   w->queue_blocks, w->trace_block, s);
+}
+
+Camera to_camera(const rgpu_camera* cam) {
+    Camera c{};
+    for (int k = 0; k < 3; ++k) {
+        c.eye[k] = cam->eye[k];
+        c.forward[k] = cam->forward[k];
+        c.up[k] = cam->up[k];
+        c.right[k] = cam->right[k];
+        c.o[k] = cam->imageplane_o[k];
+        c.r[k] = cam->imageplane_r[k];
+        c.b[k] = cam->imageplane_b[k];
+    }
+    c.width = cam->width;
+    c.height = cam->height;
+    c.lens_radius = cam->lens_radius;
+    return c;
+}
+
+// Options shared by single renders and batches.
+int parse_config(rgpu_ctx* ctx, const rgpu_wavefront_config* cfg, rgpu_wavefront* w) {
+    if (cfg->abi_version != RGPU_ABI_VERSION) return fail(ctx, RGPU_ERR_INVALID_ARG, "abi_version mismatch");
+    if (cfg->variant != RGPU_VARIANT_STACKLESS && cfg->variant != RGPU_VARIANT_THREADED)
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
+    WFParams& p = w->params;
+    p.rr_start = cfg->rr_start ? cfg->rr_start : 25u;
+    p.max_depth = cfg->max_depth;
+    p.accum64 = (cfg->flags & RGPU_WF_ACCUM64) ? 1u : 0u;
+    p.quirks = (cfg->flags & RGPU_WF_NO_REFERENCE_QUIRKS) ? 0u : 1u;
+    p.consistent = (cfg->flags & RGPU_WF_CONSISTENT_NORMALS) ? 1u : 0u;
+    p.clamp = cfg->radiance_clamp > 0.0f ? cfg->radiance_clamp : 1000.0f;
+    p.bxdf_p = cfg->bxdf_select_p > 0.0f ? cfg->bxdf_select_p : 0.5f;
+    w->seed = cfg->seed;
+    w->variant = (int)cfg->variant;
+    w->queue_blocks = cfg->queue_blocks ? (int)cfg->queue_blocks : (ctx->cu_count > 0 ? ctx->cu_count * 8 : 2048);
+    return RGPU_OK;
+}
+
+// Slot state for n path slots.
+int alloc_slots(rgpu_wavefront* w, size_t n) {
+    w->params.n = (uint32_t)n;
+    DevState& s = w->st;
+    s.nblocks = (uint32_t)((n + 255) / 256);
+    int rc = hipSetDevice(w->ctx->device) == hipSuccess ? RGPU_OK : RGPU_ERR_HIP;
+    void* ptr;
+    auto A = [&](size_t bytes) -> void* {
+        if (rc != RGPU_OK) return nullptr;
+        rc = wf_alloc(w, bytes, &ptr);
+        return rc == RGPU_OK ? ptr : nullptr;
+    };
+    s.ray = (TraceRay*)A(n * sizeof(TraceRay));
+    s.path = (PathRec*)A(n * sizeof(PathRec));
+    s.ext = (ExtRec*)A(n * sizeof(ExtRec));
+    s.shade = (ShadeRec*)A(n * sizeof(ShadeRec));
+    s.vol = (VolList*)A(n * sizeof(VolList));
+    s.rng = (uint4*)A(n * sizeof(uint4));
+    s.qsel = (uint8_t*)A(n);
+    s.items = (uint32_t*)A(n * kNumQueues * sizeof(uint32_t));
+    s.counts = (uint32_t*)A(kNumQueues * sizeof(uint32_t));
+    s.block_counts = (uint32_t*)A((size_t)s.nblocks * (kNumQueues + 1) * sizeof(uint32_t));
+    s.next_pixel = (unsigned long long*)A(sizeof(unsigned long long));
+    w->stat_total = (unsigned long long*)A(sizeof(unsigned long long));
+    return rc;
+}
+
+// (Re)start every job of a batch: cursors to zero, slots seeded, accumulation cleared.
+int restart_batch(rgpu_wavefront* w) {
+    rgpu_ctx* ctx = w->ctx;
+    std::vector<JobDesc> table(w->jobs.size());
+    for (size_t j = 0; j < w->jobs.size(); ++j) {
+        const rgpu_wavefront::Job& J = w->jobs[j];
+        JobDesc& d = table[j];
+        d.sc = J.scene->dev;
+        d.m = J.mats->dev;
+        d.env = J.env->dev;
+        d.cam = J.cam;
+        d.accum = J.accum;
+        d.next_pixel = 0;
+        d.budget = (uint64_t)J.npixels * J.spp;
+        d.seed = J.seed;
+        d.first_block = J.first_block;
+        d.nblocks = J.nblocks;
+    }
+    RGPU_HIP(ctx, hipMemcpy(w->djobs.jobs, table.data(), table.size() * sizeof(JobDesc), hipMemcpyHostToDevice));
+    RGPU_HIP(ctx, wf_initialize_jobs(w->st, w->params, w->djobs, ctx->stream));
+    for (const rgpu_wavefront::Job& J : w->jobs) {
+        DevState js = w->st;
+        js.accum = J.accum;
+        RGPU_HIP(ctx, wf_clear_accum(js, J.npixels, false, ctx->stream));
+    }
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RGPU_OK;
+}
+
+int restart(rgpu_wavefront* w, const rgpu_camera* camera) {
+    if (w->is_batch()) {
+        drop_graph(w);
+        return restart_batch(w);
+    }
+    if (!camera) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "a camera is needed to restart a single render");
+    return rgpu_wavefront_initialize(w, camera, 0, 1);
 }
 
 }  // namespace
@@ -305,11 +427,8 @@ int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_mat
     if (!ctx || !scene || !mats || !env || !cfg || !out) return RGPU_ERR_INVALID_ARG;
     *out = nullptr;
     if (!env->ctx) return fail(ctx, RGPU_ERR_INVALID_ARG, "envmap was built without a context (host tables only)");
-    if (cfg->abi_version != RGPU_ABI_VERSION) return fail(ctx, RGPU_ERR_INVALID_ARG, "abi_version mismatch");
     if (cfg->path_count == 0 || cfg->path_count > (1u << 28))
         return fail(ctx, RGPU_ERR_INVALID_ARG, "path_count must be in 1..2^28");
-    if (cfg->variant != RGPU_VARIANT_STACKLESS && cfg->variant != RGPU_VARIANT_THREADED)
-        return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
     if (mats->nprims != scene->nprims)
         return fail(ctx, RGPU_ERR_INVALID_ARG, "materials must give exactly one entry per primitive");
     rgpu_wavefront* w = new (std::nothrow) rgpu_wavefront();
@@ -318,40 +437,8 @@ int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_mat
     w->scene = scene;
     w->mats = mats;
     w->env = env;
-    WFParams& p = w->params;
-    p.n = cfg->path_count;
-    p.rr_start = cfg->rr_start ? cfg->rr_start : 25u;
-    p.max_depth = cfg->max_depth;
-    p.accum64 = (cfg->flags & RGPU_WF_ACCUM64) ? 1u : 0u;
-    p.quirks = (cfg->flags & RGPU_WF_NO_REFERENCE_QUIRKS) ? 0u : 1u;
-    p.consistent = (cfg->flags & RGPU_WF_CONSISTENT_NORMALS) ? 1u : 0u;
-    p.clamp = cfg->radiance_clamp > 0.0f ? cfg->radiance_clamp : 1000.0f;
-    p.bxdf_p = cfg->bxdf_select_p > 0.0f ? cfg->bxdf_select_p : 0.5f;
-    w->seed = cfg->seed;
-    w->variant = (int)cfg->variant;
-    w->queue_blocks = cfg->queue_blocks ? (int)cfg->queue_blocks : (ctx->cu_count > 0 ? ctx->cu_count * 8 : 2048);
-    const size_t n = p.n;
-    DevState& s = w->st;
-    s.nblocks = (uint32_t)((n + 255) / 256);
-    int rc = hipSetDevice(ctx->device) == hipSuccess ? RGPU_OK : RGPU_ERR_HIP;
-    void* ptr;
-    auto A = [&](size_t bytes) -> void* {
-        if (rc != RGPU_OK) return nullptr;
-        rc = wf_alloc(w, bytes, &ptr);
-        return rc == RGPU_OK ? ptr : nullptr;
-    };
-    s.ray = (TraceRay*)A(n * sizeof(TraceRay));
-    s.path = (PathRec*)A(n * sizeof(PathRec));
-    s.ext = (ExtRec*)A(n * sizeof(ExtRec));
-    s.shade = (ShadeRec*)A(n * sizeof(ShadeRec));
-    s.vol = (VolList*)A(n * sizeof(VolList));
-    s.rng = (uint4*)A(n * sizeof(uint4));
-    s.qsel = (uint8_t*)A(n);
-    s.items = (uint32_t*)A(n * kNumQueues * sizeof(uint32_t));
-    s.counts = (uint32_t*)A(kNumQueues * sizeof(uint32_t));
-    s.block_counts = (uint32_t*)A((size_t)s.nblocks * (kNumQueues + 1) * sizeof(uint32_t));
-    s.next_pixel = (unsigned long long*)A(sizeof(unsigned long long));
-    w->stat_total = (unsigned long long*)A(sizeof(unsigned long long));
+    int rc = parse_config(ctx, cfg, w);
+    if (rc == RGPU_OK) rc = alloc_slots(w, cfg->path_count);
     if (rc != RGPU_OK) {
         rgpu_wavefront_destroy(w);
         return rc;
@@ -360,9 +447,184 @@ int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_mat
     return RGPU_OK;
 }
 
+int rgpu_wavefront_create_batch(rgpu_ctx* ctx, const rgpu_batch_job* jobs, uint32_t njobs,
+                                const rgpu_wavefront_config* cfg, rgpu_wavefront** out) {
+    if (!ctx || !jobs || njobs == 0 || !cfg || !out) return RGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (njobs > 65535) return fail(ctx, RGPU_ERR_INVALID_ARG, "at most 65535 jobs in a batch");
+    if (cfg->flags & RGPU_WF_ACCUM64) return fail(ctx, RGPU_ERR_UNSUPPORTED, "batches accumulate in float");
+    uint64_t blocks = 0;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const rgpu_batch_job& b = jobs[j];
+        if (!b.scene || !b.materials || !b.envmap || b.path_count == 0 || b.camera.width == 0 || b.camera.height == 0)
+            return fail(ctx, RGPU_ERR_INVALID_ARG, "batch job needs a scene, materials, envmap, camera and slots");
+        if (!b.envmap->ctx) return fail(ctx, RGPU_ERR_INVALID_ARG, "envmap was built without a context");
+        if (b.materials->nprims != b.scene->nprims)
+            return fail(ctx, RGPU_ERR_INVALID_ARG, "materials must give exactly one entry per primitive");
+        if ((uint64_t)b.camera.width * b.camera.height >= (1ull << 31))
+            return fail(ctx, RGPU_ERR_INVALID_ARG, "job resolution too large");
+        blocks += (b.path_count + kJobBlock - 1) / kJobBlock;
+    }
+    if (blocks * kJobBlock > (1ull << 28)) return fail(ctx, RGPU_ERR_INVALID_ARG, "batch exceeds 2^28 slots");
+    rgpu_wavefront* w = new (std::nothrow) rgpu_wavefront();
+    if (!w) return RGPU_ERR_OUT_OF_MEMORY;
+    w->ctx = ctx;
+    int rc = parse_config(ctx, cfg, w);
+    if (rc == RGPU_OK) rc = alloc_slots(w, (size_t)blocks * kJobBlock);
+    std::vector<uint16_t> job_of_block(blocks);
+    uint32_t fb = 0;
+    for (uint32_t j = 0; j < njobs && rc == RGPU_OK; ++j) {
+        const rgpu_batch_job& b = jobs[j];
+        rgpu_wavefront::Job J{};
+        J.scene = b.scene;
+        J.mats = b.materials;
+        J.env = b.envmap;
+        J.cam = to_camera(&b.camera);
+        J.npixels = b.camera.width * b.camera.height;
+        J.spp = b.spp;
+        J.seed = b.seed;
+        J.first_block = fb;
+        J.nblocks = (b.path_count + kJobBlock - 1) / kJobBlock;
+        void* p = nullptr;
+        rc = wf_alloc(w, (size_t)J.npixels * 4 * sizeof(float), &p);
+        J.accum = (float*)p;
+        for (uint32_t k = 0; k < J.nblocks; ++k) job_of_block[fb + k] = (uint16_t)j;
+        fb += J.nblocks;
+        w->jobs.push_back(J);
+    }
+    void* p = nullptr;
+    if (rc == RGPU_OK) rc = wf_alloc(w, njobs * sizeof(JobDesc), &p);
+    w->djobs.jobs = (JobDesc*)p;
+    if (rc == RGPU_OK) rc = wf_alloc(w, blocks * sizeof(uint16_t), &p);
+    if (rc == RGPU_OK) {
+        w->djobs.job_of_block = (const uint16_t*)p;
+        w->djobs.njobs = njobs;
+        if (hipMemcpy(p, job_of_block.data(), blocks * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(ctx, RGPU_ERR_HIP, "upload of the job map failed");
+    }
+    if (rc == RGPU_OK) rc = restart_batch(w);
+    if (rc != RGPU_OK) {
+        rgpu_wavefront_destroy(w);
+        return rc;
+    }
+    w->initialized = true;
+    *out = w;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_job_count(const rgpu_wavefront* w, uint32_t* njobs) {
+    if (!w || !njobs) return RGPU_ERR_INVALID_ARG;
+    *njobs = (uint32_t)w->jobs.size();
+    return RGPU_OK;
+}
+
+// A job's queue run in queue q spans the scanned offsets of its first block and of the next
+// job's first block (the queue's total for the last job).
+int rgpu_wavefront_job_queue_counts(rgpu_wavefront* w, uint32_t job, uint32_t* counts7, uint64_t* samples_started) {
+    if (!w || !counts7 || job >= w->jobs.size()) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t nb = w->st.nblocks;
+    const bool last = job + 1 == w->jobs.size();
+    uint32_t totals[kNumQueues];
+    RGPU_HIP(ctx, hipMemcpy(totals, w->st.counts, sizeof(totals), hipMemcpyDeviceToHost));
+    for (int q = 0; q < kNumQueues; ++q) {
+        uint32_t lo = 0, hi = totals[q];
+        RGPU_HIP(ctx, hipMemcpy(&lo, w->st.block_counts + (size_t)q * nb + w->jobs[job].first_block, 4,
+                                hipMemcpyDeviceToHost));
+        if (!last)
+            RGPU_HIP(ctx, hipMemcpy(&hi, w->st.block_counts + (size_t)q * nb + w->jobs[job + 1].first_block, 4,
+                                    hipMemcpyDeviceToHost));
+        counts7[q] = hi - lo;
+    }
+    if (samples_started) {
+        unsigned long long c = 0;
+        RGPU_HIP(ctx, hipMemcpy(&c, &w->djobs.jobs[job].next_pixel, 8, hipMemcpyDeviceToHost));
+        const uint64_t budget = (uint64_t)w->jobs[job].npixels * w->jobs[job].spp;
+        *samples_started = budget != 0 && c > budget ? budget : c;
+    }
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_job_accumulation(rgpu_wavefront* w, uint32_t job, float* rgb, uint32_t* counts) {
+    if (!w || job >= w->jobs.size()) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t P = w->jobs[job].npixels;
+    std::vector<float> a(P * 4);
+    RGPU_HIP(ctx, hipMemcpy(a.data(), w->jobs[job].accum, P * 16, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; ++i) {
+        if (rgb) {
+            rgb[3 * i] = a[4 * i];
+            rgb[3 * i + 1] = a[4 * i + 1];
+            rgb[3 * i + 2] = a[4 * i + 2];
+        }
+        if (counts) std::memcpy(&counts[i], &a[4 * i + 3], 4);
+    }
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_job_stat(rgpu_wavefront* w, uint32_t job, uint64_t* total_samples) {
+    if (!w || job >= w->jobs.size() || !total_samples) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    DevState js = w->st;
+    js.accum = w->jobs[job].accum;
+    RGPU_HIP(ctx, hipMemsetAsync(w->stat_total, 0, 8, ctx->stream));
+    RGPU_HIP(ctx, wf_stat(js, w->jobs[job].npixels, false, w->stat_total, ctx->stream));
+    unsigned long long t = 0;
+    RGPU_HIP(ctx, hipMemcpyAsync(&t, w->stat_total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *total_samples = t;
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_job_tonemap(rgpu_wavefront* w, uint32_t job, uint8_t* rgba8) {
+    if (!w || job >= w->jobs.size() || !rgba8) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t P = w->jobs[job].npixels;
+    void* buf = nullptr;
+    RGPU_HIP(ctx, hipMalloc(&buf, (size_t)P * 4));
+    DevState js = w->st;
+    js.accum = w->jobs[job].accum;
+    hipError_t e = wf_tonemap(js, P, false, buf, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba8, buf, (size_t)P * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rgpu_wavefront_job_tonemap");
+    return RGPU_OK;
+}
+
+// Steps until no slot is live (every job has started and finished its samples) or
+// max_steps, checking every `check_every` steps.
+int rgpu_wavefront_run_batch(rgpu_wavefront* w, uint32_t max_steps, uint32_t check_every, uint32_t* steps_done) {
+    if (!w || !w->is_batch() || check_every == 0) return RGPU_ERR_INVALID_ARG;
+    for (int j = 0; j < (int)w->jobs.size(); ++j)
+        if (w->jobs[j].spp == 0) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "every job of a run needs an spp budget");
+    uint32_t done = 0;
+    while (done < max_steps) {
+        const uint32_t k = std::min(check_every, max_steps - done);
+        int rc = rgpu_wavefront_step(w, nullptr, k);
+        if (rc != RGPU_OK) return rc;
+        done += k;
+        uint32_t c[kNumQueues];
+        rc = rgpu_wavefront_queue_counts(w, c, nullptr);
+        if (rc != RGPU_OK) return rc;
+        uint64_t live = 0;
+        for (int q = 0; q < kNumQueues; ++q) live += c[q];
+        if (live == 0) break;
+    }
+    if (steps_done) *steps_done = done;
+    return RGPU_OK;
+}
+
 int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* cam, uint32_t lane_index, uint32_t num_lanes) {
     if (!w || !cam || num_lanes == 0 || lane_index >= num_lanes || cam->width == 0 || cam->height == 0)
         return RGPU_ERR_INVALID_ARG;
+    if (w->is_batch()) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "a batch is initialised at creation");
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -397,19 +659,7 @@ int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* cam, uint32_
         w->tonemap_buf = (uint8_t*)ptr;
         w->npixels = P;
     }
-    Camera& c = w->cam;
-    for (int k = 0; k < 3; ++k) {
-        c.eye[k] = cam->eye[k];
-        c.forward[k] = cam->forward[k];
-        c.up[k] = cam->up[k];
-        c.right[k] = cam->right[k];
-        c.o[k] = cam->imageplane_o[k];
-        c.r[k] = cam->imageplane_r[k];
-        c.b[k] = cam->imageplane_b[k];
-    }
-    c.width = cam->width;
-    c.height = cam->height;
-    c.lens_radius = cam->lens_radius;
+    w->cam = to_camera(cam);
     // WavefrontLane::initialize (common/wavefront_path_tracing.hpp:532-570): pixel offset
     // lane * (P / lanes), RNG offset lane * N, every slot in the new-path queue.
     const uint64_t next_pixel = (uint64_t)lane_index * (P / num_lanes);
@@ -427,9 +677,7 @@ int rgpu_wavefront_step(rgpu_wavefront* w, rgpu_stream stream, uint32_t steps) {
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = stream_of(ctx, stream);
-    for (uint32_t i = 0; i < steps; ++i)
-        RGPU_HIP(ctx, wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->guide,
-                              w->variant, w->queue_blocks, w->trace_block, s));
+    for (uint32_t i = 0; i < steps; ++i) RGPU_HIP(ctx, step_once(w, s));
     return RGPU_OK;
 }
 
@@ -443,9 +691,7 @@ int rgpu_wavefront_graph_capture(rgpu_wavefront* w, uint32_t steps) {
     hipStream_t cs;
     RGPU_HIP(ctx, hipStreamCreate(&cs));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    for (uint32_t i = 0; i < steps && e == hipSuccess; ++i)
-        e = wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->guide, w->variant,
-                    w->queue_blocks, w->trace_block, cs);
+    for (uint32_t i = 0; i < steps && e == hipSuccess; ++i) e = step_once(w, cs);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e == hipSuccess) e = e2;
@@ -496,7 +742,7 @@ int rgpu_wavefront_get_launch(const rgpu_wavefront* w, int* variant, uint32_t* t
 // before each trial) and keeps the fastest; the wavefront is left initialised with it.
 int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32_t warmup, uint32_t steps,
                             double* best_ms_per_step) {
-    if (!w || !camera || steps == 0) return RGPU_ERR_INVALID_ARG;
+    if (!w || (!camera && !w->is_batch()) || steps == 0) return RGPU_ERR_INVALID_ARG;
     const int cu = w->ctx->cu_count > 0 ? w->ctx->cu_count : 256;
     const int variants[2] = {RGPU_VARIANT_STACKLESS, RGPU_VARIANT_THREADED};
     const uint32_t blocks[3] = {64, 128, 256};
@@ -508,7 +754,7 @@ int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32
         for (uint32_t b : blocks)
             for (uint32_t q : qmul) {
                 int rc = rgpu_wavefront_set_launch(w, v, b, (uint32_t)cu * q);
-                if (rc == RGPU_OK) rc = rgpu_wavefront_initialize(w, camera, 0, 1);
+                if (rc == RGPU_OK) rc = restart(w, camera);
                 if (rc == RGPU_OK && warmup) rc = rgpu_wavefront_step(w, nullptr, warmup);
                 double ms = 0.0;
                 if (rc == RGPU_OK) rc = rgpu_wavefront_time_steps(w, steps, &ms);
@@ -521,7 +767,7 @@ int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32
                 }
             }
     int rc = rgpu_wavefront_set_launch(w, bv, bb, bq);
-    if (rc == RGPU_OK) rc = rgpu_wavefront_initialize(w, camera, 0, 1);
+    if (rc == RGPU_OK) rc = restart(w, camera);
     if (rc == RGPU_OK && best_ms_per_step) *best_ms_per_step = best;
     return rc;
 }
@@ -560,6 +806,7 @@ int rgpu_wavefront_queue_counts(rgpu_wavefront* w, uint32_t* counts7, uint64_t* 
 
 int rgpu_wavefront_read_accumulation(rgpu_wavefront* w, double* rgb, uint32_t* counts) {
     if (!w || !w->initialized) return RGPU_ERR_INVALID_ARG;
+    if (w->is_batch()) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "use the rgpu_wavefront_job_* readers of a batch");
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -585,6 +832,7 @@ int rgpu_wavefront_read_accumulation(rgpu_wavefront* w, double* rgb, uint32_t* c
 
 int rgpu_wavefront_tonemap(rgpu_wavefront* w, uint8_t* rgba8) {
     if (!w || !w->initialized || !rgba8) return RGPU_ERR_INVALID_ARG;
+    if (w->is_batch()) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "use the rgpu_wavefront_job_* readers of a batch");
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, wf_tonemap(w->st, w->npixels, w->params.accum64 != 0, w->tonemap_buf, ctx->stream));
@@ -595,6 +843,7 @@ int rgpu_wavefront_tonemap(rgpu_wavefront* w, uint8_t* rgba8) {
 
 int rgpu_wavefront_stat(rgpu_wavefront* w, uint64_t* total_samples) {
     if (!w || !w->initialized || !total_samples) return RGPU_ERR_INVALID_ARG;
+    if (w->is_batch()) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "use the rgpu_wavefront_job_* readers of a batch");
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipMemsetAsync(w->stat_total, 0, 8, ctx->stream));
@@ -780,6 +1029,7 @@ int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows)
 
 int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, int training) {
     if (!w) return RGPU_ERR_INVALID_ARG;
+    if (g && w->is_batch()) return fail(w->ctx, RGPU_ERR_UNSUPPORTED, "guiding a batch is not supported");
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));

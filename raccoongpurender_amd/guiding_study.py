@@ -24,6 +24,13 @@ def rel_mse(y, ref):
     return float(np.mean((y - ref) ** 2 / (ref ** 2 + 1e-2)))
 
 
+def rel_mse_trimmed(y, ref, drop=0.01):
+    """relMSE over the pixels left after dropping the `drop` fraction with the largest error
+    (caustic fireflies of the unclamped estimators)."""
+    e = np.sort((y - ref) ** 2 / (ref ** 2 + 1e-2))[::-1]
+    return float(e[int(e.size * drop):].mean())
+
+
 def render_reference(ctx, sd, spp, paths, seed=777):
     wf = build_lane(ctx, sd, paths, clamp=NO_CLAMP, flags=_lib.WF_ACCUM64, seed=seed)
     P = sd["camera"]["width"] * sd["camera"]["height"]
@@ -61,12 +68,17 @@ def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=
     return y, el, float(cnt.mean()), passes
 
 
-def equal_time_relmse(ctx, sd, seconds=2.0, ref_spp=16384, paths=1 << 20, seed=5):
-    ref, t_ref, spp_ref = render_reference(ctx, sd, ref_spp, paths)
+def equal_time_relmse(ctx, sd, seconds=4.0, ref_spp=16384, paths=1 << 20, seed=5, ref_paths=1 << 21):
+    """Plain and guided renders of `seconds` each (guided time includes inference and
+    training) scored against a `ref_spp` plain render."""
+    ref, t_ref, spp_ref = render_reference(ctx, sd, ref_spp, ref_paths)
     plain, t_p, spp_p, _ = render_timed(ctx, sd, seconds, paths, False, seed)
     guided, t_g, spp_g, passes = render_timed(ctx, sd, seconds, paths, True, seed)
-    out = {"relmse_plain": rel_mse(plain, ref), "relmse_guided": rel_mse(guided, ref),
+    out = {"relmse_plain": round(rel_mse(plain, ref), 4), "relmse_guided": round(rel_mse(guided, ref), 4),
+           "relmse_trim1pct_plain": round(rel_mse_trimmed(plain, ref), 4),
+           "relmse_trim1pct_guided": round(rel_mse_trimmed(guided, ref), 4),
            "seconds": seconds, "plain_spp": round(spp_p, 1), "guided_spp": round(spp_g, 1),
            "guided_training_passes": passes, "reference_spp": round(spp_ref, 1), "reference_seconds": round(t_ref, 1)}
-    out["ratio_plain_over_guided"] = out["relmse_plain"] / max(out["relmse_guided"], 1e-30)
+    out["ratio_plain_over_guided"] = round(out["relmse_plain"] / max(out["relmse_guided"], 1e-30), 3)
+    out["ratio_trim1pct"] = round(out["relmse_trim1pct_plain"] / max(out["relmse_trim1pct_guided"], 1e-30), 3)
     return out
