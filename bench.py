@@ -71,9 +71,11 @@ def cpu_baseline(tree, points, indices, rays, budget_s=10.0):
                       f"kernel on the reference 96-byte node layout, {threads} threads, {dt:.1f} s"}
 
 
-def render_bench(steps=32, warmup=16, paths=1 << 22):
-    """Full wavefront render at 1920x1080 (2,017,294 triangles, every material, a medium and a
-    2048x1024 sun envmap): samples/s and extension rays/s over `steps` bounces after `warmup`."""
+def render_bench(steps=32, warmup=16, paths=1 << 22, spp=64, graph_steps=8):
+    """Full wavefront render at 1920x1080, depth 8 (2,017,294 triangles, every material, a
+    medium and a 2048x1024 sun envmap): steady-state samples/s and extension rays/s over `steps`
+    bounces after `warmup`, then the whole 64-spp render (every pixel started exactly 64 times)
+    timed from the first bounce until its last path has ended."""
     from raccoongpurender_amd import _lib, render_workloads
     from raccoongpurender_amd.render import build_lane
 
@@ -88,13 +90,97 @@ def render_bench(steps=32, warmup=16, paths=1 << 22):
     ms = wf.time_steps(steps)
     s1 = wf.stat()
     sec = ms * steps * 1e-3
-    out = {"workload": "1920x1080, 2017294 tris, lambertian/ward/mirror/dielectric + medium + sun envmap",
+    out = {"workload": "1920x1080, depth 8, 2017294 tris, lambertian/ward/mirror/dielectric + medium + sun envmap",
            "paths": paths, "steps": steps, "ms_per_step": round(ms, 3),
            "samples_per_s": round((s1 - s0) / sec, 1), "ext_mrays_per_s": round(paths * steps / sec / 1e6, 1),
            "setup_s": round(setup, 1)}
     print(f"[bench] render: {ms:.3f} ms/step, {(s1 - s0) / sec / 1e6:.2f} Msamples/s, "
           f"{paths * steps / sec / 1e6:.0f} Mrays/s", file=sys.stderr, flush=True)
     wf.close()
+    # the complete 64-spp image: a one-job batch with a sample budget, bounce loop from a graph
+    bvh, scene, mats, env = wf._keep
+    c = sd["camera"]
+    cam = _lib.camera_look_at(c["eye"], c["target"], c["up"], c["fov"], c["width"], c["height"], c["lens_radius"])
+    job = _lib.BatchJob(scene, mats, env, cam, paths, spp=spp, seed=0)
+    full = _lib.Batch(ctx, [job], max_depth=sd["max_depth"])
+    full.graph_capture(graph_steps)
+    ctx.synchronize()
+    w0 = time.perf_counter()
+    launches = 0
+    while True:
+        full.graph_launch(1)
+        launches += 1
+        counts, _ = full.queue_counts()
+        if int(counts.sum()) == 0 or launches > 4096:
+            break
+    wall = time.perf_counter() - w0
+    done = full.job_stat(0)
+    out["full_render"] = {"spp": spp, "seconds": round(wall, 3), "bounces": launches * graph_steps,
+                          "samples": int(done), "samples_per_s": round(done / wall, 1),
+                          "finished": bool(int(counts.sum()) == 0)}
+    print(f"[bench] 1080p {spp} spp render: {wall:.3f} s, {done / wall / 1e6:.1f} M samples/s", file=sys.stderr,
+          flush=True)
+    full.close()
+    ctx.close()
+    return out
+
+
+def batch_bench(total_slots=1 << 22, graph_steps=8):
+    """256 render jobs over 8 variants of the full-render scene (92k-2.0M triangles), widths and
+    heights log-uniform in [64, 1024], spp from {1, 4, 16, 64, 256} (seed 99), in one persistent
+    wavefront: launch configuration autotuned, bounce loop replayed from a hipGraph, run until
+    every job has finished its samples."""
+    from raccoongpurender_amd import _lib, render_workloads as RW
+
+    t0 = time.perf_counter()
+    variants = RW.batch_scene_variants()
+    specs = RW.batch_jobs()
+    slots = RW.batch_slots(specs, total_slots)
+    ctx = _lib.Context(0)
+    dev = []
+    env = None
+    for v in variants:
+        sd = RW.full_render_scene(64, 64, seed=v["seed"], n_large=v["n_large"], n_mid=v["n_mid"])
+        bvh = _lib.BVH(sd["points"], sd["indices"])
+        if env is None:
+            env = _lib.Envmap(ctx, sd["env"])
+        dev.append(dict(bvh=bvh, scene=_lib.Scene(ctx, bvh), mats=_lib.Materials(ctx, sd["materials"]),
+                        cam=sd["camera"]))
+    jobs = []
+    for spec, n in zip(specs, slots):
+        d = dev[spec["variant"]]
+        c = d["cam"]
+        cam = _lib.camera_look_at(c["eye"], c["target"], c["up"], c["fov"], spec["width"], spec["height"],
+                                  c["lens_radius"])
+        jobs.append(_lib.BatchJob(d["scene"], d["mats"], env, cam, n, spp=spec["spp"], seed=spec["seed"]))
+    batch = _lib.Batch(ctx, jobs, max_depth=8)
+    setup = time.perf_counter() - t0
+    tune_ms = batch.autotune(None, warmup=2, steps=4)  # leaves the batch restarted with the best launch
+    variant, tb, qb = batch.get_launch()
+    batch.graph_capture(graph_steps)
+    ctx.synchronize()
+    w0 = time.perf_counter()
+    launches = 0
+    while True:
+        batch.graph_launch(1)
+        launches += 1
+        counts, _ = batch.queue_counts()
+        if int(counts.sum()) == 0 or launches > 100000 // graph_steps:
+            break
+    ctx.synchronize()
+    wall = time.perf_counter() - w0
+    samples = sum(batch.job_stat(k) for k in range(len(jobs)))
+    budget = sum(j.npixels * j.spp for j in jobs)
+    out = {"workload": "256 jobs, 8 scene variants (92k-2.0M tris), 64..1024 px per side, spp {1,4,16,64,256}, "
+                       "seed 99, one wavefront",
+           "jobs": len(jobs), "slots": batch.n, "samples": int(samples), "sample_budget": int(budget),
+           "bounces": launches * graph_steps, "seconds": round(wall, 3),
+           "samples_per_s": round(samples / wall, 1), "launch": [variant, tb, qb],
+           "autotune_ms_per_step": round(tune_ms, 3), "setup_s": round(setup, 1), "sampling": "plain"}
+    print(f"[bench] batch: {samples / 1e9:.3f} G samples in {wall:.2f} s ({samples / wall / 1e6:.1f} M samples/s), "
+          f"{launches * graph_steps} bounces, launch {variant}/{tb}/{qb}", file=sys.stderr, flush=True)
+    batch.close()
+    ctx.close()
     return out
 
 
@@ -118,6 +204,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--no-render", action="store_true", help="skip the full-render samples/s measurement")
     ap.add_argument("--no-guiding", action="store_true", help="skip the equal-time guided/plain relMSE study")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-job measurement")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched multi-job measurement")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -138,6 +226,9 @@ def main():
             ctx.copy(b, a, nb)
         ctx.synchronize()
         print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
+        return
+    if args.batch_only:
+        print(json.dumps(batch_bench()), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -228,6 +319,8 @@ def main():
         out["extra"]["render"] = render_bench()
     if not args.no_guiding:
         out["extra"]["guiding"] = guiding_bench()
+    if not args.no_batch:
+        out["extra"]["batch"] = batch_bench()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(bvh.export(), d["points"], d["indices"], d["incoherent"])

@@ -19,6 +19,8 @@ SCENES = {
 
 
 def build_lane(ctx, sd, path_count, **kw):
+    """Device scene + wavefront for a scene dict; the scene's max_depth is the default."""
+    kw.setdefault("max_depth", sd.get("max_depth", 0))
     bvh = _lib.BVH(sd["points"], sd["indices"])
     scene = _lib.Scene(ctx, bvh)
     mats = _lib.Materials(ctx, sd["materials"])
