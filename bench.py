@@ -256,14 +256,18 @@ def main():
 
     results = {}
     variants = ["stackless", "threaded"] if args.variant == "auto" else [args.variant]
+    # incoherent rays: the persistent kernel (block 0: waves refill idle lanes from their ray
+    # chunk); coherent primary rays: one thread per ray (block 256), which keeps neighbouring
+    # pixels in one wave for the whole traversal
+    blocks = {"incoherent": 0, "coherent": 256}
     for variant in variants:
         for name, rays in (("incoherent", rays_inc), ("coherent", rays_coh)):
             cnt = scene.extension_ray_cast_stats(variant, rays, hits, n)
             for _ in range(args.warmup):
-                scene.extension_ray_cast(variant, rays, hits, n)
+                scene.extension_ray_cast(variant, rays, hits, n, blocks[name])
             ctx.synchronize()
             w0 = time.perf_counter()
-            ms = scene.time_extension_ray_cast(variant, rays, hits, n, args.steps)
+            ms = scene.time_extension_ray_cast(variant, rays, hits, n, args.steps, blocks[name])
             ctx.synchronize()
             wall = (time.perf_counter() - w0) / args.steps * 1e3
             ab = algorithmic_bytes(variant, n, cnt)
@@ -274,7 +278,7 @@ def main():
 
     best = max(variants, key=lambda v: results[(v, "incoherent")]["mrays"])
     r = results[(best, "incoherent")]
-    kernel = f"k_trace_{best}"
+    kernel = f"k_trace_persistent_{best}"
     achieved = r["bytes"] / (r["ms"] * 1e-3) / 1e9
     traffic = read_traffic(f"{kernel}_c2_incoherent")
     out = {

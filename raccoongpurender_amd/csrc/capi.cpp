@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -338,8 +339,19 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
         return fail(ctx, RGPU_ERR_INVALID_ARG, "block must be a multiple of 64 up to 1024");
     if (block > 256) return fail(ctx, RGPU_ERR_INVALID_ARG, "traversal kernels are built for blocks <= 256");
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    TraceTuning tune;
+    tune.cu_count = (uint32_t)ctx->cu_count;
+    tune.waves_per_simd = ctx->trace_waves;
+    tune.refill = ctx->trace_refill;
     RGPU_HIP(ctx, launch_trace(scene->dev, variant, (const TraceRay*)rays, (TraceHit*)hits, count, block,
-                               stream_of(ctx, stream)));
+                               stream_of(ctx, stream), nullptr, &tune));
+    return RGPU_OK;
+}
+
+int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill) {
+    if (!ctx || waves_per_simd > 16 || refill > 64) return RGPU_ERR_INVALID_ARG;
+    ctx->trace_waves = waves_per_simd;
+    ctx->trace_refill = refill;
     return RGPU_OK;
 }
 

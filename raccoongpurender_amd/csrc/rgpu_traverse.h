@@ -36,9 +36,21 @@ struct TraceScene {
     int32_t sphere_begin = 0;
 };
 
+// Launch shape of the persistent ray-cast kernel: CUs to fill, resident waves per SIMD (grid =
+// cu_count * waves_per_simd workgroups of 256), and the idle-lane count that triggers a refill.
+// Zeros take the defaults (256 CUs, 8 waves; refill at 8 lanes for the stackless walk, 48 for
+// the threaded one).
+struct TraceTuning {
+    uint32_t cu_count = 0;
+    uint32_t waves_per_simd = 0;
+    uint32_t refill = 0;
+};
+
 // counters: null for the production kernels; otherwise 5 device u64 totals
 // {branch visits, leaf visits, parent steps, primitive tests, threaded node visits}.
+// block 0 without counters: the persistent kernel; 64/128/256: one thread per ray.
 hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays, TraceHit* hits, uint32_t n,
-                        uint32_t block, hipStream_t stream, unsigned long long* counters = nullptr);
+                        uint32_t block, hipStream_t stream, unsigned long long* counters = nullptr,
+                        const TraceTuning* tune = nullptr);
 
 }  // namespace rgpu

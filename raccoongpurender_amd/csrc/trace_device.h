@@ -213,4 +213,111 @@ __device__ __forceinline__ int32_t trace_threaded(const MTNode* __restrict__ tab
     return prim;
 }
 
+// ---------------------------------------------------------------- resumable traversal
+// The same two traversals cut into single steps over explicit per-lane state, so that a lane
+// whose ray has finished can take the next ray while the rest of its wave keeps going. Each
+// ray still visits nodes and tests primitives in exactly the order above.
+
+struct ThreadedRay {
+    const MTNode* tab;
+    F3 ro, rd, inv;
+    float tmin, u, v;
+    int32_t prim;
+    uint32_t node;  // kInvalid when finished
+};
+
+__device__ __forceinline__ void threaded_begin(ThreadedRay& r, const MTNode* tables, uint32_t nnodes, F3 ro, F3 rd,
+                                               float tmax) {
+    r.tab = tables + (size_t)direction_class(rd) * nnodes;
+    r.ro = ro;
+    r.rd = rd;
+    r.inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    r.tmin = tmax;
+    r.u = r.v = 0.0f;
+    r.prim = -1;
+    r.node = 0;
+}
+
+// One node of trace_threaded.
+__device__ __forceinline__ void threaded_step(ThreadedRay& r, const PrimRec* __restrict__ prims, int32_t sphere_begin) {
+    const float4* rec = reinterpret_cast<const float4*>(r.tab + r.node);
+    float4 q0 = rec[0], q1 = rec[1];
+    float lo[3] = {q0.x, q0.y, q0.z}, hi[3] = {q0.w, q1.x, q1.y};
+    uint32_t miss = __float_as_uint(q1.z), leaf = __float_as_uint(q1.w);
+    float h0, h1;
+    bool hit = slab_hit(lo, hi, r.ro, r.inv, r.tmin, &h0, &h1);
+    if (hit && leaf != 0) {
+        const uint32_t beg = leaf & 0x0FFFFFFFu;
+        const uint32_t end = beg + ((leaf >> 28) & 7u) + 1u;
+        for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
+    }
+    r.node = (hit && leaf == 0) ? r.node + 1 : miss;
+}
+
+struct StacklessRay {
+    F3 ro, rd, inv;
+    float tmin, u, v;
+    int32_t prim;
+    uint64_t bitstack;
+    uint32_t node;  // kInvalid when finished
+    uint4 q3;       // {a, b, parent, sibling} of node
+};
+
+__device__ __forceinline__ void stackless_begin(StacklessRay& r, const SLNode* __restrict__ nodes, F3 ro, F3 rd,
+                                                float tmax) {
+    r.ro = ro;
+    r.rd = rd;
+    r.inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    r.tmin = tmax;
+    r.u = r.v = 0.0f;
+    r.prim = -1;
+    r.bitstack = 0;
+    r.node = 0;
+    r.q3 = reinterpret_cast<const uint4*>(nodes)[3];
+}
+
+// One pass of the loop body of trace_stackless: a branch descent, or a leaf followed by its
+// backtrack to the next sibling (or the end of the traversal).
+__device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __restrict__ nodes,
+                                               const PrimRec* __restrict__ prims, int32_t sphere_begin) {
+    const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
+    if ((r.q3.x & kLeafFlag) == 0) {
+        const float4* rec = reinterpret_cast<const float4*>(nodes + r.node);
+        float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+        float lmin[3] = {q0.x, q0.y, q0.z}, lmax[3] = {q0.w, q1.x, q1.y};
+        float rmin[3] = {q1.z, q1.w, q2.x}, rmax[3] = {q2.y, q2.z, q2.w};
+        float Lh0, Lh1, Rh0, Rh1;
+        bool L = slab_hit(lmin, lmax, r.ro, r.inv, r.tmin, &Lh0, &Lh1);
+        bool R = slab_hit(rmin, rmax, r.ro, r.inv, r.tmin, &Rh0, &Rh1);
+        if (L || R) {
+            r.bitstack = r.bitstack << 1;
+            if (L && R) {
+                bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
+                r.node = nearL ? r.q3.x : r.q3.y;
+                r.bitstack = r.bitstack | 1;
+            } else {
+                r.node = L ? r.q3.x : r.q3.y;
+            }
+            r.q3 = q3p[4 * r.node + 3];
+            return;
+        }
+    } else {
+        const uint32_t beg = r.q3.x & ~kLeafFlag, end = r.q3.y;
+        for (uint32_t s = beg; s < end; ++s)
+            test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
+    }
+    while ((r.bitstack & 1) == 0) {
+        if (r.bitstack == 0) {
+            r.node = kInvalid;
+            return;
+        }
+        r.node = r.q3.z;
+        r.bitstack = r.bitstack >> 1;
+        if (r.bitstack != 0) r.q3 = q3p[4 * r.node + 3];
+    }
+    r.node = r.q3.w;
+    r.bitstack = r.bitstack ^ 1;
+    r.q3 = q3p[4 * r.node + 3];
+}
+
 }  // namespace rgpu

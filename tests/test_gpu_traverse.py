@@ -73,6 +73,26 @@ def test_variants_agree_and_block_sizes_identical(ctx):
                 assert H.compare_hits(g, base) == dict(prim=0, t=0, uv=0)
 
 
+def test_persistent_launch_shapes_match_per_ray_kernel(ctx):
+    # block 0 runs the persistent kernel (waves refilling idle lanes from their ray chunk);
+    # every shape must give the one-thread-per-ray kernel's bits
+    v, f = scenes.displaced_icosphere(level=5)
+    scene = _lib.Scene(ctx, _lib.BVH(v, f))
+    for rays in (scenes.incoherent_rays(30011, v.min(0), v.max(0), seed=23), scenes.primary_rays(64, 48),
+                 scenes.incoherent_rays(37, v.min(0), v.max(0), seed=24)):
+        for variant in ("stackless", "threaded"):
+            ref = _lib.trace(ctx, scene, rays, variant, block=256)
+            try:
+                for wps, refill in ((1, 1), (4, 16), (8, 64), (16, 8), (0, 0)):
+                    ctx.set_trace_tuning(wps, refill)
+                    g = _lib.trace(ctx, scene, rays, variant)
+                    assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (variant, wps, refill, len(rays))
+            finally:
+                ctx.set_trace_tuning(0, 0)
+    with pytest.raises(_lib.RgpuError):
+        ctx.set_trace_tuning(17, 0)
+
+
 def test_empty_and_single_triangle(ctx):
     v = np.zeros((0, 3), np.float32)
     f = np.zeros((0, 3), np.uint32)
