@@ -155,7 +155,7 @@ hipError_t wf_initialize(const DevState& st, const WFParams& p, uint64_t seed_of
                          uint32_t npixels, hipStream_t s);
 hipError_t wf_initialize_jobs(const DevState& st, const WFParams& p, const DevJobs& jobs, hipStream_t s);
 // One bounce. persistent_blocks: grid of the queue-driven kernels; trace_block: workgroup
-// size of the extension ray cast (64, 128 or 256). jobs: null for a single render (scene,
+// size of the extension ray cast (64, 128 or 256; 0 = persistent waves with lane refill). jobs: null for a single render (scene,
 // materials, envmap and camera as given), else the batch's job table (those are ignored).
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
                    const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,

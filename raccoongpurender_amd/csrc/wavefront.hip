@@ -20,6 +20,8 @@
 // except where accumulation order differs (pixel sums use float atomics).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <stdint.h>
 
 #include "guide_device.h"
@@ -806,76 +808,159 @@ __global__ void __launch_bounds__(256) k_shade_volume_inside(DevState s, WFParam
     }
 }
 
+// finalize_new_path (kernels/new_path.cl:85-88): advance the sample cursor(s) by the new paths
+// started this bounce (a job's share: the span of its run in queue 0). Thread gid of the grid.
+template <class J>
+__device__ __forceinline__ void advance_cursors(const DevState& s, const J& jobs, uint32_t gid) {
+    if constexpr (!J::kBatch) {
+        if (gid == 0) *s.next_pixel += s.counts[0];
+    } else if (gid < jobs.d.njobs) {
+        JobDesc& jd = jobs.d.jobs[gid];
+        const uint32_t lo = s.block_counts[jd.first_block];
+        const uint32_t hi = gid + 1 < jobs.d.njobs ? s.block_counts[jobs.d.jobs[gid + 1].first_block] : s.counts[0];
+        jd.next_pixel += hi - lo;
+    }
+}
+
+// homogeneous_volume_through: the medium the ray is in (drawn from the slot's list) and its
+// free-flight distance, which becomes the extension ray's far clip.
+__device__ __forceinline__ void volume_through(const DevState& s, const DevMaterials& m, uint32_t slot,
+                                               int32_t* hit_volume, float* tmin) {
+    uint4 st = s.rng[slot];
+    VolList v = s.vol[slot];
+    *hit_volume = -1;
+    if (0 < v.count) {
+        unsigned long long u = xo_uint62(st);
+        *hit_volume = v.volumes[u % (unsigned long long)v.count];
+    }
+    *tmin = kFltMax;
+    if (0 <= *hit_volume) {
+        const Volume vol = m.volume[m.mat[*hit_volume].index];
+        float u = xo_uniform(st);
+        *tmin = -1.0f / vol.C * flog(u);
+    }
+    s.rng[slot] = st;
+}
+
+// Extension result of a slot: the surface hit (with its geometric normal) or, failing that,
+// the medium event at the free-flight distance.
+__device__ __forceinline__ void store_extension(const DevState& s, const TraceScene& sc, uint32_t slot, F3 ro, F3 rd,
+                                                int32_t prim, float t, int32_t hit_volume, float tvol) {
+    ExtRec e = s.ext[slot];
+    e.hit_surface = -1;
+    e.hit_volume = hit_volume;
+    e.tmin = tvol;
+    if (prim >= 0) {
+        F3 n;
+        if (prim < sc.sphere_begin) {
+            const uint32_t* ix = sc.indices + 3 * (size_t)prim;
+            float4 a = sc.points[ix[0]], b = sc.points[ix[1]], c = sc.points[ix[2]];
+            F3 v0 = F3{a.x, a.y, a.z}, v1 = F3{b.x, b.y, b.z}, v2 = F3{c.x, c.y, c.z};
+            n = cross3(sub3(v2, v1), sub3(v1, v0));  // clockwise winding
+        } else {
+            float4 sp = sc.spheres[prim - sc.sphere_begin];
+            n = sub3(add3(ro, mul3(rd, t)), F3{sp.x, sp.y, sp.z});
+        }
+        n = normalize3(n);
+        e.hit_surface = prim;
+        e.hit_volume = -1;
+        e.tmin = t;
+        e.ngx = n.x;
+        e.ngy = n.y;
+        e.ngz = n.z;
+    }
+    s.ext[slot] = e;
+}
+
 // homogeneous_volume_through (kernels/homogeneous_volume_through.cl:7-45) fused with
-// extension_ray_cast (kernels/extension_ray_cast_stackless.cl:202-246) over every slot.
+// extension_ray_cast (kernels/extension_ray_cast_stackless.cl:202-246) over every slot, one
+// thread per slot.
 template <int kVariant, class J>
 __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jobs) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    // finalize_new_path (kernels/new_path.cl:85-88): advance the sample cursor(s) by the
-    // new paths started this bounce (a job's share: the span of its run in queue 0)
-    if constexpr (!J::kBatch) {
-        if (slot == 0) *s.next_pixel += s.counts[0];
-    } else if (slot < jobs.d.njobs) {
-        JobDesc& jd = jobs.d.jobs[slot];
-        const uint32_t lo = s.block_counts[jd.first_block];
-        const uint32_t hi = slot + 1 < jobs.d.njobs ? s.block_counts[jobs.d.jobs[slot + 1].first_block] : s.counts[0];
-        jd.next_pixel += hi - lo;
-    }
+    advance_cursors(s, jobs, slot);
     if (slot >= p.n) return;
     if (J::kBatch && ray_pixel(s.ray, slot) == kDeadPixel) return;
     const uint32_t jb = jobs.job(slot);
     const TraceScene sc = jobs.scene(jb);
-    const DevMaterials m = jobs.mats(jb);
-    uint4 st = s.rng[slot];
-    VolList v = s.vol[slot];
-    int32_t hit_volume = -1;
-    if (0 < v.count) {
-        unsigned long long u = xo_uint62(st);
-        hit_volume = v.volumes[u % (unsigned long long)v.count];
-    }
-    float tmin = kFltMax;
-    if (0 <= hit_volume) {
-        const Volume vol = m.volume[m.mat[hit_volume].index];
-        float u = xo_uniform(st);
-        tmin = -1.0f / vol.C * flog(u);
-    }
-    s.rng[slot] = st;
+    int32_t hit_volume;
+    float tmin;
+    volume_through(s, jobs.mats(jb), slot, &hit_volume, &tmin);
     F3 ro, rd;
     float tmax;
     uint32_t pixel;
     load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
-    ExtRec e = s.ext[slot];
-    e.hit_surface = -1;
-    e.hit_volume = hit_volume;
-    e.tmin = tmin;
+    float t = tmin, u = 0.0f, vv = 0.0f;
+    int32_t prim = -1;
     if (sc.nnodes != 0) {
-        float t = tmin, u, vv;
         Cnt cnt;
-        int32_t prim = kVariant == kVariantThreaded
-                           ? trace_threaded<false>(sc.mt_nodes, sc.nnodes, sc.prims, sc.sphere_begin, ro, rd, &t,
-                                                   &u, &vv, &cnt)
-                           : trace_stackless<false>(sc.sl_nodes, sc.prims, sc.sphere_begin, ro, rd, &t, &u, &vv, &cnt);
-        if (prim >= 0) {
-            F3 n;
-            if (prim < sc.sphere_begin) {
-                const uint32_t* ix = sc.indices + 3 * (size_t)prim;
-                float4 a = sc.points[ix[0]], b = sc.points[ix[1]], c = sc.points[ix[2]];
-                F3 v0 = F3{a.x, a.y, a.z}, v1 = F3{b.x, b.y, b.z}, v2 = F3{c.x, c.y, c.z};
-                n = cross3(sub3(v2, v1), sub3(v1, v0));  // clockwise winding
-            } else {
-                float4 sp = sc.spheres[prim - sc.sphere_begin];
-                n = sub3(add3(ro, mul3(rd, t)), F3{sp.x, sp.y, sp.z});
+        prim = kVariant == kVariantThreaded
+                   ? trace_threaded<false>(sc.mt_nodes, sc.nnodes, sc.prims, sc.sphere_begin, ro, rd, &t, &u, &vv, &cnt)
+                   : trace_stackless<false>(sc.sl_nodes, sc.prims, sc.sphere_begin, ro, rd, &t, &u, &vv, &cnt);
+    }
+    store_extension(s, sc, slot, ro, rd, prim, t, hit_volume, tmin);
+}
+
+// The same stage on persistent waves: wave w owns slots [w * chunk, (w + 1) * chunk) and hands
+// them to lanes as lanes finish (a refill when at least `refill` lanes are idle), so that lanes
+// do not wait for the longest traversal of their wave. Each slot's work and results are those
+// of k_trace_step.
+template <int kVariant, class J>
+__global__ void __launch_bounds__(256) k_trace_step_persistent(DevState s, WFParams p, J jobs, uint32_t chunk,
+                                                               uint32_t refill) {
+    advance_cursors(s, jobs, blockIdx.x * blockDim.x + threadIdx.x);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint64_t next = (uint64_t)wave * chunk;
+    const uint64_t end = next + chunk < (uint64_t)p.n ? next + chunk : (uint64_t)p.n;
+    using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
+    State st;
+    st.node = kInvalid;
+    uint32_t slot = 0;
+    int32_t hit_volume = -1;
+    float tvol = 0.0f;
+    TraceScene lsc = jobs.scene(0);  // the slot's scene (per lane in a batch)
+    for (;;) {
+        const bool idle = st.node == kInvalid;
+        const unsigned long long idle_mask = __ballot(idle);
+        const uint32_t nidle = (uint32_t)__popcll(idle_mask);
+        if (nidle >= refill || nidle == 64u) {
+            if (idle) {
+                const uint64_t r = next + (uint64_t)__popcll(idle_mask & below);
+                if (r < end && !(J::kBatch && ray_pixel(s.ray, (uint32_t)r) == kDeadPixel)) {
+                    slot = (uint32_t)r;
+                    const uint32_t jb = jobs.job(slot);
+                    if constexpr (J::kBatch) lsc = jobs.scene(jb);
+                    volume_through(s, jobs.mats(jb), slot, &hit_volume, &tvol);
+                    F3 ro, rd;
+                    float tmax;
+                    uint32_t pixel;
+                    load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+                    if (lsc.nnodes != 0) {
+                        if constexpr (kVariant == kVariantThreaded)
+                            threaded_begin(st, lsc.mt_nodes, lsc.nnodes, ro, rd, tvol);
+                        else
+                            stackless_begin(st, lsc.sl_nodes, ro, rd, tvol);
+                    } else {
+                        store_extension(s, lsc, slot, ro, rd, -1, tvol, hit_volume, tvol);
+                    }
+                }
             }
-            n = normalize3(n);
-            e.hit_surface = prim;
-            e.hit_volume = -1;
-            e.tmin = t;
-            e.ngx = n.x;
-            e.ngy = n.y;
-            e.ngz = n.z;
+            next += nidle;
+        }
+        if (__ballot(st.node != kInvalid) == 0) {
+            if (next >= end) break;
+            continue;  // every lane took a parked or empty-scene slot: hand out more
+        }
+        if (st.node != kInvalid) {
+            if constexpr (kVariant == kVariantThreaded)
+                threaded_step(st, lsc.prims, lsc.sphere_begin);
+            else
+                stackless_step(st, lsc.sl_nodes, lsc.prims, lsc.sphere_begin);
+            if (st.node == kInvalid) store_extension(s, lsc, slot, st.ro, st.rd, st.prim, st.tmin, hit_volume, tvol);
         }
     }
-    s.ext[slot] = e;
 }
 
 // logic (kernels/logic.cl:117-280): path update, Russian roulette, envmap on a miss,
@@ -1200,10 +1285,20 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
     hipLaunchKernelGGL(k_shade_dielectric<J>, qgrid, blk, 0, s, st, p, jobs);
     hipLaunchKernelGGL(k_shade_volume_surface<J>, qgrid, blk, 0, s, st, p, jobs);
     hipLaunchKernelGGL(k_shade_volume_inside<J>, qgrid, blk, 0, s, st, p, jobs);
-    if (variant == kVariantThreaded)
+    if (trace_block == 0) {
+        // persistent waves: 8 per SIMD over 256 CUs, refill as in the ray-cast kernels
+        const uint32_t blocks = 2048u, chunk = (p.n + blocks * 4u - 1) / (blocks * 4u);
+        if (variant == kVariantThreaded)
+            hipLaunchKernelGGL((k_trace_step_persistent<kVariantThreaded, J>), dim3(blocks), dim3(256), 0, s, st, p,
+                               jobs, chunk, 48u);
+        else
+            hipLaunchKernelGGL((k_trace_step_persistent<kVariantStackless, J>), dim3(blocks), dim3(256), 0, s, st, p,
+                               jobs, chunk, 8u);
+    } else if (variant == kVariantThreaded) {
         hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs);
-    else
+    } else {
         hipLaunchKernelGGL((k_trace_step<kVariantStackless, J>), tgrid, tblk, 0, s, st, p, jobs);
+    }
     hipLaunchKernelGGL(k_logic<J>, full, blk, 0, s, st, p, jobs, g);
     hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues + 1), dim3(1024), 0, s, st,
                        g.train ? g.sample_count : (uint32_t*)nullptr, g.capacity);

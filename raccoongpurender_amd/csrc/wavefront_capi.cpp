@@ -48,7 +48,7 @@ struct rgpu_wavefront {
     std::vector<void*> owned;
     unsigned long long* stat_total = nullptr;
     uint8_t* tonemap_buf = nullptr;
-    int trace_block = 256;
+    int trace_block = 256;  // 0 would select persistent extension-ray-cast waves
     hipGraphExec_t graph = nullptr;  // captured bounce loop
     uint32_t graph_steps = 0;
     DevGuide guide;                  // network + per-slot guiding buffers (guide.enabled == 0: plain)
@@ -719,8 +719,8 @@ int rgpu_wavefront_set_launch(rgpu_wavefront* w, int variant, uint32_t trace_blo
     if (!w) return RGPU_ERR_INVALID_ARG;
     if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED)
         return fail(w->ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
-    if (trace_block != 64 && trace_block != 128 && trace_block != 256)
-        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "trace_block must be 64, 128 or 256");
+    if (trace_block != 0 && trace_block != 64 && trace_block != 128 && trace_block != 256)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "trace_block must be 0 (persistent), 64, 128 or 256");
     if (queue_blocks == 0 || queue_blocks > 65536)
         return fail(w->ctx, RGPU_ERR_INVALID_ARG, "queue_blocks must be in 1..65536");
     drop_graph(w);
@@ -745,7 +745,7 @@ int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32
     if (!w || (!camera && !w->is_batch()) || steps == 0) return RGPU_ERR_INVALID_ARG;
     const int cu = w->ctx->cu_count > 0 ? w->ctx->cu_count : 256;
     const int variants[2] = {RGPU_VARIANT_STACKLESS, RGPU_VARIANT_THREADED};
-    const uint32_t blocks[3] = {64, 128, 256};
+    const uint32_t blocks[4] = {0, 64, 128, 256};
     const uint32_t qmul[4] = {2, 4, 8, 16};
     double best = 1e300;
     int bv = w->variant;
