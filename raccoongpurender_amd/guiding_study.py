@@ -82,3 +82,33 @@ def equal_time_relmse(ctx, sd, seconds=4.0, ref_spp=16384, paths=1 << 20, seed=5
     out["ratio_plain_over_guided"] = round(out["relmse_plain"] / max(out["relmse_guided"], 1e-30), 3)
     out["ratio_trim1pct"] = round(out["relmse_trim1pct_plain"] / max(out["relmse_trim1pct_guided"], 1e-30), 3)
     return out
+
+
+def main():
+    """Timed renders of the full-render scene, plain then guided (the profiling entry point),
+    or the whole equal-time study with --study."""
+    import argparse
+    import json
+
+    from .render_workloads import full_render_scene
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--width", type=int, default=480)
+    ap.add_argument("--height", type=int, default=270)
+    ap.add_argument("--seconds", type=float, default=1.0)
+    ap.add_argument("--study", action="store_true", help="equal-time relMSE against a 16k-spp plain render")
+    args = ap.parse_args()
+    sd = full_render_scene(args.width, args.height)
+    ctx = _lib.Context(0)
+    if args.study:
+        print(json.dumps(equal_time_relmse(ctx, sd, seconds=args.seconds)), flush=True)
+    else:
+        for guided in (False, True):
+            _, el, spp, passes = render_timed(ctx, sd, args.seconds, 1 << 20, guided, 5)
+            print(json.dumps(dict(guided=guided, seconds=round(el, 3), spp=round(spp, 1), training_passes=passes)),
+                  flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
