@@ -139,8 +139,10 @@ int rgpu_scene_bytes(const rgpu_scene* scene, uint64_t* out4);
  * Replaces the extension_ray_cast kernel and its launch
  * (kernels/extension_ray_cast_stackless.cl:202-246, kernels/extension_ray_cast.cl:96-132;
  * call site common/wavefront_path_tracing.hpp:832-844) for a flat ray batch:
- * closest hit in [0, ray.tmax] for `count` device-resident rays. block = 0 picks the
- * default workgroup size. */
+ * closest hit in [0, ray.tmax] for `count` device-resident rays. block = 0 runs persistent
+ * waves that refill idle lanes from per-wave ray chunks (best for incoherent rays); block =
+ * 64/128/256 runs one thread per ray in workgroups of that size (best for coherent rays). The
+ * results are bit-identical either way. */
 int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene* scene, int variant,
                             const rgpu_ray* rays, rgpu_hit* hits, uint32_t count, uint32_t block);
 /* Launch shape of the persistent ray-cast kernel used when block == 0: resident waves per

@@ -1,0 +1,410 @@
+// C++ host surface over the C ABI (include/raccoon_gpu.h), mirroring the reference's host
+// classes for the wavefront path (namespace rt):
+//   SceneManager            common/scene_manager.hpp:330-650 (geometry, materials, envmap,
+//                           buildBVH, createBuffer/createMaterialBuffer/createEnvmapBuffer)
+//   WavefrontLane           common/wavefront_path_tracing.hpp:339-1153 (setup, initialize, step,
+//                           update_intermediate, re_start, step_count, stat_avg_sample)
+//   WavefrontPathTracing    common/wavefront_path_tracing.hpp:1159-1470 (launch_fixed, re_start,
+//                           create_color_image, onColorRecieved) for one GPU
+//   StacklessBVH/ThreadedBVH  the two traversals of common/stackless_bvh.hpp and
+//                           common/threaded_bvh.hpp as GPU ray casters over one built tree
+// Scene ingestion takes arrays instead of Alembic objects (the reference's addPolymesh/addPoint
+// read houdini_alembic objects). Where the reference asserts and aborts (RAC_ASSERT,
+// common/raccoon_ocl.hpp:20-28) these classes throw rt::GpuError with the library's message.
+#pragma once
+
+#include <stdint.h>
+
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/raccoon_gpu.h"
+
+namespace rt {
+
+class GpuError : public std::runtime_error {
+public:
+    GpuError(int code, const std::string& what) : std::runtime_error(what), code(code) {}
+    int code;
+};
+
+inline void check(int rc, const rgpu_ctx* ctx, const char* what) {
+    if (rc == RGPU_OK) return;
+    std::string msg = what;
+    if (ctx) {
+        const char* detail = rgpu_last_error_string(ctx);
+        if (detail && *detail) msg += std::string(": ") + detail;
+    }
+    throw GpuError(rc, msg + " (status " + std::to_string(rc) + ")");
+}
+
+// One GPU: the OpenCLContext + OpenCLLane pair of common/raccoon_ocl.hpp for a single device.
+class GpuContext {
+public:
+    explicit GpuContext(int device_index = 0) {
+        rgpu_config cfg = {RGPU_ABI_VERSION, 0};
+        check(rgpu_create(device_index, &cfg, &h_), nullptr, "rgpu_create");
+    }
+    ~GpuContext() {
+        if (h_) rgpu_destroy(h_);
+    }
+    GpuContext(const GpuContext&) = delete;
+    GpuContext& operator=(const GpuContext&) = delete;
+    static int deviceCount() {
+        int n = 0;
+        return rgpu_device_count(&n) == RGPU_OK ? n : 0;
+    }
+    rgpu_ctx* get() const { return h_; }
+    void synchronize() const { check(rgpu_synchronize(h_, nullptr), h_, "rgpu_synchronize"); }
+
+private:
+    rgpu_ctx* h_ = nullptr;
+};
+
+// RAII handle for the library's opaque objects.
+template <class T, int (*Destroy)(T*)>
+class Handle {
+public:
+    Handle() = default;
+    explicit Handle(T* p) : p_(p) {}
+    ~Handle() { reset(); }
+    Handle(Handle&& o) noexcept : p_(o.p_) { o.p_ = nullptr; }
+    Handle& operator=(Handle&& o) noexcept {
+        if (this != &o) {
+            reset();
+            p_ = o.p_;
+            o.p_ = nullptr;
+        }
+        return *this;
+    }
+    Handle(const Handle&) = delete;
+    Handle& operator=(const Handle&) = delete;
+    void reset() {
+        if (p_) Destroy(p_);
+        p_ = nullptr;
+    }
+    T* get() const { return p_; }
+    explicit operator bool() const { return p_ != nullptr; }
+
+private:
+    T* p_ = nullptr;
+};
+
+using SceneBuffer = Handle<rgpu_scene, rgpu_scene_destroy>;
+using MaterialBuffer = Handle<rgpu_materials, rgpu_materials_destroy>;
+using EnvmapBuffer = Handle<rgpu_envmap, rgpu_envmap_destroy>;
+using BVHHandle = Handle<rgpu_bvh, rgpu_bvh_destroy>;
+using WavefrontHandle = Handle<rgpu_wavefront, rgpu_wavefront_destroy>;
+
+// 8-bit colour of the finalised image (RGBA8ValueType, wavefront_path_tracing.hpp:87-92).
+struct RGBA8ValueType {
+    uint8_t r, g, b, a;
+};
+
+// Camera setting of a render: the look-at inputs of standardCamera()
+// (wavefront_path_tracing.hpp:103-118); lens radius 0.0045 as kernels/new_path.cl:61.
+struct CameraSetting {
+    float eye[3] = {0.0f, 0.0f, 5.0f};
+    float target[3] = {0.0f, 0.0f, 0.0f};
+    float up[3] = {0.0f, 1.0f, 0.0f};
+    float fov_y_deg = 45.0f;
+    float focus_distance = 0.0f;
+    uint32_t width = 256, height = 256;
+    float lens_radius = 0.0045f;
+};
+
+inline rgpu_camera standardCamera(const CameraSetting& c) {
+    rgpu_camera cam;
+    check(rgpu_camera_look_at(c.eye, c.target, c.up, c.fov_y_deg, c.focus_distance, c.width, c.height,
+                              c.lens_radius, &cam),
+          nullptr, "rgpu_camera_look_at");
+    return cam;
+}
+
+// A material of the scene: its type (material_type.hpp) and index in the type's table.
+struct MaterialRef {
+    int32_t type = RGPU_MAT_LAMBERTIAN;
+    int32_t index = 0;
+};
+
+// SceneManager (common/scene_manager.hpp:330-650): triangles (primitives 0..ntris-1) then
+// spheres, one material per primitive, material tables per type, an optional envmap.
+class SceneManager {
+public:
+    MaterialRef addLambertian(const float Le[3], const float R[3], bool backEmission = false) {
+        rgpu_lambertian m = {{Le[0], Le[1], Le[2]}, {R[0], R[1], R[2]}, backEmission ? 1 : 0};
+        lambertians_.push_back(m);
+        return {RGPU_MAT_LAMBERTIAN, (int32_t)lambertians_.size() - 1};
+    }
+    MaterialRef addWard(float alpha_x, float alpha_y, const float reflectance[3], const float edgetint[3],
+                        float falloff) {
+        rgpu_ward w = {alpha_x, alpha_y, {reflectance[0], reflectance[1], reflectance[2]},
+                       {edgetint[0], edgetint[1], edgetint[2]}, falloff};
+        wards_.push_back(w);
+        return {RGPU_MAT_WARD, (int32_t)wards_.size() - 1};
+    }
+    MaterialRef addSpecular() { return {RGPU_MAT_SPECULAR, 0}; }
+    MaterialRef addDielectric(float ior = 1.5f) {
+        dielectrics_.push_back(rgpu_dielectric{ior});
+        return {RGPU_MAT_DIELECTRIC, (int32_t)dielectrics_.size() - 1};
+    }
+    // HomogeneousVolume boundary: C = extinction, R = scattering colour
+    MaterialRef addVolume(float C, const float R[3]) {
+        volumes_.push_back(rgpu_volume{C, {R[0], R[1], R[2]}});
+        return {RGPU_MAT_VOLUME, (int32_t)volumes_.size() - 1};
+    }
+
+    // addPolymesh (scene_manager.hpp:339-370): triangles over `points` (xyz), clockwise as seen
+    // from the front, all with material `m`.
+    void addPolymesh(const float* xyz, size_t npoints, const uint32_t* tri, size_t ntris, MaterialRef m) {
+        const uint32_t base = (uint32_t)(points_.size() / 3);
+        points_.insert(points_.end(), xyz, xyz + 3 * npoints);
+        for (size_t i = 0; i < 3 * ntris; ++i) {
+            if (tri[i] >= npoints) throw GpuError(RGPU_ERR_INVALID_ARG, "addPolymesh: index out of range");
+            indices_.push_back(base + tri[i]);
+        }
+        tri_mats_.insert(tri_mats_.end(), ntris, m);
+        bvh_.reset();
+    }
+    // Sphere primitives of addPointPrimitive (scene_manager.hpp:372-430).
+    void addSphere(const float center[3], float radius, MaterialRef m) {
+        spheres_.insert(spheres_.end(), {center[0], center[1], center[2], radius});
+        sphere_mats_.push_back(m);
+        bvh_.reset();
+    }
+    // ImageEnvmap of addPoint (scene_manager.hpp:432-497): rgb(a) floats, row 0 at the zenith.
+    void setEnvmap(const float* rgb, uint32_t channels, uint32_t width, uint32_t height) {
+        env_.assign(rgb, rgb + (size_t)channels * width * height);
+        env_channels_ = channels;
+        env_w_ = width;
+        env_h_ = height;
+    }
+
+    // buildBVH (scene_manager.hpp:517-577): binned SAH over triangles then spheres.
+    void buildBVH(uint32_t max_leaf = 4, uint32_t bins = 16, uint32_t max_depth = 63) {
+        rgpu_bvh_options opt = {RGPU_ABI_VERSION, max_leaf, bins, max_depth};
+        rgpu_bvh* b = nullptr;
+        check(rgpu_bvh_build(points_.data(), 3, (uint32_t)(points_.size() / 3), indices_.data(),
+                             (uint32_t)(indices_.size() / 3), spheres_.data(), (uint32_t)(spheres_.size() / 4), &opt,
+                             &b),
+              nullptr, "rgpu_bvh_build");
+        bvh_ = BVHHandle(b);
+    }
+    rgpu_bvh_stats bvhStats() const {
+        rgpu_bvh_stats st{};
+        check(bvh_ ? rgpu_bvh_get_stats(bvh_.get(), &st) : RGPU_ERR_INVALID_ARG, nullptr, "bvhStats (build first)");
+        return st;
+    }
+
+    // createBuffer / createMaterialBuffer / createEnvmapBuffer (scene_manager.hpp:579-650)
+    SceneBuffer createBuffer(GpuContext& ctx) const {
+        if (!bvh_) throw GpuError(RGPU_ERR_INVALID_ARG, "createBuffer: buildBVH() first");
+        rgpu_scene* s = nullptr;
+        check(rgpu_scene_create(ctx.get(), bvh_.get(), points_.data(), 3, (uint32_t)(points_.size() / 3),
+                                indices_.data(), (uint32_t)(indices_.size() / 3), spheres_.data(),
+                                (uint32_t)(spheres_.size() / 4), &s),
+              ctx.get(), "rgpu_scene_create");
+        return SceneBuffer(s);
+    }
+    MaterialBuffer createMaterialBuffer(GpuContext& ctx) const {
+        std::vector<int32_t> types, idx;
+        for (const std::vector<MaterialRef>* list : {&tri_mats_, &sphere_mats_})
+            for (const MaterialRef& m : *list) {
+                types.push_back(m.type);
+                idx.push_back(m.index);
+            }
+        rgpu_materials* m = nullptr;
+        check(rgpu_materials_create(ctx.get(), (uint32_t)types.size(), types.data(), idx.data(), lambertians_.data(),
+                                    (uint32_t)lambertians_.size(), wards_.data(), (uint32_t)wards_.size(),
+                                    dielectrics_.data(), (uint32_t)dielectrics_.size(), volumes_.data(),
+                                    (uint32_t)volumes_.size(), &m),
+              ctx.get(), "rgpu_materials_create");
+        return MaterialBuffer(m);
+    }
+    EnvmapBuffer createEnvmapBuffer(GpuContext& ctx) const {
+        static const float kBlack[3] = {0.0f, 0.0f, 0.0f};
+        const bool none = env_.empty();
+        rgpu_envmap* e = nullptr;
+        check(rgpu_envmap_create(ctx.get(), none ? kBlack : env_.data(), none ? 3 : env_channels_, none ? 1 : env_w_,
+                                 none ? 1 : env_h_, &e),
+              ctx.get(), "rgpu_envmap_create");
+        return EnvmapBuffer(e);
+    }
+
+    size_t triangleCount() const { return indices_.size() / 3; }
+    size_t sphereCount() const { return spheres_.size() / 4; }
+    const rgpu_bvh* bvh() const { return bvh_.get(); }
+
+private:
+    std::vector<float> points_;
+    std::vector<uint32_t> indices_;
+    std::vector<float> spheres_;
+    std::vector<MaterialRef> tri_mats_, sphere_mats_;
+    std::vector<rgpu_lambertian> lambertians_;
+    std::vector<rgpu_ward> wards_;
+    std::vector<rgpu_dielectric> dielectrics_;
+    std::vector<rgpu_volume> volumes_;
+    std::vector<float> env_;
+    uint32_t env_channels_ = 3, env_w_ = 0, env_h_ = 0;
+    BVHHandle bvh_;
+};
+
+// Options of a lane beyond the reference's (which hard-codes them).
+struct LaneOptions {
+    uint64_t seed = 0;
+    uint32_t max_depth = 0;        // 0 = unlimited, as the reference
+    float radiance_clamp = 0.0f;   // 0 = 1000, as the reference
+    uint32_t flags = 0;            // RGPU_WF_*
+    int variant = RGPU_VARIANT_STACKLESS;
+};
+
+// WavefrontLane (wavefront_path_tracing.hpp:339-1153) on one GPU.
+class WavefrontLane {
+public:
+    WavefrontLane(GpuContext& lane, int wavefrontPathCount, LaneOptions options = LaneOptions())
+        : lane_(lane), path_count_(wavefrontPathCount), options_(options) {}
+
+    // setup(camera, sceneManager) (wavefront_path_tracing.hpp:451-530): device buffers of the
+    // scene, materials and envmap, and the resident path state.
+    void setup(const rgpu_camera& camera, const SceneManager& sceneManager) {
+        wf_.reset();
+        scene_ = sceneManager.createBuffer(lane_);
+        materials_ = sceneManager.createMaterialBuffer(lane_);
+        envmap_ = sceneManager.createEnvmapBuffer(lane_);
+        rgpu_wavefront_config cfg = {RGPU_ABI_VERSION, (uint32_t)path_count_, options_.seed, 0, options_.max_depth,
+                                     options_.radiance_clamp, 0.0f, options_.flags, (uint32_t)options_.variant, 0};
+        rgpu_wavefront* w = nullptr;
+        check(rgpu_wavefront_create(lane_.get(), scene_.get(), materials_.get(), envmap_.get(), &cfg, &w), lane_.get(),
+              "rgpu_wavefront_create");
+        wf_ = WavefrontHandle(w);
+        camera_ = camera;
+    }
+    // initialize(lane_index, num_device) (wavefront_path_tracing.hpp:532-570)
+    void initialize(int lane_index, int num_device) {
+        check(rgpu_wavefront_initialize(wf_.get(), &camera_, (uint32_t)lane_index, (uint32_t)num_device),
+              lane_.get(), "rgpu_wavefront_initialize");
+        lane_index_ = lane_index;
+        num_device_ = num_device;
+        step_count_ = 0;
+    }
+    // step() (wavefront_path_tracing.hpp:591-939): one bounce of every resident path.
+    void step() {
+        check(rgpu_wavefront_step(wf_.get(), nullptr, 1), lane_.get(), "rgpu_wavefront_step");
+        ++step_count_;
+    }
+    // update_intermediate() + the finalize pass: mean radiance as RGBA8 (accumlation.cl:6-25,68-79)
+    std::vector<RGBA8ValueType> update_intermediate() {
+        std::vector<RGBA8ValueType> img((size_t)camera_.width * camera_.height);
+        check(rgpu_wavefront_tonemap(wf_.get(), reinterpret_cast<uint8_t*>(img.data())), lane_.get(),
+              "rgpu_wavefront_tonemap");
+        return img;
+    }
+    // re_start(lane_index, camera) (wavefront_path_tracing.hpp:1013-1018)
+    void re_start(int lane_index, const rgpu_camera& camera) {
+        camera_ = camera;
+        initialize(lane_index, num_device_);
+    }
+    int step_count() const { return step_count_; }
+    // stat_avg_sample(): samples per pixel so far (kernels/stat.cl)
+    float stat_avg_sample() const {
+        uint64_t total = 0;
+        check(rgpu_wavefront_stat(wf_.get(), &total), lane_.get(), "rgpu_wavefront_stat");
+        return (float)((double)total / ((double)camera_.width * camera_.height));
+    }
+    const rgpu_camera& camera() const { return camera_; }
+    rgpu_wavefront* handle() const { return wf_.get(); }
+
+private:
+    GpuContext& lane_;
+    int path_count_;
+    LaneOptions options_;
+    rgpu_camera camera_{};
+    int lane_index_ = 0, num_device_ = 1, step_count_ = 0;
+    SceneBuffer scene_;
+    MaterialBuffer materials_;
+    EnvmapBuffer envmap_;
+    WavefrontHandle wf_;
+};
+
+// WavefrontPathTracing (wavefront_path_tracing.hpp:1159-1470) on one GPU (RenderMode_SingleGPU).
+class WavefrontPathTracing {
+public:
+    WavefrontPathTracing(const SceneManager& sceneManager, const rgpu_camera& camera,
+                         std::function<void(RGBA8ValueType*, int, int)> colorRecieved, int wavefrontPathCount = 1 << 22,
+                         int device_index = 0, LaneOptions options = LaneOptions())
+        : onColorRecieved(std::move(colorRecieved)), context_(device_index) {
+        lane_.reset(new WavefrontLane(context_, wavefrontPathCount, options));
+        lane_->setup(camera, sceneManager);
+        lane_->initialize(0, 1);
+    }
+    // launch_fixed(steps) (wavefront_path_tracing.hpp:1443-1458)
+    void launch_fixed(int steps) {
+        for (int i = 0; i < steps; ++i) lane_->step();
+        create_color_image();
+    }
+    // re_start(camera) (wavefront_path_tracing.hpp:1460-1464)
+    void re_start(const rgpu_camera& camera) { lane_->re_start(0, camera); }
+    // create_color_image() (wavefront_path_tracing.hpp:1345-1408): with one lane there is
+    // nothing to merge; the finalised image goes to onColorRecieved.
+    void create_color_image() {
+        std::vector<RGBA8ValueType> img = lane_->update_intermediate();
+        if (onColorRecieved) onColorRecieved(img.data(), (int)lane_->camera().width, (int)lane_->camera().height);
+    }
+    WavefrontLane& lane() { return *lane_; }
+    GpuContext& context() { return context_; }
+
+    std::function<void(RGBA8ValueType*, int, int)> onColorRecieved;
+
+private:
+    GpuContext context_;
+    std::unique_ptr<WavefrontLane> lane_;
+};
+
+// A built tree cast against on the GPU with one of the two traversals: the stackless walk of
+// common/stackless_bvh.hpp (kernels/extension_ray_cast_stackless.cl) or the threaded walk of
+// common/threaded_bvh.hpp (kernels/extension_ray_cast.cl).
+class RayCaster {
+public:
+    RayCaster(GpuContext& ctx, const SceneManager& sceneManager, int variant)
+        : ctx_(ctx), scene_(sceneManager.createBuffer(ctx)), variant_(variant) {}
+    // Closest hits of host rays (prim -1 and t = tmax on a miss).
+    std::vector<rgpu_hit> intersect(const std::vector<rgpu_ray>& rays) const {
+        std::vector<rgpu_hit> hits(rays.size());
+        if (rays.empty()) return hits;
+        void* dr = nullptr;
+        void* dh = nullptr;
+        check(rgpu_alloc(ctx_.get(), rays.size() * sizeof(rgpu_ray), &dr), ctx_.get(), "rgpu_alloc");
+        int rc = rgpu_alloc(ctx_.get(), rays.size() * sizeof(rgpu_hit), &dh);
+        if (rc == RGPU_OK)
+            rc = rgpu_upload(ctx_.get(), nullptr, dr, rays.data(), rays.size() * sizeof(rgpu_ray));
+        if (rc == RGPU_OK)
+            rc = rgpu_extension_ray_cast(ctx_.get(), nullptr, scene_.get(), variant_, (const rgpu_ray*)dr,
+                                         (rgpu_hit*)dh, (uint32_t)rays.size(), 0);
+        if (rc == RGPU_OK) rc = rgpu_download(ctx_.get(), nullptr, hits.data(), dh, hits.size() * sizeof(rgpu_hit));
+        if (dh) rgpu_free(ctx_.get(), dh);
+        rgpu_free(ctx_.get(), dr);
+        check(rc, ctx_.get(), "RayCaster::intersect");
+        return hits;
+    }
+
+private:
+    GpuContext& ctx_;
+    SceneBuffer scene_;
+    int variant_;
+};
+
+class StacklessBVH : public RayCaster {
+public:
+    StacklessBVH(GpuContext& ctx, const SceneManager& sm) : RayCaster(ctx, sm, RGPU_VARIANT_STACKLESS) {}
+};
+class ThreadedBVH : public RayCaster {
+public:
+    ThreadedBVH(GpuContext& ctx, const SceneManager& sm) : RayCaster(ctx, sm, RGPU_VARIANT_THREADED) {}
+};
+
+}  // namespace rt
