@@ -147,7 +147,7 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
                             const rgpu_ray* rays, rgpu_hit* hits, uint32_t count, uint32_t block);
 /* Launch shape of the persistent ray-cast kernel used when block == 0: resident waves per
  * SIMD (grid = CUs * waves_per_simd workgroups of 256; 1..16) and the number of idle lanes of a
- * wave that makes it hand out its next rays (1..64). 0 keeps the default (8 waves; refill at 8
+ * wave that makes it hand out its next rays (1..64). 0 keeps the default (8 waves; refill at 16
  * idle lanes for stackless, 48 for threaded). No
  * counterpart in the reference (it leaves launch shapes to the OpenCL driver,
  * common/raccoon_ocl.hpp:906). */

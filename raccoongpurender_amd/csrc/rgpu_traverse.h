@@ -38,7 +38,7 @@ struct TraceScene {
 
 // Launch shape of the persistent ray-cast kernel: CUs to fill, resident waves per SIMD (grid =
 // cu_count * waves_per_simd workgroups of 256), and the idle-lane count that triggers a refill.
-// Zeros take the defaults (256 CUs, 8 waves; refill at 8 lanes for the stackless walk, 48 for
+// Zeros take the defaults (256 CUs, 8 waves; refill at 16 lanes for the stackless walk, 48 for
 // the threaded one).
 struct TraceTuning {
     uint32_t cu_count = 0;

@@ -102,6 +102,31 @@ struct Cnt {
     uint32_t branch = 0, leaf = 0, parent = 0, prim = 0, node = 0;
 };
 
+// The far children a stackless walk has deferred, newest first, in four registers. The
+// newest deferred child always belongs to the lowest set bit of the bitstack, so a backtrack
+// can jump to it without climbing the parent pointers; once more than four are pending the
+// oldest fall out, and a backtrack that finds the stack empty climbs as the reference does.
+// Only loads are saved: the visit order is the reference's.
+struct ShortStack {
+    uint32_t s0, s1, s2, s3, n;
+    __device__ __forceinline__ void clear() { n = 0; }
+    __device__ __forceinline__ void push(uint32_t x) {
+        s3 = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = x;
+        n = n < 4 ? n + 1 : 4;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        const uint32_t x = s0;
+        s0 = s1;
+        s1 = s2;
+        s2 = s3;
+        --n;
+        return x;
+    }
+};
+
 // Stackless traversal of one ray (kernels/extension_ray_cast_stackless.cl:81-200).
 template <bool kCount>
 __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ nodes,
@@ -114,6 +139,8 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
     int32_t prim = -1;
     uint64_t bitstack = 0;
     uint32_t node = 0;
+    ShortStack stk;
+    stk.clear();
     const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
     uint4 q3 = q3p[4 * node + 3];  // {a, b, parent, sibling}
     for (;;) {
@@ -131,6 +158,7 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
                 if (L && R) {
                     bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
                     node = nearL ? q3.x : q3.y;
+                    stk.push(nearL ? q3.y : q3.x);
                     bitstack = bitstack | 1;
                 } else {
                     node = L ? q3.x : q3.y;
@@ -147,6 +175,17 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
             for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, ro, rd, &tmin, &u, &v, &prim);
         }
         // Backtrack: climb while the top bit is 0, then step to the sibling.
+        if (bitstack == 0) break;
+        if (stk.n != 0) {
+            while ((bitstack & 1) == 0) {
+                if (kCount) cnt->parent++;
+                bitstack = bitstack >> 1;
+            }
+            node = stk.pop();
+            bitstack = bitstack ^ 1;
+            q3 = q3p[4 * node + 3];
+            continue;
+        }
         bool done = false;
         while ((bitstack & 1) == 0) {
             if (bitstack == 0) {
@@ -261,6 +300,7 @@ struct StacklessRay {
     uint64_t bitstack;
     uint32_t node;  // kInvalid when finished
     uint4 q3;       // {a, b, parent, sibling} of node
+    ShortStack stk;
 };
 
 __device__ __forceinline__ void stackless_begin(StacklessRay& r, const SLNode* __restrict__ nodes, F3 ro, F3 rd,
@@ -273,6 +313,7 @@ __device__ __forceinline__ void stackless_begin(StacklessRay& r, const SLNode* _
     r.prim = -1;
     r.bitstack = 0;
     r.node = 0;
+    r.stk.clear();
     r.q3 = reinterpret_cast<const uint4*>(nodes)[3];
 }
 
@@ -294,6 +335,7 @@ __device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __
             if (L && R) {
                 bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
                 r.node = nearL ? r.q3.x : r.q3.y;
+                r.stk.push(nearL ? r.q3.y : r.q3.x);
                 r.bitstack = r.bitstack | 1;
             } else {
                 r.node = L ? r.q3.x : r.q3.y;
@@ -305,6 +347,17 @@ __device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __
         const uint32_t beg = r.q3.x & ~kLeafFlag, end = r.q3.y;
         for (uint32_t s = beg; s < end; ++s)
             test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
+    }
+    if (r.bitstack == 0) {
+        r.node = kInvalid;
+        return;
+    }
+    if (r.stk.n != 0) {
+        while ((r.bitstack & 1) == 0) r.bitstack = r.bitstack >> 1;
+        r.node = r.stk.pop();
+        r.bitstack = r.bitstack ^ 1;
+        r.q3 = q3p[4 * r.node + 3];
+        return;
     }
     while ((r.bitstack & 1) == 0) {
         if (r.bitstack == 0) {

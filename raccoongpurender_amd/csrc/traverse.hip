@@ -163,7 +163,7 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
         const uint32_t wps = tune && tune->waves_per_simd ? tune->waves_per_simd : 8u;
         // measured on the 1.3M-triangle incoherent set: the stackless walk prefers early refills,
         // the threaded one late ones
-        const uint32_t refill = tune && tune->refill ? tune->refill : (variant == kVariantThreaded ? 48u : 8u);
+        const uint32_t refill = tune && tune->refill ? tune->refill : (variant == kVariantThreaded ? 48u : 16u);
         const uint32_t blocks = cus * wps;  // 4 waves (one per SIMD) per 256-thread block
         const uint32_t waves = blocks * 4u;
         const uint32_t chunk = (n + waves - 1) / waves;

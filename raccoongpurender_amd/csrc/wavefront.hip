@@ -1293,7 +1293,7 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
                                jobs, chunk, 48u);
         else
             hipLaunchKernelGGL((k_trace_step_persistent<kVariantStackless, J>), dim3(blocks), dim3(256), 0, s, st, p,
-                               jobs, chunk, 8u);
+                               jobs, chunk, 16u);
     } else if (variant == kVariantThreaded) {
         hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs);
     } else {
