@@ -24,12 +24,18 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per traversal event (the record sizes the kernels fetch).
 RAY_IO = 32 + 16  # ray in, hit out
 SL_BRANCH, SL_LEAF, SL_PARENT, PRIM, MT_NODE = 64, 16, 16, 48, 32
+# compressed stackless records: a branch 32 B, a leaf's slot range 8 B, an up word 4 B
+Q_BRANCH, Q_LEAF, Q_UP = 32, 8, 4
 
 
 def algorithmic_bytes(variant, n, c):
     if variant == "stackless":
         return (n * RAY_IO + c["branch_visits"] * SL_BRANCH + c["leaf_visits"] * SL_LEAF
                 + c["parent_steps"] * SL_PARENT + c["prim_tests"] * PRIM)
+    if variant == "compressed":
+        # the fifth counter of this variant counts the up words read
+        return (n * RAY_IO + c["branch_visits"] * Q_BRANCH + c["leaf_visits"] * Q_LEAF
+                + c["node_visits"] * Q_UP + c["prim_tests"] * PRIM)
     return n * RAY_IO + c["node_visits"] * MT_NODE + c["prim_tests"] * PRIM
 
 
@@ -125,12 +131,62 @@ def render_bench(steps=32, warmup=16, paths=1 << 22, spp=64, graph_steps=8):
     return out
 
 
-def batch_bench(total_slots=1 << 22, graph_steps=8):
+def _run_batch(ctx, jobs, guided, launch=None, graph_steps=8, capacity=1 << 18):
+    """One pass of the batch until every job has finished its samples: plain (launch
+    autotuned) or guided (the plain run's launch; a training pass after every graph replay
+    once `capacity` samples are buffered). The bounce loop replays a captured hipGraph."""
+    from raccoongpurender_amd import _lib
+    from raccoongpurender_amd import guide as G
+
+    batch = _lib.Batch(ctx, jobs, max_depth=8)
+    tune_ms, gd, passes = None, None, 0
+    if launch is None:
+        tune_ms = batch.autotune(None, warmup=2, steps=4)  # leaves the batch restarted with the best launch
+        launch = batch.get_launch()
+    else:
+        batch.set_launch(*launch)
+    if guided:
+        gd = G.Guide(ctx, capacity=capacity, minibatch=16384, seed=1)
+        G.attach(batch, gd, sampling=True, training=True)
+    batch.graph_capture(graph_steps)
+    ctx.synchronize()
+    w0 = time.perf_counter()
+    launches = 0
+    while True:
+        batch.graph_launch(1)
+        launches += 1
+        if guided:
+            passes += G.train_pass(batch, capacity) > 0
+        counts, _ = batch.queue_counts()
+        if int(counts.sum()) == 0 or launches > 100000 // graph_steps:
+            break
+    ctx.synchronize()
+    wall = time.perf_counter() - w0
+    samples = sum(batch.job_stat(k) for k in range(len(jobs)))
+    res = {"samples": int(samples), "bounces": launches * graph_steps, "seconds": round(wall, 3),
+           "samples_per_s": round(samples / wall, 1), "launch": list(launch)}
+    if tune_ms is not None:
+        res["autotune_ms_per_step"] = round(tune_ms, 3)
+    if guided:
+        res["training_passes"] = passes
+    return batch, gd, res
+
+
+def _job_luminance(batch, k):
+    rgb, cnt = batch.job_accumulation(k)
+    y = (0.2126 * rgb[:, 0] + 0.7152 * rgb[:, 1] + 0.0722 * rgb[:, 2]).astype(np.float64)
+    return y / np.maximum(cnt, 1)
+
+
+def batch_bench(total_slots=1 << 22, graph_steps=8, ref_spp=4096):
     """256 render jobs over 8 variants of the full-render scene (92k-2.0M triangles), widths and
     heights log-uniform in [64, 1024], spp from {1, 4, 16, 64, 256} (seed 99), in one persistent
     wavefront: launch configuration autotuned, bounce loop replayed from a hipGraph, run until
-    every job has finished its samples."""
+    every job has finished its samples; plain, then guided (one network for the batch, trained
+    while it renders). Error: per variant the smallest job with at least 16 spp is scored
+    (relMSE of luminance) against a `ref_spp` plain render of the same job."""
     from raccoongpurender_amd import _lib, render_workloads as RW
+    from raccoongpurender_amd.guiding_study import rel_mse
 
     t0 = time.perf_counter()
     variants = RW.batch_scene_variants()
@@ -153,33 +209,55 @@ def batch_bench(total_slots=1 << 22, graph_steps=8):
         cam = _lib.camera_look_at(c["eye"], c["target"], c["up"], c["fov"], spec["width"], spec["height"],
                                   c["lens_radius"])
         jobs.append(_lib.BatchJob(d["scene"], d["mats"], env, cam, n, spp=spec["spp"], seed=spec["seed"]))
-    batch = _lib.Batch(ctx, jobs, max_depth=8)
     setup = time.perf_counter() - t0
-    tune_ms = batch.autotune(None, warmup=2, steps=4)  # leaves the batch restarted with the best launch
-    variant, tb, qb = batch.get_launch()
-    batch.graph_capture(graph_steps)
-    ctx.synchronize()
-    w0 = time.perf_counter()
-    launches = 0
-    while True:
-        batch.graph_launch(1)
-        launches += 1
-        counts, _ = batch.queue_counts()
-        if int(counts.sum()) == 0 or launches > 100000 // graph_steps:
-            break
-    ctx.synchronize()
-    wall = time.perf_counter() - w0
-    samples = sum(batch.job_stat(k) for k in range(len(jobs)))
     budget = sum(j.npixels * j.spp for j in jobs)
+    probes = []
+    for v in range(len(variants)):
+        cand = [k for k, s in enumerate(specs) if s["variant"] == v and s["spp"] >= 16]
+        if cand:
+            probes.append(min(cand, key=lambda k: jobs[k].npixels))
+    images = {}
+    runs = {}
+    for guided in (False, True):
+        batch, gd, res = _run_batch(ctx, jobs, guided, launch=None if not guided else runs[False]["launch"],
+                                    graph_steps=graph_steps)
+        images[guided] = [_job_luminance(batch, k) for k in probes]
+        runs[guided] = res
+        batch.close()
+        if gd is not None:
+            gd.close()
+    # references: each probe job alone, plain, ref_spp samples per pixel
+    errs = {False: [], True: []}
+    for i, k in enumerate(probes):
+        j = jobs[k]
+        rj = _lib.BatchJob(j.scene, j.materials, j.envmap, j.camera, min(1 << 20, j.npixels * 64), spp=ref_spp,
+                           seed=12345 + k)
+        rb = _lib.Batch(ctx, [rj], max_depth=8)
+        rb.set_launch(*runs[False]["launch"])
+        rb.run(max_steps=100000, check_every=16)
+        ref = _job_luminance(rb, 0)
+        rb.close()
+        for guided in (False, True):
+            errs[guided].append(rel_mse(images[guided][i], ref))
+    plain, guided = runs[False], runs[True]
     out = {"workload": "256 jobs, 8 scene variants (92k-2.0M tris), 64..1024 px per side, spp {1,4,16,64,256}, "
                        "seed 99, one wavefront",
-           "jobs": len(jobs), "slots": batch.n, "samples": int(samples), "sample_budget": int(budget),
-           "bounces": launches * graph_steps, "seconds": round(wall, 3),
-           "samples_per_s": round(samples / wall, 1), "launch": [variant, tb, qb],
-           "autotune_ms_per_step": round(tune_ms, 3), "setup_s": round(setup, 1), "sampling": "plain"}
-    print(f"[bench] batch: {samples / 1e9:.3f} G samples in {wall:.2f} s ({samples / wall / 1e6:.1f} M samples/s), "
-          f"{launches * graph_steps} bounces, launch {variant}/{tb}/{qb}", file=sys.stderr, flush=True)
-    batch.close()
+           "jobs": len(jobs), "slots": sum(j.slots for j in jobs), "sample_budget": int(budget),
+           "samples": plain["samples"], "bounces": plain["bounces"], "seconds": plain["seconds"],
+           "samples_per_s": plain["samples_per_s"], "launch": plain["launch"],
+           "autotune_ms_per_step": plain["autotune_ms_per_step"], "setup_s": round(setup, 1), "sampling": "plain",
+           "guided": guided,
+           "probe_relmse": {"jobs": probes, "reference_spp": ref_spp,
+                            "plain": round(float(np.mean(errs[False])), 4),
+                            "guided": round(float(np.mean(errs[True])), 4),
+                            "per_job_plain": [round(e, 4) for e in errs[False]],
+                            "per_job_guided": [round(e, 4) for e in errs[True]]}}
+    print(f"[bench] batch: {plain['samples'] / 1e9:.3f} G samples in {plain['seconds']:.2f} s "
+          f"({plain['samples_per_s'] / 1e6:.1f} M samples/s), {plain['bounces']} bounces, launch "
+          f"{'/'.join(str(x) for x in plain['launch'])}; guided {guided['seconds']:.2f} s "
+          f"({guided['samples_per_s'] / 1e6:.1f} M samples/s, {guided['training_passes']} training passes); "
+          f"equal-spp relMSE of {len(probes)} probe jobs plain {out['probe_relmse']['plain']:.4f} "
+          f"guided {out['probe_relmse']['guided']:.4f}", file=sys.stderr, flush=True)
     ctx.close()
     return out
 
@@ -211,7 +289,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level (8 = 1,310,720 triangles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--variant", choices=["auto", "stackless", "threaded"], default="auto")
+    ap.add_argument("--variant", choices=["auto", "stackless", "threaded", "compressed"], default="auto")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run the library's copy kernel on MiB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -255,7 +333,7 @@ def main():
     hits = ctx.alloc(n * 16)
 
     results = {}
-    variants = ["stackless", "threaded"] if args.variant == "auto" else [args.variant]
+    variants = ["stackless", "threaded", "compressed"] if args.variant == "auto" else [args.variant]
     # incoherent rays: the persistent kernel (block 0: waves refill idle lanes from their ray
     # chunk); coherent primary rays: one thread per ray (block 256), which keeps neighbouring
     # pixels in one wave for the whole traversal

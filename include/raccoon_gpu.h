@@ -38,6 +38,11 @@ extern "C" {
 /* traversal variants */
 #define RGPU_VARIANT_STACKLESS 0 /* parent-pointer BVH2, bitstack (kernels/extension_ray_cast_stackless.cl) */
 #define RGPU_VARIANT_THREADED 1  /* skip-pointer BVH2, 6 direction tables (kernels/extension_ray_cast.cl) */
+/* The stackless walk of variant 0 over 32-byte records whose child boxes are quantised to 8
+ * bits per plane, rounded outward (no counterpart in the reference, whose nodes hold float
+ * boxes). The same closest hit as variant 0 except where two primitives tie in t: the looser
+ * boxes may enter children in another order, and a later primitive wins an exact tie. */
+#define RGPU_VARIANT_COMPRESSED 2
 
 typedef struct rgpu_ctx rgpu_ctx;
 typedef struct rgpu_bvh rgpu_bvh;
@@ -120,6 +125,10 @@ int rgpu_bvh_get_stats(const rgpu_bvh* bvh, rgpu_bvh_stats* out);
 /* The built tree in depth-first pre-order: per node its box (6 floats: min xyz, max xyz),
  * children (-1 for a leaf) and leaf slot range; prim_ids maps slots to primitives.
  * Arrays are sized from rgpu_bvh_get_stats (nodes, slots). Any output may be NULL. */
+/* The tree as RGPU_VARIANT_COMPRESSED stores it on the device: *count records of 32 bytes
+ * (nodes + 1; record 0 the root, record 1 unused, children in pairs) and one up word per
+ * record. Call with records = up = NULL for the count. Host only. */
+int rgpu_bvh_export_compressed(const rgpu_bvh* bvh, void* records, uint32_t* up, uint32_t* count);
 int rgpu_bvh_export(const rgpu_bvh* bvh, float* bounds, int32_t* left, int32_t* right, uint32_t* prim_beg,
                     uint32_t* prim_cnt, uint32_t* prim_ids);
 
@@ -155,7 +164,8 @@ int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refil
 
 /* Instrumented extension ray cast: the same traversal (identical hits) with per-ray event
  * counting, reduced to 5 totals {stackless branch visits, stackless leaf visits, stackless
- * parent steps, primitive tests, threaded node visits}. The reference measures kernels with
+ * parent steps, primitive tests, threaded node visits}; for RGPU_VARIANT_COMPRESSED the fifth
+ * is the number of 4-byte up words read. The reference measures kernels with
  * profiling events only (common/raccoon_ocl.hpp:172-189); these counts give the algorithmic
  * bytes the roofline is priced on. Synchronous; allocates a 40-byte device buffer. */
 int rgpu_extension_ray_cast_stats(rgpu_ctx* ctx, const rgpu_scene* scene, int variant, const rgpu_ray* rays,
@@ -328,8 +338,11 @@ typedef struct rgpu_batch_job {
 
 /* Creates and initialises the batch (cfg->path_count and cfg->seed are ignored; RGPU_WF_ACCUM64
  * is not supported). step, time_steps, queue_counts (totals), read_state, graph_capture/launch,
- * set_launch and autotune (camera may be NULL) apply to a batch; initialize, the single-render
- * readers and guiding do not. The scenes, materials and envmaps must outlive it. */
+ * set_launch, autotune (camera may be NULL) and guiding apply to a batch; initialize and the
+ * single-render readers do not. The scenes, materials and envmaps must outlive it. A guide
+ * attached to a batch is one network for all jobs: a vertex's position is normalised to its
+ * job's scene box (the BVH root box grown by 1e-3) and its scene is an input (one-hot class,
+ * in order of first use; scenes past the 8th share "no class"). */
 int rgpu_wavefront_create_batch(rgpu_ctx* ctx, const rgpu_batch_job* jobs, uint32_t njobs,
                                 const rgpu_wavefront_config* cfg, rgpu_wavefront** out);
 int rgpu_wavefront_job_count(const rgpu_wavefront* w, uint32_t* njobs);
@@ -340,7 +353,8 @@ int rgpu_wavefront_job_accumulation(rgpu_wavefront* w, uint32_t job, float* rgb_
 int rgpu_wavefront_job_stat(rgpu_wavefront* w, uint32_t job, uint64_t* total_samples);
 int rgpu_wavefront_job_tonemap(rgpu_wavefront* w, uint32_t job, uint8_t* rgba8);
 /* Steps (check_every at a time) until every job has finished its samples or max_steps have
- * run; every job needs spp > 0. */
+ * run; every job needs spp > 0. With a training guide attached, a training pass runs after
+ * every check_every steps once `capacity` samples are buffered. */
 int rgpu_wavefront_run_batch(rgpu_wavefront* w, uint32_t max_steps, uint32_t check_every, uint32_t* steps_done);
 
 /* ================================================================ path guiding
@@ -360,6 +374,9 @@ typedef struct rgpu_guide_config {
     float scene_lo[3];     /* scene box that normalises positions */
     float scene_hi[3];
     uint32_t max_depth;    /* guide the first max_depth vertices of a path; 0 = every vertex */
+    float weight_cap;      /* training weights F/p above it are clamped to it; 0 = no clamp.
+                              Affects what the network learns only: the estimator stays unbiased */
+    float log_kappa_max;   /* upper clamp of ln(kappa) of a lobe, in (-3, 12]; 0 = 7.5 */
 } rgpu_guide_config;
 
 int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** out);
@@ -367,12 +384,16 @@ int rgpu_guide_destroy(rgpu_guide* g);
 int rgpu_guide_param_count(void);
 /* Copy the fp32 weights out (get) and/or in (set; refreshes the bf16 copies). */
 int rgpu_guide_params(rgpu_guide* g, float* get, const float* set);
-/* Test hooks over host rows of 16 floats (position in [0,1]^3, normal, wo, wi, weight, pad):
+/* Test hooks over host rows of 16 floats (position in [0,1]^3, normal, wo, wi, weight, scene
+ * class code (0 = none, 1..8: one-hot input), pad 2):
  * raw head outputs (64 per row), the gradient of the mean loss -weight*log q(wi) (at most
  * minibatch rows), and training (minibatch Adam steps over at most capacity rows). */
 int rgpu_guide_forward(rgpu_guide* g, const float* samples, uint32_t rows, float* out);
 int rgpu_guide_gradients(rgpu_guide* g, const float* samples, uint32_t rows, float* grad);
 int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows);
+/* Test hook: copy the first `rows` rows (16 floats, as above) of the training buffer a
+ * wavefront has filled; *buffered = samples in it (capped at capacity). Synchronous. */
+int rgpu_guide_read_samples(rgpu_guide* g, float* out, uint32_t rows, uint32_t* buffered);
 /* Attach (or detach with g = NULL) a guide: `sampling` enables guided sampling, `training`
  * records vertices and collects training samples. */
 int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, int training);

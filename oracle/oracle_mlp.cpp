@@ -37,8 +37,10 @@ struct Net {
     double R(double x) const { return emulate ? bf16(x) : x; }
 };
 
-// features: 12-band sin/cos of the normalised position, then normal and outgoing direction
-void encode(const double* p01, const double* n, const double* wo, double* x) {
+// features: 12-band sin/cos of the normalised position, then normal and outgoing direction,
+// then the scene class of a batch job one-hot in features 78..85 (class code c = 1..8 sets
+// feature 77 + c; c = 0, a single render, leaves them zero)
+void encode(const double* p01, const double* n, const double* wo, uint32_t cls, double* x) {
     for (int k = 0; k < kBands; ++k)
         for (int a = 0; a < 3; ++a) {
             double arg = (double)(1u << k) * kPi * p01[a];
@@ -50,6 +52,7 @@ void encode(const double* p01, const double* n, const double* wo, double* x) {
         x[75 + a] = wo[a];
     }
     for (int i = 78; i < kIn; ++i) x[i] = 0.0;
+    if (cls >= 1 && cls <= 8) x[77 + cls] = 1.0;
 }
 
 struct Fwd {
@@ -159,8 +162,11 @@ extern "C" {
 
 int orc_mlp_param_count() { return kParams; }
 
-void orc_mlp_encode(uint32_t n, const double* p01, const double* nrm, const double* wo, double* x) {
-    for (uint32_t i = 0; i < n; ++i) encode(p01 + 3 * i, nrm + 3 * i, wo + 3 * i, x + (size_t)kIn * i);
+// cls: per-row class codes (0 = none), or null for none
+void orc_mlp_encode(uint32_t n, const double* p01, const double* nrm, const double* wo, const uint32_t* cls,
+                    double* x) {
+    for (uint32_t i = 0; i < n; ++i)
+        encode(p01 + 3 * i, nrm + 3 * i, wo + 3 * i, cls ? cls[i] : 0u, x + (size_t)kIn * i);
 }
 
 // out: n x 64 raw head outputs

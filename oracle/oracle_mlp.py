@@ -9,7 +9,7 @@ import oracle as O
 lib = O.lib
 vp, u32 = C.c_void_p, C.c_uint32
 O._sig("orc_mlp_param_count", C.c_int)
-O._sig("orc_mlp_encode", None, u32, vp, vp, vp, vp)
+O._sig("orc_mlp_encode", None, u32, vp, vp, vp, vp, vp)
 O._sig("orc_mlp_forward", None, vp, u32, vp, vp, C.c_int)
 O._sig("orc_mlp_pdf", None, u32, vp, vp, vp)
 O._sig("orc_mlp_loss_grad", C.c_double, vp, u32, vp, vp, vp, vp, C.c_int)
@@ -21,10 +21,12 @@ PARAM_COUNT = lib.orc_mlp_param_count()
 N_IN, N_OUT = 96, 64
 
 
-def encode(p01, n, wo):
+def encode(p01, n, wo, cls=None):
+    """cls: per-row scene class codes (0 = none, 1..8 = one-hot feature 77 + c), or None."""
     p01, n, wo = [np.ascontiguousarray(a, np.float64) for a in (p01, n, wo)]
     x = np.zeros((len(p01), N_IN))
-    lib.orc_mlp_encode(len(p01), P(p01), P(n), P(wo), P(x))
+    c = None if cls is None else np.ascontiguousarray(cls, np.uint32)
+    lib.orc_mlp_encode(len(p01), P(p01), P(n), P(wo), None if c is None else c.ctypes.data, P(x))
     return x
 
 

@@ -79,8 +79,10 @@ __device__ void fwd_layer(const uint16_t* X, int px, const uint16_t* W, const fl
 
 // Encoded input features of one row (written as bf16 into X): for band k and axis a,
 // sin and cos of 2^k pi p_a. 2^k p_a is exact in fp32 and so is its reduction modulo 2,
-// which leaves sinpi/cospi a small argument.
-__device__ void encode_row(uint16_t* X, const float* p01, const float* n, const float* wo, int part) {
+// which leaves sinpi/cospi a small argument. Then the normal, the outgoing direction and
+// the scene class one-hot (class code c = 1..kGClasses sets feature 77 + c).
+__device__ void encode_row(uint16_t* X, const float* p01, const float* n, const float* wo, uint32_t cls,
+                           int part) {
     // part 0..3: each of four lanes writes 24 of the 96 features
     for (int f = part * 24; f < part * 24 + 24; ++f) {
         float v;
@@ -94,16 +96,21 @@ __device__ void encode_row(uint16_t* X, const float* p01, const float* n, const 
         } else if (f < 78) {
             v = wo[f - 75];
         } else {
-            v = 0.0f;
+            v = (uint32_t)(f - 77) == cls ? 1.0f : 0.0f;
         }
         X[f] = f2bf(v);
     }
 }
 
+// Class code of a stored sample; anything but 1..kGClasses reads as none.
+__device__ __forceinline__ uint32_t sample_class(float c) {
+    return (c >= 1.0f && c <= (float)kGClasses) ? (uint32_t)c : 0u;
+}
+
 // Lobe k of one row from its raw outputs o[0..4] and the row's softmax constants.
-__device__ __forceinline__ void lobe_params(const float* o, float mx, float lz, float* L) {
+__device__ __forceinline__ void lobe_params(const float* o, float mx, float lz, float lkmax, float* L) {
     const float m = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
-    const float kap = expf(fminf(fmaxf(o[3], kGLogKappaMin), kGLogKappaMax));
+    const float kap = expf(fminf(fmaxf(o[3], kGLogKappaMin), lkmax));
     const float lw = o[4] - mx - lz;
     L[0] = o[0] / m;
     L[1] = o[1] / m;
@@ -125,20 +132,20 @@ __device__ __forceinline__ float quad_sum(float v) {
 }
 
 // Raw head outputs of one row -> this lane's two lobes (mu, kappa, weight, log(weight C(kappa))).
-__device__ void head_lobes_part(const float* out, int part, float* L) {
+__device__ void head_lobes_part(const float* out, int part, float lkmax, float* L) {
     const float* o0 = out + 5 * (2 * part);
     const float* o1 = o0 + 5;
     const float mx = quad_max(fmaxf(o0[4], o1[4]));
     const float lz = logf(quad_sum(expf(o0[4] - mx) + expf(o1[4] - mx)));
-    lobe_params(o0, mx, lz, L);
-    lobe_params(o1, mx, lz, L + kGLobeFloats);
+    lobe_params(o0, mx, lz, lkmax, L);
+    lobe_params(o1, mx, lz, lkmax, L + kGLobeFloats);
 }
 
 // d(-weight * log q(wi)) / d(raw outputs) for this lane's two lobes: d[j] is the derivative
 // by output 10 part + j (the 24 unused outputs have none).
-__device__ void head_grad_part(const float* out, int part, const float* wi, float weight, float* d) {
+__device__ void head_grad_part(const float* out, int part, const float* wi, float weight, float lkmax, float* d) {
     float L[2 * kGLobeFloats], a[2], cm1[2];
-    head_lobes_part(out, part, L);
+    head_lobes_part(out, part, lkmax, L);
     for (int q = 0; q < 2; ++q) {
         const float* Lk = L + kGLobeFloats * q;
         const float dx = wi[0] - Lk[0], dy = wi[1] - Lk[1], dz = wi[2] - Lk[2];
@@ -156,7 +163,7 @@ __device__ void head_grad_part(const float* out, int part, const float* wi, floa
         const float kap = Lk[3];
         const float e = expf(-2.0f * kap);
         const float dlogf = 1.0f / kap - 2.0f * e / (1.0f - e) + cm1[q];
-        d[5 * q + 3] = (o[3] > kGLogKappaMin && o[3] < kGLogKappaMax) ? -weight * r * dlogf * kap : 0.0f;
+        d[5 * q + 3] = (o[3] > kGLogKappaMin && o[3] < lkmax) ? -weight * r * dlogf * kap : 0.0f;
         const float m = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
         const float c = cm1[q] + 1.0f;
         for (int j = 0; j < 3; ++j) d[5 * q + j] = -weight * r * kap * (wi[j] - c * Lk[j]) / m;
@@ -180,7 +187,7 @@ __device__ void tile_forward(TileLds& t, const DevGuide& g, int r0, int lane) {
 }
 
 // Inference over the lambertian and ward queues: lobe parameters for every guided vertex.
-__global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, DevGuide g) {
+__global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, DevGuide g, DevJobs jobs) {
     __shared__ TileLds t;
     const uint32_t c1 = s.counts[kMatLambertian], c4 = s.counts[kMatWard];
     const uint32_t rows = c1 + c4;
@@ -192,14 +199,19 @@ __global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, Dev
             const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
             const uint32_t i = base + row;
             float p01[3] = {0, 0, 0}, n[3] = {0, 0, 0}, wo[3] = {0, 0, 0};
+            uint32_t cls = 0;
             if (i < rows) {
                 const uint32_t slot = i < c1 ? s.items[(size_t)kMatLambertian * p.n + i]
                                              : s.items[(size_t)kMatWard * p.n + (i - c1)];
                 const TraceRay r = s.ray[slot];
                 const ExtRec e = s.ext[slot];
+                const GuideFrame fr = jobs.njobs ? jobs.jobs[jobs.job_of_block[slot / kJobBlock]].gframe
+                                                 : GuideFrame{{g.lo[0], g.lo[1], g.lo[2]},
+                                                              {g.inv_ext[0], g.inv_ext[1], g.inv_ext[2]}, 0u};
+                cls = fr.cls;
                 for (int a = 0; a < 3; ++a) {
                     float P = r.ro[a] + r.rd[a] * e.tmin;
-                    p01[a] = (P - g.lo[a]) * g.inv_ext[a];
+                    p01[a] = (P - fr.lo[a]) * fr.inv_ext[a];
                     wo[a] = -r.rd[a];
                 }
                 n[0] = e.ngx;
@@ -211,7 +223,7 @@ __global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, Dev
                     n[2] = -n[2];
                 }
             }
-            encode_row(t.X + row * kPX, p01, n, wo, part);
+            encode_row(t.X + row * kPX, p01, n, wo, cls, part);
         }
         __syncthreads();
         tile_forward(t, g, r0, lane);
@@ -220,7 +232,7 @@ __global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, Dev
             const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
             const uint32_t i = base + row;
             float L[2 * kGLobeFloats];
-            head_lobes_part(t.F + row * kPF, part, L);  // all four lanes of a row take part
+            head_lobes_part(t.F + row * kPF, part, g.log_kappa_max, L);  // all four lanes of a row take part
             if (i < rows) {
                 const uint32_t slot = i < c1 ? s.items[(size_t)kMatLambertian * p.n + i]
                                              : s.items[(size_t)kMatWard * p.n + (i - c1)];
@@ -283,6 +295,7 @@ __global__ void __launch_bounds__(128) k_guide_fwdbwd(DevGuide g, uint32_t first
     {
         const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
         float p01[3] = {0, 0, 0}, n[3] = {0, 0, 0}, wo[3] = {0, 0, 0};
+        uint32_t cls = 0;
         if (base + row < rows) {
             const GuideSample& sm = g.samples[first + base + row];
             for (int a = 0; a < 3; ++a) {
@@ -290,8 +303,9 @@ __global__ void __launch_bounds__(128) k_guide_fwdbwd(DevGuide g, uint32_t first
                 n[a] = sm.n[a];
                 wo[a] = sm.wo[a];
             }
+            cls = sample_class(sm.scene_class);
         }
-        encode_row(t.X + row * kPX, p01, n, wo, part);
+        encode_row(t.X + row * kPX, p01, n, wo, cls, part);
     }
     __syncthreads();
     tile_forward(t, g, r0, lane);
@@ -305,10 +319,10 @@ __global__ void __launch_bounds__(128) k_guide_fwdbwd(DevGuide g, uint32_t first
             wi[0] = sm.wi[0];
             wi[1] = sm.wi[1];
             wi[2] = sm.wi[2];
-            wgt = sm.weight * inv_batch;
+            wgt = (g.weight_cap > 0.0f ? fminf(sm.weight, g.weight_cap) : sm.weight) * inv_batch;
         }
         float d[10];
-        head_grad_part(t.F + row * kPF, part, wi, wgt, d);
+        head_grad_part(t.F + row * kPF, part, wi, wgt, g.log_kappa_max, d);
         uint16_t* drow = dA + row * kPH;
 #pragma unroll
         for (int i = 0; i < 10; ++i) drow[10 * part + i] = f2bf(d[i]);
@@ -397,13 +411,16 @@ __global__ void __launch_bounds__(128) k_guide_forward(DevGuide g, const GuideSa
     {
         const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
         float p01[3] = {0, 0, 0}, n[3] = {0, 0, 0}, wo[3] = {0, 0, 0};
-        if (base + row < rows)
+        uint32_t cls = 0;
+        if (base + row < rows) {
             for (int a = 0; a < 3; ++a) {
                 p01[a] = in[base + row].p[a];
                 n[a] = in[base + row].n[a];
                 wo[a] = in[base + row].wo[a];
             }
-        encode_row(t.X + row * kPX, p01, n, wo, part);
+            cls = sample_class(in[base + row].scene_class);
+        }
+        encode_row(t.X + row * kPX, p01, n, wo, cls, part);
     }
     __syncthreads();
     tile_forward(t, g, wave * 16, lane);
@@ -467,8 +484,10 @@ __global__ void __launch_bounds__(256) k_adam(DevGuide g, uint32_t groups, float
 
 }  // namespace
 
-hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_infer, dim3(g.infer_blocks ? g.infer_blocks : 2048u), dim3(128), 0, s, st, p, g);
+hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, const DevJobs& jobs,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_infer, dim3(g.infer_blocks ? g.infer_blocks : 2048u), dim3(128), 0, s, st, p, g,
+                       jobs);
     return hipGetLastError();
 }
 

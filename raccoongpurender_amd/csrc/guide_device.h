@@ -25,10 +25,11 @@ constexpr int kGLobeFloats = 6;
 constexpr int kGSlotFloats = kGLobes * kGLobeFloats;  // 48 floats of lobe parameters per slot
 
 // One training sample: position (normalised to the scene box), normal, outgoing and
-// incident directions, and the weight F/p of the direction (luminance of the incident
-// radiance times BSDF cosine over the sampling pdf).
+// incident directions, the weight F/p of the direction (luminance of the incident radiance
+// times BSDF cosine over the sampling pdf) and the scene class code of a batch job
+// (0 for a single render, 1..kGClasses for the job's scene; an exact small float).
 struct GuideSample {
-    float p[3], n[3], wo[3], wi[3], weight, pad[3];
+    float p[3], n[3], wo[3], wi[3], weight, scene_class, pad[2];
 };
 static_assert(sizeof(GuideSample) == 64, "guide sample is 64 bytes");
 
@@ -40,6 +41,14 @@ struct GuideVertex {
 };
 static_assert(sizeof(GuideVertex) == 48, "guide vertex is 48 bytes");
 constexpr int kGRecords = 4;  // guided vertices recorded per path
+constexpr int kGClasses = 8;  // distinct scenes one network tells apart (one-hot inputs 78..85)
+
+// Where a vertex's guide inputs come from: the box that normalises its position and the
+// class code of its scene (0 = none: a single render; c = 1..kGClasses: one-hot input 77 + c).
+struct GuideFrame {
+    float lo[3], inv_ext[3];
+    uint32_t cls;
+};
 
 struct DevGuide {
     float* params = nullptr;          // fp32 master weights
@@ -65,6 +74,8 @@ struct DevGuide {
     float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};  // scene box normalisation
     float select_p = 0.5f;            // one-sample MIS selection probability of the guide
     uint32_t max_depth = 0;           // guided vertices per path (0 = all)
+    float weight_cap = 0.0f;          // training weights are clamped to it (0 = not clamped)
+    float log_kappa_max = kGLogKappaMax;  // upper clamp of a lobe's log concentration
     uint32_t enabled = 0;             // guided sampling on
     uint32_t train = 0;               // record training samples
 };

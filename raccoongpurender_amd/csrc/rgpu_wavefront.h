@@ -143,6 +143,7 @@ struct JobDesc {
     unsigned long long seed = 0;          // RNG seed of the job's first slot
     uint32_t first_block = 0;
     uint32_t nblocks = 0;
+    GuideFrame gframe = {{0.0f, 0.0f, 0.0f}, {1.0f, 1.0f, 1.0f}, 0u};  // guide inputs of the job's vertices
 };
 
 struct DevJobs {
@@ -161,7 +162,8 @@ hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, 
                    const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
                    int trace_block, hipStream_t s, const DevJobs* jobs = nullptr);
 // Path-guiding network (guide.hip).
-hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, hipStream_t s);
+hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, const DevJobs& jobs,
+                       hipStream_t s);
 hipError_t guide_train_batch(const DevGuide& g, uint32_t first, uint32_t rows, float lr, uint32_t step,
                              hipStream_t s);
 hipError_t guide_refresh(const DevGuide& g, hipStream_t s);

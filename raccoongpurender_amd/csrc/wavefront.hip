@@ -323,6 +323,10 @@ struct OneJob {
     __device__ __forceinline__ DevEnvmap envmap(uint32_t) const { return env; }
     __device__ __forceinline__ const Camera& camera(uint32_t) const { return cam; }
     __device__ __forceinline__ float* accum(uint32_t, const DevState& s) const { return s.accum; }
+    __device__ __forceinline__ GuideFrame guide_frame(uint32_t, const DevGuide& g) const {
+        return GuideFrame{{g.lo[0], g.lo[1], g.lo[2]}, {g.inv_ext[0], g.inv_ext[1], g.inv_ext[2]}, 0u};
+    }
+    DevJobs table() const { return DevJobs{}; }
 };
 
 struct ManyJobs {
@@ -334,6 +338,8 @@ struct ManyJobs {
     __device__ __forceinline__ DevEnvmap envmap(uint32_t j) const { return d.jobs[j].env; }
     __device__ __forceinline__ const Camera& camera(uint32_t j) const { return d.jobs[j].cam; }
     __device__ __forceinline__ float* accum(uint32_t j, const DevState&) const { return d.jobs[j].accum; }
+    __device__ __forceinline__ GuideFrame guide_frame(uint32_t j, const DevGuide&) const { return d.jobs[j].gframe; }
+    DevJobs table() const { return d; }
 };
 
 // ---------------------------------------------------------------- kernels
@@ -481,14 +487,14 @@ __device__ __forceinline__ int select_strategy(uint4& st, float bxdf_p, bool gui
 }
 
 // Remember a guided vertex until its path ends (throughput before it, radiance after its emission).
-__device__ __forceinline__ void record_vertex(const DevGuide& g, uint32_t slot, F3 hit, F3 n, F3 wo, F3 wi,
-                                              const PathRec& pr, F3 Le) {
+__device__ __forceinline__ void record_vertex(const DevGuide& g, const GuideFrame& fr, uint32_t slot, F3 hit, F3 n,
+                                              F3 wo, F3 wi, const PathRec& pr, F3 Le) {
     const uint32_t c = g.rec_count[slot];
     if (c >= (uint32_t)kGRecords) return;
     GuideVertex r;
-    r.p[0] = (hit.x - g.lo[0]) * g.inv_ext[0];
-    r.p[1] = (hit.y - g.lo[1]) * g.inv_ext[1];
-    r.p[2] = (hit.z - g.lo[2]) * g.inv_ext[2];
+    r.p[0] = (hit.x - fr.lo[0]) * fr.inv_ext[0];
+    r.p[1] = (hit.y - fr.lo[1]) * fr.inv_ext[1];
+    r.p[2] = (hit.z - fr.lo[2]) * fr.inv_ext[2];
     r.n[0] = n.x; r.n[1] = n.y; r.n[2] = n.z;
     r.wo[0] = wo.x; r.wo[1] = wo.y; r.wo[2] = wo.z;
     r.wi[0] = wi.x; r.wi[1] = wi.y; r.wi[2] = wi.z;
@@ -568,7 +574,9 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
             T.z = L.R[2] / kPiF * c / pdf;
         }
         store_shade(s.shade, slot, load3(L.Le), T);
-        if (guide_here && g.train) record_vertex(g, slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot], load3(L.Le));
+        if (guide_here && g.train)
+            record_vertex(g, jobs.guide_frame(jb, g), slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot],
+                          load3(L.Le));
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
     }
@@ -642,7 +650,8 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jo
         }
         store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, T);
         if (guide_here && g.train)
-            record_vertex(g, slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot], F3{0.0f, 0.0f, 0.0f});
+            record_vertex(g, jobs.guide_frame(jb, g), slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot],
+                          F3{0.0f, 0.0f, 0.0f});
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
     }
@@ -1131,7 +1140,7 @@ __global__ void __launch_bounds__(1024) k_scan_queues(DevState s, uint32_t* samp
 
 // Scatter slots into their queues in slot order; with training on, also write the finished
 // paths' training samples (slot order) into the guide's buffer.
-__global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p, DevGuide g) {
+__global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p, DevGuide g, DevJobs jobs) {
     __shared__ uint32_t wpre[kNumQueues][4];
     __shared__ uint32_t wsam[4];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1168,6 +1177,7 @@ __global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p, 
         for (uint32_t w = 0; w < wave; ++w) idx += wsam[w];
         const float4 wts = g.emit_w[slot];
         const float wj[kGRecords] = {wts.x, wts.y, wts.z, wts.w};
+        const float cls = jobs.njobs ? (float)jobs.jobs[jobs.job_of_block[slot / kJobBlock]].gframe.cls : 0.0f;
 #pragma unroll
         for (int j = 0; j < kGRecords; ++j) {
             if (!((mask >> j) & 1u)) continue;
@@ -1178,7 +1188,7 @@ __global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p, 
                 o[0] = r[0];
                 o[1] = r[1];
                 o[2] = r[2];
-                o[3] = make_float4(wj[j], 0.0f, 0.0f, 0.0f);
+                o[3] = make_float4(wj[j], cls, 0.0f, 0.0f);
             }
             ++idx;
         }
@@ -1273,7 +1283,7 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
     const dim3 tgrid((p.n + tb - 1) / tb), tblk(tb);
     hipLaunchKernelGGL(k_new_path<J>, qgrid, blk, 0, s, st, p, jobs, g);
     if (g.enabled) {
-        hipError_t e = guide_infer(st, p, g, s);
+        hipError_t e = guide_infer(st, p, g, jobs.table(), s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_shade_lambertian<true, J>), qgrid, blk, 0, s, st, p, jobs, g);
         hipLaunchKernelGGL((k_shade_ward<true, J>), qgrid, blk, 0, s, st, p, jobs, g);
@@ -1302,7 +1312,7 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
     hipLaunchKernelGGL(k_logic<J>, full, blk, 0, s, st, p, jobs, g);
     hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues + 1), dim3(1024), 0, s, st,
                        g.train ? g.sample_count : (uint32_t*)nullptr, g.capacity);
-    hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p, g);
+    hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p, g, jobs.table());
     return hipGetLastError();
 }
 }  // namespace

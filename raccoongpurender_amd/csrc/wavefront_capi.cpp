@@ -62,6 +62,7 @@ struct rgpu_wavefront {
         float* accum;
         uint32_t npixels, spp, first_block, nblocks;
         uint64_t seed;
+        GuideFrame gframe;
     };
     std::vector<Job> jobs;
     DevJobs djobs;
@@ -201,6 +202,7 @@ int restart_batch(rgpu_wavefront* w) {
         d.seed = J.seed;
         d.first_block = J.first_block;
         d.nblocks = J.nblocks;
+        d.gframe = J.gframe;
     }
     RGPU_HIP(ctx, hipMemcpy(w->djobs.jobs, table.data(), table.size() * sizeof(JobDesc), hipMemcpyHostToDevice));
     RGPU_HIP(ctx, wf_initialize_jobs(w->st, w->params, w->djobs, ctx->stream));
@@ -472,10 +474,26 @@ int rgpu_wavefront_create_batch(rgpu_ctx* ctx, const rgpu_batch_job* jobs, uint3
     int rc = parse_config(ctx, cfg, w);
     if (rc == RGPU_OK) rc = alloc_slots(w, (size_t)blocks * kJobBlock);
     std::vector<uint16_t> job_of_block(blocks);
+    std::vector<const rgpu_scene*> classes;  // distinct scenes in order of first use
     uint32_t fb = 0;
     for (uint32_t j = 0; j < njobs && rc == RGPU_OK; ++j) {
         const rgpu_batch_job& b = jobs[j];
         rgpu_wavefront::Job J{};
+        // guide inputs: positions normalised to the scene's root box (grown by 1e-3 on each
+        // side), the scene's class code 1..kGClasses; scenes past the kGClasses-th get none
+        uint32_t cls = 0;
+        for (size_t c = 0; c < classes.size(); ++c)
+            if (classes[c] == b.scene) cls = (uint32_t)c + 1;
+        if (cls == 0 && classes.size() < (size_t)kGClasses) {
+            classes.push_back(b.scene);
+            cls = (uint32_t)classes.size();
+        }
+        J.gframe.cls = cls;
+        for (int k = 0; k < 3; ++k) {
+            const float lo = b.scene->lo[k] - 1e-3f, ext = b.scene->hi[k] + 1e-3f - lo;
+            J.gframe.lo[k] = lo;
+            J.gframe.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 1.0f;
+        }
         J.scene = b.scene;
         J.mats = b.materials;
         J.env = b.envmap;
@@ -610,6 +628,10 @@ int rgpu_wavefront_run_batch(rgpu_wavefront* w, uint32_t max_steps, uint32_t che
         int rc = rgpu_wavefront_step(w, nullptr, k);
         if (rc != RGPU_OK) return rc;
         done += k;
+        if (w->guide_obj && w->guide.train) {
+            rc = rgpu_wavefront_guide_train(w, w->guide_obj->dev.capacity, nullptr);
+            if (rc != RGPU_OK) return rc;
+        }
         uint32_t c[kNumQueues];
         rc = rgpu_wavefront_queue_counts(w, c, nullptr);
         if (rc != RGPU_OK) return rc;
@@ -904,6 +926,13 @@ int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** 
     d.capacity = cfg->capacity ? cfg->capacity : (1u << 18);
     d.select_p = cfg->select_p;
     d.max_depth = cfg->max_depth;
+    if (!(cfg->weight_cap >= 0.0f) || !(cfg->log_kappa_max == 0.0f ||
+                                        (cfg->log_kappa_max > kGLogKappaMin && cfg->log_kappa_max <= 12.0f))) {
+        delete g;
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "weight_cap must be >= 0 and log_kappa_max in (-3, 12]");
+    }
+    d.weight_cap = cfg->weight_cap;
+    d.log_kappa_max = cfg->log_kappa_max != 0.0f ? cfg->log_kappa_max : kGLogKappaMax;
     for (int k = 0; k < 3; ++k) {
         float ext = cfg->scene_hi[k] - cfg->scene_lo[k];
         d.lo[k] = cfg->scene_lo[k];
@@ -1029,7 +1058,6 @@ int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows)
 
 int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, int training) {
     if (!w) return RGPU_ERR_INVALID_ARG;
-    if (g && w->is_batch()) return fail(w->ctx, RGPU_ERR_UNSUPPORTED, "guiding a batch is not supported");
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1088,6 +1116,21 @@ int rgpu_wavefront_guide_pending(rgpu_wavefront* w, uint32_t* pending) {
 
 // One training pass if at least `min_samples` samples are buffered: minibatch Adam steps over
 // the buffer, then the buffer is emptied. *trained = samples used (0 if none).
+int rgpu_guide_read_samples(rgpu_guide* g, float* out, uint32_t rows, uint32_t* buffered) {
+    if (!g || (rows && !out)) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = g->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t n = 0;
+    RGPU_HIP(ctx, hipMemcpy(&n, g->dev.sample_count, 4, hipMemcpyDeviceToHost));
+    n = std::min(n, g->dev.capacity);
+    if (buffered) *buffered = n;
+    if (rows > n) return fail(ctx, RGPU_ERR_INVALID_ARG, "fewer samples buffered than asked for");
+    if (rows)
+        RGPU_HIP(ctx, hipMemcpy(out, g->dev.samples, (size_t)rows * sizeof(GuideSample), hipMemcpyDeviceToHost));
+    return RGPU_OK;
+}
+
 int rgpu_wavefront_guide_train(rgpu_wavefront* w, uint32_t min_samples, uint32_t* trained) {
     if (!w || !w->guide_obj) return RGPU_ERR_INVALID_ARG;
     rgpu_guide* g = w->guide_obj;

@@ -8,7 +8,7 @@ from ._lib import RGPU_ABI_VERSION, ptr
 
 vp, u32 = C.c_void_p, C.c_uint32
 SAMPLE_DTYPE = np.dtype([("p", np.float32, 3), ("n", np.float32, 3), ("wo", np.float32, 3), ("wi", np.float32, 3),
-                         ("weight", np.float32), ("pad", np.float32, 3)])
+                         ("weight", np.float32), ("scene_class", np.float32), ("pad", np.float32, 2)])
 assert SAMPLE_DTYPE.itemsize == 64
 N_OUT = 64
 LOBES = 8
@@ -17,7 +17,8 @@ LOBES = 8
 class rgpu_guide_config(C.Structure):
     _fields_ = [("abi_version", u32), ("select_p", C.c_float), ("capacity", u32), ("minibatch", u32),
                 ("learning_rate", C.c_float), ("seed", u32), ("scene_lo", C.c_float * 3),
-                ("scene_hi", C.c_float * 3), ("max_depth", u32)]
+                ("scene_hi", C.c_float * 3), ("max_depth", u32), ("weight_cap", C.c_float),
+                ("log_kappa_max", C.c_float)]
 
 
 _sig = _lib._sig
@@ -28,21 +29,23 @@ _sig("rgpu_guide_params", C.c_int, vp, vp, vp)
 _sig("rgpu_guide_forward", C.c_int, vp, vp, u32, vp)
 _sig("rgpu_guide_gradients", C.c_int, vp, vp, u32, vp)
 _sig("rgpu_guide_train_samples", C.c_int, vp, vp, u32)
+_sig("rgpu_guide_read_samples", C.c_int, vp, vp, u32, C.POINTER(u32))
 _sig("rgpu_wavefront_attach_guide", C.c_int, vp, vp, C.c_int, C.c_int)
 _sig("rgpu_wavefront_guide_pending", C.c_int, vp, C.POINTER(u32))
 _sig("rgpu_wavefront_guide_train", C.c_int, vp, u32, C.POINTER(u32))
 _lib.EXPORTED += ["rgpu_guide_create", "rgpu_guide_destroy", "rgpu_guide_param_count", "rgpu_guide_params",
-                  "rgpu_guide_forward", "rgpu_guide_gradients", "rgpu_guide_train_samples",
+                  "rgpu_guide_forward", "rgpu_guide_gradients", "rgpu_guide_train_samples", "rgpu_guide_read_samples",
                   "rgpu_wavefront_attach_guide", "rgpu_wavefront_guide_pending", "rgpu_wavefront_guide_train"]
 PARAM_COUNT = _lib.lib.rgpu_guide_param_count()
 
 
 class Guide:
     def __init__(self, ctx, scene_lo=(0, 0, 0), scene_hi=(1, 1, 1), select_p=0.5, capacity=1 << 18,
-                 minibatch=16384, lr=1e-3, seed=1, max_depth=0):
+                 minibatch=16384, lr=1e-3, seed=1, max_depth=0, weight_cap=0.0, log_kappa_max=0.0):
         self.ctx = ctx
         cfg = rgpu_guide_config(RGPU_ABI_VERSION, select_p, capacity, minibatch, lr, seed,
-                                (C.c_float * 3)(*scene_lo), (C.c_float * 3)(*scene_hi), max_depth)
+                                (C.c_float * 3)(*scene_lo), (C.c_float * 3)(*scene_hi), max_depth, weight_cap,
+                                log_kappa_max)
         h = vp()
         ctx.check(_lib.lib.rgpu_guide_create(ctx.h, C.byref(cfg), C.byref(h)), "rgpu_guide_create")
         self.h = h
@@ -75,6 +78,16 @@ class Guide:
         samples = np.ascontiguousarray(samples, SAMPLE_DTYPE)
         self.ctx.check(_lib.lib.rgpu_guide_train_samples(self.h, ptr(samples), len(samples)),
                        "rgpu_guide_train_samples")
+
+    def buffered_samples(self, rows=None):
+        """The training samples a wavefront has buffered (all of them, or the first `rows`)."""
+        n = u32(0)
+        self.ctx.check(_lib.lib.rgpu_guide_read_samples(self.h, None, 0, C.byref(n)), "rgpu_guide_read_samples")
+        rows = n.value if rows is None else int(rows)
+        out = np.zeros(rows, SAMPLE_DTYPE)
+        self.ctx.check(_lib.lib.rgpu_guide_read_samples(self.h, ptr(out), rows, C.byref(n)),
+                       "rgpu_guide_read_samples")
+        return out
 
     def close(self):
         if self.h and self.ctx.h:

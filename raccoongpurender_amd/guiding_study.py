@@ -44,14 +44,14 @@ def render_reference(ctx, sd, spp, paths, seed=777):
 
 
 def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=1 << 18, select_p=0.5, lr=1e-3,
-                 minibatch=16384, max_depth=0, bxdf_p=0.0):
+                 minibatch=16384, max_depth=0, bxdf_p=0.0, weight_cap=0.0, log_kappa_max=0.0):
     wf = build_lane(ctx, sd, paths, clamp=NO_CLAMP, seed=seed, bxdf_p=bxdf_p)
     g = None
     if guided:
         lo = sd["points"].min(0) - 1e-3
         hi = sd["points"].max(0) + 1e-3
         g = Guide(ctx, lo, hi, select_p=select_p, capacity=capacity, lr=lr, seed=seed, minibatch=minibatch,
-                  max_depth=max_depth)
+                  max_depth=max_depth, weight_cap=weight_cap, log_kappa_max=log_kappa_max)
         attach(wf, g, sampling=True, training=True)
     ctx.synchronize()
     t0 = time.perf_counter()

@@ -132,3 +132,44 @@ def test_batch_graph_replays_and_launch_shapes_agree(ctx):
             assert np.allclose(ro, re, rtol=1e-4, atol=1e-6)
     ms = eager.autotune(None, warmup=1, steps=2)
     assert ms > 0.0
+
+
+def _furnace_batch(ctx, guided, seed=5):
+    from raccoongpurender_amd import guide as G
+    specs = [(R.furnace_scene("lambertian", 32, 32, value=0.5, fill=True), 8192, 48, seed),
+             (R.furnace_scene("lambertian", 24, 16, value=0.5, fill=True, flip=True), 4096, 64, seed + 1)]
+    batch = _lib.Batch(ctx, _jobs(ctx, specs), clamp=1e30)
+    gd = None
+    if guided:
+        gd = G.Guide(ctx, capacity=1 << 14, minibatch=4096, seed=seed)
+        G.attach(batch, gd, sampling=True, training=True)
+    return batch, gd
+
+
+def test_guided_batch_samples_carry_the_job_scene(ctx):
+    # one network for the batch: each training sample names its job's scene (class 1 or 2 in
+    # order of first use) and its position is normalised to that scene's box
+    batch, gd = _furnace_batch(ctx, guided=True)
+    batch.step(3)
+    s = gd.buffered_samples()
+    cls, cnt = np.unique(s["scene_class"], return_counts=True)
+    print(f"{len(s)} samples, classes {dict(zip(cls.tolist(), cnt.tolist()))}", flush=True)
+    assert set(cls.tolist()) == {1.0, 2.0}
+    assert (s["p"] >= 0).all() and (s["p"] <= 1).all()
+    assert np.isfinite(s["weight"]).all() and (s["weight"] > 0).all()
+
+
+@pytest.mark.parametrize("guided", [False, True])
+def test_guided_batch_furnace_is_exact(ctx, guided):
+    # Two lambertian furnaces (R = 0.5 under a sky of 0.5: every sample's expectation is 0.25)
+    # in one guided batch that trains while it renders: each job's mean must stay at 0.25.
+    batch, gd = _furnace_batch(ctx, guided)
+    steps = batch.run(max_steps=4000, check_every=4)
+    for k, job in enumerate(batch.jobs):
+        rgb, cnt = batch.job_accumulation(k)
+        assert (cnt == job.spp).all()
+        m = rgb[:, 0] / cnt
+        se = m.std(ddof=1) / np.sqrt(m.size)
+        z = abs(m.mean() - 0.25) / se
+        print(f"guided={guided} job {k}: {steps} steps, mean {m.mean():.6f} se {se:.3g} z={z:.2f}", flush=True)
+        assert z < 5.0

@@ -45,6 +45,49 @@ def test_forward_matches_oracle(ctx):
     assert np.quantile(rel, 0.99) < 1e-4 and rel.max() < 2e-2
 
 
+def test_forward_with_scene_classes_matches_oracle(ctx):
+    # batch jobs feed their scene as a one-hot class; the network must see it as the oracle does
+    rng = np.random.default_rng(21)
+    g = G.Guide(ctx, seed=3)
+    # the class columns of the first layer start at zero (every class the same function until
+    # training tells them apart): give them weights so that the comparison exercises them
+    w = g.params()
+    w0 = w[:64 * 96].reshape(64, 96)
+    w0[:, 78:86] = rng.uniform(-0.25, 0.25, (64, 8)).astype(np.float32)
+    g.set_params(w)
+    w = g.params().astype(np.float64)
+    s = _samples(rng, 320)
+    s["scene_class"] = rng.integers(0, 9, len(s))
+    out = g.forward(s)
+    x = M.encode(s["p"].astype(np.float64), s["n"].astype(np.float64), s["wo"].astype(np.float64),
+                 s["scene_class"].astype(np.uint32))
+    ref = M.forward(w, x, emulate=True)
+    rel = np.abs(out - ref) / (1.0 + np.abs(ref))
+    print("forward (classes) rel err q99 %.3g max %.3g" % (np.quantile(rel, 0.99), rel.max()), flush=True)
+    assert np.quantile(rel, 0.99) < 1e-4 and rel.max() < 2e-2
+    # the class changes the output of an otherwise identical row
+    one = np.repeat(s[:1], 2)
+    one["scene_class"] = (0, 3)
+    o = g.forward(one)
+    assert np.abs(o[0] - o[1]).max() > 1e-4
+
+
+def test_single_render_samples_are_in_the_box_with_no_class(ctx):
+    sd = R.material_test_scene(32, 24)
+    lo, hi = sd["points"].min(0) - 1e-3, sd["points"].max(0) + 1e-3
+    wf = build_lane(ctx, sd, 32 * 24 * 8, clamp=1e30, seed=3)
+    gd = G.Guide(ctx, lo, hi, capacity=1 << 15, minibatch=4096, seed=3)
+    G.attach(wf, gd, sampling=True, training=True)
+    wf.step(12)
+    s = gd.buffered_samples()
+    print(f"{len(s)} samples buffered", flush=True)
+    assert len(s) > 100
+    assert (s["scene_class"] == 0).all()
+    assert (s["p"] >= 0).all() and (s["p"] <= 1).all()
+    assert np.isfinite(s["weight"]).all() and (s["weight"] > 0).all()
+    assert np.allclose(np.linalg.norm(s["wi"], axis=1), 1.0, atol=1e-4)
+
+
 def test_gradients_match_oracle(ctx):
     rng = np.random.default_rng(2)
     g = G.Guide(ctx, seed=4)

@@ -27,6 +27,22 @@ def test_encoding_values():
     assert list(x[72:78]) == [0, 0, 1, 1, 0, 0] and not x[78:].any()
 
 
+def test_scene_class_is_one_hot_after_the_directions():
+    # class code c = 1..8 (a batch job's scene) sets feature 77 + c and nothing else; 0 and
+    # out-of-range codes leave the encoding of a single render
+    rng = np.random.default_rng(11)
+    p, n, wo = rng.random((10, 3)), _unit(rng, 10), _unit(rng, 10)
+    base = M.encode(p, n, wo)
+    cls = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+    x = M.encode(p, n, wo, cls)
+    assert np.array_equal(x[:, :78], base[:, :78])
+    for i, c in enumerate(cls):
+        expect = np.zeros(18)
+        if 1 <= c <= 8:
+            expect[c - 1] = 1.0
+        assert np.array_equal(x[i, 78:], expect), c
+
+
 def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(0)
     params = _rand_params(1)
