@@ -24,18 +24,12 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per traversal event (the record sizes the kernels fetch).
 RAY_IO = 32 + 16  # ray in, hit out
 SL_BRANCH, SL_LEAF, SL_PARENT, PRIM, MT_NODE = 64, 16, 16, 48, 32
-# compressed stackless records: a branch 32 B, a leaf's slot range 8 B, an up word 4 B
-Q_BRANCH, Q_LEAF, Q_UP = 32, 8, 4
 
 
 def algorithmic_bytes(variant, n, c):
     if variant == "stackless":
         return (n * RAY_IO + c["branch_visits"] * SL_BRANCH + c["leaf_visits"] * SL_LEAF
                 + c["parent_steps"] * SL_PARENT + c["prim_tests"] * PRIM)
-    if variant == "compressed":
-        # the fifth counter of this variant counts the up words read
-        return (n * RAY_IO + c["branch_visits"] * Q_BRANCH + c["leaf_visits"] * Q_LEAF
-                + c["node_visits"] * Q_UP + c["prim_tests"] * PRIM)
     return n * RAY_IO + c["node_visits"] * MT_NODE + c["prim_tests"] * PRIM
 
 
@@ -289,7 +283,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level (8 = 1,310,720 triangles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--variant", choices=["auto", "stackless", "threaded", "compressed"], default="auto")
+    ap.add_argument("--variant", choices=["auto", "stackless", "threaded"], default="auto")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run the library's copy kernel on MiB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -333,7 +327,7 @@ def main():
     hits = ctx.alloc(n * 16)
 
     results = {}
-    variants = ["stackless", "threaded", "compressed"] if args.variant == "auto" else [args.variant]
+    variants = ["stackless", "threaded"] if args.variant == "auto" else [args.variant]
     # incoherent rays: the persistent kernel (block 0: waves refill idle lanes from their ray
     # chunk); coherent primary rays: one thread per ray (block 256), which keeps neighbouring
     # pixels in one wave for the whole traversal

@@ -38,11 +38,6 @@ extern "C" {
 /* traversal variants */
 #define RGPU_VARIANT_STACKLESS 0 /* parent-pointer BVH2, bitstack (kernels/extension_ray_cast_stackless.cl) */
 #define RGPU_VARIANT_THREADED 1  /* skip-pointer BVH2, 6 direction tables (kernels/extension_ray_cast.cl) */
-/* The stackless walk of variant 0 over 32-byte records whose child boxes are quantised to 8
- * bits per plane, rounded outward (no counterpart in the reference, whose nodes hold float
- * boxes). The same closest hit as variant 0 except where two primitives tie in t: the looser
- * boxes may enter children in another order, and a later primitive wins an exact tie. */
-#define RGPU_VARIANT_COMPRESSED 2
 
 typedef struct rgpu_ctx rgpu_ctx;
 typedef struct rgpu_bvh rgpu_bvh;
@@ -125,10 +120,6 @@ int rgpu_bvh_get_stats(const rgpu_bvh* bvh, rgpu_bvh_stats* out);
 /* The built tree in depth-first pre-order: per node its box (6 floats: min xyz, max xyz),
  * children (-1 for a leaf) and leaf slot range; prim_ids maps slots to primitives.
  * Arrays are sized from rgpu_bvh_get_stats (nodes, slots). Any output may be NULL. */
-/* The tree as RGPU_VARIANT_COMPRESSED stores it on the device: *count records of 32 bytes
- * (nodes + 1; record 0 the root, record 1 unused, children in pairs) and one up word per
- * record. Call with records = up = NULL for the count. Host only. */
-int rgpu_bvh_export_compressed(const rgpu_bvh* bvh, void* records, uint32_t* up, uint32_t* count);
 int rgpu_bvh_export(const rgpu_bvh* bvh, float* bounds, int32_t* left, int32_t* right, uint32_t* prim_beg,
                     uint32_t* prim_cnt, uint32_t* prim_ids);
 
@@ -164,8 +155,7 @@ int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refil
 
 /* Instrumented extension ray cast: the same traversal (identical hits) with per-ray event
  * counting, reduced to 5 totals {stackless branch visits, stackless leaf visits, stackless
- * parent steps, primitive tests, threaded node visits}; for RGPU_VARIANT_COMPRESSED the fifth
- * is the number of 4-byte up words read. The reference measures kernels with
+ * parent steps, primitive tests, threaded node visits}. The reference measures kernels with
  * profiling events only (common/raccoon_ocl.hpp:172-189); these counts give the algorithmic
  * bytes the roofline is priced on. Synchronous; allocates a 40-byte device buffer. */
 int rgpu_extension_ray_cast_stats(rgpu_ctx* ctx, const rgpu_scene* scene, int variant, const rgpu_ray* rays,

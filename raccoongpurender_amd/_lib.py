@@ -24,8 +24,7 @@ ERRORS = {
 }
 VARIANT_STACKLESS = 0
 VARIANT_THREADED = 1
-VARIANT_COMPRESSED = 2
-VARIANTS = {"stackless": VARIANT_STACKLESS, "threaded": VARIANT_THREADED, "compressed": VARIANT_COMPRESSED}
+VARIANTS = {"stackless": VARIANT_STACKLESS, "threaded": VARIANT_THREADED}
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -86,7 +85,6 @@ _sig("rgpu_bvh_build", C.c_int, vp, u32, u32, vp, u32, vp, u32, C.POINTER(rgpu_b
 _sig("rgpu_bvh_destroy", C.c_int, vp)
 _sig("rgpu_bvh_get_stats", C.c_int, vp, C.POINTER(rgpu_bvh_stats))
 _sig("rgpu_bvh_export", C.c_int, vp, vp, vp, vp, vp, vp, vp)
-_sig("rgpu_bvh_export_compressed", C.c_int, vp, vp, vp, C.POINTER(C.c_uint32))
 _sig("rgpu_scene_create", C.c_int, vp, vp, vp, u32, u32, vp, u32, vp, u32, C.POINTER(vp))
 _sig("rgpu_scene_destroy", C.c_int, vp)
 _sig("rgpu_set_trace_tuning", C.c_int, vp, C.c_uint32, C.c_uint32)
@@ -99,7 +97,7 @@ EXPORTED = [
     "rgpu_create", "rgpu_destroy", "rgpu_last_error_string", "rgpu_device_count",
     "rgpu_alloc", "rgpu_free", "rgpu_upload", "rgpu_download", "rgpu_fill_u32", "rgpu_synchronize", "rgpu_copy",
     "rgpu_time_extension_ray_cast",
-    "rgpu_bvh_build", "rgpu_bvh_destroy", "rgpu_bvh_get_stats", "rgpu_bvh_export", "rgpu_bvh_export_compressed",
+    "rgpu_bvh_build", "rgpu_bvh_destroy", "rgpu_bvh_get_stats", "rgpu_bvh_export",
     "rgpu_scene_create", "rgpu_scene_destroy", "rgpu_scene_bytes", "rgpu_set_trace_tuning",
     "rgpu_extension_ray_cast", "rgpu_extension_ray_cast_stats",
 ]
@@ -251,17 +249,6 @@ class BVH:
         pids = np.zeros(max(s, 1), np.uint32)
         lib.rgpu_bvh_export(self.h, ptr(bounds), ptr(left), ptr(right), ptr(beg), ptr(cnt), ptr(pids))
         return dict(bounds=bounds, left=left, right=right, prim_beg=beg, prim_cnt=cnt, prim_ids=pids[:s])
-
-    def export_compressed(self):
-        """Records of the compressed stackless layout (n x 32 bytes) and their up words."""
-        n = C.c_uint32(0)
-        if lib.rgpu_bvh_export_compressed(self.h, None, None, C.byref(n)) != 0:
-            raise RgpuError(-1, "rgpu_bvh_export_compressed")
-        rec = np.zeros((n.value, 32), np.uint8)
-        up = np.zeros(n.value, np.uint32)
-        if lib.rgpu_bvh_export_compressed(self.h, ptr(rec), ptr(up), C.byref(n)) != 0:
-            raise RgpuError(-1, "rgpu_bvh_export_compressed")
-        return rec, up
 
     def __del__(self):
         try:

@@ -320,107 +320,6 @@ void pack_stackless(const BVHTree& t, std::vector<SLNode>* out) {
     }
 }
 
-float dequantize_plane(float origin, uint32_t biased_exp, uint32_t q) {
-    uint32_t bits = biased_exp << 23;
-    float scale;
-    std::memcpy(&scale, &bits, 4);
-    const float step = (float)q * scale;  // exact: q < 2^8, scale a power of two
-    return origin + step;
-}
-
-namespace {
-
-// 8-bit planes of child boxes [clo, chi] on the grid origin + q * 2^(e - 127), rounded
-// outward. Picks the smallest exponent whose 255 steps cover the node's extent and for which
-// every rounded plane still contains its exact value.
-uint32_t quantize_axis(float origin, float extent, const float* clo, const float* chi, uint8_t* qlo, uint8_t* qhi) {
-    if (!(extent >= 0.0f) || !(extent < 1e30f)) throw std::invalid_argument("BVH box extent out of range");
-    int e = 1;  // scale 2^-126
-    if (extent > 0.0f) {
-        e = (int)std::ceil(std::log2((double)extent / 255.0)) + 127;
-        e = std::max(1, std::min(254, e));
-    }
-    for (;; ++e) {
-        if (e > 254) throw std::invalid_argument("BVH box cannot be quantised");
-        const double s = std::ldexp(1.0, e - 127);
-        bool ok = true;
-        for (int c = 0; c < 2 && ok; ++c) {
-            long lo = (long)std::floor(((double)clo[c] - (double)origin) / s);
-            long hi = (long)std::ceil(((double)chi[c] - (double)origin) / s);
-            lo = std::max(0L, std::min(255L, lo));
-            hi = std::max(0L, std::min(255L, hi));
-            while (lo > 0 && dequantize_plane(origin, (uint32_t)e, (uint32_t)lo) > clo[c]) --lo;
-            while (hi < 255 && dequantize_plane(origin, (uint32_t)e, (uint32_t)hi) < chi[c]) ++hi;
-            if (dequantize_plane(origin, (uint32_t)e, (uint32_t)lo) > clo[c] ||
-                dequantize_plane(origin, (uint32_t)e, (uint32_t)hi) < chi[c] || lo > hi) {
-                ok = false;
-                break;
-            }
-            qlo[c] = (uint8_t)lo;
-            qhi[c] = (uint8_t)hi;
-        }
-        if (ok) return (uint32_t)e;
-    }
-}
-
-}  // namespace
-
-void pack_stackless_q(const BVHTree& t, std::vector<SLQNode>* out, std::vector<uint32_t>* up) {
-    const size_t nn = t.nodes.size();
-    out->clear();
-    up->clear();
-    if (nn == 0) return;
-    out->assign(nn + 1, SLQNode());
-    up->assign(nn + 1, kInvalid);
-    std::memset(out->data(), 0, out->size() * sizeof(SLQNode));
-    // record of each built node: the root at 0, then children pairs in depth-first order
-    std::vector<uint32_t> rec(nn, kInvalid);
-    rec[0] = 0;
-    uint32_t next = 2;
-    std::vector<uint32_t> todo = {0};
-    while (!todo.empty()) {
-        const uint32_t i = todo.back();
-        todo.pop_back();
-        const BuildNode& b = t.nodes[i];
-        if (b.left < 0) continue;
-        rec[b.left] = next;
-        rec[b.right] = next + 1;
-        next += 2;
-        todo.push_back((uint32_t)b.right);
-        todo.push_back((uint32_t)b.left);
-    }
-    if (next != nn + 1) throw std::logic_error("compressed BVH layout: unexpected record count");
-    for (size_t i = 0; i < nn; ++i) {
-        const BuildNode& b = t.nodes[i];
-        SLQNode& s = (*out)[rec[i]];
-        if (b.left < 0) {
-            const uint32_t range[2] = {b.prim_beg, b.prim_beg + b.prim_cnt};
-            std::memcpy(&s, range, sizeof(range));
-            continue;
-        }
-        const BuildNode& L = t.nodes[b.left];
-        const BuildNode& R = t.nodes[b.right];
-        uint32_t exps = 0;
-        for (int k = 0; k < 3; ++k) {
-            s.origin[k] = b.bmin[k];
-            const float clo[2] = {L.bmin[k], R.bmin[k]}, chi[2] = {L.bmax[k], R.bmax[k]};
-            uint8_t qlo[2], qhi[2];
-            const uint32_t e = quantize_axis(b.bmin[k], b.bmax[k] - b.bmin[k], clo, chi, qlo, qhi);
-            exps |= e << (8 * k);
-            s.q[k] = qlo[0];
-            s.q[3 + k] = qhi[0];
-            s.q[6 + k] = qlo[1];
-            s.q[9 + k] = qhi[1];
-        }
-        const bool lleaf = L.left < 0, rleaf = R.left < 0;
-        exps |= (uint32_t)((lleaf ? 1u : 0u) | (rleaf ? 2u : 0u)) << 24;
-        s.exps = exps;
-        s.child = rec[b.left];
-        (*up)[rec[b.left]] = rec[i] | (rleaf ? kLeafFlag : 0u);
-        (*up)[rec[b.right]] = rec[i] | (lleaf ? kLeafFlag : 0u);
-    }
-}
-
 void pack_threaded(const BVHTree& t, std::vector<MTNode>* out) {
     const uint32_t nn = (uint32_t)t.nodes.size();
     out->assign((size_t)6 * nn, MTNode());

@@ -42,11 +42,6 @@ void build_bvh(const float* points, uint32_t stride, uint32_t npoints,
 
 // Device layouts.
 void pack_stackless(const BVHTree& t, std::vector<SLNode>* out);
-// Compressed records (root 0, record 1 unused, children in pairs laid out depth first) and
-// the up words beside them.
-void pack_stackless_q(const BVHTree& t, std::vector<SLQNode>* out, std::vector<uint32_t>* up);
-// Plane value a compressed record decodes to, computed as the kernels compute it.
-float dequantize_plane(float origin, uint32_t biased_exp, uint32_t q);
 // out: 6 tables of t.nodes.size() records, direction-major (+X,-X,+Y,-Y,+Z,-Z).
 void pack_threaded(const BVHTree& t, std::vector<MTNode>* out);
 void pack_prims(const BVHTree& t, const float* points, uint32_t stride,

@@ -228,21 +228,6 @@ int rgpu_bvh_get_stats(const rgpu_bvh* bvh, rgpu_bvh_stats* out) {
     return RGPU_OK;
 }
 
-int rgpu_bvh_export_compressed(const rgpu_bvh* bvh, void* records, uint32_t* up, uint32_t* count) {
-    if (!bvh || !count) return RGPU_ERR_INVALID_ARG;
-    try {
-        std::vector<SLQNode> sq;
-        std::vector<uint32_t> sq_up;
-        pack_stackless_q(bvh->tree, &sq, &sq_up);
-        *count = (uint32_t)sq.size();
-        if (records) std::memcpy(records, sq.data(), sq.size() * sizeof(SLQNode));
-        if (up) std::memcpy(up, sq_up.data(), sq_up.size() * sizeof(uint32_t));
-    } catch (const std::exception&) {
-        return RGPU_ERR_INVALID_ARG;
-    }
-    return RGPU_OK;
-}
-
 int rgpu_bvh_export(const rgpu_bvh* bvh, float* bounds, int32_t* left, int32_t* right, uint32_t* prim_beg,
                     uint32_t* prim_cnt, uint32_t* prim_ids) {
     if (!bvh) return RGPU_ERR_INVALID_ARG;
@@ -280,7 +265,7 @@ static int upload_new(rgpu_ctx* ctx, const void* src, size_t bytes, void** dst) 
 int rgpu_scene_destroy(rgpu_scene* s) {
     if (!s) return RGPU_ERR_INVALID_ARG;
     if (s->ctx) (void)hipSetDevice(s->ctx->device);
-    for (void* p : {s->sl, s->mt, s->prims, s->points, s->indices, s->spheres, s->sq, s->sq_up})
+    for (void* p : {s->sl, s->mt, s->prims, s->points, s->indices, s->spheres})
         if (p) (void)hipFree(p);
     delete s;
     return RGPU_OK;
@@ -300,10 +285,7 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     std::vector<SLNode> sl;
     std::vector<MTNode> mt;
     std::vector<PrimRec> pr;
-    std::vector<SLQNode> sq;
-    std::vector<uint32_t> sq_up;
     pack_stackless(bvh->tree, &sl);
-    pack_stackless_q(bvh->tree, &sq, &sq_up);
     pack_threaded(bvh->tree, &mt);
     pack_prims(bvh->tree, points, stride, indices, ntris, spheres, nspheres, &pr);
     std::vector<float> p4((size_t)npoints * 4, 0.0f);
@@ -315,8 +297,6 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     if (rc == RGPU_OK) rc = upload_new(ctx, sl.data(), sl.size() * sizeof(SLNode), &s->sl);
     if (rc == RGPU_OK) rc = upload_new(ctx, mt.data(), mt.size() * sizeof(MTNode), &s->mt);
     if (rc == RGPU_OK) rc = upload_new(ctx, pr.data(), pr.size() * sizeof(PrimRec), &s->prims);
-    if (rc == RGPU_OK) rc = upload_new(ctx, sq.data(), sq.size() * sizeof(SLQNode), &s->sq);
-    if (rc == RGPU_OK) rc = upload_new(ctx, sq_up.data(), sq_up.size() * sizeof(uint32_t), &s->sq_up);
     if (rc == RGPU_OK) rc = upload_new(ctx, p4.data(), p4.size() * sizeof(float), &s->points);
     if (rc == RGPU_OK) rc = upload_new(ctx, indices, (size_t)ntris * 3 * sizeof(uint32_t), &s->indices);
     if (rc == RGPU_OK) rc = upload_new(ctx, spheres, (size_t)nspheres * 4 * sizeof(float), &s->spheres);
@@ -332,9 +312,6 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     s->dev.spheres = (const float4*)s->spheres;
     s->dev.nnodes = (uint32_t)bvh->tree.nodes.size();
     s->dev.sphere_begin = (int32_t)ntris;
-    s->dev.q_nodes = (const SLQNode*)s->sq;
-    s->dev.q_up = (const uint32_t*)s->sq_up;
-    s->dev.q_root = (!bvh->tree.nodes.empty() && bvh->tree.nodes[0].left < 0) ? kLeafFlag : 0u;
     s->nprims = ntris + nspheres;
     if (!bvh->tree.nodes.empty())
         for (int k = 0; k < 3; ++k) {
@@ -361,7 +338,7 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
                             const rgpu_ray* rays, rgpu_hit* hits, uint32_t count, uint32_t block) {
     if (!ctx || !scene) return RGPU_ERR_INVALID_ARG;
     if (count && (!rays || !hits)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null ray or hit buffer");
-    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED && variant != RGPU_VARIANT_COMPRESSED)
+    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED)
         return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
     if (block != 0 && (block % 64 != 0 || block > 1024))
         return fail(ctx, RGPU_ERR_INVALID_ARG, "block must be a multiple of 64 up to 1024");
@@ -387,7 +364,7 @@ int rgpu_extension_ray_cast_stats(rgpu_ctx* ctx, const rgpu_scene* scene, int va
                                   rgpu_hit* hits, uint32_t count, uint64_t* counters5) {
     if (!ctx || !scene || !counters5) return RGPU_ERR_INVALID_ARG;
     if (count && (!rays || !hits)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null ray or hit buffer");
-    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED && variant != RGPU_VARIANT_COMPRESSED)
+    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED)
         return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     unsigned long long* d = nullptr;

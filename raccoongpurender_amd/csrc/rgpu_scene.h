@@ -14,8 +14,6 @@ struct rgpu_scene {
     void* points = nullptr;
     void* indices = nullptr;
     void* spheres = nullptr;
-    void* sq = nullptr;     // compressed stackless records
-    void* sq_up = nullptr;  // their up words
     uint32_t nprims = 0;  // triangles + spheres
     uint64_t bytes[4] = {0, 0, 0, 0};
     float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {1.0f, 1.0f, 1.0f};  // root box of the BVH

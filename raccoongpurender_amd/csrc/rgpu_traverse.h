@@ -22,7 +22,7 @@ struct alignas(16) TraceHit {
     float v;
 };
 
-enum { kVariantStackless = 0, kVariantThreaded = 1, kVariantCompressed = 2 };
+enum { kVariantStackless = 0, kVariantThreaded = 1 };
 
 // Device-resident scene as the kernels see it.
 struct TraceScene {
@@ -34,9 +34,6 @@ struct TraceScene {
     const float4* spheres = nullptr;
     uint32_t nnodes = 0;
     int32_t sphere_begin = 0;
-    const SLQNode* q_nodes = nullptr;  // compressed stackless records (nnodes + 1)
-    const uint32_t* q_up = nullptr;    // their up words
-    uint32_t q_root = 0;               // record 0, with kLeafFlag when the root is a leaf
 };
 
 // Launch shape of the persistent ray-cast kernel: CUs to fill, resident waves per SIMD (grid =

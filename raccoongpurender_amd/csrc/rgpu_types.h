@@ -34,29 +34,6 @@ struct alignas(16) SLNode {
 };
 static_assert(sizeof(SLNode) == 64, "stackless node record must be 64 bytes");
 
-// Compressed stackless BVH2 record (32 B): the same tree, its child boxes quantised to 8 bits
-// per plane on a grid anchored at the node's own lower corner, rounded outward so that every
-// decoded box contains the exact one (a ray that hits a child's exact box hits its decoded
-// box, so no hit is lost; only more boxes may be entered).
-//
-//   q0 = { origin.x, origin.y, origin.z, ex | ey << 8 | ez << 16 | flags << 24 }
-//   q1 = { Lmin.xyz Lmax.x, Lmax.yz Rmin.xy, Rmin.z Rmax.xyz (bytes), child }
-//
-// Plane value = origin + q * 2^(e - 127) evaluated in float (the product is exact, the sum
-// rounds once); flags bit 0 / bit 1: the left / right child is a leaf. The children of a
-// branch are the records child and child + 1 (child even), so a node's sibling is its id ^ 1;
-// the root is record 0 and record 1 is unused. A leaf record holds {first primitive slot,
-// one past the last} in its first 8 bytes. Beside the records, up[i] = parent id of record i
-// | kLeafFlag when the sibling of i is a leaf (kInvalid for the root): a backtrack that has
-// to climb reads those 4-byte words only.
-struct alignas(16) SLQNode {
-    float origin[3];
-    uint32_t exps;
-    uint8_t q[12];
-    uint32_t child;
-};
-static_assert(sizeof(SLQNode) == 32, "compressed stackless record must be 32 bytes");
-
 static const uint32_t kLeafFlag = 0x80000000u;
 static const uint32_t kInvalid = 0xFFFFFFFFu;
 

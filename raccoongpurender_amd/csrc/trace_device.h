@@ -208,111 +208,6 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
     return prim;
 }
 
-// Slab test of child c (0 = left, 1 = right) of a compressed record, its planes decoded as
-// origin + q * 2^(e - 127) (the product is exact, so each plane equals the packer's
-// rounding-checked value). One child at a time keeps six planes live, not twelve.
-__device__ __forceinline__ bool child_hit(float4 q0, uint4 q1, int c, F3 ro, F3 inv, float farclip, float* h0,
-                                          float* h1) {
-    const uint32_t ex = __float_as_uint(q0.w);
-    const float o[3] = {q0.x, q0.y, q0.z};
-    const uint32_t w[3] = {q1.x, q1.y, q1.z};
-    float lo[3], hi[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float s = __uint_as_float(((ex >> (8 * k)) & 0xFFu) << 23);
-        const int bl = 6 * c + k, bh = 6 * c + 3 + k;  // byte positions of the two planes of axis k
-        lo[k] = o[k] + (float)((w[bl >> 2] >> (8 * (bl & 3))) & 0xFFu) * s;
-        hi[k] = o[k] + (float)((w[bh >> 2] >> (8 * (bh & 3))) & 0xFFu) * s;
-    }
-    return slab_hit(lo, hi, ro, inv, farclip, h0, h1);
-}
-
-// Stackless walk over compressed records: the reference's visit rule (near child first by
-// entry distance, bitstack, parent climb) on the decoded boxes. Node ids carry kLeafFlag for
-// leaves, taken from the parent's flags, so a leaf costs one 8-byte load and a branch two
-// 16-byte loads.
-template <bool kCount>
-__device__ __forceinline__ int32_t trace_stackless_q(const SLQNode* __restrict__ nodes,
-                                                     const uint32_t* __restrict__ up, uint32_t root,
-                                                     const PrimRec* __restrict__ prims, int32_t sphere_begin,
-                                                     F3 ro, F3 rd, float* tmin_io, float* uo, float* vo,
-                                                     Cnt* cnt) {
-    const F3 inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-    float tmin = *tmin_io;
-    float u = 0.0f, v = 0.0f;
-    int32_t prim = -1;
-    uint64_t bitstack = 0;
-    uint32_t node = root;
-    ShortStack stk;
-    stk.clear();
-    for (;;) {
-        if ((node & kLeafFlag) == 0) {
-            if (kCount) cnt->branch++;
-            const float4 q0 = reinterpret_cast<const float4*>(nodes + node)[0];
-            const uint4 q1 = reinterpret_cast<const uint4*>(nodes + node)[1];
-            float Lh0, Lh1, Rh0, Rh1;
-            bool L = child_hit(q0, q1, 0, ro, inv, tmin, &Lh0, &Lh1);
-            bool R = child_hit(q0, q1, 1, ro, inv, tmin, &Rh0, &Rh1);
-            if (L || R) {
-                const uint32_t fl = __float_as_uint(q0.w) >> 24;
-                const uint32_t a = q1.w | ((fl & 1u) ? kLeafFlag : 0u);
-                const uint32_t b = (q1.w + 1u) | ((fl & 2u) ? kLeafFlag : 0u);
-                bitstack = bitstack << 1;
-                if (L && R) {
-                    bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
-                    node = nearL ? a : b;
-                    stk.push(nearL ? b : a);
-                    bitstack = bitstack | 1;
-                } else {
-                    node = L ? a : b;
-                }
-                continue;
-            }
-        } else {
-            const uint2 range = reinterpret_cast<const uint2*>(nodes + (node & ~kLeafFlag))[0];
-            if (kCount) {
-                cnt->leaf++;
-                cnt->prim += range.y - range.x;
-            }
-            for (uint32_t s = range.x; s < range.y; ++s) test_prim(prims, s, sphere_begin, ro, rd, &tmin, &u, &v, &prim);
-        }
-        if (bitstack == 0) break;
-        if (stk.n != 0) {
-            while ((bitstack & 1) == 0) {
-                if (kCount) cnt->parent++;
-                bitstack = bitstack >> 1;
-            }
-            node = stk.pop();
-            bitstack = bitstack ^ 1;
-            continue;
-        }
-        uint32_t id = node & ~kLeafFlag;
-        uint32_t w = up[id];
-        if (kCount) cnt->node++;
-        bool done = false;
-        while ((bitstack & 1) == 0) {
-            if (bitstack == 0) {
-                done = true;
-                break;
-            }
-            id = w & ~kLeafFlag;
-            if (kCount) cnt->parent++;
-            bitstack = bitstack >> 1;
-            if (bitstack != 0) {
-                w = up[id];
-                if (kCount) cnt->node++;
-            }
-        }
-        if (done) break;
-        node = (id ^ 1u) | (w & kLeafFlag);
-        bitstack = bitstack ^ 1;
-    }
-    *tmin_io = tmin;
-    *uo = u;
-    *vo = v;
-    return prim;
-}
-
 // Ray-direction class of kernels/extension_ray_cast.cl:45-58.
 __device__ __forceinline__ uint32_t direction_class(F3 rd) {
     float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
@@ -476,83 +371,6 @@ __device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __
     r.node = r.q3.w;
     r.bitstack = r.bitstack ^ 1;
     r.q3 = q3p[4 * r.node + 3];
-}
-
-// Resumable form of trace_stackless_q for the persistent kernels.
-struct StacklessQRay {
-    F3 ro, rd, inv;
-    float tmin, u, v;
-    int32_t prim;
-    uint64_t bitstack;
-    uint32_t node;  // kInvalid when finished; kLeafFlag marks a leaf
-    ShortStack stk;
-};
-
-__device__ __forceinline__ void stackless_q_begin(StacklessQRay& r, uint32_t root, F3 ro, F3 rd, float tmax) {
-    r.ro = ro;
-    r.rd = rd;
-    r.inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-    r.tmin = tmax;
-    r.u = r.v = 0.0f;
-    r.prim = -1;
-    r.bitstack = 0;
-    r.node = root;
-    r.stk.clear();
-}
-
-// One pass of the loop body of trace_stackless_q.
-__device__ __forceinline__ void stackless_q_step(StacklessQRay& r, const SLQNode* __restrict__ nodes,
-                                                 const uint32_t* __restrict__ up, const PrimRec* __restrict__ prims,
-                                                 int32_t sphere_begin) {
-    if ((r.node & kLeafFlag) == 0) {
-        const float4 q0 = reinterpret_cast<const float4*>(nodes + r.node)[0];
-        const uint4 q1 = reinterpret_cast<const uint4*>(nodes + r.node)[1];
-        float Lh0, Lh1, Rh0, Rh1;
-        bool L = child_hit(q0, q1, 0, r.ro, r.inv, r.tmin, &Lh0, &Lh1);
-        bool R = child_hit(q0, q1, 1, r.ro, r.inv, r.tmin, &Rh0, &Rh1);
-        if (L || R) {
-            const uint32_t fl = __float_as_uint(q0.w) >> 24;
-            const uint32_t a = q1.w | ((fl & 1u) ? kLeafFlag : 0u);
-            const uint32_t b = (q1.w + 1u) | ((fl & 2u) ? kLeafFlag : 0u);
-            r.bitstack = r.bitstack << 1;
-            if (L && R) {
-                bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
-                r.node = nearL ? a : b;
-                r.stk.push(nearL ? b : a);
-                r.bitstack = r.bitstack | 1;
-            } else {
-                r.node = L ? a : b;
-            }
-            return;
-        }
-    } else {
-        const uint2 range = reinterpret_cast<const uint2*>(nodes + (r.node & ~kLeafFlag))[0];
-        for (uint32_t s = range.x; s < range.y; ++s)
-            test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
-    }
-    if (r.bitstack == 0) {
-        r.node = kInvalid;
-        return;
-    }
-    if (r.stk.n != 0) {
-        while ((r.bitstack & 1) == 0) r.bitstack = r.bitstack >> 1;
-        r.node = r.stk.pop();
-        r.bitstack = r.bitstack ^ 1;
-        return;
-    }
-    uint32_t id = r.node & ~kLeafFlag;
-    uint32_t w = up[id];
-    while ((r.bitstack & 1) == 0) {
-        if (r.bitstack == 0) {
-            r.node = kInvalid;
-            return;
-        }
-        id = w & ~kLeafFlag;
-        r.bitstack = r.bitstack >> 1;
-        if (r.bitstack != 0) w = up[id];
-    }
-    r.node = (id ^ 1u) | (w & kLeafFlag);
-    r.bitstack = r.bitstack ^ 1;
 }
 
 }  // namespace rgpu

@@ -95,29 +95,6 @@ __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restri
     if (kCount) flush_counts(counters, c);
 }
 
-template <bool kCount>
-__global__ void __launch_bounds__(256) k_trace_compressed(const TraceRay* __restrict__ rays,
-                                                          TraceHit* __restrict__ hits, uint32_t n, TraceScene sc,
-                                                          unsigned long long* counters) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    Cnt c;
-    if (i < n) {
-        F3 ro, rd;
-        float t;
-        load_ray(rays, i, &ro, &rd, &t);
-        float u, v;
-        int32_t prim = trace_stackless_q<kCount>(sc.q_nodes, sc.q_up, sc.q_root, sc.prims, sc.sphere_begin, ro, rd,
-                                                 &t, &u, &v, &c);
-        store_hit(hits, i, prim, t, u, v);
-    }
-    if (kCount) flush_counts(counters, c);
-}
-
-// Per-ray walk state of each variant for the persistent kernels.
-template <int kVariant> struct WalkState { using type = StacklessRay; };
-template <> struct WalkState<kVariantThreaded> { using type = ThreadedRay; };
-template <> struct WalkState<kVariantCompressed> { using type = StacklessQRay; };
-
 // Persistent wave64 traversal: wave w owns rays [w * chunk, (w + 1) * chunk) and hands them to
 // its lanes as lanes finish, refilling whenever at least `refill` lanes are idle (all at once
 // when the wave is idle), so that lanes do not wait for the slowest ray of the wave. No
@@ -132,7 +109,8 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
     uint64_t next = (uint64_t)wave * chunk;
     const uint64_t end = next + chunk < (uint64_t)n ? next + chunk : (uint64_t)n;
     uint32_t ray = kInvalid;
-    typename WalkState<kVariant>::type st;
+    using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
+    State st;
     st.node = kInvalid;
     for (;;) {
         const bool idle = st.node == kInvalid;
@@ -148,8 +126,6 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
                     load_ray(rays, ray, &ro, &rd, &tmax);
                     if constexpr (kVariant == kVariantThreaded)
                         threaded_begin(st, sc.mt_nodes, sc.nnodes, ro, rd, tmax);
-                    else if constexpr (kVariant == kVariantCompressed)
-                        stackless_q_begin(st, sc.q_root, ro, rd, tmax);
                     else
                         stackless_begin(st, sc.sl_nodes, ro, rd, tmax);
                 }
@@ -160,8 +136,6 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
         if (st.node != kInvalid) {
             if constexpr (kVariant == kVariantThreaded)
                 threaded_step(st, sc.prims, sc.sphere_begin);
-            else if constexpr (kVariant == kVariantCompressed)
-                stackless_q_step(st, sc.q_nodes, sc.q_up, sc.prims, sc.sphere_begin);
             else
                 stackless_step(st, sc.sl_nodes, sc.prims, sc.sphere_begin);
             if (st.node == kInvalid) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
@@ -190,16 +164,12 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
         // measured on the 1.3M-triangle incoherent set: the stackless walk prefers early refills,
         // the threaded one late ones
         const uint32_t refill = tune && tune->refill ? tune->refill : (variant == kVariantThreaded ? 48u : 16u);
-        if (variant == kVariantCompressed && !sc.q_nodes) return hipErrorInvalidValue;
         const uint32_t blocks = cus * wps;  // 4 waves (one per SIMD) per 256-thread block
         const uint32_t waves = blocks * 4u;
         const uint32_t chunk = (n + waves - 1) / waves;
         if (variant == kVariantThreaded)
             hipLaunchKernelGGL(k_trace_persistent<kVariantThreaded>, dim3(blocks), dim3(256), 0, stream, rays, hits,
                                n, sc, chunk, refill);
-        else if (variant == kVariantCompressed)
-            hipLaunchKernelGGL(k_trace_persistent<kVariantCompressed>, dim3(blocks), dim3(256), 0, stream, rays,
-                               hits, n, sc, chunk, refill);
         else
             hipLaunchKernelGGL(k_trace_persistent<kVariantStackless>, dim3(blocks), dim3(256), 0, stream, rays,
                                hits, n, sc, chunk, refill);
@@ -209,12 +179,6 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
     dim3 grid((n + block - 1) / block);
     if (sc.nnodes == 0) {
         hipLaunchKernelGGL(k_trace_empty, grid, dim3(block), 0, stream, rays, hits, n);
-    } else if (variant == kVariantCompressed) {
-        if (!sc.q_nodes) return hipErrorInvalidValue;
-        if (counters)
-            hipLaunchKernelGGL(k_trace_compressed<true>, grid, dim3(block), 0, stream, rays, hits, n, sc, counters);
-        else
-            hipLaunchKernelGGL(k_trace_compressed<false>, grid, dim3(block), 0, stream, rays, hits, n, sc, counters);
     } else if (variant == kVariantThreaded) {
         if (counters)
             hipLaunchKernelGGL(k_trace_threaded<true>, grid, dim3(block), 0, stream, rays, hits, n, sc.mt_nodes,

@@ -543,8 +543,11 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         }
         // The reference samples and evaluates the cosine lobe and the envmap pdf with the
         // unflipped normal (kernels/lambertian.cl:59-64, envmap_sampling.cl:263-265); the
-        // consistent-normals option flips it toward wo like the shading does.
-        const F3 Ns = p.consistent ? flip_toward(Ng, wo) : Ng;
+        // consistent-normals option flips it toward wo like the shading does. A guided vertex
+        // always flips: at a back face the reference's envmap pdf of a direction the envmap
+        // did not sample comes from the opposite tables, which only the guide's samples (not
+        // the cosine lobe's, which point into the surface) would expose as bias.
+        const F3 Ns = (p.consistent || guide_here) ? flip_toward(Ng, wo) : Ng;
         float cosTheta;
         if (bxdf) {
             float u0 = xo_uniform(st);
@@ -624,7 +627,7 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jo
             wi = sample_ward(w, wo, Ngf, u0, u1);
         }
         float pdf_b = pdf_ward(w, wo, wi, Ngf);
-        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, p.consistent ? Ngf : Ng);
+        if (strat != kStratEnv) pdf_e = pdf_env_6axis(env, wi, (p.consistent || guide_here) ? Ngf : Ng);
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;

@@ -1,11 +1,8 @@
-"""GPU parity of extension ray casting (the traversal variants) against the oracle.
+"""GPU parity of extension ray casting (both traversal variants) against the oracle.
 
 Bit-exact bar: primitive index, t, u and v of every ray equal the oracle's bits when
 the oracle runs the reference algorithm (kernels/extension_ray_cast_stackless.cl,
 common/threaded_bvh.hpp:527-569) on the reference layouts restated from the same tree.
-The compressed variant (8-bit child planes) is held to the tie-excepted bar: no ray may
-differ from the oracle's stackless walk unless its t ties within 1e-6 relative; where the
-primitive agrees, t, u and v are the same bits; ties are counted.
 """
 import numpy as np
 import pytest
@@ -61,61 +58,6 @@ def test_bit_exact_small_scenes(ctx, variant):
         scene.close()
 
 
-def _compressed_vs_oracle(g, o, label):
-    ties, wrong = H.tie_excepted_mismatch(g, o)
-    same = g["prim"] == o["prim"]
-    tbits = int((g["t"].view(np.uint32) != o["t"].view(np.uint32))[same].sum())
-    hit = same & (o["prim"] >= 0)
-    uvbits = int(((g["u"].view(np.uint32) != o["u"].view(np.uint32)) |
-                  (g["v"].view(np.uint32) != o["v"].view(np.uint32)))[hit].sum())
-    print(f"{label}: ties {ties} of {len(g)}", flush=True)
-    assert wrong == 0 and tbits == 0 and uvbits == 0, (label, wrong, tbits, uvbits)
-    return ties
-
-
-def test_compressed_small_scenes_against_oracle(ctx):
-    total, ties = 0, 0
-    for name, v, f, sph in _meshes():
-        bvh = _lib.BVH(v, f, sph)
-        scene = _lib.Scene(ctx, bvh)
-        tree = bvh.export()
-        for rname, rays in _ray_sets(v).items():
-            print(f"case {name}/{rname}/compressed n={len(rays)}", flush=True)
-            g = _lib.trace(ctx, scene, rays, "compressed", block=256)
-            o, _ = _oracle(tree, v, f, sph, rays, "stackless")
-            ties += _compressed_vs_oracle(g, o, f"{name}/{rname}")
-            total += len(rays)
-            # the persistent kernel walks the same records in the same order
-            p = _lib.trace(ctx, scene, rays, "compressed")
-            assert H.compare_hits(p, g) == dict(prim=0, t=0, uv=0), (name, rname)
-        scene.close()
-    assert ties <= 1e-3 * total
-
-
-def test_compressed_persistent_shapes_and_counts(ctx):
-    v, f = scenes.displaced_icosphere(level=5)
-    scene = _lib.Scene(ctx, _lib.BVH(v, f))
-    rays = scenes.incoherent_rays(30011, v.min(0), v.max(0), seed=23)
-    ref = _lib.trace(ctx, scene, rays, "compressed", block=64)
-    try:
-        for wps, refill in ((1, 1), (8, 64), (16, 8), (0, 0)):
-            ctx.set_trace_tuning(wps, refill)
-            g = _lib.trace(ctx, scene, rays, "compressed")
-            assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (wps, refill)
-    finally:
-        ctx.set_trace_tuning(0, 0)
-    # the counting kernel returns the same hits; looser boxes mean at least as many branch visits
-    rd = ctx.to_device(rays)
-    hd = ctx.alloc(len(rays) * 16)
-    cq = scene.extension_ray_cast_stats("compressed", rd, hd, len(rays))
-    hq = hd.download(_lib.HIT_DTYPE, (len(rays),))
-    assert H.compare_hits(hq, ref) == dict(prim=0, t=0, uv=0)
-    ce = scene.extension_ray_cast_stats("stackless", rd, hd, len(rays))
-    print(f"exact {ce} compressed {cq}", flush=True)
-    assert ce["branch_visits"] <= cq["branch_visits"] <= 1.5 * ce["branch_visits"]
-    assert cq["prim_tests"] >= ce["prim_tests"]
-
-
 def test_variants_agree_and_block_sizes_identical(ctx):
     v, f = scenes.displaced_icosphere(level=5)
     bvh = _lib.BVH(v, f)
@@ -157,7 +99,7 @@ def test_empty_and_single_triangle(ctx):
     bvh = _lib.BVH(v, f)
     scene = _lib.Scene(ctx, bvh)
     rays = scenes.incoherent_rays(1000, [-1, -1, -1], [1, 1, 1], seed=1)
-    for variant in ("stackless", "threaded", "compressed"):
+    for variant in ("stackless", "threaded"):
         g = _lib.trace(ctx, scene, rays, variant)
         assert np.all(g["prim"] == -1) and np.array_equal(g["t"], rays["tmax"])
     v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
@@ -166,7 +108,7 @@ def test_empty_and_single_triangle(ctx):
     scene = _lib.Scene(ctx, bvh)
     rays = scenes.make_rays(np.array([[0.2, 0.2, 1.0], [2, 2, 1.0], [0.1, 0.1, -1.0]]),
                             np.array([[0, 0, -1.0], [0, 0, -1.0], [0, 0, 1.0]]))
-    for variant in ("stackless", "threaded", "compressed"):
+    for variant in ("stackless", "threaded"):
         g = _lib.trace(ctx, scene, rays, variant)
         assert g["prim"].tolist() == [0, -1, 0]
         assert g["t"][0] == 1.0 and g["t"][2] == 1.0
@@ -211,18 +153,4 @@ def test_bit_exact_raycast_benchmark_full_size(ctx, c2, variant, rayset):
     bf = O.intersect_bruteforce(v, f, rays[idx])
     ties, wrong = H.tie_excepted_mismatch(o[idx], bf)
     assert wrong == 0 and ties <= 2
-    scene.close()
-
-
-@pytest.mark.parametrize("rayset", ["coherent", "incoherent"])
-def test_compressed_raycast_benchmark_full_size(ctx, c2, rayset):
-    """1,310,720 triangles, 4,194,304 rays per set: compressed walk == oracle up to t-ties."""
-    v, f = c2["points"], c2["indices"]
-    scene = _lib.Scene(ctx, c2["bvh"])
-    rays = c2[rayset]
-    print(f"case c2/{rayset}/compressed", flush=True)
-    g = _lib.trace(ctx, scene, rays, "compressed")
-    o, _ = _oracle(c2["tree"], v, f, None, rays, "stackless")
-    ties = _compressed_vs_oracle(g, o, f"c2/{rayset}")
-    assert ties <= 1e-4 * len(rays)
     scene.close()
