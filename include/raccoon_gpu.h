@@ -146,12 +146,13 @@ int rgpu_scene_bytes(const rgpu_scene* scene, uint64_t* out4);
 int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene* scene, int variant,
                             const rgpu_ray* rays, rgpu_hit* hits, uint32_t count, uint32_t block);
 /* Launch shape of the persistent ray-cast kernel used when block == 0: resident waves per
- * SIMD (grid = CUs * waves_per_simd workgroups of 256; 1..16) and the number of idle lanes of a
- * wave that makes it hand out its next rays (1..64). 0 keeps the default (8 waves; refill at 16
- * idle lanes for stackless, 48 for threaded). No
- * counterpart in the reference (it leaves launch shapes to the OpenCL driver,
+ * SIMD (grid = CUs * waves_per_simd workgroups of 256; 1..16), the number of idle lanes of a
+ * wave that makes it hand out its next rays (1..64), and the number of lanes waiting at leaves
+ * that makes it test their primitives (1..64). 0 keeps the default (8 waves; refill at 32 idle
+ * lanes for stackless, 48 for threaded; leaves at 8 waiting lanes). Results do not depend on
+ * them. No counterpart in the reference (it leaves launch shapes to the OpenCL driver,
  * common/raccoon_ocl.hpp:906). */
-int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill);
+int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill, uint32_t leaf_batch);
 
 /* Instrumented extension ray cast: the same traversal (identical hits) with per-ray event
  * counting, reduced to 5 totals {stackless branch visits, stackless leaf visits, stackless

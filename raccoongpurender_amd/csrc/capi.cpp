@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -312,6 +313,10 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     s->dev.spheres = (const float4*)s->spheres;
     s->dev.nnodes = (uint32_t)bvh->tree.nodes.size();
     s->dev.sphere_begin = (int32_t)ntris;
+    bool finite = true;
+    for (const BuildNode& b : bvh->tree.nodes)
+        for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(b.bmin[k]) && std::isfinite(b.bmax[k]);
+    s->dev.boxes_finite = finite ? 1u : 0u;
     s->nprims = ntris + nspheres;
     if (!bvh->tree.nodes.empty())
         for (int k = 0; k < 3; ++k) {
@@ -348,15 +353,17 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
     tune.cu_count = (uint32_t)ctx->cu_count;
     tune.waves_per_simd = ctx->trace_waves;
     tune.refill = ctx->trace_refill;
+    tune.leaf_batch = ctx->trace_leaf_batch;
     RGPU_HIP(ctx, launch_trace(scene->dev, variant, (const TraceRay*)rays, (TraceHit*)hits, count, block,
                                stream_of(ctx, stream), nullptr, &tune));
     return RGPU_OK;
 }
 
-int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill) {
-    if (!ctx || waves_per_simd > 16 || refill > 64) return RGPU_ERR_INVALID_ARG;
+int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill, uint32_t leaf_batch) {
+    if (!ctx || waves_per_simd > 16 || refill > 64 || leaf_batch > 64) return RGPU_ERR_INVALID_ARG;
     ctx->trace_waves = waves_per_simd;
     ctx->trace_refill = refill;
+    ctx->trace_leaf_batch = leaf_batch;
     return RGPU_OK;
 }
 

@@ -34,17 +34,21 @@ struct TraceScene {
     const float4* spheres = nullptr;
     uint32_t nnodes = 0;
     int32_t sphere_begin = 0;
+    uint32_t boxes_finite = 0;  // every node box finite (the slab test may drop its NaN guard)
 };
 
 // Launch shape of the persistent ray-cast kernel: CUs to fill, resident waves per SIMD (grid =
-// cu_count * waves_per_simd workgroups of 256), and the idle-lane count that triggers a refill.
-// Zeros take the defaults (256 CUs, 8 waves; refill at 16 lanes for the stackless walk, 48 for
-// the threaded one).
+// cu_count * waves_per_simd workgroups of 256), the idle-lane count that triggers a refill and
+// the count of lanes waiting at leaves that triggers their primitive tests. Zeros take the
+// defaults (256 CUs, 8 waves; refill at 32 lanes for the stackless walk, 48 for the threaded
+// one; leaves at 8), measured on 4M incoherent rays over 1.3M triangles.
 struct TraceTuning {
     uint32_t cu_count = 0;
     uint32_t waves_per_simd = 0;
     uint32_t refill = 0;
+    uint32_t leaf_batch = 0;
 };
+constexpr uint32_t kDefaultLeafBatch = 8;
 
 // counters: null for the production kernels; otherwise 5 device u64 totals
 // {branch visits, leaf visits, parent steps, primitive tests, threaded node visits}.

@@ -19,27 +19,64 @@ __device__ __forceinline__ F3 cross3(F3 a, F3 b) {
 }
 
 // One axis of the slab test: entry/exit parameters with the NaN substitution of
-// kernels/slab.cl:15-16 (t0 first, then t1 from the updated t0).
+// kernels/slab.cl:15-16 (t0 first, then t1 from the updated t0). kPlain leaves the
+// substitution out; callers choose it only where neither product can be NaN, so both forms
+// give the same bits there.
+template <bool kPlain = false>
 __device__ __forceinline__ void slab_axis(float lo, float hi, float o, float inv, float* tn, float* tf) {
     float t0 = (lo - o) * inv;
     float t1 = (hi - o) * inv;
-    t0 = __builtin_isnan(t0) ? -t1 : t0;
-    t1 = __builtin_isnan(t1) ? -t0 : t1;
+    if (!kPlain) {
+        t0 = __builtin_isnan(t0) ? -t1 : t0;
+        t1 = __builtin_isnan(t1) ? -t0 : t1;
+    }
     *tn = fminf(t0, t1);
     *tf = fmaxf(t0, t1);
 }
 
+// (plane - o) * inv is never NaN when the planes and the origin are finite and every inverse
+// direction component is finite and non-zero: a finite or overflowed difference times a finite
+// non-zero factor. Then the NaN substitution of the slab test never fires. boxes_finite is a
+// property of the scene (all node boxes finite).
+__device__ __forceinline__ bool plain_slabs(bool boxes_finite, F3 o, F3 inv) {
+    const bool fo = __builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z);
+    const bool fi = __builtin_isfinite(inv.x) && __builtin_isfinite(inv.y) && __builtin_isfinite(inv.z) &&
+                    inv.x != 0.0f && inv.y != 0.0f && inv.z != 0.0f;
+    return boxes_finite && fo && fi;
+}
+
+template <bool kPlain = false>
 __device__ __forceinline__ bool slab_hit(const float* lo, const float* hi, F3 o, F3 inv, float farclip,
                                          float* h0, float* h1) {
     float nx, fx, ny, fy, nz, fz;
-    slab_axis(lo[0], hi[0], o.x, inv.x, &nx, &fx);
-    slab_axis(lo[1], hi[1], o.y, inv.y, &ny, &fy);
-    slab_axis(lo[2], hi[2], o.z, inv.z, &nz, &fz);
+    slab_axis<kPlain>(lo[0], hi[0], o.x, inv.x, &nx, &fx);
+    slab_axis<kPlain>(lo[1], hi[1], o.y, inv.y, &ny, &fy);
+    slab_axis<kPlain>(lo[2], hi[2], o.z, inv.z, &nz, &fz);
     float rmin = fmaxf(fmaxf(nx, ny), nz);
     float rmax = fminf(fminf(fx, fy), fz);
     *h0 = fmaxf(0.0f, rmin);
     *h1 = fminf(rmax, farclip);
     return rmin <= rmax && 0.0f <= rmax && rmin <= farclip;
+}
+
+// Both child boxes of a stackless node; the wave takes the plain slab form when every active
+// lane may (a wave-uniform branch, so no lane runs both forms).
+__device__ __forceinline__ void slab_pair(const float* lmin, const float* lmax, const float* rmin, const float* rmax,
+                                          F3 o, F3 inv, float farclip, bool plain, bool* L, bool* R, float* Lh0,
+                                          float* Lh1, float* Rh0, float* Rh1) {
+    if (__all(plain)) {
+        *L = slab_hit<true>(lmin, lmax, o, inv, farclip, Lh0, Lh1);
+        *R = slab_hit<true>(rmin, rmax, o, inv, farclip, Rh0, Rh1);
+    } else {
+        *L = slab_hit<false>(lmin, lmax, o, inv, farclip, Lh0, Lh1);
+        *R = slab_hit<false>(rmin, rmax, o, inv, farclip, Rh0, Rh1);
+    }
+}
+
+__device__ __forceinline__ bool slab_one(const float* lo, const float* hi, F3 o, F3 inv, float farclip, bool plain,
+                                         float* h0, float* h1) {
+    if (__all(plain)) return slab_hit<true>(lo, hi, o, inv, farclip, h0, h1);
+    return slab_hit<false>(lo, hi, o, inv, farclip, h0, h1);
 }
 
 // Test one primitive record; updates (tmin, u, v, prim) when it is the closest so far.
@@ -131,9 +168,10 @@ struct ShortStack {
 template <bool kCount>
 __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ nodes,
                                                    const PrimRec* __restrict__ prims, int32_t sphere_begin,
-                                                   F3 ro, F3 rd, float* tmin_io, float* uo, float* vo,
-                                                   Cnt* cnt) {
+                                                   bool boxes_finite, F3 ro, F3 rd, float* tmin_io, float* uo,
+                                                   float* vo, Cnt* cnt) {
     const F3 inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    const bool plain = plain_slabs(boxes_finite, ro, inv);
     float tmin = *tmin_io;
     float u = 0.0f, v = 0.0f;
     int32_t prim = -1;
@@ -142,17 +180,20 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
     ShortStack stk;
     stk.clear();
     const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
-    uint4 q3 = q3p[4 * node + 3];  // {a, b, parent, sibling}
     for (;;) {
+        // the whole record in one round trip: the boxes are needed unless the node is a leaf,
+        // and waiting for {a, b, parent, sibling} before asking for them would double the
+        // latency of every branch visit
+        const float4* rec = reinterpret_cast<const float4*>(nodes + node);
+        const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+        const uint4 q3 = q3p[4 * node + 3];  // {a, b, parent, sibling}
         if ((q3.x & kLeafFlag) == 0) {
             if (kCount) cnt->branch++;
-            const float4* rec = reinterpret_cast<const float4*>(nodes + node);
-            float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
             float lmin[3] = {q0.x, q0.y, q0.z}, lmax[3] = {q0.w, q1.x, q1.y};
             float rmin[3] = {q1.z, q1.w, q2.x}, rmax[3] = {q2.y, q2.z, q2.w};
             float Lh0, Lh1, Rh0, Rh1;
-            bool L = slab_hit(lmin, lmax, ro, inv, tmin, &Lh0, &Lh1);
-            bool R = slab_hit(rmin, rmax, ro, inv, tmin, &Rh0, &Rh1);
+            bool L, R;
+            slab_pair(lmin, lmax, rmin, rmax, ro, inv, tmin, plain, &L, &R, &Lh0, &Lh1, &Rh0, &Rh1);
             if (L || R) {
                 bitstack = bitstack << 1;
                 if (L && R) {
@@ -163,7 +204,6 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
                 } else {
                     node = L ? q3.x : q3.y;
                 }
-                q3 = q3p[4 * node + 3];
                 continue;
             }
         } else {
@@ -183,24 +223,23 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
             }
             node = stk.pop();
             bitstack = bitstack ^ 1;
-            q3 = q3p[4 * node + 3];
             continue;
         }
         bool done = false;
+        uint4 q = q3;
         while ((bitstack & 1) == 0) {
             if (bitstack == 0) {
                 done = true;
                 break;
             }
-            node = q3.z;
+            node = q.z;
             if (kCount) cnt->parent++;
             bitstack = bitstack >> 1;
-            if (bitstack != 0) q3 = q3p[4 * node + 3];
+            if (bitstack != 0) q = q3p[4 * node + 3];
         }
         if (done) break;
-        node = q3.w;
+        node = q.w;
         bitstack = bitstack ^ 1;
-        q3 = q3p[4 * node + 3];
     }
     *tmin_io = tmin;
     *uo = u;
@@ -222,10 +261,11 @@ __device__ __forceinline__ uint32_t direction_class(F3 rd) {
 template <bool kCount>
 __device__ __forceinline__ int32_t trace_threaded(const MTNode* __restrict__ tables, uint32_t nnodes,
                                                   const PrimRec* __restrict__ prims, int32_t sphere_begin,
-                                                  F3 ro, F3 rd, float* tmin_io, float* uo, float* vo,
-                                                  Cnt* cnt) {
+                                                  bool boxes_finite, F3 ro, F3 rd, float* tmin_io, float* uo,
+                                                  float* vo, Cnt* cnt) {
     const MTNode* tab = tables + (size_t)direction_class(rd) * nnodes;
     const F3 inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    const bool plain = plain_slabs(boxes_finite, ro, inv);
     float tmin = *tmin_io;
     float u = 0.0f, v = 0.0f;
     int32_t prim = -1;
@@ -237,7 +277,7 @@ __device__ __forceinline__ int32_t trace_threaded(const MTNode* __restrict__ tab
         uint32_t miss = __float_as_uint(q1.z), leaf = __float_as_uint(q1.w);
         float h0, h1;
         if (kCount) cnt->node++;
-        bool hit = slab_hit(lo, hi, ro, inv, tmin, &h0, &h1);
+        bool hit = slab_one(lo, hi, ro, inv, tmin, plain, &h0, &h1);
         if (hit && leaf != 0) {
             const uint32_t beg = leaf & 0x0FFFFFFFu;
             const uint32_t end = beg + ((leaf >> 28) & 7u) + 1u;
@@ -263,6 +303,7 @@ struct ThreadedRay {
     float tmin, u, v;
     int32_t prim;
     uint32_t node;  // kInvalid when finished
+    uint32_t leaf;  // primitive word of an entered leaf not yet tested (0: none)
 };
 
 __device__ __forceinline__ void threaded_begin(ThreadedRay& r, const MTNode* tables, uint32_t nnodes, F3 ro, F3 rd,
@@ -275,22 +316,35 @@ __device__ __forceinline__ void threaded_begin(ThreadedRay& r, const MTNode* tab
     r.u = r.v = 0.0f;
     r.prim = -1;
     r.node = 0;
+    r.leaf = 0;
 }
 
-// One node of trace_threaded.
-__device__ __forceinline__ void threaded_step(ThreadedRay& r, const PrimRec* __restrict__ prims, int32_t sphere_begin) {
+// Box test of the current node of trace_threaded: a branch moves on (hit: next record, miss:
+// miss link); an entered leaf is marked pending with its primitive word kept in r.leaf.
+__device__ __forceinline__ void threaded_visit(ThreadedRay& r, bool boxes_finite) {
     const float4* rec = reinterpret_cast<const float4*>(r.tab + r.node);
     float4 q0 = rec[0], q1 = rec[1];
     float lo[3] = {q0.x, q0.y, q0.z}, hi[3] = {q0.w, q1.x, q1.y};
     uint32_t miss = __float_as_uint(q1.z), leaf = __float_as_uint(q1.w);
     float h0, h1;
-    bool hit = slab_hit(lo, hi, r.ro, r.inv, r.tmin, &h0, &h1);
-    if (hit && leaf != 0) {
-        const uint32_t beg = leaf & 0x0FFFFFFFu;
-        const uint32_t end = beg + ((leaf >> 28) & 7u) + 1u;
-        for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
-    }
+    bool hit = slab_one(lo, hi, r.ro, r.inv, r.tmin, plain_slabs(boxes_finite, r.ro, r.inv), &h0, &h1);
+    if (hit && leaf != 0) r.leaf = leaf;  // the miss link is where the walk goes on afterwards
     r.node = (hit && leaf == 0) ? r.node + 1 : miss;
+}
+
+// The pending leaf of a lane: its primitives; the walk goes on at the miss link already in node.
+__device__ __forceinline__ void threaded_leaf(ThreadedRay& r, const PrimRec* __restrict__ prims, int32_t sphere_begin) {
+    const uint32_t beg = r.leaf & 0x0FFFFFFFu;
+    const uint32_t end = beg + ((r.leaf >> 28) & 7u) + 1u;
+    for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
+    r.leaf = 0;
+}
+
+// One node of trace_threaded.
+__device__ __forceinline__ void threaded_step(ThreadedRay& r, const PrimRec* __restrict__ prims, int32_t sphere_begin,
+                                              bool boxes_finite) {
+    threaded_visit(r, boxes_finite);
+    if (r.leaf != 0) threaded_leaf(r, prims, sphere_begin);
 }
 
 struct StacklessRay {
@@ -299,12 +353,10 @@ struct StacklessRay {
     int32_t prim;
     uint64_t bitstack;
     uint32_t node;  // kInvalid when finished
-    uint4 q3;       // {a, b, parent, sibling} of node
     ShortStack stk;
 };
 
-__device__ __forceinline__ void stackless_begin(StacklessRay& r, const SLNode* __restrict__ nodes, F3 ro, F3 rd,
-                                                float tmax) {
+__device__ __forceinline__ void stackless_begin(StacklessRay& r, F3 ro, F3 rd, float tmax) {
     r.ro = ro;
     r.rd = rd;
     r.inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
@@ -314,40 +366,17 @@ __device__ __forceinline__ void stackless_begin(StacklessRay& r, const SLNode* _
     r.bitstack = 0;
     r.node = 0;
     r.stk.clear();
-    r.q3 = reinterpret_cast<const uint4*>(nodes)[3];
 }
 
 // One pass of the loop body of trace_stackless: a branch descent, or a leaf followed by its
 // backtrack to the next sibling (or the end of the traversal).
-__device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __restrict__ nodes,
-                                               const PrimRec* __restrict__ prims, int32_t sphere_begin) {
-    const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
-    if ((r.q3.x & kLeafFlag) == 0) {
-        const float4* rec = reinterpret_cast<const float4*>(nodes + r.node);
-        float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-        float lmin[3] = {q0.x, q0.y, q0.z}, lmax[3] = {q0.w, q1.x, q1.y};
-        float rmin[3] = {q1.z, q1.w, q2.x}, rmax[3] = {q2.y, q2.z, q2.w};
-        float Lh0, Lh1, Rh0, Rh1;
-        bool L = slab_hit(lmin, lmax, r.ro, r.inv, r.tmin, &Lh0, &Lh1);
-        bool R = slab_hit(rmin, rmax, r.ro, r.inv, r.tmin, &Rh0, &Rh1);
-        if (L || R) {
-            r.bitstack = r.bitstack << 1;
-            if (L && R) {
-                bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
-                r.node = nearL ? r.q3.x : r.q3.y;
-                r.stk.push(nearL ? r.q3.y : r.q3.x);
-                r.bitstack = r.bitstack | 1;
-            } else {
-                r.node = L ? r.q3.x : r.q3.y;
-            }
-            r.q3 = q3p[4 * r.node + 3];
-            return;
-        }
-    } else {
-        const uint32_t beg = r.q3.x & ~kLeafFlag, end = r.q3.y;
-        for (uint32_t s = beg; s < end; ++s)
-            test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
-    }
+// Pending-leaf mark of a resumable walk: the lane has reached a leaf and waits for its wave to
+// run the leaves together (node ids stay below 2^31; kInvalid is never marked).
+static const uint32_t kPendingLeaf = 0x80000000u;
+
+// The end of a stackless step: the backtrack from the current node (record word q3) to the
+// next sibling, or the end of the walk.
+__device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const uint4* __restrict__ q3p, uint4 q3) {
     if (r.bitstack == 0) {
         r.node = kInvalid;
         return;
@@ -356,21 +385,72 @@ __device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __
         while ((r.bitstack & 1) == 0) r.bitstack = r.bitstack >> 1;
         r.node = r.stk.pop();
         r.bitstack = r.bitstack ^ 1;
-        r.q3 = q3p[4 * r.node + 3];
         return;
     }
+    uint4 q = q3;
     while ((r.bitstack & 1) == 0) {
         if (r.bitstack == 0) {
             r.node = kInvalid;
             return;
         }
-        r.node = r.q3.z;
+        r.node = q.z;
         r.bitstack = r.bitstack >> 1;
-        if (r.bitstack != 0) r.q3 = q3p[4 * r.node + 3];
+        if (r.bitstack != 0) q = q3p[4 * r.node + 3];
     }
-    r.node = r.q3.w;
+    r.node = q.w;
     r.bitstack = r.bitstack ^ 1;
-    r.q3 = q3p[4 * r.node + 3];
+}
+
+// Visit of a branch node of trace_stackless (descent, or backtrack when neither child is
+// entered). Reaching a leaf marks it pending instead of testing its primitives.
+__device__ __forceinline__ void stackless_visit(StacklessRay& r, const SLNode* __restrict__ nodes, bool boxes_finite) {
+    const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
+    const float4* rec = reinterpret_cast<const float4*>(nodes + r.node);
+    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
+    const uint4 q3 = q3p[4 * r.node + 3];  // the whole record in one round trip
+    if (q3.x & kLeafFlag) {
+        r.node |= kPendingLeaf;
+        return;
+    }
+    float lmin[3] = {q0.x, q0.y, q0.z}, lmax[3] = {q0.w, q1.x, q1.y};
+    float rmin[3] = {q1.z, q1.w, q2.x}, rmax[3] = {q2.y, q2.z, q2.w};
+    float Lh0, Lh1, Rh0, Rh1;
+    bool L, R;
+    slab_pair(lmin, lmax, rmin, rmax, r.ro, r.inv, r.tmin, plain_slabs(boxes_finite, r.ro, r.inv), &L, &R, &Lh0, &Lh1,
+              &Rh0, &Rh1);
+    if (L || R) {
+        r.bitstack = r.bitstack << 1;
+        if (L && R) {
+            bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
+            r.node = nearL ? q3.x : q3.y;
+            r.stk.push(nearL ? q3.y : q3.x);
+            r.bitstack = r.bitstack | 1;
+        } else {
+            r.node = L ? q3.x : q3.y;
+        }
+        return;
+    }
+    stackless_backtrack(r, q3p, q3);
+}
+
+// The pending leaf of a lane: its primitives, then the backtrack.
+__device__ __forceinline__ void stackless_leaf(StacklessRay& r, const SLNode* __restrict__ nodes,
+                                               const PrimRec* __restrict__ prims, int32_t sphere_begin) {
+    const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
+    r.node &= ~kPendingLeaf;
+    const uint4 q3 = q3p[4 * r.node + 3];
+    const uint32_t beg = q3.x & ~kLeafFlag, end = q3.y;
+    for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
+    stackless_backtrack(r, q3p, q3);
+}
+
+// One pass of the loop body of trace_stackless: a branch descent, or a leaf followed by its
+// backtrack to the next sibling (or the end of the traversal).
+__device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __restrict__ nodes,
+                                               const PrimRec* __restrict__ prims, int32_t sphere_begin,
+                                               bool boxes_finite) {
+    stackless_visit(r, nodes, boxes_finite);
+    if (r.node != kInvalid && (r.node & kPendingLeaf)) stackless_leaf(r, nodes, prims, sphere_begin);
 }
 
 }  // namespace rgpu

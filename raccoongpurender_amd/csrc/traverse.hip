@@ -62,7 +62,8 @@ __global__ void __launch_bounds__(256) k_trace_stackless(const TraceRay* __restr
                                                          TraceHit* __restrict__ hits, uint32_t n,
                                                          const SLNode* __restrict__ nodes,
                                                          const PrimRec* __restrict__ prims,
-                                                         int32_t sphere_begin, unsigned long long* counters) {
+                                                         int32_t sphere_begin, uint32_t boxes_finite,
+                                                         unsigned long long* counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Cnt c;
     if (i < n) {
@@ -70,7 +71,7 @@ __global__ void __launch_bounds__(256) k_trace_stackless(const TraceRay* __restr
         float t;
         load_ray(rays, i, &ro, &rd, &t);
         float u, v;
-        int32_t prim = trace_stackless<kCount>(nodes, prims, sphere_begin, ro, rd, &t, &u, &v, &c);
+        int32_t prim = trace_stackless<kCount>(nodes, prims, sphere_begin, boxes_finite != 0, ro, rd, &t, &u, &v, &c);
         store_hit(hits, i, prim, t, u, v);
     }
     if (kCount) flush_counts(counters, c);
@@ -81,7 +82,8 @@ __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restri
                                                         TraceHit* __restrict__ hits, uint32_t n,
                                                         const MTNode* __restrict__ tables, uint32_t nnodes,
                                                         const PrimRec* __restrict__ prims,
-                                                        int32_t sphere_begin, unsigned long long* counters) {
+                                                        int32_t sphere_begin, uint32_t boxes_finite,
+                                                        unsigned long long* counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Cnt c;
     if (i < n) {
@@ -89,20 +91,34 @@ __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restri
         float t;
         load_ray(rays, i, &ro, &rd, &t);
         float u, v;
-        int32_t prim = trace_threaded<kCount>(tables, nnodes, prims, sphere_begin, ro, rd, &t, &u, &v, &c);
+        int32_t prim =
+            trace_threaded<kCount>(tables, nnodes, prims, sphere_begin, boxes_finite != 0, ro, rd, &t, &u, &v, &c);
         store_hit(hits, i, prim, t, u, v);
     }
     if (kCount) flush_counts(counters, c);
 }
 
+// Lane states of the persistent walks: live (a ray in progress), pending (stopped at an entered
+// leaf whose primitives are not yet tested).
+__device__ __forceinline__ bool walk_live(const StacklessRay& r) { return r.node != kInvalid; }
+__device__ __forceinline__ bool walk_pending(const StacklessRay& r) {
+    return r.node != kInvalid && (r.node & kPendingLeaf) != 0;
+}
+__device__ __forceinline__ bool walk_live(const ThreadedRay& r) { return r.node != kInvalid || r.leaf != 0; }
+__device__ __forceinline__ bool walk_pending(const ThreadedRay& r) { return r.leaf != 0; }
+
 // Persistent wave64 traversal: wave w owns rays [w * chunk, (w + 1) * chunk) and hands them to
 // its lanes as lanes finish, refilling whenever at least `refill` lanes are idle (all at once
 // when the wave is idle), so that lanes do not wait for the slowest ray of the wave. No
 // atomics: a wave's cursor is wave-uniform.
+// Leaves are batched: a lane that enters a leaf stops there, and the wave tests primitives only
+// once `leaf_batch` lanes wait at leaves (or none can walk on), so that the primitive tests run
+// with many lanes at once instead of taxing every node visit of the wave. Each ray still
+// visits its nodes and tests its primitives in the order of the one-ray walk.
 template <int kVariant>
 __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __restrict__ rays,
                                                           TraceHit* __restrict__ hits, uint32_t n, TraceScene sc,
-                                                          uint32_t chunk, uint32_t refill) {
+                                                          uint32_t chunk, uint32_t refill, uint32_t leaf_batch) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -112,8 +128,10 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
     using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
     State st;
     st.node = kInvalid;
+    if constexpr (kVariant == kVariantThreaded) st.leaf = 0;
+    const bool finite = sc.boxes_finite != 0;
     for (;;) {
-        const bool idle = st.node == kInvalid;
+        const bool idle = !walk_live(st);
         const unsigned long long idle_mask = __ballot(idle);
         const uint32_t nidle = (uint32_t)__popcll(idle_mask);
         if (nidle >= refill || nidle == 64u) {
@@ -127,18 +145,29 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
                     if constexpr (kVariant == kVariantThreaded)
                         threaded_begin(st, sc.mt_nodes, sc.nnodes, ro, rd, tmax);
                     else
-                        stackless_begin(st, sc.sl_nodes, ro, rd, tmax);
+                        stackless_begin(st, ro, rd, tmax);
                 }
             }
             next += nidle;
         }
-        if (__ballot(st.node != kInvalid) == 0) break;
-        if (st.node != kInvalid) {
+        if (__ballot(walk_live(st)) == 0) break;
+        const bool pending = walk_pending(st);
+        const bool walking = walk_live(st) && !pending;
+        const uint32_t npending = (uint32_t)__popcll(__ballot(pending));
+        if (npending >= leaf_batch || __ballot(walking) == 0) {
+            if (pending) {
+                if constexpr (kVariant == kVariantThreaded)
+                    threaded_leaf(st, sc.prims, sc.sphere_begin);
+                else
+                    stackless_leaf(st, sc.sl_nodes, sc.prims, sc.sphere_begin);
+                if (!walk_live(st)) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
+            }
+        } else if (walking) {
             if constexpr (kVariant == kVariantThreaded)
-                threaded_step(st, sc.prims, sc.sphere_begin);
+                threaded_visit(st, finite);
             else
-                stackless_step(st, sc.sl_nodes, sc.prims, sc.sphere_begin);
-            if (st.node == kInvalid) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
+                stackless_visit(st, sc.sl_nodes, finite);
+            if (!walk_live(st)) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
         }
     }
 }
@@ -161,18 +190,19 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
         // persistent kernel: as many resident waves as fit, each with a chunk of the rays
         const uint32_t cus = tune && tune->cu_count ? tune->cu_count : 256u;
         const uint32_t wps = tune && tune->waves_per_simd ? tune->waves_per_simd : 8u;
-        // measured on the 1.3M-triangle incoherent set: the stackless walk prefers early refills,
-        // the threaded one late ones
-        const uint32_t refill = tune && tune->refill ? tune->refill : (variant == kVariantThreaded ? 48u : 16u);
+        // measured on the 1.3M-triangle incoherent set: refills at 32 (stackless) or 48 (threaded)
+        // idle lanes, primitive tests once 8 lanes wait at leaves
+        const uint32_t refill = tune && tune->refill ? tune->refill : (variant == kVariantThreaded ? 48u : 32u);
+        const uint32_t leaf_batch = tune && tune->leaf_batch ? tune->leaf_batch : kDefaultLeafBatch;
         const uint32_t blocks = cus * wps;  // 4 waves (one per SIMD) per 256-thread block
         const uint32_t waves = blocks * 4u;
         const uint32_t chunk = (n + waves - 1) / waves;
         if (variant == kVariantThreaded)
             hipLaunchKernelGGL(k_trace_persistent<kVariantThreaded>, dim3(blocks), dim3(256), 0, stream, rays, hits,
-                               n, sc, chunk, refill);
+                               n, sc, chunk, refill, leaf_batch);
         else
             hipLaunchKernelGGL(k_trace_persistent<kVariantStackless>, dim3(blocks), dim3(256), 0, stream, rays,
-                               hits, n, sc, chunk, refill);
+                               hits, n, sc, chunk, refill, leaf_batch);
         return hipGetLastError();
     }
     if (block == 0) block = 256;
@@ -182,17 +212,17 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
     } else if (variant == kVariantThreaded) {
         if (counters)
             hipLaunchKernelGGL(k_trace_threaded<true>, grid, dim3(block), 0, stream, rays, hits, n, sc.mt_nodes,
-                               sc.nnodes, sc.prims, sc.sphere_begin, counters);
+                               sc.nnodes, sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
         else
             hipLaunchKernelGGL(k_trace_threaded<false>, grid, dim3(block), 0, stream, rays, hits, n, sc.mt_nodes,
-                               sc.nnodes, sc.prims, sc.sphere_begin, counters);
+                               sc.nnodes, sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
     } else {
         if (counters)
             hipLaunchKernelGGL(k_trace_stackless<true>, grid, dim3(block), 0, stream, rays, hits, n, sc.sl_nodes,
-                               sc.prims, sc.sphere_begin, counters);
+                               sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
         else
             hipLaunchKernelGGL(k_trace_stackless<false>, grid, dim3(block), 0, stream, rays, hits, n, sc.sl_nodes,
-                               sc.prims, sc.sphere_begin, counters);
+                               sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
     }
     return hipGetLastError();
 }

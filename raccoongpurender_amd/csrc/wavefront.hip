@@ -907,8 +907,10 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jo
     if (sc.nnodes != 0) {
         Cnt cnt;
         prim = kVariant == kVariantThreaded
-                   ? trace_threaded<false>(sc.mt_nodes, sc.nnodes, sc.prims, sc.sphere_begin, ro, rd, &t, &u, &vv, &cnt)
-                   : trace_stackless<false>(sc.sl_nodes, sc.prims, sc.sphere_begin, ro, rd, &t, &u, &vv, &cnt);
+                   ? trace_threaded<false>(sc.mt_nodes, sc.nnodes, sc.prims, sc.sphere_begin, sc.boxes_finite != 0, ro,
+                                           rd, &t, &u, &vv, &cnt)
+                   : trace_stackless<false>(sc.sl_nodes, sc.prims, sc.sphere_begin, sc.boxes_finite != 0, ro, rd, &t,
+                                            &u, &vv, &cnt);
     }
     store_extension(s, sc, slot, ro, rd, prim, t, hit_volume, tmin);
 }
@@ -953,7 +955,7 @@ __global__ void __launch_bounds__(256) k_trace_step_persistent(DevState s, WFPar
                         if constexpr (kVariant == kVariantThreaded)
                             threaded_begin(st, lsc.mt_nodes, lsc.nnodes, ro, rd, tvol);
                         else
-                            stackless_begin(st, lsc.sl_nodes, ro, rd, tvol);
+                            stackless_begin(st, ro, rd, tvol);
                     } else {
                         store_extension(s, lsc, slot, ro, rd, -1, tvol, hit_volume, tvol);
                     }
@@ -967,9 +969,9 @@ __global__ void __launch_bounds__(256) k_trace_step_persistent(DevState s, WFPar
         }
         if (st.node != kInvalid) {
             if constexpr (kVariant == kVariantThreaded)
-                threaded_step(st, lsc.prims, lsc.sphere_begin);
+                threaded_step(st, lsc.prims, lsc.sphere_begin, lsc.boxes_finite != 0);
             else
-                stackless_step(st, lsc.sl_nodes, lsc.prims, lsc.sphere_begin);
+                stackless_step(st, lsc.sl_nodes, lsc.prims, lsc.sphere_begin, lsc.boxes_finite != 0);
             if (st.node == kInvalid) store_extension(s, lsc, slot, st.ro, st.rd, st.prim, st.tmin, hit_volume, tvol);
         }
     }

@@ -83,14 +83,16 @@ def test_persistent_launch_shapes_match_per_ray_kernel(ctx):
         for variant in ("stackless", "threaded"):
             ref = _lib.trace(ctx, scene, rays, variant, block=256)
             try:
-                for wps, refill in ((1, 1), (4, 16), (8, 64), (16, 8), (0, 0)):
-                    ctx.set_trace_tuning(wps, refill)
+                for wps, refill, lb in ((1, 1, 1), (4, 16, 64), (8, 64, 8), (16, 8, 32), (0, 0, 0)):
+                    ctx.set_trace_tuning(wps, refill, lb)
                     g = _lib.trace(ctx, scene, rays, variant)
-                    assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (variant, wps, refill, len(rays))
+                    assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (variant, wps, refill, lb, len(rays))
             finally:
-                ctx.set_trace_tuning(0, 0)
+                ctx.set_trace_tuning(0, 0, 0)
     with pytest.raises(_lib.RgpuError):
         ctx.set_trace_tuning(17, 0)
+    with pytest.raises(_lib.RgpuError):
+        ctx.set_trace_tuning(0, 0, 65)
 
 
 def test_empty_and_single_triangle(ctx):
