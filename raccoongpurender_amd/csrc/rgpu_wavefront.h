@@ -121,6 +121,8 @@ struct DevState {
     float* accum = nullptr;           // 4 floats per pixel (r, g, b, count bits) or
     double* accum64 = nullptr;        // 3 doubles per pixel + counts in accum_count
     uint32_t* accum_count = nullptr;
+    TraceRay* xray = nullptr;         // single render, persistent trace: extension rays (far clip, medium)
+    TraceHit* xhit = nullptr;         // and their hits
     uint32_t nblocks = 0;             // logic blocks (n / 256 rounded up)
 };
 

@@ -977,6 +977,38 @@ __global__ void __launch_bounds__(256) k_trace_step_persistent(DevState s, WFPar
     }
 }
 
+// The same stage for a single render as three launches: the medium draw and far clip of every
+// slot written as a flat ray (the medium id in the spare word), the library's persistent
+// ray-cast kernel over those rays (leaf batching included), and the extension records from the
+// hits. Each slot's traversal and results are those of k_trace_step.
+template <class J>
+__global__ void __launch_bounds__(256) k_trace_setup(DevState s, WFParams p, J jobs) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    advance_cursors(s, jobs, slot);
+    if (slot >= p.n) return;
+    int32_t hit_volume;
+    float tvol;
+    volume_through(s, jobs.mats(0), slot, &hit_volume, &tvol);
+    F3 ro, rd;
+    float tmax;
+    uint32_t pixel;
+    load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
+    float4* x = reinterpret_cast<float4*>(s.xray + slot);
+    x[0] = make_float4(ro.x, ro.y, ro.z, tvol);
+    x[1] = make_float4(rd.x, rd.y, rd.z, __int_as_float(hit_volume));
+}
+
+template <class J>
+__global__ void __launch_bounds__(256) k_trace_finish(DevState s, WFParams p, J jobs) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= p.n) return;
+    const float4* x = reinterpret_cast<const float4*>(s.xray + slot);
+    const float4 a = x[0], b = x[1];
+    const TraceHit h = s.xhit[slot];
+    store_extension(s, jobs.scene(0), slot, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, h.prim, h.t,
+                    __float_as_int(b.w), a.w);
+}
+
 // logic (kernels/logic.cl:117-280): path update, Russian roulette, envmap on a miss,
 // accumulation of finished paths, and the per-block queue counts of a stable compaction.
 template <class J>
@@ -1301,14 +1333,22 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
     hipLaunchKernelGGL(k_shade_volume_surface<J>, qgrid, blk, 0, s, st, p, jobs);
     hipLaunchKernelGGL(k_shade_volume_inside<J>, qgrid, blk, 0, s, st, p, jobs);
     if (trace_block == 0) {
-        // persistent waves: 8 per SIMD over 256 CUs, refill as in the ray-cast kernels
-        const uint32_t blocks = 2048u, chunk = (p.n + blocks * 4u - 1) / (blocks * 4u);
-        if (variant == kVariantThreaded)
-            hipLaunchKernelGGL((k_trace_step_persistent<kVariantThreaded, J>), dim3(blocks), dim3(256), 0, s, st, p,
-                               jobs, chunk, 48u);
-        else
-            hipLaunchKernelGGL((k_trace_step_persistent<kVariantStackless, J>), dim3(blocks), dim3(256), 0, s, st, p,
-                               jobs, chunk, 16u);
+        if constexpr (!J::kBatch) {
+            hipLaunchKernelGGL(k_trace_setup<J>, full, blk, 0, s, st, p, jobs);
+            hipError_t e = launch_trace(jobs.sc, variant, st.xray, st.xhit, p.n, 0, s, nullptr, nullptr);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_trace_finish<J>, full, blk, 0, s, st, p, jobs);
+        } else {
+            // several scenes: persistent waves over the slots, 8 per SIMD over 256 CUs, refill as
+            // in the ray-cast kernels
+            const uint32_t blocks = 2048u, chunk = (p.n + blocks * 4u - 1) / (blocks * 4u);
+            if (variant == kVariantThreaded)
+                hipLaunchKernelGGL((k_trace_step_persistent<kVariantThreaded, J>), dim3(blocks), dim3(256), 0, s, st,
+                                   p, jobs, chunk, 48u);
+            else
+                hipLaunchKernelGGL((k_trace_step_persistent<kVariantStackless, J>), dim3(blocks), dim3(256), 0, s, st,
+                                   p, jobs, chunk, 16u);
+        }
     } else if (variant == kVariantThreaded) {
         hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs);
     } else {

@@ -441,6 +441,15 @@ int rgpu_wavefront_create(rgpu_ctx* ctx, const rgpu_scene* scene, const rgpu_mat
     w->env = env;
     int rc = parse_config(ctx, cfg, w);
     if (rc == RGPU_OK) rc = alloc_slots(w, cfg->path_count);
+    if (rc == RGPU_OK) {
+        // the persistent trace of a single render casts the extension rays through the ray-cast
+        // kernel: 32 + 16 bytes per slot
+        void* ptr = nullptr;
+        rc = wf_alloc(w, (size_t)cfg->path_count * sizeof(TraceRay), &ptr);
+        w->st.xray = (TraceRay*)ptr;
+        if (rc == RGPU_OK) rc = wf_alloc(w, (size_t)cfg->path_count * sizeof(TraceHit), &ptr);
+        w->st.xhit = (TraceHit*)ptr;
+    }
     if (rc != RGPU_OK) {
         rgpu_wavefront_destroy(w);
         return rc;
