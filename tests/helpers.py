@@ -46,12 +46,18 @@ def with_tmax(rays, tmax):
     return r
 
 
+def _bits_differ(x, y):
+    # bitwise, except that two NaNs agree: IEEE 754 leaves the sign and payload of a NaN that an
+    # operation creates to the implementation (x86 and the GPU pick different ones)
+    return (x.view(np.uint32) != y.view(np.uint32)) & ~(np.isnan(x) & np.isnan(y))
+
+
 def compare_hits(a, b):
-    """Counts of disagreements between two hit arrays (bitwise on t, u, v)."""
+    """Counts of disagreements between two hit arrays (bitwise on t, u, v; NaN equals NaN)."""
     prim = int((a["prim"] != b["prim"]).sum())
-    t = int((a["t"].view(np.uint32) != b["t"].view(np.uint32)).sum())
+    t = int(_bits_differ(a["t"], b["t"]).sum())
     hit = a["prim"] >= 0
-    uv = int(((a["u"].view(np.uint32) != b["u"].view(np.uint32)) | (a["v"].view(np.uint32) != b["v"].view(np.uint32)))[hit].sum())
+    uv = int((_bits_differ(a["u"], b["u"]) | _bits_differ(a["v"], b["v"]))[hit].sum())
     return dict(prim=prim, t=t, uv=uv)
 
 

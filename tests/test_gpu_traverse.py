@@ -95,6 +95,32 @@ def test_persistent_launch_shapes_match_per_ray_kernel(ctx):
         ctx.set_trace_tuning(0, 0, 65)
 
 
+def test_nan_guarded_slabs_in_mixed_waves(ctx):
+    # Waves take the slab test without its NaN substitution only when no active lane can form a
+    # NaN product. Rays that can (zero direction components from box-plane origins, a NaN origin)
+    # are spread one in seven among ordinary rays, so most waves hold both kinds; a scene with
+    # an infinite vertex has non-finite boxes and always keeps the substitution.
+    v, f = scenes.displaced_icosphere(level=4)
+    lo, hi = v.min(0), v.max(0)
+    rays = scenes.incoherent_rays(14000, lo, hi, seed=31)
+    deg = H.degenerate_rays(v, 2000, 5)
+    rays[::7] = deg[:len(rays[::7])]
+    rays["ro"][3::97] = np.nan
+    bad_v = np.concatenate([v, np.array([[np.inf, 0.0, 0.0]], np.float32)])
+    bad_f = np.concatenate([f, np.array([[0, 1, len(v)]], np.uint32)])
+    for name, vv, ff in (("finite", v, f), ("infinite vertex", bad_v, bad_f)):
+        bvh = _lib.BVH(vv, ff)
+        scene = _lib.Scene(ctx, bvh)
+        tree = bvh.export()
+        for variant in ("stackless", "threaded"):
+            o, _ = _oracle(tree, vv, ff, None, rays, variant)
+            for block in (0, 64, 256):
+                print(f"case {name}/{variant}/block {block}", flush=True)
+                g = _lib.trace(ctx, scene, rays, variant, block=block)
+                assert H.compare_hits(g, o) == dict(prim=0, t=0, uv=0), (name, variant, block)
+        scene.close()
+
+
 def test_empty_and_single_triangle(ctx):
     v = np.zeros((0, 3), np.float32)
     f = np.zeros((0, 3), np.uint32)
