@@ -149,8 +149,9 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
  * SIMD (grid = CUs * waves_per_simd workgroups of 256; 1..16), the number of idle lanes of a
  * wave that makes it hand out its next rays (1..64), and the number of lanes waiting at leaves
  * that makes it test their primitives (1..64). 0 keeps the default (8 waves; refill at 32 idle
- * lanes for stackless, 48 for threaded; leaves at 8 waiting lanes). Results do not depend on
- * them. No counterpart in the reference (it leaves launch shapes to the OpenCL driver,
+ * lanes for stackless, 48 for threaded; leaves at 8 waiting lanes). A leaf_batch above 1 also
+ * batches leaves in the one-thread-per-ray kernels (block != 0), which by default do not (their
+ * usual rays are coherent). Results do not depend on any of them. No counterpart in the reference (it leaves launch shapes to the OpenCL driver,
  * common/raccoon_ocl.hpp:906). */
 int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill, uint32_t leaf_batch);
 

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
+#include "rgpu_traverse.h"
 #include "rgpu_types.h"
 
 namespace rgpu {
@@ -451,6 +454,54 @@ __device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __
                                                bool boxes_finite) {
     stackless_visit(r, nodes, boxes_finite);
     if (r.node != kInvalid && (r.node & kPendingLeaf)) stackless_leaf(r, nodes, prims, sphere_begin);
+}
+
+// Lane states of the resumable walks: live (a ray in progress), pending (stopped at an entered
+// leaf whose primitives are not yet tested).
+__device__ __forceinline__ bool walk_live(const StacklessRay& r) { return r.node != kInvalid; }
+__device__ __forceinline__ bool walk_pending(const StacklessRay& r) {
+    return r.node != kInvalid && (r.node & kPendingLeaf) != 0;
+}
+__device__ __forceinline__ bool walk_live(const ThreadedRay& r) { return r.node != kInvalid || r.leaf != 0; }
+__device__ __forceinline__ bool walk_pending(const ThreadedRay& r) { return r.leaf != 0; }
+
+// One ray per lane with leaves batched across the wave: a lane that enters a leaf waits until
+// leaf_batch lanes (or every lane still walking) wait too, then the wave tests their primitives
+// together. The lane's own visits and tests are those of trace_stackless / trace_threaded.
+// Lanes whose ray has ended stay in the loop, masked, until the wave is done.
+template <int kVariant>
+__device__ __forceinline__ int32_t trace_batched(const TraceScene& sc, F3 ro, F3 rd, float* tmin_io, float* uo,
+                                                 float* vo, uint32_t leaf_batch) {
+    using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
+    State st;
+    if constexpr (kVariant == kVariantThreaded)
+        threaded_begin(st, sc.mt_nodes, sc.nnodes, ro, rd, *tmin_io);
+    else
+        stackless_begin(st, ro, rd, *tmin_io);
+    const bool finite = sc.boxes_finite != 0;
+    for (;;) {
+        const bool live = walk_live(st);
+        if (__ballot(live) == 0) break;
+        const bool pending = walk_pending(st);
+        const bool walking = live && !pending;
+        if ((uint32_t)__popcll(__ballot(pending)) >= leaf_batch || __ballot(walking) == 0) {
+            if (pending) {
+                if constexpr (kVariant == kVariantThreaded)
+                    threaded_leaf(st, sc.prims, sc.sphere_begin);
+                else
+                    stackless_leaf(st, sc.sl_nodes, sc.prims, sc.sphere_begin);
+            }
+        } else if (walking) {
+            if constexpr (kVariant == kVariantThreaded)
+                threaded_visit(st, finite);
+            else
+                stackless_visit(st, sc.sl_nodes, finite);
+        }
+    }
+    *tmin_io = st.tmin;
+    *uo = st.u;
+    *vo = st.v;
+    return st.prim;
 }
 
 }  // namespace rgpu

@@ -57,13 +57,16 @@ __device__ __forceinline__ void flush_counts(unsigned long long* counters, const
     wave_add(counters + 4, c.node);
 }
 
-template <bool kCount>
+// One thread per ray, in three forms of the same walk (same visits, same results): plain, the
+// walk as one loop (best for coherent rays, whose lanes reach leaves together anyway); count,
+// the same with event counters; batched, leaves batched across the wave (better for incoherent
+// rays, chosen by a leaf_batch above 1).
+enum { kWalkPlain = 0, kWalkCount = 1, kWalkBatched = 2 };
+
+template <int kMode>
 __global__ void __launch_bounds__(256) k_trace_stackless(const TraceRay* __restrict__ rays,
-                                                         TraceHit* __restrict__ hits, uint32_t n,
-                                                         const SLNode* __restrict__ nodes,
-                                                         const PrimRec* __restrict__ prims,
-                                                         int32_t sphere_begin, uint32_t boxes_finite,
-                                                         unsigned long long* counters) {
+                                                         TraceHit* __restrict__ hits, uint32_t n, TraceScene sc,
+                                                         uint32_t leaf_batch, unsigned long long* counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Cnt c;
     if (i < n) {
@@ -71,19 +74,19 @@ __global__ void __launch_bounds__(256) k_trace_stackless(const TraceRay* __restr
         float t;
         load_ray(rays, i, &ro, &rd, &t);
         float u, v;
-        int32_t prim = trace_stackless<kCount>(nodes, prims, sphere_begin, boxes_finite != 0, ro, rd, &t, &u, &v, &c);
+        int32_t prim = kMode == kWalkBatched ? trace_batched<kVariantStackless>(sc, ro, rd, &t, &u, &v, leaf_batch)
+                                         : trace_stackless<kMode == kWalkCount>(sc.sl_nodes, sc.prims, sc.sphere_begin,
+                                                                             sc.boxes_finite != 0, ro, rd, &t, &u,
+                                                                             &v, &c);
         store_hit(hits, i, prim, t, u, v);
     }
-    if (kCount) flush_counts(counters, c);
+    if (kMode == kWalkCount) flush_counts(counters, c);
 }
 
-template <bool kCount>
+template <int kMode>
 __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restrict__ rays,
-                                                        TraceHit* __restrict__ hits, uint32_t n,
-                                                        const MTNode* __restrict__ tables, uint32_t nnodes,
-                                                        const PrimRec* __restrict__ prims,
-                                                        int32_t sphere_begin, uint32_t boxes_finite,
-                                                        unsigned long long* counters) {
+                                                        TraceHit* __restrict__ hits, uint32_t n, TraceScene sc,
+                                                        uint32_t leaf_batch, unsigned long long* counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Cnt c;
     if (i < n) {
@@ -91,21 +94,14 @@ __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restri
         float t;
         load_ray(rays, i, &ro, &rd, &t);
         float u, v;
-        int32_t prim =
-            trace_threaded<kCount>(tables, nnodes, prims, sphere_begin, boxes_finite != 0, ro, rd, &t, &u, &v, &c);
+        int32_t prim = kMode == kWalkBatched ? trace_batched<kVariantThreaded>(sc, ro, rd, &t, &u, &v, leaf_batch)
+                                         : trace_threaded<kMode == kWalkCount>(sc.mt_nodes, sc.nnodes, sc.prims,
+                                                                            sc.sphere_begin, sc.boxes_finite != 0,
+                                                                            ro, rd, &t, &u, &v, &c);
         store_hit(hits, i, prim, t, u, v);
     }
-    if (kCount) flush_counts(counters, c);
+    if (kMode == kWalkCount) flush_counts(counters, c);
 }
-
-// Lane states of the persistent walks: live (a ray in progress), pending (stopped at an entered
-// leaf whose primitives are not yet tested).
-__device__ __forceinline__ bool walk_live(const StacklessRay& r) { return r.node != kInvalid; }
-__device__ __forceinline__ bool walk_pending(const StacklessRay& r) {
-    return r.node != kInvalid && (r.node & kPendingLeaf) != 0;
-}
-__device__ __forceinline__ bool walk_live(const ThreadedRay& r) { return r.node != kInvalid || r.leaf != 0; }
-__device__ __forceinline__ bool walk_pending(const ThreadedRay& r) { return r.leaf != 0; }
 
 // Persistent wave64 traversal: wave w owns rays [w * chunk, (w + 1) * chunk) and hands them to
 // its lanes as lanes finish, refilling whenever at least `refill` lanes are idle (all at once
@@ -207,22 +203,31 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
     }
     if (block == 0) block = 256;
     dim3 grid((n + block - 1) / block);
+    // one thread per ray: leaves are batched only when asked for (leaf_batch > 1)
+    const uint32_t leaf_batch = tune && tune->leaf_batch ? tune->leaf_batch : 1u;
+    const int mode = counters ? kWalkCount : (leaf_batch > 1 ? kWalkBatched : kWalkPlain);
     if (sc.nnodes == 0) {
         hipLaunchKernelGGL(k_trace_empty, grid, dim3(block), 0, stream, rays, hits, n);
     } else if (variant == kVariantThreaded) {
-        if (counters)
-            hipLaunchKernelGGL(k_trace_threaded<true>, grid, dim3(block), 0, stream, rays, hits, n, sc.mt_nodes,
-                               sc.nnodes, sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
+        if (mode == kWalkCount)
+            hipLaunchKernelGGL(k_trace_threaded<kWalkCount>, grid, dim3(block), 0, stream, rays, hits, n, sc, leaf_batch,
+                               counters);
+        else if (mode == kWalkBatched)
+            hipLaunchKernelGGL(k_trace_threaded<kWalkBatched>, grid, dim3(block), 0, stream, rays, hits, n, sc,
+                               leaf_batch, counters);
         else
-            hipLaunchKernelGGL(k_trace_threaded<false>, grid, dim3(block), 0, stream, rays, hits, n, sc.mt_nodes,
-                               sc.nnodes, sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
+            hipLaunchKernelGGL(k_trace_threaded<kWalkPlain>, grid, dim3(block), 0, stream, rays, hits, n, sc, leaf_batch,
+                               counters);
     } else {
-        if (counters)
-            hipLaunchKernelGGL(k_trace_stackless<true>, grid, dim3(block), 0, stream, rays, hits, n, sc.sl_nodes,
-                               sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
+        if (mode == kWalkCount)
+            hipLaunchKernelGGL(k_trace_stackless<kWalkCount>, grid, dim3(block), 0, stream, rays, hits, n, sc,
+                               leaf_batch, counters);
+        else if (mode == kWalkBatched)
+            hipLaunchKernelGGL(k_trace_stackless<kWalkBatched>, grid, dim3(block), 0, stream, rays, hits, n, sc,
+                               leaf_batch, counters);
         else
-            hipLaunchKernelGGL(k_trace_stackless<false>, grid, dim3(block), 0, stream, rays, hits, n, sc.sl_nodes,
-                               sc.prims, sc.sphere_begin, sc.boxes_finite, counters);
+            hipLaunchKernelGGL(k_trace_stackless<kWalkPlain>, grid, dim3(block), 0, stream, rays, hits, n, sc,
+                               leaf_batch, counters);
     }
     return hipGetLastError();
 }

@@ -904,14 +904,7 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jo
     load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
     float t = tmin, u = 0.0f, vv = 0.0f;
     int32_t prim = -1;
-    if (sc.nnodes != 0) {
-        Cnt cnt;
-        prim = kVariant == kVariantThreaded
-                   ? trace_threaded<false>(sc.mt_nodes, sc.nnodes, sc.prims, sc.sphere_begin, sc.boxes_finite != 0, ro,
-                                           rd, &t, &u, &vv, &cnt)
-                   : trace_stackless<false>(sc.sl_nodes, sc.prims, sc.sphere_begin, sc.boxes_finite != 0, ro, rd, &t,
-                                            &u, &vv, &cnt);
-    }
+    if (sc.nnodes != 0) prim = trace_batched<kVariant>(sc, ro, rd, &t, &u, &vv, kDefaultLeafBatch);
     store_extension(s, sc, slot, ro, rd, prim, t, hit_volume, tmin);
 }
 

@@ -85,8 +85,11 @@ def test_persistent_launch_shapes_match_per_ray_kernel(ctx):
             try:
                 for wps, refill, lb in ((1, 1, 1), (4, 16, 64), (8, 64, 8), (16, 8, 32), (0, 0, 0)):
                     ctx.set_trace_tuning(wps, refill, lb)
-                    g = _lib.trace(ctx, scene, rays, variant)
-                    assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (variant, wps, refill, lb, len(rays))
+                    # block 0: the persistent kernel; 64: one thread per ray, leaves batched when lb > 1
+                    for block in (0, 64):
+                        g = _lib.trace(ctx, scene, rays, variant, block=block)
+                        assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (variant, wps, refill, lb, block,
+                                                                                   len(rays))
             finally:
                 ctx.set_trace_tuning(0, 0, 0)
     with pytest.raises(_lib.RgpuError):
