@@ -42,11 +42,18 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--seconds", type=float, default=0.0)
     ap.add_argument("--variant", default="stackless")
+    ap.add_argument("--trace-block", type=int, default=256, help="0: persistent trace; 64/128/256: one thread per slot")
+    ap.add_argument("--queue-blocks", type=int, default=0, help="workgroups of the queue kernels (0: default)")
     ap.add_argument("--out", default="render.png")
     a = ap.parse_args()
     sd = SCENES[a.scene](a.width, a.height)
     ctx = _lib.Context(0)
     wf = build_lane(ctx, sd, a.paths, variant=a.variant)
+    if a.trace_block != 256 or a.queue_blocks:
+        wf.set_launch(a.variant, a.trace_block, a.queue_blocks or 1024)
+        c = sd["camera"]
+        wf.initialize(_lib.camera_look_at(c["eye"], c["target"], c["up"], c["fov"], c["width"], c["height"],
+                                          c.get("lens_radius", 0.0)))
     t0 = time.perf_counter()
     steps = 0
     while True:
