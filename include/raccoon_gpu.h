@@ -299,8 +299,9 @@ int rgpu_wavefront_graph_capture(rgpu_wavefront* w, uint32_t steps);
 int rgpu_wavefront_graph_launch(rgpu_wavefront* w, rgpu_stream stream, uint32_t launches);
 
 /* Launch configuration: traversal variant, extension-ray-cast workgroup size (64/128/256 threads,
- * one per slot, 256 by default; 0 = persistent waves that refill idle lanes) and grid of the
- * queue-driven kernels. The reference leaves workgroup sizes to the driver
+ * one per slot, 256 by default; 0 = the persistent ray-cast kernel over the slots' rays, for a
+ * single render; a batch traces 0 as 256) and grid of the queue-driven kernels. Results do not
+ * depend on it. The reference leaves workgroup sizes to the driver
  * (common/raccoon_ocl.hpp:906). */
 int rgpu_wavefront_set_launch(rgpu_wavefront* w, int variant, uint32_t trace_block, uint32_t queue_blocks);
 int rgpu_wavefront_get_launch(const rgpu_wavefront* w, int* variant, uint32_t* trace_block,

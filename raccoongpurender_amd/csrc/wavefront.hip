@@ -908,68 +908,6 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jo
     store_extension(s, sc, slot, ro, rd, prim, t, hit_volume, tmin);
 }
 
-// The same stage on persistent waves: wave w owns slots [w * chunk, (w + 1) * chunk) and hands
-// them to lanes as lanes finish (a refill when at least `refill` lanes are idle), so that lanes
-// do not wait for the longest traversal of their wave. Each slot's work and results are those
-// of k_trace_step.
-template <int kVariant, class J>
-__global__ void __launch_bounds__(256) k_trace_step_persistent(DevState s, WFParams p, J jobs, uint32_t chunk,
-                                                               uint32_t refill) {
-    advance_cursors(s, jobs, blockIdx.x * blockDim.x + threadIdx.x);
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    uint64_t next = (uint64_t)wave * chunk;
-    const uint64_t end = next + chunk < (uint64_t)p.n ? next + chunk : (uint64_t)p.n;
-    using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
-    State st;
-    st.node = kInvalid;
-    uint32_t slot = 0;
-    int32_t hit_volume = -1;
-    float tvol = 0.0f;
-    TraceScene lsc = jobs.scene(0);  // the slot's scene (per lane in a batch)
-    for (;;) {
-        const bool idle = st.node == kInvalid;
-        const unsigned long long idle_mask = __ballot(idle);
-        const uint32_t nidle = (uint32_t)__popcll(idle_mask);
-        if (nidle >= refill || nidle == 64u) {
-            if (idle) {
-                const uint64_t r = next + (uint64_t)__popcll(idle_mask & below);
-                if (r < end && !(J::kBatch && ray_pixel(s.ray, (uint32_t)r) == kDeadPixel)) {
-                    slot = (uint32_t)r;
-                    const uint32_t jb = jobs.job(slot);
-                    if constexpr (J::kBatch) lsc = jobs.scene(jb);
-                    volume_through(s, jobs.mats(jb), slot, &hit_volume, &tvol);
-                    F3 ro, rd;
-                    float tmax;
-                    uint32_t pixel;
-                    load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
-                    if (lsc.nnodes != 0) {
-                        if constexpr (kVariant == kVariantThreaded)
-                            threaded_begin(st, lsc.mt_nodes, lsc.nnodes, ro, rd, tvol);
-                        else
-                            stackless_begin(st, ro, rd, tvol);
-                    } else {
-                        store_extension(s, lsc, slot, ro, rd, -1, tvol, hit_volume, tvol);
-                    }
-                }
-            }
-            next += nidle;
-        }
-        if (__ballot(st.node != kInvalid) == 0) {
-            if (next >= end) break;
-            continue;  // every lane took a parked or empty-scene slot: hand out more
-        }
-        if (st.node != kInvalid) {
-            if constexpr (kVariant == kVariantThreaded)
-                threaded_step(st, lsc.prims, lsc.sphere_begin, lsc.boxes_finite != 0);
-            else
-                stackless_step(st, lsc.sl_nodes, lsc.prims, lsc.sphere_begin, lsc.boxes_finite != 0);
-            if (st.node == kInvalid) store_extension(s, lsc, slot, st.ro, st.rd, st.prim, st.tmin, hit_volume, tvol);
-        }
-    }
-}
-
 // The same stage for a single render as three launches: the medium draw and far clip of every
 // slot written as a flat ray (the medium id in the spare word), the library's persistent
 // ray-cast kernel over those rays (leaf batching included), and the extension records from the
@@ -1325,22 +1263,14 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
     hipLaunchKernelGGL(k_shade_dielectric<J>, qgrid, blk, 0, s, st, p, jobs);
     hipLaunchKernelGGL(k_shade_volume_surface<J>, qgrid, blk, 0, s, st, p, jobs);
     hipLaunchKernelGGL(k_shade_volume_inside<J>, qgrid, blk, 0, s, st, p, jobs);
-    if (trace_block == 0) {
+    // trace_block 0: a single render casts through the persistent ray-cast kernel; a batch (several
+    // scenes) keeps one thread per slot (tgrid is 256 wide then)
+    if (!J::kBatch && trace_block == 0) {
         if constexpr (!J::kBatch) {
             hipLaunchKernelGGL(k_trace_setup<J>, full, blk, 0, s, st, p, jobs);
             hipError_t e = launch_trace(jobs.sc, variant, st.xray, st.xhit, p.n, 0, s, nullptr, nullptr);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_trace_finish<J>, full, blk, 0, s, st, p, jobs);
-        } else {
-            // several scenes: persistent waves over the slots, 8 per SIMD over 256 CUs, refill as
-            // in the ray-cast kernels
-            const uint32_t blocks = 2048u, chunk = (p.n + blocks * 4u - 1) / (blocks * 4u);
-            if (variant == kVariantThreaded)
-                hipLaunchKernelGGL((k_trace_step_persistent<kVariantThreaded, J>), dim3(blocks), dim3(256), 0, s, st,
-                                   p, jobs, chunk, 48u);
-            else
-                hipLaunchKernelGGL((k_trace_step_persistent<kVariantStackless, J>), dim3(blocks), dim3(256), 0, s, st,
-                                   p, jobs, chunk, 16u);
         }
     } else if (variant == kVariantThreaded) {
         hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs);

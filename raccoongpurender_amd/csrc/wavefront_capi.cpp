@@ -784,6 +784,7 @@ int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32
     for (int v : variants)
         for (uint32_t b : blocks)
             for (uint32_t q : qmul) {
+                if (b == 0 && w->is_batch()) continue;  // a batch traces 0 as 256
                 int rc = rgpu_wavefront_set_launch(w, v, b, (uint32_t)cu * q);
                 if (rc == RGPU_OK) rc = restart(w, camera);
                 if (rc == RGPU_OK && warmup) rc = rgpu_wavefront_step(w, nullptr, warmup);
