@@ -343,12 +343,6 @@ __device__ __forceinline__ void threaded_leaf(ThreadedRay& r, const PrimRec* __r
     r.leaf = 0;
 }
 
-// One node of trace_threaded.
-__device__ __forceinline__ void threaded_step(ThreadedRay& r, const PrimRec* __restrict__ prims, int32_t sphere_begin,
-                                              bool boxes_finite) {
-    threaded_visit(r, boxes_finite);
-    if (r.leaf != 0) threaded_leaf(r, prims, sphere_begin);
-}
 
 struct StacklessRay {
     F3 ro, rd, inv;
@@ -371,8 +365,6 @@ __device__ __forceinline__ void stackless_begin(StacklessRay& r, F3 ro, F3 rd, f
     r.stk.clear();
 }
 
-// One pass of the loop body of trace_stackless: a branch descent, or a leaf followed by its
-// backtrack to the next sibling (or the end of the traversal).
 // Pending-leaf mark of a resumable walk: the lane has reached a leaf and waits for its wave to
 // run the leaves together (node ids stay below 2^31; kInvalid is never marked).
 static const uint32_t kPendingLeaf = 0x80000000u;
@@ -447,14 +439,6 @@ __device__ __forceinline__ void stackless_leaf(StacklessRay& r, const SLNode* __
     stackless_backtrack(r, q3p, q3);
 }
 
-// One pass of the loop body of trace_stackless: a branch descent, or a leaf followed by its
-// backtrack to the next sibling (or the end of the traversal).
-__device__ __forceinline__ void stackless_step(StacklessRay& r, const SLNode* __restrict__ nodes,
-                                               const PrimRec* __restrict__ prims, int32_t sphere_begin,
-                                               bool boxes_finite) {
-    stackless_visit(r, nodes, boxes_finite);
-    if (r.node != kInvalid && (r.node & kPendingLeaf)) stackless_leaf(r, nodes, prims, sphere_begin);
-}
 
 // Lane states of the resumable walks: live (a ray in progress), pending (stopped at an entered
 // leaf whose primitives are not yet tested).
