@@ -1104,8 +1104,9 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
         d.rec_count = w->guide.rec_count;
     }
     d.nslots = (uint32_t)n;
-    // four 29 KB-LDS inference workgroups per CU, each looping over tiles
-    d.infer_blocks = (uint32_t)(ctx->cu_count > 0 ? 4 * ctx->cu_count : 1024);
+    // eight inference workgroups per CU (13 KB of LDS and 111 VGPRs per lane allow 8), each
+    // looping over tiles
+    d.infer_blocks = (uint32_t)(ctx->cu_count > 0 ? 8 * ctx->cu_count : 2048);
     d.enabled = sampling ? 1u : 0u;
     d.train = training ? 1u : 0u;
     w->guide = d;
