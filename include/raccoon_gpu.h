@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RGPU_ABI_VERSION 1
+#define RGPU_ABI_VERSION 2
 
 /* status codes */
 #define RGPU_OK 0
@@ -70,6 +70,7 @@ typedef struct rgpu_bvh_options {
     uint32_t max_leaf;    /* primitives per leaf, 1..8 (default 4) */
     uint32_t bins;        /* SAH bins per axis (default 16) */
     uint32_t max_depth;   /* deepest leaf, <= 64 (default 63) */
+    float traversal_cost; /* SAH cost of a node visit relative to a primitive test; 0 = 1 */
 } rgpu_bvh_options;
 
 typedef struct rgpu_bvh_stats {

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libraccoongpu.so")
 
-RGPU_ABI_VERSION = 1
+RGPU_ABI_VERSION = 2
 RGPU_OK = 0
 ERRORS = {
     -1: "invalid argument",
@@ -48,7 +48,8 @@ class rgpu_config(C.Structure):
 
 
 class rgpu_bvh_options(C.Structure):
-    _fields_ = [("abi_version", C.c_uint32), ("max_leaf", C.c_uint32), ("bins", C.c_uint32), ("max_depth", C.c_uint32)]
+    _fields_ = [("abi_version", C.c_uint32), ("max_leaf", C.c_uint32), ("bins", C.c_uint32), ("max_depth", C.c_uint32),
+                ("traversal_cost", C.c_float)]
 
 
 class rgpu_bvh_stats(C.Structure):
@@ -219,7 +220,7 @@ class DeviceBuffer:
 class BVH:
     """Host-built binned-SAH BVH2 (rgpu_bvh_build). Runs without a GPU."""
 
-    def __init__(self, points, indices, spheres=None, max_leaf=4, bins=16, max_depth=63):
+    def __init__(self, points, indices, spheres=None, max_leaf=4, bins=16, max_depth=63, traversal_cost=0.0):
         points = np.ascontiguousarray(points, dtype=np.float32)
         indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
         assert points.ndim == 2 and points.shape[1] in (3, 4)
@@ -227,7 +228,7 @@ class BVH:
         self.points, self.indices, self.spheres = points, indices, spheres
         self.ntris = indices.size // 3
         self.nspheres = 0 if spheres is None else spheres.shape[0]
-        opt = rgpu_bvh_options(RGPU_ABI_VERSION, max_leaf, bins, max_depth)
+        opt = rgpu_bvh_options(RGPU_ABI_VERSION, max_leaf, bins, max_depth, traversal_cost)
         h = vp()
         rc = lib.rgpu_bvh_build(ptr(points), points.shape[1], points.shape[0], ptr(indices), self.ntris,
                                 ptr(spheres), self.nspheres, C.byref(opt), C.byref(h))

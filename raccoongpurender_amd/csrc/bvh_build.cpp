@@ -154,7 +154,7 @@ struct Builder {
 
         uint32_t mid;
         if (best_axis >= 0) {
-            double split_cost = 1.0 + (parea > 0.0 ? best_cost / parea : (double)n);
+            double split_cost = opt.traversal_cost + (parea > 0.0 ? best_cost / parea : (double)n);
             if (n <= opt.max_leaf && (double)n <= split_cost) return make_leaf();
             const int k = best_axis;
             const float scale = (float)B / (cb.hi[k] - cb.lo[k]);

@@ -22,6 +22,7 @@ struct BuildOptions {
     uint32_t max_leaf = 4;    // largest leaf
     uint32_t bins = 16;       // SAH bins per axis
     uint32_t max_depth = 63;  // deepest leaf; the stackless bitstack holds 64 levels
+    double traversal_cost = 1.0;  // SAH cost of a node visit (a primitive test costs 1)
 };
 
 struct BVHTree {

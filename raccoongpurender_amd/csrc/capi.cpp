@@ -195,6 +195,8 @@ int rgpu_bvh_build(const float* points, uint32_t stride, uint32_t npoints, const
         if (opt->max_leaf) bo.max_leaf = opt->max_leaf;
         if (opt->bins) bo.bins = opt->bins;
         if (opt->max_depth) bo.max_depth = opt->max_depth;
+        if (!(opt->traversal_cost >= 0.0f && opt->traversal_cost <= 64.0f)) return RGPU_ERR_INVALID_ARG;
+        if (opt->traversal_cost > 0.0f) bo.traversal_cost = opt->traversal_cost;
     }
     try {
         rgpu_bvh* b = new rgpu_bvh();
