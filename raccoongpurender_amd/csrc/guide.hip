@@ -382,7 +382,6 @@ __global__ void __launch_bounds__(256) k_guide_dw(DevGuide g, uint32_t ntiles, u
     f32x4 acc[6];
     for (int j = 0; j < 6; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float bsum = 0.0f;  // column (threadIdx & 63) of dZ, rows 8 (threadIdx / 64) .. + 7 of each tile
-#pragma unroll 2
     for (uint32_t tt = t0; tt < t1; ++tt) {
         const uint16_t* blk = g.act + (size_t)tt * kAcCols * kTile;
 #pragma unroll
