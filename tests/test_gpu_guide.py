@@ -213,3 +213,24 @@ def test_guided_render_close_to_plain(ctx):
         means[guided] = float((y * cnt).sum() / cnt.sum())
     print(f"plain {means[False]:.6f} guided {means[True]:.6f}", flush=True)
     assert abs(means[True] - means[False]) < 0.01 * means[False]
+
+
+def test_guide_config_and_hooks_reject_bad_values(ctx):
+    from raccoongpurender_amd import _lib
+
+    for kw in (dict(weight_cap=-1.0), dict(log_kappa_max=12.5), dict(log_kappa_max=-3.0), dict(select_p=1.0)):
+        with pytest.raises(_lib.RgpuError) as e:
+            G.Guide(ctx, **kw)
+        assert e.value.code == -1, kw
+    g = G.Guide(ctx, seed=2, weight_cap=5.0, log_kappa_max=10.0)
+    assert len(g.buffered_samples()) == 0  # nothing rendered into it yet
+    with pytest.raises(_lib.RgpuError):
+        g.buffered_samples(rows=1)
+    # the cap limits what a sample contributes to the gradient
+    rng = np.random.default_rng(8)
+    s = _samples(rng, 64)
+    s["weight"] = 50.0
+    capped = g.gradients(s).astype(np.float64)
+    s["weight"] = 5.0
+    at_cap = g.gradients(s).astype(np.float64)
+    assert np.allclose(capped, at_cap, rtol=1e-6, atol=1e-9)
