@@ -272,6 +272,12 @@ def guiding_bench(width=480, height=270, seconds=4.0):
     print(f"[bench] guiding: relMSE plain {out['relmse_plain']:.4f} guided {out['relmse_guided']:.4f} "
           f"(trimmed 1%: {out['relmse_trim1pct_plain']:.4f} / {out['relmse_trim1pct_guided']:.4f})",
           file=sys.stderr, flush=True)
+    # the same comparison on the Cornell box (small area light, no caustics), with a training
+    # warm-up inside the guided budget
+    cb = guiding_study.cornell_equal_time(ctx)
+    out["cornell"] = cb
+    print(f"[bench] guiding, Cornell box: relMSE plain {cb['relmse_plain']:.3g} guided {cb['relmse_guided']:.3g} "
+          f"({cb['ratio_plain_over_guided']:.1f}x lower at equal time)", file=sys.stderr, flush=True)
     ctx.close()
     return out
 

@@ -44,7 +44,11 @@ def render_reference(ctx, sd, spp, paths, seed=777):
 
 
 def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=1 << 18, select_p=0.5, lr=1e-3,
-                 minibatch=16384, max_depth=0, bxdf_p=0.0, weight_cap=0.0, log_kappa_max=0.0):
+                 minibatch=16384, max_depth=0, bxdf_p=0.0, weight_cap=0.0, log_kappa_max=0.0, warmup=0.0):
+    """`seconds` of rendering, plain or guided. A guided render trains while it renders; with
+    `warmup` > 0 the image accumulated during the first `warmup` seconds (drawn while the
+    network was still untrained) is discarded and accumulation restarts, training going on.
+    The wall-clock budget includes the warm-up and all training."""
     wf = build_lane(ctx, sd, paths, clamp=NO_CLAMP, seed=seed, bxdf_p=bxdf_p)
     g = None
     if guided:
@@ -56,7 +60,13 @@ def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=
     ctx.synchronize()
     t0 = time.perf_counter()
     passes = 0
+    restart = guided and warmup > 0.0
     while time.perf_counter() - t0 < seconds:
+        if restart and time.perf_counter() - t0 >= warmup:
+            c = sd["camera"]
+            wf.initialize(_lib.camera_look_at(c["eye"], c["target"], c["up"], c["fov"], c["width"], c["height"],
+                                              c.get("lens_radius", 0.0)))
+            restart = False
         wf.step(train_every)
         if guided:
             passes += train_pass(wf, capacity) > 0
@@ -81,6 +91,28 @@ def equal_time_relmse(ctx, sd, seconds=4.0, ref_spp=16384, paths=1 << 20, seed=5
            "guided_training_passes": passes, "reference_spp": round(spp_ref, 1), "reference_seconds": round(t_ref, 1)}
     out["ratio_plain_over_guided"] = round(out["relmse_plain"] / max(out["relmse_guided"], 1e-30), 3)
     out["ratio_trim1pct"] = round(out["relmse_trim1pct_plain"] / max(out["relmse_trim1pct_guided"], 1e-30), 3)
+    return out
+
+
+def cornell_equal_time(ctx, size=64, seconds=1.0, warmup=0.25, ref_spp=1 << 20, seed=5):
+    """The Cornell box (a small ceiling light, black environment, depth 5) at size x size:
+    plain against guided sampling at equal wall-clock, the guided budget including a training
+    warm-up whose image is discarded; scored against a `ref_spp` plain render."""
+    from .render_scenes import cornell_box
+
+    sd = cornell_box(size, size)
+    sd["max_depth"] = 5
+    paths = 1 << 18
+    ref, t_ref, spp_ref = render_reference(ctx, sd, ref_spp, paths)
+    plain, _, spp_p, _ = render_timed(ctx, sd, seconds, paths, False, seed)
+    guided, _, spp_g, passes = render_timed(ctx, sd, seconds, paths, True, seed, capacity=1 << 16, minibatch=8192,
+                                            warmup=warmup)
+    out = {"workload": f"Cornell box {size}x{size}, depth 5, {seconds} s each (guided: {warmup} s of training "
+                       f"warm-up discarded, training continued), relMSE vs a {ref_spp}-spp plain render",
+           "relmse_plain": float(f"{rel_mse(plain, ref):.4g}"), "relmse_guided": float(f"{rel_mse(guided, ref):.4g}"),
+           "plain_spp": round(spp_p, 1), "guided_spp": round(spp_g, 1), "guided_training_passes": passes,
+           "reference_spp": round(spp_ref, 1), "reference_seconds": round(t_ref, 1)}
+    out["ratio_plain_over_guided"] = round(out["relmse_plain"] / max(out["relmse_guided"], 1e-30), 3)
     return out
 
 
