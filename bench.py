@@ -260,6 +260,56 @@ def batch_bench(total_slots=1 << 22, graph_steps=8, ref_spp=4096):
     return out
 
 
+# Algorithmic bytes per environment sample: RNG state in and out (32), the alias bucket (8), the
+# fragment (16) and the output direction + pdf (16), plus one pdf (4) from the uniform table or,
+# six-axis, the normal (12) and three pdfs (12). Per pdf evaluation: direction (12), pdf out (4)
+# and one table pdf (4), or the normal (12) and three table pdfs (12).
+ENV_SAMPLE_BYTES = {"uniform": 32 + 8 + 16 + 16 + 4, "six_axis": 32 + 8 + 16 + 16 + 12 + 12}
+ENV_PDF_BYTES = {"uniform": 12 + 4 + 4, "six_axis": 12 + 4 + 12 + 12}
+
+
+def envmap_bench(samples=1 << 24, iters=20, width=2048, height=1024):
+    """Flat-batch environment sampling and pdf evaluation on a 2048x1024 sky: samples/s and the
+    HBM roofline fraction of algorithmic bytes. Each launch takes a few hundred microseconds, so
+    `iters` launches queued back to back between two synchronisations time the kernels (the
+    library's own stream; torch carries a HIP runtime of its own whose streams it cannot take)."""
+    from raccoongpurender_amd import _lib, render_scenes
+
+    ctx = _lib.Context(0)
+    env = _lib.Envmap(ctx, render_scenes.sky_envmap(width, height))
+    rng = np.random.default_rng(9)
+    states = ctx.to_device(rng.integers(0, 1 << 32, (samples, 4), dtype=np.uint32))
+    nrm = rng.normal(size=(samples, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    normals = ctx.to_device(nrm)
+    del nrm
+    out = ctx.alloc(samples * 16)
+    pdfs = ctx.alloc(samples * 4)
+    dirs = ctx.to_device(rng.normal(size=(samples, 3)).astype(np.float32) / np.float32(1.7320508))
+    res = {"workload": f"{width}x{height} sky, {samples} samples per launch, {iters} launches"}
+    for kind, nb in (("uniform", None), ("six_axis", normals)):
+        for name, launch, per in (
+                ("sample", lambda nb=nb: env.sample_device(states, out, samples, nb), ENV_SAMPLE_BYTES),
+                ("pdf", lambda nb=nb: env.pdf_device(dirs, pdfs, samples, nb), ENV_PDF_BYTES)):
+            for _ in range(3):
+                launch()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(iters):
+                launch()
+            ctx.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / iters
+            gbs = samples * per[kind] / (ms * 1e-3) / 1e9
+            res[f"{name}_{kind}"] = {"ms": round(ms, 4), "msamples_s": round(samples / (ms * 1e-3) / 1e6, 1),
+                                     "bytes_per_sample": per[kind], "achieved_gbs": round(gbs, 1),
+                                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            print(f"[bench] envmap {name:6s} {kind:8s} {ms:.3f} ms  {samples / (ms * 1e-3) / 1e6:.0f} M/s  "
+                  f"{gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    ctx.close()
+    return res
+
+
 def guiding_bench(width=480, height=270, seconds=4.0):
     """Equal-time relMSE of guided against plain sampling on the full-render scene at a reduced
     resolution, scored against a 16k-spp plain render (all unclamped)."""
@@ -287,6 +337,8 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the full-render samples/s measurement")
     ap.add_argument("--no-guiding", action="store_true", help="skip the equal-time guided/plain relMSE study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-job measurement")
+    ap.add_argument("--no-envmap", action="store_true", help="skip the flat-batch envmap sampling measurement")
+    ap.add_argument("--envmap-only", action="store_true", help="run only the envmap sampling measurement")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched multi-job measurement")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -311,6 +363,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(batch_bench()), flush=True)
+        return
+    if args.envmap_only:
+        print(json.dumps(envmap_bench()), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -407,6 +462,8 @@ def main():
         out["extra"]["guiding"] = guiding_bench()
     if not args.no_batch:
         out["extra"]["batch"] = batch_bench()
+    if not args.no_envmap:
+        out["extra"]["envmap"] = envmap_bench()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(bvh.export(), d["points"], d["indices"], d["incoherent"])

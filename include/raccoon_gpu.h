@@ -255,6 +255,20 @@ int rgpu_envmap_destroy(rgpu_envmap* e);
  * table fell back from all-zero weights. Outputs may be NULL. */
 int rgpu_envmap_export(const rgpu_envmap* e, int table, float* pdf, float* heights, int32_t* aliases,
                        float* fragments, int32_t* degenerate);
+/* Flat-batch environment sampling: replaces sample_envmap_6axis_stage and sample_envmap_stage
+ * (kernels/envmap_sampling.cl:169-248) over `count` device-resident samples with no queue
+ * indirection. states: 4 words per sample (the xoshiro128** state), advanced in place by the
+ * draws of one sample. normals: 3 floats per sample selecting the six-axis tables (the caller
+ * flips them to the outgoing side, as the reference stage does at :199-203), or NULL for the
+ * uniform-weight table. out: 4 floats per sample (direction x, y, z, pdf in solid angle).
+ * states and out 16-byte aligned. The envmap must have been created on ctx. */
+int rgpu_envmap_sample(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_envmap* e, uint32_t* states,
+                       const float* normals, uint32_t count, float* out);
+/* The matching densities of given directions (3 floats each): replaces
+ * evaluate_envmap_6axis_pdf_stage and evaluate_envmap_pdf_stage
+ * (kernels/envmap_sampling.cl:250-283). normals as above. */
+int rgpu_envmap_pdf(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_envmap* e, const float* dirs, const float* normals,
+                    uint32_t count, float* pdfs);
 
 /* Pinhole/thin-lens camera from a look-at (the standardCamera image-plane construction of
  * common/wavefront_path_tracing.hpp:103-118). focus_distance 0 = distance to target. */

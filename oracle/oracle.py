@@ -229,6 +229,8 @@ _sig("orc_envmap_destroy", None, vp)
 _sig("orc_envmap_lookup", None, vp, u32, vp, C.c_int, vp)
 _sig("orc_envmap_sample", None, vp, vp, u32, vp, vp, vp)
 _sig("orc_envmap_pdf", None, vp, u32, vp, vp, vp)
+_sig("orc_envmap_sample_each", None, vp, vp, u32, vp, vp, vp)
+_sig("orc_envmap_pdf_each", None, vp, u32, vp, vp, vp)
 
 EXT_DTYPE = np.dtype([("hit_surface", np.int32), ("hit_volume", np.int32), ("tmin", np.float32), ("Ng", np.float32, 3),
                       ("pad", np.float32, 2)])
@@ -270,6 +272,23 @@ class OracleEnvmap:
         nrm = None if normal is None else np.ascontiguousarray(normal, np.float32)
         out = np.zeros(len(dirs), np.float32)
         lib.orc_envmap_pdf(self.h, len(dirs), p(dirs), p(nrm), p(out))
+        return out
+
+    def sample_each(self, states, normals=None):
+        """One sample per state row (n x 4 u32): (directions, pdfs, advanced states)."""
+        st = np.array(states, np.uint32).reshape(-1, 4)
+        n = len(st)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(n, 3)
+        dirs = np.zeros((n, 3), np.float32)
+        pdfs = np.zeros(n, np.float32)
+        lib.orc_envmap_sample_each(self.h, p(st), n, p(nrm), p(dirs), p(pdfs))
+        return dirs, pdfs, st
+
+    def pdf_each(self, dirs, normals=None):
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(len(dirs), 3)
+        out = np.zeros(len(dirs), np.float32)
+        lib.orc_envmap_pdf_each(self.h, len(dirs), p(dirs), p(nrm), p(out))
         return out
 
     def __del__(self):

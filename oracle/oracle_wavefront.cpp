@@ -1155,6 +1155,14 @@ void orc_envmap_sample(void* h, uint32_t* state, uint32_t n, const float* normal
     }
     memcpy(state, st.s, 16);
 }
+// One sample per state (4 words each, advanced in place), the normal of sample i at
+// normals[3i] (NULL = uniform table): sample_envmap_6axis_stage / sample_envmap_stage
+// (kernels/envmap_sampling.cl:169-248) over a flat batch.
+void orc_envmap_sample_each(void* h, uint32_t* states, uint32_t n, const float* normals, float* dirs, float* pdfs) {
+    for (uint32_t i = 0; i < n; ++i)
+        orc_envmap_sample(h, states + 4 * (size_t)i, 1, normals ? normals + 3 * (size_t)i : nullptr,
+                          dirs + 3 * (size_t)i, pdfs + i);
+}
 void orc_envmap_pdf(void* h, uint32_t n, const float* dirs, const float* normal, float* pdfs) {
     Envmap* e = (Envmap*)h;
     for (uint32_t i = 0; i < n; ++i) {
@@ -1162,6 +1170,11 @@ void orc_envmap_pdf(void* h, uint32_t n, const float* dirs, const float* normal,
         pdfs[i] = normal ? envmap_pdf_sixAxis(*e, wi, F(normal[0], normal[1], normal[2]))
                          : e->pdf[0][envmap_index(*e, wi)];
     }
+}
+// pdf of direction i under the normal of row i (NULL = uniform table).
+void orc_envmap_pdf_each(void* h, uint32_t n, const float* dirs, const float* normals, float* pdfs) {
+    for (uint32_t i = 0; i < n; ++i)
+        orc_envmap_pdf(h, 1, dirs + 3 * (size_t)i, normals ? normals + 3 * (size_t)i : nullptr, pdfs + i);
 }
 
 }  // extern "C"

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libraccoongpu.so")
 
 RGPU_ABI_VERSION = 2
 RGPU_OK = 0
+RGPU_ERR_INVALID_ARG = -1
 ERRORS = {
     -1: "invalid argument",
     -2: "out of memory",
@@ -375,6 +376,8 @@ _sig("rgpu_materials_destroy", C.c_int, vp)
 _sig("rgpu_envmap_create", C.c_int, vp, vp, u32, u32, u32, C.POINTER(vp))
 _sig("rgpu_envmap_destroy", C.c_int, vp)
 _sig("rgpu_envmap_export", C.c_int, vp, C.c_int, vp, vp, vp, vp, vp)
+_sig("rgpu_envmap_sample", C.c_int, vp, vp, vp, vp, vp, u32, vp)
+_sig("rgpu_envmap_pdf", C.c_int, vp, vp, vp, vp, vp, u32, vp)
 _sig("rgpu_camera_look_at", C.c_int, vp, vp, vp, C.c_float, C.c_float, u32, u32, C.c_float, C.POINTER(rgpu_camera))
 _sig("rgpu_wavefront_create", C.c_int, vp, vp, vp, vp, C.POINTER(rgpu_wavefront_config), C.POINTER(vp))
 _sig("rgpu_wavefront_destroy", C.c_int, vp)
@@ -413,7 +416,7 @@ EXPORTED += ["rgpu_wavefront_create_batch", "rgpu_wavefront_job_count", "rgpu_wa
              "rgpu_wavefront_run_batch"]
 EXPORTED += [
     "rgpu_materials_create", "rgpu_materials_destroy", "rgpu_envmap_create", "rgpu_envmap_destroy",
-    "rgpu_envmap_export", "rgpu_camera_look_at", "rgpu_wavefront_create", "rgpu_wavefront_destroy",
+    "rgpu_envmap_export", "rgpu_envmap_sample", "rgpu_envmap_pdf", "rgpu_camera_look_at", "rgpu_wavefront_create", "rgpu_wavefront_destroy",
     "rgpu_wavefront_initialize", "rgpu_wavefront_step", "rgpu_wavefront_time_steps", "rgpu_wavefront_queue_counts",
     "rgpu_wavefront_read_accumulation", "rgpu_wavefront_tonemap", "rgpu_wavefront_stat", "rgpu_wavefront_read_state",
 ]
@@ -525,6 +528,38 @@ class Envmap:
         deg = C.c_int32(0)
         lib.rgpu_envmap_export(self.h, table, ptr(pdf), ptr(hts), ptr(al), ptr(fr), C.byref(deg))
         return dict(pdf=pdf, height=hts, alias=al, fragments=fr, degenerate=bool(deg.value))
+
+    def sample_device(self, states, out, count, normals=None, stream=None):
+        """Device buffers: states (count x 4 u32, advanced in place), out (count x 4 f32:
+        direction, pdf), normals (count x 3 f32, six-axis tables) or None (uniform table)."""
+        self.ctx.check(lib.rgpu_envmap_sample(self.ctx.h, stream, self.h, states.ptr,
+                                              normals.ptr if normals is not None else None, count, out.ptr),
+                       "rgpu_envmap_sample")
+
+    def pdf_device(self, dirs, pdfs, count, normals=None, stream=None):
+        self.ctx.check(lib.rgpu_envmap_pdf(self.ctx.h, stream, self.h, dirs.ptr,
+                                           normals.ptr if normals is not None else None, count, pdfs.ptr),
+                       "rgpu_envmap_pdf")
+
+    def sample(self, states, normals=None):
+        """Host arrays in, host arrays out: (directions, pdfs, advanced states)."""
+        states = np.ascontiguousarray(states, np.uint32).reshape(-1, 4)
+        n = len(states)
+        s_dev = self.ctx.to_device(states)
+        out = self.ctx.alloc(max(n, 1) * 16)
+        n_dev = None if normals is None else self.ctx.to_device(np.ascontiguousarray(normals, np.float32))
+        self.sample_device(s_dev, out, n, n_dev)
+        res = out.download(np.float32, (n, 4))
+        return res[:, :3].copy(), res[:, 3].copy(), s_dev.download(np.uint32, (n, 4))
+
+    def pdf(self, dirs, normals=None):
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = len(dirs)
+        d_dev = self.ctx.to_device(dirs)
+        out = self.ctx.alloc(max(n, 1) * 4)
+        n_dev = None if normals is None else self.ctx.to_device(np.ascontiguousarray(normals, np.float32))
+        self.pdf_device(d_dev, out, n, n_dev)
+        return out.download(np.float32, n)
 
     def close(self):
         if self.h and (self.ctx is None or self.ctx.h):

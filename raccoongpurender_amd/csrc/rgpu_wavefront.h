@@ -174,5 +174,12 @@ hipError_t guide_gradients(const DevGuide& g, uint32_t rows, hipStream_t s);
 hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hipStream_t s);
 hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s);
 hipError_t wf_stat(const DevState& st, uint32_t npixels, bool accum64, unsigned long long* total, hipStream_t s);
+// Flat-batch environment sampling: states (4 words per sample) advance in place; normals (3 floats
+// per sample) select the six-axis tables, null the uniform-weight one; out holds 4 floats per
+// sample (direction, pdf). envmap_pdf evaluates the same densities for given directions.
+hipError_t envmap_sample(const DevEnvmap& e, uint32_t* states, const float* normals, uint32_t n, float* out,
+                         hipStream_t s);
+hipError_t envmap_pdf(const DevEnvmap& e, const float* dirs, const float* normals, uint32_t n, float* pdfs,
+                      hipStream_t s);
 
 }  // namespace rgpu

@@ -215,6 +215,42 @@ __device__ F3 sample_env_uniform(const DevEnvmap& e, uint4& st, float* pdf) {
 }
 __device__ __forceinline__ float pdf_env_uniform(const DevEnvmap& e, F3 wi) { return e.pdf[envmap_index(e, wi)]; }
 
+// sample_envmap_6axis_stage / sample_envmap_stage (kernels/envmap_sampling.cl:169-248) over a
+// flat batch: one sample per RNG state (advanced in place); given normals (already facing the
+// outgoing side), from the six-axis tables mixed by n^2, else from the uniform-weight table.
+// out: direction and pdf per sample.
+__global__ void __launch_bounds__(256) k_envmap_sample(DevEnvmap e, uint4* states, const float* normals, uint32_t n,
+                                                       float4* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint4 st = states[i];
+    float pdf;
+    F3 wi;
+    if (normals) {
+        const F3 nn = F3{normals[3 * (size_t)i], normals[3 * (size_t)i + 1], normals[3 * (size_t)i + 2]};
+        wi = sample_env_6axis(e, nn, st, &pdf);
+    } else {
+        wi = sample_env_uniform(e, st, &pdf);
+    }
+    states[i] = st;
+    out[i] = make_float4(wi.x, wi.y, wi.z, pdf);
+}
+
+// evaluate_envmap_6axis_pdf_stage / evaluate_envmap_pdf_stage (kernels/envmap_sampling.cl:250-283)
+// for given directions.
+__global__ void __launch_bounds__(256) k_envmap_pdf(DevEnvmap e, const float* dirs, const float* normals, uint32_t n,
+                                                    float* pdfs) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const F3 wi = F3{dirs[3 * (size_t)i], dirs[3 * (size_t)i + 1], dirs[3 * (size_t)i + 2]};
+    if (normals) {
+        const F3 nn = F3{normals[3 * (size_t)i], normals[3 * (size_t)i + 1], normals[3 * (size_t)i + 2]};
+        pdfs[i] = pdf_env_6axis(e, wi, nn);
+    } else {
+        pdfs[i] = e.pdf[envmap_index(e, wi)];
+    }
+}
+
 // ---------------------------------------------------------------- BSDF helpers
 // Duff et al. 2017 orthonormal basis, z up (kernels/lambertian.cl:24-32)
 __device__ __forceinline__ void onb(F3 z, F3* x, F3* y) {
@@ -1206,6 +1242,23 @@ __global__ void __launch_bounds__(256) k_stat(DevState s, uint32_t npixels, uint
 }
 
 }  // namespace
+
+hipError_t envmap_sample(const DevEnvmap& e, uint32_t* states, const float* normals, uint32_t n, float* out,
+                         hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((uint32_t)(((uint64_t)n + 255) / 256));
+    hipLaunchKernelGGL(k_envmap_sample, grid, dim3(256), 0, s, e, reinterpret_cast<uint4*>(states),
+                       normals, n, reinterpret_cast<float4*>(out));
+    return hipGetLastError();
+}
+
+hipError_t envmap_pdf(const DevEnvmap& e, const float* dirs, const float* normals, uint32_t n, float* pdfs,
+                      hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((uint32_t)(((uint64_t)n + 255) / 256));
+    hipLaunchKernelGGL(k_envmap_pdf, grid, dim3(256), 0, s, e, dirs, normals, n, pdfs);
+    return hipGetLastError();
+}
 
 hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s) {
     dim3 grid((npixels + 255) / 256);

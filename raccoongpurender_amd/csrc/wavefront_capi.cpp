@@ -375,6 +375,30 @@ int rgpu_envmap_export(const rgpu_envmap* e, int table, float* pdf, float* heigh
     return RGPU_OK;
 }
 
+int rgpu_envmap_sample(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_envmap* e, uint32_t* states,
+                       const float* normals, uint32_t count, float* out) {
+    if (!ctx || !e) return RGPU_ERR_INVALID_ARG;
+    if (e->ctx != ctx) return fail(ctx, RGPU_ERR_INVALID_ARG, "envmap has no device tables on this context");
+    if (count && (!states || !out)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null state or output buffer");
+    if ((((uintptr_t)states | (uintptr_t)out) & 15) || ((uintptr_t)normals & 3))
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "states and out need 16-byte alignment, normals 4-byte");
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, envmap_sample(e->dev, states, normals, count, out, stream_of(ctx, stream)));
+    return RGPU_OK;
+}
+
+int rgpu_envmap_pdf(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_envmap* e, const float* dirs, const float* normals,
+                    uint32_t count, float* pdfs) {
+    if (!ctx || !e) return RGPU_ERR_INVALID_ARG;
+    if (e->ctx != ctx) return fail(ctx, RGPU_ERR_INVALID_ARG, "envmap has no device tables on this context");
+    if (count && (!dirs || !pdfs)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null direction or pdf buffer");
+    if (((uintptr_t)dirs | (uintptr_t)normals | (uintptr_t)pdfs) & 3)
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "directions, normals and pdfs need 4-byte alignment");
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, envmap_pdf(e->dev, dirs, normals, count, pdfs, stream_of(ctx, stream)));
+    return RGPU_OK;
+}
+
 int rgpu_camera_look_at(const float* eye, const float* target, const float* up, float fov_y_deg,
                         float focus_distance, uint32_t width, uint32_t height, float lens_radius, rgpu_camera* out) {
     if (!eye || !target || !up || !out || width == 0 || height == 0 || !(fov_y_deg > 0.0f) || !(fov_y_deg < 180.0f))

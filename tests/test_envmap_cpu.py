@@ -110,3 +110,25 @@ def test_lookup_nearest_and_bilinear():
     d = np.array([[math.sin(th) * math.sin(ph), math.cos(th), math.sin(th) * math.cos(ph)]], np.float32)
     assert np.allclose(orc.lookup(d, False)[0], (1, 2, 3))
     assert np.allclose(orc.lookup(d, True)[0], (1, 2, 3), atol=1e-5)  # centre: weight 1 on that texel
+
+
+def test_oracle_per_state_batch_is_first_draw_of_each_stream():
+    """orc_envmap_sample_each (one sample per state, the GPU entry point's batch form) equals
+    the pinned sequential sampler's first draw from each state, and advances each state as
+    that draw does."""
+    img = R.sky_envmap(32, 16, sun_peak=200.0, sun_width_deg=10.0)
+    orc = O.OracleEnvmap(img)
+    rng = np.random.default_rng(5)
+    st = rng.integers(0, 1 << 32, (64, 4), dtype=np.uint32)
+    nrm = rng.normal(size=(64, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    for normals in (None, nrm):
+        d, pdf, st2 = orc.sample_each(st, normals)
+        assert np.all(pdf > 0)
+        assert np.array_equal(orc.pdf_each(d, normals) > 0, np.ones(64, bool))
+        for i in (0, 17, 63):
+            s1 = st[i].copy()
+            ds, ps = orc.sample(s1, 2, None if normals is None else normals[i])
+            assert np.array_equal(ds[0], d[i]) and ps[0] == pdf[i]
+            d3, _, _ = orc.sample_each(st2[i:i + 1], None if normals is None else normals[i:i + 1])
+            assert np.array_equal(d3[0], ds[1])
