@@ -98,10 +98,21 @@ struct DevMaterials {
     const Volume* volume = nullptr;
 };
 
+// One 64-byte record per texel: the pdfs of the seven tables (uniform, +X, -X, +Y, -Y, +Z, -Z;
+// the eighth word unused) and the texel's sampling fragment, so that drawing a direction or
+// pricing one gathers a single cache line for the texel instead of one per table.
+struct alignas(64) EnvTexelRec {
+    float pdf[8];
+    EnvFragment frag;
+    float pad[4];
+};
+static_assert(sizeof(EnvTexelRec) == 64, "texel record is one 64-byte line");
+
 struct DevEnvmap {
-    const float4* texels = nullptr;  // width*height RGBA
-    const EnvFragment* fragments = nullptr;
-    const float* pdf = nullptr;           // 7 tables of width*height
+    const float4* texels = nullptr;       // width*height RGBA
+    const EnvTexelRec* rec = nullptr;     // width*height
+    const float* pdf0 = nullptr;          // width*height: the uniform table's pdfs again, densely
+                                          // packed for pricing arbitrary directions (16 per line)
     const AliasBucket* alias = nullptr;   // 7 tables of width*height
     uint32_t width = 0, height = 0;
 };

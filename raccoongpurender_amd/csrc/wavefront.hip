@@ -171,49 +171,45 @@ __device__ __forceinline__ F3 sample_from_fragment(const EnvFragment& f, uint4& 
     return p;
 }
 
-__device__ __forceinline__ void six_axis_tables(const DevEnvmap& e, F3 n, uint32_t* tx, uint32_t* ty, uint32_t* tz) {
-    *tx = 0.0f < n.x ? 1u : 2u;
-    *ty = 0.0f < n.y ? 3u : 4u;
-    *tz = 0.0f < n.z ? 5u : 6u;
+// The six-axis mixture n.x^2 p(+-X) + n.y^2 p(+-Y) + n.z^2 p(+-Z) of one texel, summed in the
+// reference's order (envmap_sampling.cl:117-121, 152-155); the sign of each normal component
+// picks the table (0 < n.c: the + table). Two 16-byte loads of the texel's record.
+__device__ __forceinline__ float six_axis_mix(const DevEnvmap& e, uint32_t idx, F3 n, F3 ap) {
+    const float4* r = reinterpret_cast<const float4*>(e.rec + idx);
+    const float4 lo = r[0], hi = r[1];  // pdf[0..3], pdf[4..7]
+    const float px = 0.0f < n.x ? lo.y : lo.z;
+    const float py = 0.0f < n.y ? lo.w : hi.x;
+    const float pz = 0.0f < n.z ? hi.y : hi.z;
+    float p = 0.0f;
+    p += ap.x * px;
+    p += ap.y * py;
+    p += ap.z * pz;
+    return p;
 }
 
 // sample_envmap_6axis + sample_from_fragment (envmap_sampling.cl:126-157, 169-217)
 __device__ F3 sample_env_6axis(const DevEnvmap& e, F3 n, uint4& st, float* pdf) {
     const uint32_t count = e.width * e.height;
-    uint32_t tx, ty, tz;
-    six_axis_tables(e, n, &tx, &ty, &tz);
+    const uint32_t tx = 0.0f < n.x ? 1u : 2u, ty = 0.0f < n.y ? 3u : 4u, tz = 0.0f < n.z ? 5u : 6u;
     float u = xo_uniform(st);
     F3 ap = mul3v(n, n);
     uint32_t sel = (u < ap.x) ? tx : ((u < ap.x + ap.y) ? ty : tz);
     uint32_t idx = alias_sample(st, e.alias + (size_t)sel * count, count);
-    float p = 0.0f;
-    p += ap.x * e.pdf[(size_t)tx * count + idx];
-    p += ap.y * e.pdf[(size_t)ty * count + idx];
-    p += ap.z * e.pdf[(size_t)tz * count + idx];
-    *pdf = p;
-    return sample_from_fragment(e.fragments[idx], st);
+    *pdf = six_axis_mix(e, idx, n, ap);
+    return sample_from_fragment(e.rec[idx].frag, st);
 }
 // envmap_pdf_sixAxis (envmap_sampling.cl:100-124)
 __device__ float pdf_env_6axis(const DevEnvmap& e, F3 wi, F3 n) {
-    const uint32_t count = e.width * e.height;
-    uint32_t tx, ty, tz;
-    six_axis_tables(e, n, &tx, &ty, &tz);
-    F3 ap = mul3v(n, n);
-    int idx = envmap_index(e, wi);
-    float p = 0.0f;
-    p += ap.x * e.pdf[(size_t)tx * count + idx];
-    p += ap.y * e.pdf[(size_t)ty * count + idx];
-    p += ap.z * e.pdf[(size_t)tz * count + idx];
-    return p;
+    return six_axis_mix(e, (uint32_t)envmap_index(e, wi), n, mul3v(n, n));
 }
 // sample_envmap_stage (envmap_sampling.cl:219-248): the uniform-weight table
 __device__ F3 sample_env_uniform(const DevEnvmap& e, uint4& st, float* pdf) {
     const uint32_t count = e.width * e.height;
     uint32_t idx = alias_sample(st, e.alias, count);
-    *pdf = e.pdf[idx];
-    return sample_from_fragment(e.fragments[idx], st);
+    *pdf = e.rec[idx].pdf[0];
+    return sample_from_fragment(e.rec[idx].frag, st);
 }
-__device__ __forceinline__ float pdf_env_uniform(const DevEnvmap& e, F3 wi) { return e.pdf[envmap_index(e, wi)]; }
+__device__ __forceinline__ float pdf_env_uniform(const DevEnvmap& e, F3 wi) { return e.pdf0[envmap_index(e, wi)]; }
 
 // sample_envmap_6axis_stage / sample_envmap_stage (kernels/envmap_sampling.cl:169-248) over a
 // flat batch: one sample per RNG state (advanced in place); given normals (already facing the
@@ -247,7 +243,7 @@ __global__ void __launch_bounds__(256) k_envmap_pdf(DevEnvmap e, const float* di
         const F3 nn = F3{normals[3 * (size_t)i], normals[3 * (size_t)i + 1], normals[3 * (size_t)i + 2]};
         pdfs[i] = pdf_env_6axis(e, wi, nn);
     } else {
-        pdfs[i] = e.pdf[envmap_index(e, wi)];
+        pdfs[i] = pdf_env_uniform(e, wi);
     }
 }
 

@@ -321,25 +321,30 @@ int rgpu_envmap_create(rgpu_ctx* ctx, const float* rgb, uint32_t channels, uint3
         return RGPU_OK;
     }
     const size_t P = (size_t)width * height;
-    std::vector<float> pdf(7 * P);
+    std::vector<EnvTexelRec> rec(P);
     std::vector<AliasBucket> alias(7 * P);
-    for (int t = 0; t < 7; ++t) {
-        std::memcpy(pdf.data() + t * P, e->host.pdf[t].data(), P * sizeof(float));
-        std::memcpy(alias.data() + t * P, e->host.alias[t].data(), P * sizeof(AliasBucket));
+    for (size_t i = 0; i < P; ++i) {
+        EnvTexelRec& r = rec[i];
+        for (int t = 0; t < 7; ++t) r.pdf[t] = e->host.pdf[t][i];
+        r.pdf[7] = 0.0f;
+        r.frag = e->host.fragments[i];
+        r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0.0f;
     }
+    for (int t = 0; t < 7; ++t)
+        std::memcpy(alias.data() + t * P, e->host.alias[t].data(), P * sizeof(AliasBucket));
     int rc = hipSetDevice(ctx->device) == hipSuccess ? RGPU_OK : RGPU_ERR_HIP;
     if (rc == RGPU_OK) rc = upload_vec(ctx, e->host.texels.data(), e->host.texels.size(), &e->bufs[0]);
-    if (rc == RGPU_OK) rc = upload_vec(ctx, e->host.fragments.data(), P, &e->bufs[1]);
-    if (rc == RGPU_OK) rc = upload_vec(ctx, pdf.data(), pdf.size(), &e->bufs[2]);
-    if (rc == RGPU_OK) rc = upload_vec(ctx, alias.data(), alias.size(), &e->bufs[3]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, rec.data(), rec.size(), &e->bufs[1]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, alias.data(), alias.size(), &e->bufs[2]);
+    if (rc == RGPU_OK) rc = upload_vec(ctx, e->host.pdf[0].data(), P, &e->bufs[3]);
     if (rc != RGPU_OK) {
         rgpu_envmap_destroy(e);
         return rc;
     }
     e->dev.texels = (const float4*)e->bufs[0];
-    e->dev.fragments = (const EnvFragment*)e->bufs[1];
-    e->dev.pdf = (const float*)e->bufs[2];
-    e->dev.alias = (const AliasBucket*)e->bufs[3];
+    e->dev.rec = (const EnvTexelRec*)e->bufs[1];
+    e->dev.alias = (const AliasBucket*)e->bufs[2];
+    e->dev.pdf0 = (const float*)e->bufs[3];
     e->dev.width = width;
     e->dev.height = height;
     *out = e;
