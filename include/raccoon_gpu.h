@@ -152,9 +152,14 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
  * that makes it test their primitives (1..64). 0 keeps the default (8 waves; refill at 32 idle
  * lanes for stackless, 48 for threaded; leaves at 8 waiting lanes). A leaf_batch above 1 also
  * batches leaves in the one-thread-per-ray kernels (block != 0), which by default do not (their
- * usual rays are coherent). Results do not depend on any of them. No counterpart in the reference (it leaves launch shapes to the OpenCL driver,
+ * usual rays are coherent). treelet: the number of top-of-tree stackless node records each
+ * workgroup of the persistent kernel copies to LDS and reads from there (1..1024, 64 bytes
+ * each; 0 = default 320; RGPU_TREELET_OFF = none). Results do not depend on any of them. No
+ * counterpart in the reference (it leaves launch shapes to the OpenCL driver,
  * common/raccoon_ocl.hpp:906). */
-int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill, uint32_t leaf_batch);
+#define RGPU_TREELET_OFF 0xFFFFFFFFu
+int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill, uint32_t leaf_batch,
+                          uint32_t treelet);
 
 /* Instrumented extension ray cast: the same traversal (identical hits) with per-ray event
  * counting, reduced to 5 totals {stackless branch visits, stackless leaf visits, stackless

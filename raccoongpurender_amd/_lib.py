@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libraccoongpu.so")
 RGPU_ABI_VERSION = 2
 RGPU_OK = 0
 RGPU_ERR_INVALID_ARG = -1
+TREELET_OFF = 0xFFFFFFFF
 ERRORS = {
     -1: "invalid argument",
     -2: "out of memory",
@@ -89,7 +90,7 @@ _sig("rgpu_bvh_get_stats", C.c_int, vp, C.POINTER(rgpu_bvh_stats))
 _sig("rgpu_bvh_export", C.c_int, vp, vp, vp, vp, vp, vp, vp)
 _sig("rgpu_scene_create", C.c_int, vp, vp, vp, u32, u32, vp, u32, vp, u32, C.POINTER(vp))
 _sig("rgpu_scene_destroy", C.c_int, vp)
-_sig("rgpu_set_trace_tuning", C.c_int, vp, C.c_uint32, C.c_uint32, C.c_uint32)
+_sig("rgpu_set_trace_tuning", C.c_int, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32)
 _sig("rgpu_scene_bytes", C.c_int, vp, C.POINTER(C.c_uint64))
 _sig("rgpu_extension_ray_cast", C.c_int, vp, vp, vp, C.c_int, vp, vp, u32, u32)
 _sig("rgpu_extension_ray_cast_stats", C.c_int, vp, vp, C.c_int, vp, vp, u32, C.POINTER(C.c_uint64))
@@ -164,8 +165,10 @@ class Context:
     def synchronize(self):
         self.check(lib.rgpu_synchronize(self.h, None), "rgpu_synchronize")
 
-    def set_trace_tuning(self, waves_per_simd=0, refill=0, leaf_batch=0):
-        self.check(lib.rgpu_set_trace_tuning(self.h, waves_per_simd, refill, leaf_batch), "rgpu_set_trace_tuning")
+    def set_trace_tuning(self, waves_per_simd=0, refill=0, leaf_batch=0, treelet=0):
+        """treelet: stackless top-of-tree records per workgroup in LDS (0 default, TREELET_OFF none)."""
+        self.check(lib.rgpu_set_trace_tuning(self.h, waves_per_simd, refill, leaf_batch, treelet),
+                   "rgpu_set_trace_tuning")
 
     def copy(self, dst, src, nbytes):
         self.check(lib.rgpu_copy(self.h, None, dst.ptr, src.ptr, nbytes), "rgpu_copy")

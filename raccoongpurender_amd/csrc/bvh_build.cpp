@@ -320,6 +320,39 @@ void pack_stackless(const BVHTree& t, std::vector<SLNode>* out) {
     }
 }
 
+void number_top_breadth_first(std::vector<SLNode>* nodes, uint32_t top) {
+    std::vector<SLNode>& sl = *nodes;
+    const size_t nn = sl.size();
+    if (nn == 0 || top <= 1) return;
+    std::vector<uint32_t> order;
+    order.reserve(top);
+    order.push_back(0);
+    for (size_t h = 0; h < order.size() && order.size() < top; ++h) {
+        const SLNode& s = sl[order[h]];
+        if (s.a & kLeafFlag) continue;
+        order.push_back(s.a);
+        if (order.size() < top) order.push_back(s.b);
+    }
+    const uint32_t none = 0xFFFFFFFFu;
+    std::vector<uint32_t> id(nn, none);
+    for (size_t k = 0; k < order.size(); ++k) id[order[k]] = (uint32_t)k;
+    uint32_t next = (uint32_t)order.size();
+    for (size_t i = 0; i < nn; ++i)
+        if (id[i] == none) id[i] = next++;
+    std::vector<SLNode> out(nn);
+    for (size_t i = 0; i < nn; ++i) {
+        SLNode s = sl[i];
+        if (!(s.a & kLeafFlag)) {
+            s.a = id[s.a];
+            s.b = id[s.b];
+        }
+        s.parent = id[s.parent];
+        s.sibling = id[s.sibling];
+        out[id[i]] = s;
+    }
+    sl.swap(out);
+}
+
 void pack_threaded(const BVHTree& t, std::vector<MTNode>* out) {
     const uint32_t nn = (uint32_t)t.nodes.size();
     out->assign((size_t)6 * nn, MTNode());

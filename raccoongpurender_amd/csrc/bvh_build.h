@@ -43,6 +43,10 @@ void build_bvh(const float* points, uint32_t stride, uint32_t npoints,
 
 // Device layouts.
 void pack_stackless(const BVHTree& t, std::vector<SLNode>* out);
+// Renumbers the stackless records so that the first `top` ids are the top of the tree in
+// breadth-first order (root stays 0); the other records keep their relative order. Child,
+// parent and sibling links follow; the walk and its results do not change.
+void number_top_breadth_first(std::vector<SLNode>* nodes, uint32_t top);
 // out: 6 tables of t.nodes.size() records, direction-major (+X,-X,+Y,-Y,+Z,-Z).
 void pack_threaded(const BVHTree& t, std::vector<MTNode>* out);
 void pack_prims(const BVHTree& t, const float* points, uint32_t stride,

@@ -289,6 +289,7 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     std::vector<MTNode> mt;
     std::vector<PrimRec> pr;
     pack_stackless(bvh->tree, &sl);
+    number_top_breadth_first(&sl, kTreeletMaxNodes);
     pack_threaded(bvh->tree, &mt);
     pack_prims(bvh->tree, points, stride, indices, ntris, spheres, nspheres, &pr);
     std::vector<float> p4((size_t)npoints * 4, 0.0f);
@@ -356,16 +357,20 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
     tune.waves_per_simd = ctx->trace_waves;
     tune.refill = ctx->trace_refill;
     tune.leaf_batch = ctx->trace_leaf_batch;
+    tune.treelet = ctx->trace_treelet;
     RGPU_HIP(ctx, launch_trace(scene->dev, variant, (const TraceRay*)rays, (TraceHit*)hits, count, block,
                                stream_of(ctx, stream), nullptr, &tune));
     return RGPU_OK;
 }
 
-int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill, uint32_t leaf_batch) {
+int rgpu_set_trace_tuning(rgpu_ctx* ctx, uint32_t waves_per_simd, uint32_t refill, uint32_t leaf_batch,
+                          uint32_t treelet) {
     if (!ctx || waves_per_simd > 16 || refill > 64 || leaf_batch > 64) return RGPU_ERR_INVALID_ARG;
+    if (treelet > kTreeletMaxNodes && treelet != RGPU_TREELET_OFF) return RGPU_ERR_INVALID_ARG;
     ctx->trace_waves = waves_per_simd;
     ctx->trace_refill = refill;
     ctx->trace_leaf_batch = leaf_batch;
+    ctx->trace_treelet = treelet;
     return RGPU_OK;
 }
 

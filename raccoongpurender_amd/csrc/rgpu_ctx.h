@@ -11,6 +11,7 @@ struct rgpu_ctx {
     int cu_count = 0;
     uint32_t trace_waves = 0;   // persistent ray cast: resident waves per SIMD (0 = default)
     uint32_t trace_refill = 0;  // persistent ray cast: idle lanes that trigger a refill (0 = default)
+    uint32_t trace_treelet = 0;     // persistent stackless ray cast: records staged in LDS (0 = default)
     uint32_t trace_leaf_batch = 0;  // persistent ray cast: waiting lanes that trigger leaf tests (0 = default)
     std::string arch;
     std::mutex err_mu;

@@ -47,8 +47,14 @@ struct TraceTuning {
     uint32_t waves_per_simd = 0;
     uint32_t refill = 0;
     uint32_t leaf_batch = 0;
+    uint32_t treelet = 0;  // stackless records staged in LDS per workgroup; kTreeletOff: none
 };
 constexpr uint32_t kDefaultLeafBatch = 8;
+// The scene packer numbers this many top records breadth-first, so that any LDS treelet of up
+// to this size is a prefix of the node array (64 KB of LDS).
+constexpr uint32_t kTreeletMaxNodes = 1024;
+constexpr uint32_t kTreeletOff = 0xFFFFFFFFu;
+constexpr uint32_t kDefaultTreelet = 320;
 
 // counters: null for the production kernels; otherwise 5 device u64 totals
 // {branch visits, leaf visits, parent steps, primitive tests, threaded node visits}.

@@ -369,9 +369,57 @@ __device__ __forceinline__ void stackless_begin(StacklessRay& r, F3 ro, F3 rd, f
 // run the leaves together (node ids stay below 2^31; kInvalid is never marked).
 static const uint32_t kPendingLeaf = 0x80000000u;
 
+// Where the resumable stackless walk reads node records from. SLGlobal: the node array in
+// device memory. SLTreelet: the first `top` records (the top levels of the tree, numbered
+// breadth-first by the scene packer) from a copy in LDS that the persistent kernel stages at
+// its start, the rest from device memory; a lane's record comes from one or the other, the
+// values are the same.
+extern __shared__ float4 sl_treelet[];
+
+struct SLGlobal {
+    const float4* __restrict__ g;
+    __device__ __forceinline__ void record(uint32_t i, float4* q0, float4* q1, float4* q2, uint4* q3) const {
+        const float4* p = g + 4 * (size_t)i;
+        *q0 = p[0];
+        *q1 = p[1];
+        *q2 = p[2];
+        *q3 = reinterpret_cast<const uint4*>(p)[3];
+    }
+    __device__ __forceinline__ uint4 links(uint32_t i) const { return reinterpret_cast<const uint4*>(g)[4 * (size_t)i + 3]; }
+};
+
+struct SLTreelet {
+    const float4* __restrict__ g;
+    uint32_t top;
+    __device__ __forceinline__ void record(uint32_t i, float4* q0, float4* q1, float4* q2, uint4* q3) const {
+        if (i < top) {
+            const float4* p = sl_treelet + 4 * i;
+            *q0 = p[0];
+            *q1 = p[1];
+            *q2 = p[2];
+            const float4 w = p[3];
+            *q3 = make_uint4(__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z), __float_as_uint(w.w));
+        } else {
+            const float4* p = g + 4 * (size_t)i;
+            *q0 = p[0];
+            *q1 = p[1];
+            *q2 = p[2];
+            *q3 = reinterpret_cast<const uint4*>(p)[3];
+        }
+    }
+    __device__ __forceinline__ uint4 links(uint32_t i) const {
+        if (i < top) {
+            const float4 w = sl_treelet[4 * i + 3];
+            return make_uint4(__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z), __float_as_uint(w.w));
+        }
+        return reinterpret_cast<const uint4*>(g)[4 * (size_t)i + 3];
+    }
+};
+
 // The end of a stackless step: the backtrack from the current node (record word q3) to the
 // next sibling, or the end of the walk.
-__device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const uint4* __restrict__ q3p, uint4 q3) {
+template <class Src>
+__device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const Src& src, uint4 q3) {
     if (r.bitstack == 0) {
         r.node = kInvalid;
         return;
@@ -390,7 +438,7 @@ __device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const uint4
         }
         r.node = q.z;
         r.bitstack = r.bitstack >> 1;
-        if (r.bitstack != 0) q = q3p[4 * r.node + 3];
+        if (r.bitstack != 0) q = src.links(r.node);
     }
     r.node = q.w;
     r.bitstack = r.bitstack ^ 1;
@@ -398,11 +446,11 @@ __device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const uint4
 
 // Visit of a branch node of trace_stackless (descent, or backtrack when neither child is
 // entered). Reaching a leaf marks it pending instead of testing its primitives.
-__device__ __forceinline__ void stackless_visit(StacklessRay& r, const SLNode* __restrict__ nodes, bool boxes_finite) {
-    const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
-    const float4* rec = reinterpret_cast<const float4*>(nodes + r.node);
-    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2];
-    const uint4 q3 = q3p[4 * r.node + 3];  // the whole record in one round trip
+template <class Src>
+__device__ __forceinline__ void stackless_visit(StacklessRay& r, const Src& src, bool boxes_finite) {
+    float4 q0, q1, q2;
+    uint4 q3;
+    src.record(r.node, &q0, &q1, &q2, &q3);  // the whole record in one round trip
     if (q3.x & kLeafFlag) {
         r.node |= kPendingLeaf;
         return;
@@ -425,18 +473,25 @@ __device__ __forceinline__ void stackless_visit(StacklessRay& r, const SLNode* _
         }
         return;
     }
-    stackless_backtrack(r, q3p, q3);
+    stackless_backtrack(r, src, q3);
+}
+__device__ __forceinline__ void stackless_visit(StacklessRay& r, const SLNode* __restrict__ nodes, bool boxes_finite) {
+    stackless_visit(r, SLGlobal{reinterpret_cast<const float4*>(nodes)}, boxes_finite);
 }
 
 // The pending leaf of a lane: its primitives, then the backtrack.
-__device__ __forceinline__ void stackless_leaf(StacklessRay& r, const SLNode* __restrict__ nodes,
-                                               const PrimRec* __restrict__ prims, int32_t sphere_begin) {
-    const uint4* q3p = reinterpret_cast<const uint4*>(nodes);
+template <class Src>
+__device__ __forceinline__ void stackless_leaf(StacklessRay& r, const Src& src, const PrimRec* __restrict__ prims,
+                                               int32_t sphere_begin) {
     r.node &= ~kPendingLeaf;
-    const uint4 q3 = q3p[4 * r.node + 3];
+    const uint4 q3 = src.links(r.node);
     const uint32_t beg = q3.x & ~kLeafFlag, end = q3.y;
     for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
-    stackless_backtrack(r, q3p, q3);
+    stackless_backtrack(r, src, q3);
+}
+__device__ __forceinline__ void stackless_leaf(StacklessRay& r, const SLNode* __restrict__ nodes,
+                                               const PrimRec* __restrict__ prims, int32_t sphere_begin) {
+    stackless_leaf(r, SLGlobal{reinterpret_cast<const float4*>(nodes)}, prims, sphere_begin);
 }
 
 

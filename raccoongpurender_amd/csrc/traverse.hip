@@ -111,10 +111,18 @@ __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restri
 // once `leaf_batch` lanes wait at leaves (or none can walk on), so that the primitive tests run
 // with many lanes at once instead of taxing every node visit of the wave. Each ray still
 // visits its nodes and tests its primitives in the order of the one-ray walk.
+// Stackless: the first `top` node records (the top of the tree, numbered breadth-first) are
+// copied to LDS at the start (top * 64 bytes of dynamic shared memory) and read from there.
 template <int kVariant>
 __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __restrict__ rays,
                                                           TraceHit* __restrict__ hits, uint32_t n, TraceScene sc,
-                                                          uint32_t chunk, uint32_t refill, uint32_t leaf_batch) {
+                                                          uint32_t chunk, uint32_t refill, uint32_t leaf_batch,
+                                                          uint32_t top) {
+    const SLTreelet src{reinterpret_cast<const float4*>(sc.sl_nodes), top};
+    if constexpr (kVariant == kVariantStackless) {
+        for (uint32_t j = threadIdx.x; j < 4 * top; j += blockDim.x) sl_treelet[j] = src.g[j];
+        __syncthreads();
+    }
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -155,14 +163,14 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
                 if constexpr (kVariant == kVariantThreaded)
                     threaded_leaf(st, sc.prims, sc.sphere_begin);
                 else
-                    stackless_leaf(st, sc.sl_nodes, sc.prims, sc.sphere_begin);
+                    stackless_leaf(st, src, sc.prims, sc.sphere_begin);
                 if (!walk_live(st)) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
             }
         } else if (walking) {
             if constexpr (kVariant == kVariantThreaded)
                 threaded_visit(st, finite);
             else
-                stackless_visit(st, sc.sl_nodes, finite);
+                stackless_visit(st, src, finite);
             if (!walk_live(st)) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
         }
     }
@@ -193,12 +201,19 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
         const uint32_t blocks = cus * wps;  // 4 waves (one per SIMD) per 256-thread block
         const uint32_t waves = blocks * 4u;
         const uint32_t chunk = (n + waves - 1) / waves;
-        if (variant == kVariantThreaded)
+        if (variant == kVariantThreaded) {
             hipLaunchKernelGGL(k_trace_persistent<kVariantThreaded>, dim3(blocks), dim3(256), 0, stream, rays, hits,
-                               n, sc, chunk, refill, leaf_batch);
-        else
-            hipLaunchKernelGGL(k_trace_persistent<kVariantStackless>, dim3(blocks), dim3(256), 0, stream, rays,
-                               hits, n, sc, chunk, refill, leaf_batch);
+                               n, sc, chunk, refill, leaf_batch, 0u);
+        } else {
+            // the top of the tree in LDS: 320 records (20 KB) let the 8 workgroups per CU of the
+            // default launch share the CU's 160 KB
+            uint32_t top = tune && tune->treelet ? tune->treelet : kDefaultTreelet;
+            if (top == kTreeletOff) top = 0;
+            top = top < sc.nnodes ? top : sc.nnodes;
+            top = top < kTreeletMaxNodes ? top : kTreeletMaxNodes;
+            hipLaunchKernelGGL(k_trace_persistent<kVariantStackless>, dim3(blocks), dim3(256), (size_t)top * 64u,
+                               stream, rays, hits, n, sc, chunk, refill, leaf_batch, top);
+        }
         return hipGetLastError();
     }
     if (block == 0) block = 256;

@@ -83,19 +83,33 @@ def test_persistent_launch_shapes_match_per_ray_kernel(ctx):
         for variant in ("stackless", "threaded"):
             ref = _lib.trace(ctx, scene, rays, variant, block=256)
             try:
-                for wps, refill, lb in ((1, 1, 1), (4, 16, 64), (8, 64, 8), (16, 8, 32), (0, 0, 0)):
-                    ctx.set_trace_tuning(wps, refill, lb)
+                for wps, refill, lb, tl in ((1, 1, 1, 1), (4, 16, 64, 7), (8, 64, 8, 1024), (16, 8, 32, 0),
+                                            (0, 0, 0, _lib.TREELET_OFF), (0, 0, 0, 0)):
+                    # tl: top-of-tree records in LDS (stackless), up to the 1024 maximum
+                    ctx.set_trace_tuning(wps, refill, lb, tl)
                     # block 0: the persistent kernel; 64: one thread per ray, leaves batched when lb > 1
                     for block in (0, 64):
                         g = _lib.trace(ctx, scene, rays, variant, block=block)
-                        assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (variant, wps, refill, lb, block,
-                                                                                   len(rays))
+                        assert H.compare_hits(g, ref) == dict(prim=0, t=0, uv=0), (variant, wps, refill, lb, tl,
+                                                                                   block, len(rays))
             finally:
                 ctx.set_trace_tuning(0, 0, 0)
     with pytest.raises(_lib.RgpuError):
         ctx.set_trace_tuning(17, 0)
     with pytest.raises(_lib.RgpuError):
         ctx.set_trace_tuning(0, 0, 65)
+    with pytest.raises(_lib.RgpuError):
+        ctx.set_trace_tuning(0, 0, 0, 1025)
+    # a tree smaller than the treelet: every record comes from LDS
+    v, f = scenes.displaced_icosphere(level=1)
+    small = _lib.Scene(ctx, _lib.BVH(v, f))
+    rays = scenes.incoherent_rays(5000, v.min(0), v.max(0), seed=25)
+    ref = _lib.trace(ctx, small, rays, "stackless", block=256)
+    try:
+        ctx.set_trace_tuning(0, 0, 0, 1024)
+        assert H.compare_hits(_lib.trace(ctx, small, rays, "stackless", block=0), ref) == dict(prim=0, t=0, uv=0)
+    finally:
+        ctx.set_trace_tuning(0, 0, 0, 0)
 
 
 def test_nan_guarded_slabs_in_mixed_waves(ctx):
