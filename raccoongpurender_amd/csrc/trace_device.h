@@ -305,6 +305,7 @@ struct ThreadedRay {
     F3 ro, rd, inv;
     float tmin, u, v;
     int32_t prim;
+    bool plain;     // origin and inverse direction allow the plain slab form (set once per ray)
     uint32_t node;  // kInvalid when finished
     uint32_t leaf;  // primitive word of an entered leaf not yet tested (0: none)
 };
@@ -315,6 +316,7 @@ __device__ __forceinline__ void threaded_begin(ThreadedRay& r, const MTNode* tab
     r.ro = ro;
     r.rd = rd;
     r.inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    r.plain = plain_slabs(true, ro, r.inv);
     r.tmin = tmax;
     r.u = r.v = 0.0f;
     r.prim = -1;
@@ -330,7 +332,7 @@ __device__ __forceinline__ void threaded_visit(ThreadedRay& r, bool boxes_finite
     float lo[3] = {q0.x, q0.y, q0.z}, hi[3] = {q0.w, q1.x, q1.y};
     uint32_t miss = __float_as_uint(q1.z), leaf = __float_as_uint(q1.w);
     float h0, h1;
-    bool hit = slab_one(lo, hi, r.ro, r.inv, r.tmin, plain_slabs(boxes_finite, r.ro, r.inv), &h0, &h1);
+    bool hit = slab_one(lo, hi, r.ro, r.inv, r.tmin, boxes_finite && r.plain, &h0, &h1);
     if (hit && leaf != 0) r.leaf = leaf;  // the miss link is where the walk goes on afterwards
     r.node = (hit && leaf == 0) ? r.node + 1 : miss;
 }
@@ -348,6 +350,7 @@ struct StacklessRay {
     F3 ro, rd, inv;
     float tmin, u, v;
     int32_t prim;
+    bool plain;     // origin and inverse direction allow the plain slab form (set once per ray)
     uint64_t bitstack;
     uint32_t node;  // kInvalid when finished
     ShortStack stk;
@@ -357,6 +360,7 @@ __device__ __forceinline__ void stackless_begin(StacklessRay& r, F3 ro, F3 rd, f
     r.ro = ro;
     r.rd = rd;
     r.inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    r.plain = plain_slabs(true, ro, r.inv);
     r.tmin = tmax;
     r.u = r.v = 0.0f;
     r.prim = -1;
@@ -459,8 +463,7 @@ __device__ __forceinline__ void stackless_visit(StacklessRay& r, const Src& src,
     float rmin[3] = {q1.z, q1.w, q2.x}, rmax[3] = {q2.y, q2.z, q2.w};
     float Lh0, Lh1, Rh0, Rh1;
     bool L, R;
-    slab_pair(lmin, lmax, rmin, rmax, r.ro, r.inv, r.tmin, plain_slabs(boxes_finite, r.ro, r.inv), &L, &R, &Lh0, &Lh1,
-              &Rh0, &Rh1);
+    slab_pair(lmin, lmax, rmin, rmax, r.ro, r.inv, r.tmin, boxes_finite && r.plain, &L, &R, &Lh0, &Lh1, &Rh0, &Rh1);
     if (L || R) {
         r.bitstack = r.bitstack << 1;
         if (L && R) {
