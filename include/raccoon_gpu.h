@@ -149,7 +149,7 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
 /* Launch shape of the persistent ray-cast kernel used when block == 0: resident waves per
  * SIMD (grid = CUs * waves_per_simd workgroups of 256; 1..16), the number of idle lanes of a
  * wave that makes it hand out its next rays (1..64), and the number of lanes waiting at leaves
- * that makes it test their primitives (1..64). 0 keeps the default (8 waves; refill at 32 idle
+ * that makes it test their primitives (1..64). 0 keeps the default (8 waves; refill at 16 idle
  * lanes for stackless, 48 for threaded; leaves at 8 waiting lanes). A leaf_batch above 1 also
  * batches leaves in the one-thread-per-ray kernels (block != 0), which by default do not (their
  * usual rays are coherent). treelet: the number of top-of-tree stackless node records each

@@ -40,7 +40,7 @@ struct TraceScene {
 // Launch shape of the persistent ray-cast kernel: CUs to fill, resident waves per SIMD (grid =
 // cu_count * waves_per_simd workgroups of 256), the idle-lane count that triggers a refill and
 // the count of lanes waiting at leaves that triggers their primitive tests. Zeros take the
-// defaults (256 CUs, 8 waves; refill at 32 lanes for the stackless walk, 48 for the threaded
+// defaults (256 CUs, 8 waves; refill at 16 lanes for the stackless walk, 48 for the threaded
 // one; leaves at 8), measured on 4M incoherent rays over 1.3M triangles.
 struct TraceTuning {
     uint32_t cu_count = 0;
