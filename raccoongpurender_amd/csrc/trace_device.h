@@ -511,9 +511,12 @@ __device__ __forceinline__ bool walk_pending(const ThreadedRay& r) { return r.le
 // leaf_batch lanes (or every lane still walking) wait too, then the wave tests their primitives
 // together. The lane's own visits and tests are those of trace_stackless / trace_threaded.
 // Lanes whose ray has ended stay in the loop, masked, until the wave is done.
-template <int kVariant>
+// kTreelet (stackless): the kernel has staged the first `top` node records in LDS.
+template <int kVariant, bool kTreelet = false>
 __device__ __forceinline__ int32_t trace_batched(const TraceScene& sc, F3 ro, F3 rd, float* tmin_io, float* uo,
-                                                 float* vo, uint32_t leaf_batch) {
+                                                 float* vo, uint32_t leaf_batch, uint32_t top = 0) {
+    const SLTreelet tsrc{reinterpret_cast<const float4*>(sc.sl_nodes), top};
+    const SLGlobal gsrc{reinterpret_cast<const float4*>(sc.sl_nodes)};
     using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
     State st;
     if constexpr (kVariant == kVariantThreaded)
@@ -530,14 +533,18 @@ __device__ __forceinline__ int32_t trace_batched(const TraceScene& sc, F3 ro, F3
             if (pending) {
                 if constexpr (kVariant == kVariantThreaded)
                     threaded_leaf(st, sc.prims, sc.sphere_begin);
+                else if constexpr (kTreelet)
+                    stackless_leaf(st, tsrc, sc.prims, sc.sphere_begin);
                 else
-                    stackless_leaf(st, sc.sl_nodes, sc.prims, sc.sphere_begin);
+                    stackless_leaf(st, gsrc, sc.prims, sc.sphere_begin);
             }
         } else if (walking) {
             if constexpr (kVariant == kVariantThreaded)
                 threaded_visit(st, finite);
+            else if constexpr (kTreelet)
+                stackless_visit(st, tsrc, finite);
             else
-                stackless_visit(st, sc.sl_nodes, finite);
+                stackless_visit(st, gsrc, finite);
         }
     }
     *tmin_io = st.tmin;

@@ -920,8 +920,19 @@ __device__ __forceinline__ void store_extension(const DevState& s, const TraceSc
 // extension_ray_cast (kernels/extension_ray_cast_stackless.cl:202-246) over every slot, one
 // thread per slot.
 template <int kVariant, class J>
-__global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jobs) {
+__global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jobs, uint32_t top) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (kVariant == kVariantStackless) {
+        // the top `top` records of the block's scene in LDS (a job owns whole 256-slot blocks,
+        // so every slot of a workgroup traces the same scene)
+        const uint32_t first = blockIdx.x * blockDim.x;
+        const TraceScene sb = jobs.scene(jobs.job(first < p.n ? first : p.n - 1));
+        const uint32_t t = top < sb.nnodes ? top : sb.nnodes;
+        const float4* g = reinterpret_cast<const float4*>(sb.sl_nodes);
+        for (uint32_t j = threadIdx.x; j < 4 * t; j += blockDim.x) sl_treelet[j] = g[j];
+        __syncthreads();
+        top = t;
+    }
     advance_cursors(s, jobs, slot);
     if (slot >= p.n) return;
     if (J::kBatch && ray_pixel(s.ray, slot) == kDeadPixel) return;
@@ -936,7 +947,7 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jo
     load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
     float t = tmin, u = 0.0f, vv = 0.0f;
     int32_t prim = -1;
-    if (sc.nnodes != 0) prim = trace_batched<kVariant>(sc, ro, rd, &t, &u, &vv, kDefaultLeafBatch);
+    if (sc.nnodes != 0) prim = trace_batched<kVariant, true>(sc, ro, rd, &t, &u, &vv, kDefaultLeafBatch, top);
     store_extension(s, sc, slot, ro, rd, prim, t, hit_volume, tmin);
 }
 
@@ -1322,9 +1333,12 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
             hipLaunchKernelGGL(k_trace_finish<J>, full, blk, 0, s, st, p, jobs);
         }
     } else if (variant == kVariantThreaded) {
-        hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs);
+        hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs, 0u);
     } else {
-        hipLaunchKernelGGL((k_trace_step<kVariantStackless, J>), tgrid, tblk, 0, s, st, p, jobs);
+        // the top of the tree in LDS: 80 records per wave, so that 8 waves per SIMD fit the 160 KB
+        const uint32_t top = kDefaultTreelet * tb / 256u;
+        hipLaunchKernelGGL((k_trace_step<kVariantStackless, J>), tgrid, tblk, (size_t)top * 64u, s, st, p, jobs,
+                           top);
     }
     hipLaunchKernelGGL(k_logic<J>, full, blk, 0, s, st, p, jobs, g);
     hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues + 1), dim3(1024), 0, s, st,
