@@ -311,6 +311,19 @@ int rgpu_wavefront_stat(rgpu_wavefront* w, uint64_t* total_samples);
 int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, float* paths, void* ext,
                               float* shade, int32_t* vol, uint32_t* items);
 
+/* Checkpoint / resume of a single render (no batch, no attached guide): a host snapshot of
+ * everything a lane carries from one step to the next (rays, path, extension and shading
+ * records, medium lists, RNG states, queues, pixel cursor, accumulation, camera). Resuming it
+ * into a lane of the same path count and accumulation mode (the same lane or a new one over the
+ * same scene, materials and envmap) continues the render exactly where the snapshot was taken:
+ * the following steps give the slot states the uninterrupted lane would have. The reference has
+ * none (its state lives only in device buffers, common/wavefront_path_tracing.hpp:592-604);
+ * re_start there re-initialises. Synchronous. checkpoint_size: bytes the snapshot needs. A
+ * failed resume leaves the lane uninitialised. */
+int rgpu_wavefront_checkpoint_size(rgpu_wavefront* w, uint64_t* bytes);
+int rgpu_wavefront_checkpoint(rgpu_wavefront* w, void* host, uint64_t bytes);
+int rgpu_wavefront_resume(rgpu_wavefront* w, const void* host, uint64_t bytes);
+
 /* hipGraph of the bounce loop: captures `steps` consecutive steps once; each launch replays
  * them. Replaces the reference's per-step host launch list (wavefront_path_tracing.hpp:606-875).
  * The graph holds the camera and buffers by value; initialize() and set_launch() drop it.

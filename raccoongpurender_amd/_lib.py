@@ -392,6 +392,10 @@ _sig("rgpu_wavefront_read_accumulation", C.c_int, vp, vp, vp)
 _sig("rgpu_wavefront_tonemap", C.c_int, vp, vp)
 _sig("rgpu_wavefront_stat", C.c_int, vp, C.POINTER(C.c_uint64))
 _sig("rgpu_wavefront_read_state", C.c_int, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("rgpu_wavefront_checkpoint_size", C.c_int, vp, C.POINTER(C.c_uint64))
+_sig("rgpu_wavefront_checkpoint", C.c_int, vp, vp, C.c_uint64)
+_sig("rgpu_wavefront_resume", C.c_int, vp, vp, C.c_uint64)
+EXPORTED += ["rgpu_wavefront_checkpoint_size", "rgpu_wavefront_checkpoint", "rgpu_wavefront_resume"]
 _sig("rgpu_wavefront_graph_capture", C.c_int, vp, u32)
 _sig("rgpu_wavefront_graph_launch", C.c_int, vp, vp, u32)
 _sig("rgpu_wavefront_set_launch", C.c_int, vp, C.c_int, u32, u32)
@@ -639,6 +643,19 @@ class Wavefront:
                                                                 ("rng", "rays", "paths", "ext", "shade", "vol",
                                                                  "items")]), "rgpu_wavefront_read_state")
         return st
+
+    def checkpoint(self):
+        """Host snapshot of the lane (numpy uint8 array) that resume() continues exactly."""
+        nb = C.c_uint64(0)
+        self.ctx.check(lib.rgpu_wavefront_checkpoint_size(self.h, C.byref(nb)), "rgpu_wavefront_checkpoint_size")
+        blob = np.zeros(nb.value, np.uint8)
+        self.ctx.check(lib.rgpu_wavefront_checkpoint(self.h, ptr(blob), nb.value), "rgpu_wavefront_checkpoint")
+        return blob
+
+    def resume(self, blob):
+        blob = np.ascontiguousarray(blob, np.uint8)
+        self.ctx.check(lib.rgpu_wavefront_resume(self.h, ptr(blob), blob.nbytes), "rgpu_wavefront_resume")
+        self.npixels = int(blob[16:20].view(np.uint32)[0])  # header: magic, version, path count, pixels
 
     def graph_capture(self, steps):
         self.ctx.check(lib.rgpu_wavefront_graph_capture(self.h, steps), "rgpu_wavefront_graph_capture")

@@ -443,6 +443,40 @@ int rgpu_camera_look_at(const float* eye, const float* target, const float* up, 
     return RGPU_OK;
 }
 
+// (Re)allocates the accumulation and tonemap buffers of a single render for P pixels.
+static int ensure_accum(rgpu_wavefront* w, uint32_t P) {
+    if (P == w->npixels) return RGPU_OK;
+    for (void* q : {(void*)w->st.accum, (void*)w->st.accum64, (void*)w->st.accum_count, (void*)w->tonemap_buf}) {
+        if (!q) continue;
+        for (auto it = w->owned.begin(); it != w->owned.end(); ++it)
+            if (*it == q) {
+                (void)hipFree(q);
+                w->owned.erase(it);
+                break;
+            }
+    }
+    w->st.accum = nullptr;
+    w->st.accum64 = nullptr;
+    w->st.accum_count = nullptr;
+    w->tonemap_buf = nullptr;
+    w->npixels = 0;
+    void* ptr;
+    int rc;
+    if (w->params.accum64) {
+        if ((rc = wf_alloc(w, (size_t)P * 3 * sizeof(double), &ptr)) != RGPU_OK) return rc;
+        w->st.accum64 = (double*)ptr;
+        if ((rc = wf_alloc(w, (size_t)P * sizeof(uint32_t), &ptr)) != RGPU_OK) return rc;
+        w->st.accum_count = (uint32_t*)ptr;
+    } else {
+        if ((rc = wf_alloc(w, (size_t)P * 4 * sizeof(float), &ptr)) != RGPU_OK) return rc;
+        w->st.accum = (float*)ptr;
+    }
+    if ((rc = wf_alloc(w, (size_t)P * 4, &ptr)) != RGPU_OK) return rc;
+    w->tonemap_buf = (uint8_t*)ptr;
+    w->npixels = P;
+    return RGPU_OK;
+}
+
 int rgpu_wavefront_destroy(rgpu_wavefront* w) {
     if (!w) return RGPU_ERR_INVALID_ARG;
     (void)hipSetDevice(w->ctx->device);
@@ -690,35 +724,8 @@ int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* cam, uint32_
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(w);  // a captured graph holds the old camera and buffers by value
     const uint32_t P = cam->width * cam->height;
-    if (P != w->npixels) {
-        // (re)allocate the accumulation buffers for this resolution
-        for (void* q : {(void*)w->st.accum, (void*)w->st.accum64, (void*)w->st.accum_count, (void*)w->tonemap_buf}) {
-            if (!q) continue;
-            for (auto it = w->owned.begin(); it != w->owned.end(); ++it)
-                if (*it == q) {
-                    (void)hipFree(q);
-                    w->owned.erase(it);
-                    break;
-                }
-        }
-        w->st.accum = nullptr;
-        w->st.accum64 = nullptr;
-        w->st.accum_count = nullptr;
-        void* ptr;
-        int rc;
-        if (w->params.accum64) {
-            if ((rc = wf_alloc(w, (size_t)P * 3 * sizeof(double), &ptr)) != RGPU_OK) return rc;
-            w->st.accum64 = (double*)ptr;
-            if ((rc = wf_alloc(w, (size_t)P * sizeof(uint32_t), &ptr)) != RGPU_OK) return rc;
-            w->st.accum_count = (uint32_t*)ptr;
-        } else {
-            if ((rc = wf_alloc(w, (size_t)P * 4 * sizeof(float), &ptr)) != RGPU_OK) return rc;
-            w->st.accum = (float*)ptr;
-        }
-        if ((rc = wf_alloc(w, (size_t)P * 4, &ptr)) != RGPU_OK) return rc;
-        w->tonemap_buf = (uint8_t*)ptr;
-        w->npixels = P;
-    }
+    int arc = ensure_accum(w, P);
+    if (arc != RGPU_OK) return arc;
     w->cam = to_camera(cam);
     // WavefrontLane::initialize (common/wavefront_path_tracing.hpp:532-570): pixel offset
     // lane * (P / lanes), RNG offset lane * N, every slot in the new-path queue.
@@ -930,6 +937,127 @@ int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, flo
     if (shade) RGPU_HIP(ctx, hipMemcpy(shade, w->st.shade, n * 32, hipMemcpyDeviceToHost));
     if (vol) RGPU_HIP(ctx, hipMemcpy(vol, w->st.vol, n * 32, hipMemcpyDeviceToHost));
     if (items) RGPU_HIP(ctx, hipMemcpy(items, w->st.items, n * kNumQueues * 4, hipMemcpyDeviceToHost));
+    return RGPU_OK;
+}
+
+// ------------------------------------------------------------------ checkpoint / resume
+namespace {
+struct CheckpointHeader {
+    char magic[8];  // "RGPUCKPT"
+    uint32_t version;
+    uint32_t path_count;
+    uint32_t npixels;
+    uint32_t accum64;
+    Camera cam;
+    uint64_t seed;     // the lane's configured seed (provenance; the RNG states are in the payload)
+    uint64_t payload;  // bytes after the header
+};
+constexpr uint32_t kCheckpointVersion = 1;
+
+struct Section {
+    void* dev;
+    size_t bytes;
+};
+
+// The device buffers a single render carries from one step to the next, in checkpoint order.
+std::vector<Section> checkpoint_sections(const rgpu_wavefront* w, uint32_t npixels) {
+    const size_t n = w->params.n, P = npixels;
+    const DevState& s = w->st;
+    std::vector<Section> v = {
+        {s.ray, n * sizeof(TraceRay)},   {s.path, n * sizeof(PathRec)},  {s.ext, n * sizeof(ExtRec)},
+        {s.shade, n * sizeof(ShadeRec)}, {s.vol, n * sizeof(VolList)},   {s.rng, n * sizeof(uint4)},
+        {s.qsel, n},                     {s.items, n * kNumQueues * 4},  {s.counts, kNumQueues * 4},
+        {s.block_counts, (size_t)s.nblocks * (kNumQueues + 1) * 4},      {s.next_pixel, 8},
+    };
+    if (w->params.accum64) {
+        v.push_back({s.accum64, P * 3 * sizeof(double)});
+        v.push_back({s.accum_count, P * 4});
+    } else {
+        v.push_back({s.accum, P * 4 * sizeof(float)});
+    }
+    return v;
+}
+
+uint64_t checkpoint_bytes(const rgpu_wavefront* w, uint32_t npixels) {
+    uint64_t b = sizeof(CheckpointHeader);
+    for (const Section& s : checkpoint_sections(w, npixels)) b += s.bytes;
+    return b;
+}
+
+int checkpoint_usable(rgpu_wavefront* w) {
+    if (w->is_batch()) return fail(w->ctx, RGPU_ERR_UNSUPPORTED, "checkpoints cover single renders, not batches");
+    if (w->guide.enabled) return fail(w->ctx, RGPU_ERR_UNSUPPORTED, "checkpoints do not cover an attached guide");
+    return RGPU_OK;
+}
+}  // namespace
+
+int rgpu_wavefront_checkpoint_size(rgpu_wavefront* w, uint64_t* bytes) {
+    if (!w || !bytes) return RGPU_ERR_INVALID_ARG;
+    if (!w->initialized) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_initialize was not called");
+    int rc = checkpoint_usable(w);
+    if (rc != RGPU_OK) return rc;
+    *bytes = checkpoint_bytes(w, w->npixels);
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_checkpoint(rgpu_wavefront* w, void* host, uint64_t bytes) {
+    if (!w || !host) return RGPU_ERR_INVALID_ARG;
+    if (!w->initialized) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_initialize was not called");
+    int rc = checkpoint_usable(w);
+    if (rc != RGPU_OK) return rc;
+    const uint64_t need = checkpoint_bytes(w, w->npixels);
+    if (bytes < need) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint buffer too small");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CheckpointHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, "RGPUCKPT", 8);
+    h.version = kCheckpointVersion;
+    h.path_count = w->params.n;
+    h.npixels = w->npixels;
+    h.accum64 = w->params.accum64 ? 1u : 0u;
+    h.cam = w->cam;
+    h.seed = w->seed;
+    h.payload = need - sizeof(h);
+    std::memcpy(host, &h, sizeof(h));
+    uint8_t* p = static_cast<uint8_t*>(host) + sizeof(h);
+    for (const Section& s : checkpoint_sections(w, w->npixels)) {
+        RGPU_HIP(ctx, hipMemcpyAsync(p, s.dev, s.bytes, hipMemcpyDeviceToHost, ctx->stream));
+        p += s.bytes;
+    }
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_resume(rgpu_wavefront* w, const void* host, uint64_t bytes) {
+    if (!w || !host) return RGPU_ERR_INVALID_ARG;
+    int rc = checkpoint_usable(w);
+    if (rc != RGPU_OK) return rc;
+    CheckpointHeader h;
+    if (bytes < sizeof(h)) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "not a checkpoint (too short)");
+    std::memcpy(&h, host, sizeof(h));
+    if (std::memcmp(h.magic, "RGPUCKPT", 8) != 0 || h.version != kCheckpointVersion)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "not a checkpoint of this library version");
+    if (h.path_count != w->params.n || h.accum64 != (w->params.accum64 ? 1u : 0u))
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint of a lane with another path count or accumulation");
+    if (h.npixels == 0 || bytes < sizeof(h) + h.payload ||
+        sizeof(h) + h.payload != checkpoint_bytes(w, h.npixels))
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint truncated or inconsistent");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(w);  // a captured graph holds the old camera by value
+    w->initialized = false;  // until every buffer holds the checkpoint
+    if ((rc = ensure_accum(w, h.npixels)) != RGPU_OK) return rc;
+    const uint8_t* p = static_cast<const uint8_t*>(host) + sizeof(h);
+    for (const Section& s : checkpoint_sections(w, h.npixels)) {
+        RGPU_HIP(ctx, hipMemcpyAsync(s.dev, p, s.bytes, hipMemcpyHostToDevice, ctx->stream));
+        p += s.bytes;
+    }
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    w->cam = h.cam;
+    w->initialized = true;
     return RGPU_OK;
 }
 

@@ -310,6 +310,25 @@ public:
         initialize(lane_index, num_device_);
     }
     int step_count() const { return step_count_; }
+    // Snapshot of the lane that resume() continues exactly (no counterpart in the reference,
+    // whose lane state lives only in device buffers).
+    struct Checkpoint {
+        rgpu_camera camera;
+        int step_count;
+        std::vector<uint8_t> bytes;
+    };
+    Checkpoint checkpoint() const {
+        uint64_t n = 0;
+        check(rgpu_wavefront_checkpoint_size(wf_.get(), &n), lane_.get(), "rgpu_wavefront_checkpoint_size");
+        Checkpoint c{camera_, step_count_, std::vector<uint8_t>(n)};
+        check(rgpu_wavefront_checkpoint(wf_.get(), c.bytes.data(), n), lane_.get(), "rgpu_wavefront_checkpoint");
+        return c;
+    }
+    void resume(const Checkpoint& c) {
+        check(rgpu_wavefront_resume(wf_.get(), c.bytes.data(), c.bytes.size()), lane_.get(), "rgpu_wavefront_resume");
+        camera_ = c.camera;
+        step_count_ = c.step_count;
+    }
     // stat_avg_sample(): samples per pixel so far (kernels/stat.cl)
     float stat_avg_sample() const {
         uint64_t total = 0;

@@ -152,6 +152,18 @@ int main(int argc, char** argv) {
         pt.re_start(cam);
         pt.launch_fixed(12);
         const std::vector<rt::RGBA8ValueType> again = image;
+        // a checkpoint resumed after more steps continues as the uninterrupted lane: the sample
+        // totals (integer counts) after the same further steps agree exactly
+        {
+            rt::WavefrontLane& lane = pt.lane();
+            const rt::WavefrontLane::Checkpoint ck = lane.checkpoint();
+            for (int i = 0; i < 3; ++i) lane.step();
+            const float spp_a = lane.stat_avg_sample();
+            lane.resume(ck);
+            for (int i = 0; i < 3; ++i) lane.step();
+            if (lane.stat_avg_sample() != spp_a || lane.step_count() != ck.step_count + 3)
+                throw std::runtime_error("resumed lane diverged from the uninterrupted one");
+        }
         // ray casts through both BVH walks
         std::vector<rgpu_ray> rays;
         for (int i = 0; i < 1000; ++i) {
