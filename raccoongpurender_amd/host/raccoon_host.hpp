@@ -15,8 +15,11 @@
 
 #include <stdint.h>
 
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <functional>
+#include <sstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -134,6 +137,69 @@ struct MaterialRef {
 
 // SceneManager (common/scene_manager.hpp:330-650): triangles (primitives 0..ntris-1) then
 // spheres, one material per primitive, material tables per type, an optional envmap.
+// Triangle mesh of a Wavefront OBJ file: `v` and `f` records (corners `i`, `i/t`, `i//n`,
+// `i/t/n`; negative indices count back from the last vertex; polygons split into a fan
+// (0, k, k+1)); other records are skipped. OBJ faces are counter-clockwise seen from the front;
+// with `clockwise` the corners are reversed to addPolymesh's clockwise order. Coordinates are
+// read as doubles and rounded to float. Throws GpuError(RGPU_ERR_INVALID_ARG) naming the line of
+// a bad record.
+struct ObjMesh {
+    std::vector<float> points;     // xyz
+    std::vector<uint32_t> indices;  // 3 per triangle
+};
+inline ObjMesh parse_obj(std::istream& in, bool clockwise = true) {
+    ObjMesh m;
+    std::string line;
+    size_t ln = 0;
+    auto bad = [&](const std::string& what) {
+        return GpuError(RGPU_ERR_INVALID_ARG, "OBJ line " + std::to_string(ln) + ": " + what);
+    };
+    std::vector<uint32_t> corner;
+    while (std::getline(in, line)) {
+        ++ln;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        std::istringstream ss(line);
+        std::string tag;
+        if (!(ss >> tag)) continue;
+        if (tag == "v") {
+            std::string c[3];
+            if (!(ss >> c[0] >> c[1] >> c[2])) throw bad("a vertex needs three coordinates");
+            for (const std::string& s : c) {
+                char* end = nullptr;
+                const double d = std::strtod(s.c_str(), &end);
+                if (end == s.c_str() || *end != 0) throw bad("bad vertex coordinate");
+                m.points.push_back((float)d);
+            }
+        } else if (tag == "f") {
+            corner.clear();
+            std::string c;
+            const long nv = (long)(m.points.size() / 3);
+            while (ss >> c) {
+                const std::string head = c.substr(0, c.find('/'));
+                char* end = nullptr;
+                const long i = std::strtol(head.c_str(), &end, 10);
+                if (head.empty() || *end != 0) throw bad("bad face corner '" + c + "'");
+                const long j = i > 0 ? i - 1 : nv + i;
+                if (i == 0 || j < 0 || j >= nv) throw bad("vertex index " + head + " out of range");
+                corner.push_back((uint32_t)j);
+            }
+            if (corner.size() < 3) throw bad("a face needs three corners");
+            for (size_t k = 1; k + 1 < corner.size(); ++k) {
+                m.indices.push_back(corner[0]);
+                m.indices.push_back(clockwise ? corner[k + 1] : corner[k]);
+                m.indices.push_back(clockwise ? corner[k] : corner[k + 1]);
+            }
+        }
+    }
+    return m;
+}
+inline ObjMesh load_obj(const std::string& path, bool clockwise = true) {
+    std::ifstream in(path);
+    if (!in) throw GpuError(RGPU_ERR_INVALID_ARG, "cannot open OBJ file " + path);
+    return parse_obj(in, clockwise);
+}
+
 class SceneManager {
 public:
     MaterialRef addLambertian(const float Le[3], const float R[3], bool backEmission = false) {
@@ -170,6 +236,11 @@ public:
         }
         tri_mats_.insert(tri_mats_.end(), ntris, m);
         bvh_.reset();
+    }
+    // A Wavefront OBJ mesh (load_obj) as one polymesh with material `m`.
+    void addObj(const std::string& path, MaterialRef m, bool clockwise = true) {
+        const ObjMesh o = load_obj(path, clockwise);
+        addPolymesh(o.points.data(), o.points.size() / 3, o.indices.data(), o.indices.size() / 3, m);
     }
     // Sphere primitives of addPointPrimitive (scene_manager.hpp:372-430).
     void addSphere(const float center[3], float radius, MaterialRef m) {

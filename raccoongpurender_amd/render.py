@@ -8,7 +8,7 @@ import time
 
 import numpy as np
 
-from . import _lib, render_scenes, render_workloads
+from . import _lib, render_scenes, render_workloads, scenes
 from .png import write_png
 
 SCENES = {
@@ -36,6 +36,7 @@ def build_lane(ctx, sd, path_count, **kw):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--scene", choices=sorted(SCENES), default="cornell")
+    ap.add_argument("--obj", default="", help="render this Wavefront OBJ mesh on a ground plane under a sky")
     ap.add_argument("--width", type=int, default=0)
     ap.add_argument("--height", type=int, default=0)
     ap.add_argument("--paths", type=int, default=1 << 22)
@@ -46,7 +47,10 @@ def main():
     ap.add_argument("--queue-blocks", type=int, default=0, help="workgroups of the queue kernels (0: default)")
     ap.add_argument("--out", default="render.png")
     a = ap.parse_args()
-    sd = SCENES[a.scene](a.width, a.height)
+    if a.obj:
+        sd = render_scenes.mesh_scene(*scenes.load_obj(a.obj), a.width or 320, a.height or 240)
+    else:
+        sd = SCENES[a.scene](a.width, a.height)
     ctx = _lib.Context(0)
     wf = build_lane(ctx, sd, a.paths, variant=a.variant)
     if a.trace_block != 256 or a.queue_blocks:

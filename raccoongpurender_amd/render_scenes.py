@@ -146,6 +146,27 @@ def cornell_box(width=256, height=256, segments=64, rings=32, lens_radius=0.0045
                 env=constant_envmap(0.0, 8, 4), camera=cam)
 
 
+def mesh_scene(points, indices, width=320, height=240, albedo=0.75, env_size=(256, 128)):
+    """A loaded mesh (clockwise triangles, e.g. scenes.load_obj) on a lambertian ground under a
+    sky with a sun, the camera framing the mesh's bounding box from its front-left."""
+    p = np.asarray(points, np.float64)
+    lo, hi = p.min(axis=0), p.max(axis=0)
+    c, r = 0.5 * (lo + hi), 0.5 * float(np.linalg.norm(hi - lo))
+    mb = MeshBuilder()
+    lam = [((0, 0, 0), (albedo, albedo, albedo)), ((0, 0, 0), (0.5, 0.5, 0.5))]
+    mb.add(p, np.asarray(indices, np.int64), MAT_LAMBERTIAN, 0)
+    e = 4.0 * r
+    y = lo[1] - 1e-3 * r
+    g = quad((c[0] - e, y, c[2] + e), (c[0] + e, y, c[2] + e), (c[0] + e, y, c[2] - e), (c[0] - e, y, c[2] - e))
+    mb.add(g[0], g[1], MAT_LAMBERTIAN, 1)
+    v, f, types, idx = mb.build()
+    env = sky_envmap(env_size[0], env_size[1], sun_peak=200.0, sun_width_deg=4.0)
+    eye = c + r * np.array([-1.2, 0.8, 2.4])
+    cam = dict(eye=tuple(eye.tolist()), target=tuple(c.tolist()), up=(0.0, 1.0, 0.0), fov=40.0, width=width,
+               height=height, lens_radius=0.0)
+    return dict(points=v, indices=f, materials=MaterialSet(types, idx, lambertians=lam), env=env, camera=cam)
+
+
 def material_test_scene(width=64, height=48, env_size=(64, 32), sun_width_deg=6.0, volume=True):
     """Every material type: a lambertian ground, icospheres of lambertian, isotropic Ward,
     anisotropic Ward, mirror and dielectric, and a homogeneous-medium box, under a sky with

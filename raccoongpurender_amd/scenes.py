@@ -152,3 +152,47 @@ def raycast_benchmark_scene(level=8, n_rays=None, width=2048, height=2048, seed_
     coherent = primary_rays(width, height)
     incoherent = incoherent_rays(n, lo, hi, seed=seed_rays)
     return dict(points=v, indices=f, coherent=coherent, incoherent=incoherent, aabb=(lo, hi))
+
+
+def load_obj(source, clockwise=True):
+    """Triangle mesh of a Wavefront OBJ file (a path, or the text itself when it contains a
+    newline): (points float32 (n, 3), indices uint32 (m, 3)). Reads `v` and `f` records; a face
+    may give `i`, `i/t`, `i//n` or `i/t/n` per corner, negative indices count back from the
+    last vertex, and polygons are split into a fan (0, k, k+1). Other records (vt, vn, o, g, s,
+    usemtl, mtllib, comments) are skipped. OBJ faces are counter-clockwise seen from the front;
+    with `clockwise` the corners are reversed to the clockwise order of addPolymesh
+    (common/scene_manager.hpp:339-370). Raises ValueError naming the line of a bad record."""
+    if "\n" in source:
+        lines = source.splitlines()
+    else:
+        with open(source, "r") as fh:
+            lines = fh.read().splitlines()
+    pts, tris = [], []
+    for ln, line in enumerate(lines, 1):
+        rec = line.split("#", 1)[0].split()
+        if not rec:
+            continue
+        if rec[0] == "v":
+            if len(rec) < 4:
+                raise ValueError(f"line {ln}: a vertex needs three coordinates")
+            try:
+                pts.append((float(rec[1]), float(rec[2]), float(rec[3])))
+            except ValueError:
+                raise ValueError(f"line {ln}: bad vertex coordinate") from None
+        elif rec[0] == "f":
+            if len(rec) < 4:
+                raise ValueError(f"line {ln}: a face needs three corners")
+            corner = []
+            for c in rec[1:]:
+                try:
+                    i = int(c.split("/", 1)[0])
+                except ValueError:
+                    raise ValueError(f"line {ln}: bad face corner {c!r}") from None
+                j = i - 1 if i > 0 else len(pts) + i
+                if i == 0 or not 0 <= j < len(pts):
+                    raise ValueError(f"line {ln}: vertex index {i} out of range")
+                corner.append(j)
+            for k in range(1, len(corner) - 1):
+                t = (corner[0], corner[k], corner[k + 1])
+                tris.append((t[0], t[2], t[1]) if clockwise else t)
+    return np.asarray(pts, np.float32).reshape(-1, 3), np.asarray(tris, np.uint32).reshape(-1, 3)

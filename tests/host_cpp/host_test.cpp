@@ -92,6 +92,25 @@ void dump(const std::string& path, const std::vector<T>& v) {
 
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "cpu";
+    if (mode == "obj" && argc > 2) {
+        // host_test obj <file>: the mesh rt::load_obj reads, floats as their bit patterns
+        try {
+            const rt::ObjMesh m = rt::load_obj(argv[2]);
+            std::printf("{\"points_bits\": [");
+            for (size_t i = 0; i < m.points.size(); ++i) {
+                uint32_t b;
+                std::memcpy(&b, &m.points[i], 4);
+                std::printf("%s%u", i ? ", " : "", b);
+            }
+            std::printf("], \"indices\": [");
+            for (size_t i = 0; i < m.indices.size(); ++i) std::printf("%s%u", i ? ", " : "", m.indices[i]);
+            std::printf("]}\n");
+            return 0;
+        } catch (const rt::GpuError& e) {
+            std::printf("{\"error\": %d}\n", e.code);
+            return 0;
+        }
+    }
     try {
         Scene s;
         build(s);

@@ -138,3 +138,19 @@ def test_per_pixel_means_agree_with_the_oracle_at_other_seeds(ctx):
     rel = abs(g_means.mean() - o_means.mean()) / o_means.mean()
     print(f"{p.size} tests, {rejected} rejected at 1e-3; image means differ by {rel:.2e} relative", flush=True)
     assert rejected <= 5
+
+
+def test_obj_mesh_scene(ctx):
+    # a mesh read through the OBJ loader (an icosphere written as OBJ text, counter-clockwise
+    # faces, one quad split into a fan) renders like every other scene
+    from raccoongpurender_amd import scenes
+    v, f = scenes.icosphere(2)
+    lines = [f"v {x:.6f} {y:.6f} {z:.6f}" for x, y, z in v.tolist()]
+    lines += [f"f {a + 1} {b + 1} {c + 1}" for a, b, c in f.tolist()]
+    n = len(v)
+    lines += [f"v {x} -1.2 {z}" for x, z in ((-0.5, 0.5), (0.5, 0.5), (0.5, -0.5), (-0.5, -0.5))]
+    lines += [f"f {n + 1} {n + 2} {n + 3} {n + 4}"]
+    pts, tri = scenes.load_obj("\n".join(lines) + "\n")
+    sd = R.mesh_scene(pts, tri, 40, 30, env_size=(64, 32))
+    print("case OBJ mesh scene", flush=True)
+    _compare(ctx, sd, 40 * 30 * 2, 10)

@@ -1,7 +1,9 @@
 """The C++ host surface (raccoongpurender_amd/host/raccoon_host.hpp) through its test program
 tests/host_cpp/host_test (built by __graft_entry__.build()).
 
-CPU: scene assembly, BVH build, argument checks and the missing-device error.
+CPU: scene assembly, BVH build, argument checks and the missing-device error; the OBJ reader
+(rt::load_obj) against the Python one (scenes.load_obj) on one file, bit for bit, and both
+refusing the same bad records.
 GPU: SceneManager -> WavefrontPathTracing.launch_fixed/re_start and the two BVH ray casters;
 the program dumps its scene, and the Python binding redoes the same render and ray casts
 through the C ABI: sample counts equal, images within one 8-bit step (float atomics add the
@@ -14,7 +16,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from raccoongpurender_amd import _lib
+from raccoongpurender_amd import _lib, scenes
 
 BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "host_cpp", "host_test")
 needs_bin = pytest.mark.skipif(not os.path.exists(BIN), reason="host_test not built (run __graft_entry__.build())")
@@ -71,3 +73,50 @@ def test_host_surface_renders_like_the_binding(ctx, tmp_path):
         g = _lib.trace(ctx, scene, rays, variant)
         host = _load(d, f"hits_{variant}.i32", np.int32).view(_lib.HIT_DTYPE)
         assert np.array_equal(g.view(np.int32), host.view(np.int32)), variant
+
+
+OBJ_TEXT = """# a quad, a pentagon and mixed corner forms
+mtllib none.mtl
+o thing
+v 0 0 0
+v 1.0 0 0
+v 1 1 0 1.0
+v 0 1 0
+vt 0 0
+vn 0 0 1
+v 0.1 0.2 0.30000001192092896
+v -2.5e-3 7 1e1
+g part
+usemtl m
+s off
+f 1 2 3 4
+f 1/1 2//1 5/1/1
+f -1 -2 -3 -4 -5   # pentagon, counted back from the last vertex
+"""
+
+
+@needs_bin
+def test_obj_loader_cpp_and_python_agree(tmp_path):
+    path = tmp_path / "mesh.obj"
+    path.write_text(OBJ_TEXT)
+    out = json.loads(subprocess.run([BIN, "obj", str(path)], check=True, capture_output=True, text=True).stdout)
+    pts, tri = scenes.load_obj(str(path))
+    assert np.array_equal(np.asarray(out["points_bits"], np.uint32), pts.reshape(-1).view(np.uint32))
+    assert np.array_equal(np.asarray(out["indices"], np.uint32), tri.reshape(-1))
+    # fans of (0, k, k+1), reversed to clockwise; negative indices from the end
+    ccw = scenes.load_obj(OBJ_TEXT, clockwise=False)[1]
+    assert ccw.tolist() == [[0, 1, 2], [0, 2, 3], [0, 1, 4], [5, 4, 3], [5, 3, 2], [5, 2, 1]]
+    assert tri.tolist() == [[a, c, b] for a, b, c in ccw.tolist()]
+    assert pts.shape == (6, 3) and pts[5].tolist() == [np.float32(-2.5e-3), 7.0, 10.0]
+
+
+@needs_bin
+@pytest.mark.parametrize("bad", ["v 1 2\n", "v 1 2 x\n", "v 0 0 0\nf 1 2\n", "v 0 0 0\nf 1 2 3\n",
+                                 "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 0 2\n", "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 -4\n"])
+def test_obj_loader_rejects_bad_records(tmp_path, bad):
+    path = tmp_path / "bad.obj"
+    path.write_text(bad)
+    out = json.loads(subprocess.run([BIN, "obj", str(path)], check=True, capture_output=True, text=True).stdout)
+    assert out == {"error": -1}
+    with pytest.raises(ValueError):
+        scenes.load_obj(bad)
