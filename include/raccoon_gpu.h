@@ -116,6 +116,15 @@ int rgpu_time_extension_ray_cast(rgpu_ctx* ctx, const rgpu_scene* scene, int var
  * (>= 3); spheres: cx, cy, cz, r. */
 int rgpu_bvh_build(const float* points, uint32_t stride, uint32_t npoints, const uint32_t* indices, uint32_t ntris,
                    const float* spheres, uint32_t nspheres, const rgpu_bvh_options* opt, rgpu_bvh** out);
+/* The same build on the GPU of ctx: level by level binned SAH, giving rgpu_bvh_build's tree node
+ * for node (same boxes, children, leaves and primitive order). Bins are limited to 2..32 and
+ * coordinates must be finite. device_ms (may be NULL): time on the device from the primitive
+ * boxes to the last level, uploads and the final renumbering excluded. Replaces the Embree build
+ * of SceneManager::buildBVH (common/scene_manager.hpp:517-577) as rgpu_bvh_build does; the host
+ * build stays for scenes outside those limits. Synchronous. */
+int rgpu_bvh_build_device(rgpu_ctx* ctx, const float* points, uint32_t stride, uint32_t npoints,
+                          const uint32_t* indices, uint32_t ntris, const float* spheres, uint32_t nspheres,
+                          const rgpu_bvh_options* opt, rgpu_bvh** out, double* device_ms);
 int rgpu_bvh_destroy(rgpu_bvh* bvh);
 int rgpu_bvh_get_stats(const rgpu_bvh* bvh, rgpu_bvh_stats* out);
 /* The built tree in depth-first pre-order: per node its box (6 floats: min xyz, max xyz),

@@ -85,6 +85,8 @@ _sig("rgpu_synchronize", C.c_int, vp, vp)
 _sig("rgpu_copy", C.c_int, vp, vp, vp, vp, C.c_size_t)
 _sig("rgpu_time_extension_ray_cast", C.c_int, vp, vp, C.c_int, vp, vp, u32, u32, C.c_int, C.POINTER(C.c_double))
 _sig("rgpu_bvh_build", C.c_int, vp, u32, u32, vp, u32, vp, u32, C.POINTER(rgpu_bvh_options), C.POINTER(vp))
+_sig("rgpu_bvh_build_device", C.c_int, vp, vp, u32, u32, vp, u32, vp, u32, C.POINTER(rgpu_bvh_options),
+     C.POINTER(vp), C.POINTER(C.c_double))
 _sig("rgpu_bvh_destroy", C.c_int, vp)
 _sig("rgpu_bvh_get_stats", C.c_int, vp, C.POINTER(rgpu_bvh_stats))
 _sig("rgpu_bvh_export", C.c_int, vp, vp, vp, vp, vp, vp, vp)
@@ -224,7 +226,9 @@ class DeviceBuffer:
 class BVH:
     """Host-built binned-SAH BVH2 (rgpu_bvh_build). Runs without a GPU."""
 
-    def __init__(self, points, indices, spheres=None, max_leaf=4, bins=16, max_depth=63, traversal_cost=0.0):
+    def __init__(self, points, indices, spheres=None, max_leaf=4, bins=16, max_depth=63, traversal_cost=0.0,
+                 device=None):
+        """device: a Context to build on its GPU (rgpu_bvh_build_device; the same tree), else the host."""
         points = np.ascontiguousarray(points, dtype=np.float32)
         indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
         assert points.ndim == 2 and points.shape[1] in (3, 4)
@@ -234,10 +238,18 @@ class BVH:
         self.nspheres = 0 if spheres is None else spheres.shape[0]
         opt = rgpu_bvh_options(RGPU_ABI_VERSION, max_leaf, bins, max_depth, traversal_cost)
         h = vp()
-        rc = lib.rgpu_bvh_build(ptr(points), points.shape[1], points.shape[0], ptr(indices), self.ntris,
-                                ptr(spheres), self.nspheres, C.byref(opt), C.byref(h))
-        if rc != RGPU_OK:
-            raise RgpuError(rc, "rgpu_bvh_build")
+        self.device_ms = None
+        if device is not None:
+            ms = C.c_double(0.0)
+            device.check(lib.rgpu_bvh_build_device(device.h, ptr(points), points.shape[1], points.shape[0],
+                                                   ptr(indices), self.ntris, ptr(spheres), self.nspheres,
+                                                   C.byref(opt), C.byref(h), C.byref(ms)), "rgpu_bvh_build_device")
+            self.device_ms = ms.value
+        else:
+            rc = lib.rgpu_bvh_build(ptr(points), points.shape[1], points.shape[0], ptr(indices), self.ntris,
+                                    ptr(spheres), self.nspheres, C.byref(opt), C.byref(h))
+            if rc != RGPU_OK:
+                raise RgpuError(rc, "rgpu_bvh_build")
         self.h = h
         st = rgpu_bvh_stats()
         lib.rgpu_bvh_get_stats(h, C.byref(st))
@@ -395,6 +407,7 @@ _sig("rgpu_wavefront_read_state", C.c_int, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("rgpu_wavefront_checkpoint_size", C.c_int, vp, C.POINTER(C.c_uint64))
 _sig("rgpu_wavefront_checkpoint", C.c_int, vp, vp, C.c_uint64)
 _sig("rgpu_wavefront_resume", C.c_int, vp, vp, C.c_uint64)
+EXPORTED += ["rgpu_bvh_build_device"]
 EXPORTED += ["rgpu_wavefront_checkpoint_size", "rgpu_wavefront_checkpoint", "rgpu_wavefront_resume"]
 _sig("rgpu_wavefront_graph_capture", C.c_int, vp, u32)
 _sig("rgpu_wavefront_graph_launch", C.c_int, vp, vp, u32)

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libraccoongpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.cpp", "bvh_build.cpp", "envmap_build.cpp", "traverse.hip", "util.hip", "wavefront.hip", "wavefront_capi.cpp", "guide.hip"]
+SOURCES = ["capi.cpp", "bvh_build.cpp", "bvh_gpu.hip", "envmap_build.cpp", "traverse.hip", "util.hip", "wavefront.hip", "wavefront_capi.cpp", "guide.hip"]
 
 # -ffp-contract=off keeps every float expression unfused so results match the CPU
 # oracle bit for bit; packed fp32 instructions are disabled for correctness.

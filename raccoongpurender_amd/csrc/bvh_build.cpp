@@ -159,7 +159,9 @@ struct Builder {
             const int k = best_axis;
             const float scale = (float)B / (cb.hi[k] - cb.lo[k]);
             const float lo = cb.lo[k];
-            auto it = std::partition(ids.begin() + beg, ids.begin() + end, [&](uint32_t p) {
+            // stable, so that the order of a leaf's primitives is a function of the input alone
+            // (the device builder partitions the same way)
+            auto it = std::stable_partition(ids.begin() + beg, ids.begin() + end, [&](uint32_t p) {
                 int b = (int)((cen[3 * (size_t)p + k] - lo) * scale);
                 b = std::min(std::max(b, 0), (int)B - 1);
                 return (uint32_t)b <= best_bin;
@@ -174,12 +176,12 @@ struct Builder {
                 float e = cb.hi[a] - cb.lo[a];
                 if (e > w) { w = e; k = a; }
             }
+            // sorted by (centroid, id), a total order: both halves come out in one defined order
             mid = beg + n / 2;
-            std::nth_element(ids.begin() + beg, ids.begin() + mid, ids.begin() + end,
-                             [&](uint32_t x, uint32_t y) {
-                                 float cx = cen[3 * (size_t)x + k], cy = cen[3 * (size_t)y + k];
-                                 return cx < cy || (cx == cy && x < y);
-                             });
+            std::sort(ids.begin() + beg, ids.begin() + end, [&](uint32_t x, uint32_t y) {
+                float cx = cen[3 * (size_t)x + k], cy = cen[3 * (size_t)y + k];
+                return cx < cy || (cx == cy && x < y);
+            });
         }
         if (mid == beg || mid == end) mid = beg + n / 2;  // defensive: never an empty child
 
@@ -260,14 +262,18 @@ void build_bvh(const float* points, uint32_t stride, uint32_t npoints,
             if (v >= npoints) throw std::invalid_argument("triangle index out of range");
             bx.grow(points + (size_t)v * stride);
         }
+        for (int k = 0; k < 3; ++k) {  // -0 as +0: box planes do not depend on which corner came first
+            bx.lo[k] += 0.0f;
+            bx.hi[k] += 0.0f;
+        }
         b.pbox[i] = bx;
     }
     for (uint32_t s = 0; s < nspheres; ++s) {
         const float* sp = spheres + 4 * (size_t)s;
         Box bx;
         for (int k = 0; k < 3; ++k) {
-            bx.lo[k] = sp[k] - sp[3];
-            bx.hi[k] = sp[k] + sp[3];
+            bx.lo[k] = (sp[k] - sp[3]) + 0.0f;
+            bx.hi[k] = (sp[k] + sp[3]) + 0.0f;
         }
         b.pbox[ntris + s] = bx;
     }

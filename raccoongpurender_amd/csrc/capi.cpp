@@ -15,6 +15,7 @@
 
 #include "../../include/raccoon_gpu.h"
 #include "bvh_build.h"
+#include "bvh_gpu.h"
 #include "rgpu_traverse.h"
 #include "rgpu_ctx.h"
 #include "rgpu_scene.h"
@@ -212,6 +213,47 @@ int rgpu_bvh_build(const float* points, uint32_t stride, uint32_t npoints, const
     } catch (const std::exception&) {
         return RGPU_ERR_INVALID_ARG;
     }
+    return RGPU_OK;
+}
+
+int rgpu_bvh_build_device(rgpu_ctx* ctx, const float* points, uint32_t stride, uint32_t npoints,
+                          const uint32_t* indices, uint32_t ntris, const float* spheres, uint32_t nspheres,
+                          const rgpu_bvh_options* opt, rgpu_bvh** out, double* device_ms) {
+    if (!ctx || !out) return RGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (ntris && (!points || !indices || stride < 3)) return fail(ctx, RGPU_ERR_INVALID_ARG, "bad triangle arrays");
+    if (nspheres && !spheres) return fail(ctx, RGPU_ERR_INVALID_ARG, "bad sphere array");
+    if ((uint64_t)ntris + nspheres >= (1ull << 28)) return fail(ctx, RGPU_ERR_INVALID_ARG, "too many primitives");
+    BuildOptions bo;
+    if (opt) {
+        if (opt->abi_version != RGPU_ABI_VERSION) return fail(ctx, RGPU_ERR_INVALID_ARG, "ABI version mismatch");
+        if (opt->max_leaf) bo.max_leaf = opt->max_leaf;
+        if (opt->bins) bo.bins = opt->bins;
+        if (opt->max_depth) bo.max_depth = opt->max_depth;
+        if (!(opt->traversal_cost >= 0.0f && opt->traversal_cost <= 64.0f))
+            return fail(ctx, RGPU_ERR_INVALID_ARG, "traversal_cost must be in [0, 64]");
+        if (opt->traversal_cost > 0.0f) bo.traversal_cost = opt->traversal_cost;
+    }
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    rgpu_bvh* b = new (std::nothrow) rgpu_bvh();
+    if (!b) return RGPU_ERR_OUT_OF_MEMORY;
+    BuildTimes t;
+    int rc = RGPU_OK;
+    try {
+        build_bvh_device(points, stride, npoints, indices, ntris, spheres, nspheres, bo, ctx->stream, &b->tree, &t);
+    } catch (const std::bad_alloc&) {
+        rc = fail(ctx, RGPU_ERR_OUT_OF_MEMORY, "device BVH build: out of memory");
+    } catch (const DeviceBuildError& e) {
+        rc = fail(ctx, RGPU_ERR_HIP, e.what());
+    } catch (const std::exception& e) {
+        rc = fail(ctx, RGPU_ERR_INVALID_ARG, e.what());
+    }
+    if (rc != RGPU_OK) {
+        delete b;
+        return rc;
+    }
+    if (device_ms) *device_ms = t.device_ms;
+    *out = b;
     return RGPU_OK;
 }
 

@@ -1,0 +1,88 @@
+"""Device BVH build (rgpu_bvh_build_device) against the host builder (rgpu_bvh_build).
+
+The device build must give the host builder's tree node for node: the same boxes (bit for
+bit), children, leaf ranges and primitive order, hence the same depth, leaf count and (up to
+the order of a double sum) SAH cost, and the same hits. Cases cover the binned path at several
+bin counts, leaf sizes and node costs, spheres after triangles, tiny inputs, the object-median
+path (coincident centroids; a depth limit that forces it) and the argument checks.
+"""
+import numpy as np
+import pytest
+
+from raccoongpurender_amd import _lib, render_scenes as R, scenes
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_tree(a, b):
+    ea, eb = a.export(), b.export()
+    assert (a.nodes, a.leaves, a.depth, a.slots) == (b.nodes, b.leaves, b.depth, b.slots)
+    assert np.array_equal(ea["bounds"].view(np.uint32), eb["bounds"].view(np.uint32))
+    for k in ("left", "right", "prim_beg", "prim_cnt", "prim_ids"):
+        assert np.array_equal(ea[k], eb[k]), k
+    assert abs(a.sah_cost - b.sah_cost) <= 1e-9 * max(1.0, abs(b.sah_cost))
+
+
+def _soup(n, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 3)) * 10.0
+    v = (c[:, None, :] + rng.normal(scale=0.2, size=(n, 3, 3))).reshape(-1, 3).astype(np.float32)
+    return v, np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(bins=2), dict(bins=7), dict(bins=32), dict(max_leaf=1),
+                                dict(max_leaf=8), dict(traversal_cost=2.5)])
+def test_device_build_equals_host_build(ctx, kw):
+    print(f"case device BVH, icosphere level 5, {kw}", flush=True)
+    v, f = scenes.displaced_icosphere(level=5)
+    _same_tree(_lib.BVH(v, f, device=ctx, **kw), _lib.BVH(v, f, **kw))
+
+
+@pytest.mark.parametrize("case", ["soup", "spheres", "tiny1", "tiny2", "tiny5", "coincident", "depth_limit"])
+def test_device_build_special_inputs(ctx, case):
+    print(f"case device BVH {case}", flush=True)
+    sph = None
+    kw = {}
+    if case == "soup":
+        v, f = _soup(5000, 3)
+    elif case == "spheres":
+        sd = R.material_test_scene(16, 16)
+        v, f = sd["points"], sd["indices"]
+        rng = np.random.default_rng(4)
+        sph = np.concatenate([rng.random((300, 3)) * 4 - 2, rng.random((300, 1)) * 0.1 + 0.01], axis=1)
+    elif case.startswith("tiny"):
+        v, f = _soup(int(case[4:]), 5)
+    elif case == "coincident":
+        # identical triangles: no centroid extent on any axis, object median all the way down
+        v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (200, 1))
+        f = np.arange(600, dtype=np.uint32).reshape(-1, 3)
+    else:
+        v, f = _soup(2000, 6)
+        kw = dict(max_depth=6)
+    _same_tree(_lib.BVH(v, f, spheres=sph, device=ctx, **kw), _lib.BVH(v, f, spheres=sph, **kw))
+
+
+def test_device_build_full_size_and_hits(ctx):
+    print("case device BVH, icosphere level 8 (1.3M triangles), hits", flush=True)
+    d = scenes.raycast_benchmark_scene(level=8, n_rays=200000)
+    dev = _lib.BVH(d["points"], d["indices"], device=ctx)
+    host = _lib.BVH(d["points"], d["indices"])
+    _same_tree(dev, host)
+    assert dev.device_ms is not None and dev.device_ms > 0.0
+    rays = d["incoherent"]
+    hits = [_lib.trace(ctx, _lib.Scene(ctx, b), rays, "stackless", block=0) for b in (dev, host)]
+    assert H.compare_hits(hits[0], hits[1]) == dict(prim=0, t=0, uv=0)
+
+
+def test_device_build_refusals(ctx):
+    print("case device BVH refusals", flush=True)
+    v, f = _soup(10, 7)
+    with pytest.raises(_lib.RgpuError):
+        _lib.BVH(v, f, bins=33, device=ctx)
+    bad = v.copy()
+    bad[4, 1] = np.nan
+    with pytest.raises(_lib.RgpuError):
+        _lib.BVH(bad, f, device=ctx)
+    with pytest.raises(_lib.RgpuError):
+        _lib.BVH(v, np.array([[0, 1, 99]], np.uint32), device=ctx)
