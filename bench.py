@@ -385,6 +385,16 @@ def main():
           f"built in {t_bvh:.1f}s", file=sys.stderr, flush=True)
 
     ctx = _lib.Context(0)
+    # the same tree built on the GPU (the second build is timed; the first warms the kernels)
+    _lib.BVH(d["points"], d["indices"], device=ctx)
+    t0 = time.perf_counter()
+    dbvh = _lib.BVH(d["points"], d["indices"], device=ctx)
+    t_dev = time.perf_counter() - t0
+    same_tree = all(np.array_equal(a, b) for a, b in zip(dbvh.export().values(), bvh.export().values()))
+    print(f"[bench] device BVH build {dbvh.device_ms:.1f} ms on the GPU, {t_dev * 1e3:.0f} ms with copies "
+          f"(host {t_bvh * 1e3:.0f} ms), same tree {same_tree}", file=sys.stderr, flush=True)
+    dbvh_ms = dbvh.device_ms
+    del dbvh
     scene = _lib.Scene(ctx, bvh)
     n = len(d["incoherent"])
     rays_inc = ctx.to_device(d["incoherent"])
@@ -455,6 +465,9 @@ def main():
         },
     }
     out["extra"]["primary_mrays"] = round(max(results[(v, "coherent")]["mrays"] for v in variants), 2)
+    out["extra"]["bvh_build"] = {"host_ms": round(t_bvh * 1e3, 1), "device_ms": round(dbvh_ms, 2),
+                                 "device_with_copies_ms": round(t_dev * 1e3, 1), "same_tree": same_tree,
+                                 "triangles": int(len(d["indices"]))}
     if not args.no_render:
         scene.close()
         out["extra"]["render"] = render_bench()

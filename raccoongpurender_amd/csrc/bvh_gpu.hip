@@ -28,6 +28,10 @@ namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBins = 32;
+// Tasks above this many primitives are bounded and binned by the position-parallel kernels (bins
+// in device memory at the task's big-task rank); the others by one wave each (k_small_tasks),
+// which also decides their split.
+constexpr uint32_t kSmallTask = 2048;
 
 struct PBox {
     float lo[3], hi[3];
@@ -119,7 +123,9 @@ __global__ void k_clear_bins(uint32_t* __restrict__ bins, size_t words) {
 }
 
 // The task of every position (kNone: a finished leaf); tasks are sorted by their first position.
-__global__ void k_task_of(const Task* __restrict__ tasks, uint32_t ntasks, uint32_t n, uint32_t* __restrict__ pos_task) {
+// pos_big: the same for the tasks of more than kSmallTask primitives, kNone elsewhere.
+__global__ void k_task_of(const Task* __restrict__ tasks, uint32_t ntasks, uint32_t n, uint32_t* __restrict__ pos_task,
+                          uint32_t* __restrict__ pos_big) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t lo = 0, hi = ntasks;  // first task with beg > i
@@ -128,9 +134,13 @@ __global__ void k_task_of(const Task* __restrict__ tasks, uint32_t ntasks, uint3
         if (tasks[m].beg <= i) lo = m + 1;
         else hi = m;
     }
-    uint32_t t = kNone;
-    if (lo > 0 && i < tasks[lo - 1].end) t = lo - 1;
+    uint32_t t = kNone, big = kNone;
+    if (lo > 0 && i < tasks[lo - 1].end) {
+        t = lo - 1;
+        if (tasks[t].end - tasks[t].beg > kSmallTask) big = t;
+    }
     pos_task[i] = t;
+    pos_big[i] = big;
 }
 
 // Min (kMin) or max of v over the wave64.
@@ -207,7 +217,8 @@ __device__ __forceinline__ uint32_t bin_of(float c, float clo, float scale, uint
 
 __global__ void __launch_bounds__(256) k_bin(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ pos_task,
                                              uint32_t n, const PBox* __restrict__ pb, const float* __restrict__ cen,
-                                             const uint32_t* __restrict__ tb, uint32_t B, uint32_t* __restrict__ bins) {
+                                             const uint32_t* __restrict__ tb, uint32_t B, const uint32_t* __restrict__ slot,
+                                             uint32_t* __restrict__ bins) {
     __shared__ uint32_t lb[3 * kMaxBins * 7];
     const uint32_t first = blockIdx.x * blockDim.x;
     const uint32_t i = first + threadIdx.x;
@@ -226,7 +237,7 @@ __global__ void __launch_bounds__(256) k_bin(const uint32_t* __restrict__ ids, c
     if (t != kNone) {
         const uint32_t p = ids[i];
         const PBox b = pb[p];
-        uint32_t* dst = uniform ? lb : bins + (size_t)t * words;
+        uint32_t* dst = uniform ? lb : bins + (size_t)slot[t] * words;
         for (int k = 0; k < 3; ++k) {
             const float clo = fdec(tb[12 * (size_t)t + 6 + k]), chi = fdec(tb[12 * (size_t)t + 9 + k]);
             const float ext = chi - clo;
@@ -243,7 +254,7 @@ __global__ void __launch_bounds__(256) k_bin(const uint32_t* __restrict__ ids, c
     }
     if (uniform) {
         __syncthreads();
-        uint32_t* g = bins + (size_t)t0 * words;
+        uint32_t* g = bins + (size_t)slot[t0] * words;
         for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) {
             const uint32_t r = w % 7;
             const uint32_t v = lb[w];
@@ -272,21 +283,134 @@ __device__ __forceinline__ uint32_t levels_needed(uint32_t n, uint32_t max_leaf)
     return l;
 }
 
-// One thread per task: the host builder's split decision (Builder::build in bvh_build.cpp).
-__global__ void k_split(const Task* __restrict__ tasks, uint32_t ntasks, const uint32_t* __restrict__ tb,
-                        const uint32_t* __restrict__ bins, Params prm, Decision* __restrict__ dec,
-                        float* __restrict__ node_box, uint32_t* __restrict__ node_leaf, double* __restrict__ node_sah,
-                        uint32_t* __restrict__ node_depth, uint32_t* __restrict__ any_median) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntasks) return;
-    const Task tk = tasks[t];
+// Best SAH split over the bins: the first (axis, bin) in the host builder's loop order whose
+// cost is below every earlier one.
+struct Best {
+    int axis;
+    uint32_t bin;
+    double cost;
+};
+
+// Serial form, one thread (bins in device memory).
+__device__ Best serial_best(const uint32_t* tbins, uint32_t B, const float* clo, const float* chi) {
+    Best best{-1, 0, DBL_MAX};
+    double rA[kMaxBins];
+    uint32_t rN[kMaxBins];
+    for (int k = 0; k < 3; ++k) {
+        const float ext = chi[k] - clo[k];
+        if (!(ext > 0.0f)) continue;
+        // suffix areas and counts, then the prefix on the way up (the host's lA/rA loops)
+        float alo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ahi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        uint32_t cnt = 0;
+        for (int b = (int)B - 1; b >= 0; --b) {
+            const uint32_t* q = tbins + (k * B + (uint32_t)b) * 7;
+            if (q[6]) {
+                for (int a = 0; a < 3; ++a) {
+                    alo[a] = fminf(alo[a], fdec(q[a]));
+                    ahi[a] = fmaxf(ahi[a], fdec(q[3 + a]));
+                }
+            }
+            cnt += q[6];
+            rA[b] = box_area(alo, ahi);
+            rN[b] = cnt;
+        }
+        for (int a = 0; a < 3; ++a) {
+            alo[a] = FLT_MAX;
+            ahi[a] = -FLT_MAX;
+        }
+        cnt = 0;
+        for (uint32_t b = 0; b + 1 < B; ++b) {
+            const uint32_t* q = tbins + (k * B + b) * 7;
+            if (q[6]) {
+                for (int a = 0; a < 3; ++a) {
+                    alo[a] = fminf(alo[a], fdec(q[a]));
+                    ahi[a] = fmaxf(ahi[a], fdec(q[3 + a]));
+                }
+            }
+            cnt += q[6];
+            const double lA = box_area(alo, ahi);
+            if (cnt == 0 || rN[b + 1] == 0) continue;
+            const double c = lA * cnt + rA[b + 1] * rN[b + 1];
+            if (c < best.cost) best = Best{k, b, c};
+        }
+    }
+    return best;
+}
+
+// Wave form (every lane of the wave calls it; bins in LDS): lane b holds bin b of an axis, the
+// prefix and suffix boxes and counts are wave scans (min, max and sums, exact in any order),
+// and the first minimum is a wave reduction over (cost, bin).
+__device__ Best wave_best(const uint32_t* L, uint32_t B, const float* clo, const float* chi, uint32_t lane) {
+    Best best{-1, 0, DBL_MAX};
+    for (int k = 0; k < 3; ++k) {
+        const float ext = chi[k] - clo[k];
+        if (!(ext > 0.0f)) continue;
+        float pl[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ph[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        uint32_t pc = 0;
+        if (lane < B) {
+            const uint32_t* q = L + (k * B + lane) * 7;
+            pc = q[6];
+            if (pc)
+                for (int a = 0; a < 3; ++a) {
+                    pl[a] = fdec(q[a]);
+                    ph[a] = fdec(q[3 + a]);
+                }
+        }
+        float sl[3] = {pl[0], pl[1], pl[2]}, sh[3] = {ph[0], ph[1], ph[2]};
+        uint32_t sc = pc;
+        for (int off = 1; off < 64; off <<= 1) {
+            for (int a = 0; a < 3; ++a) {
+                const float ul = __shfl_up(pl[a], off, 64), uh = __shfl_up(ph[a], off, 64);
+                const float dl = __shfl_down(sl[a], off, 64), dh = __shfl_down(sh[a], off, 64);
+                if (lane >= (uint32_t)off) {
+                    pl[a] = fminf(pl[a], ul);
+                    ph[a] = fmaxf(ph[a], uh);
+                }
+                if (lane + off < 64) {
+                    sl[a] = fminf(sl[a], dl);
+                    sh[a] = fmaxf(sh[a], dh);
+                }
+            }
+            const uint32_t uc = __shfl_up(pc, off, 64), dc = __shfl_down(sc, off, 64);
+            if (lane >= (uint32_t)off) pc += uc;
+            if (lane + off < 64) sc += dc;
+        }
+        // the right side of the split after bin `lane` is the suffix from lane + 1
+        float rl[3], rh[3];
+        for (int a = 0; a < 3; ++a) {
+            rl[a] = __shfl_down(sl[a], 1, 64);
+            rh[a] = __shfl_down(sh[a], 1, 64);
+        }
+        const uint32_t rc = __shfl_down(sc, 1, 64);
+        double c = DBL_MAX;
+        if (lane + 1 < B && pc != 0 && rc != 0) c = box_area(pl, ph) * pc + box_area(rl, rh) * rc;
+        uint32_t b = lane;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double oc = __shfl_xor(c, off, 64);
+            const uint32_t ob = __shfl_xor(b, off, 64);
+            if (oc < c || (oc == c && ob < b)) {
+                c = oc;
+                b = ob;
+            }
+        }
+        if (c < best.cost) best = Best{k, b, c};
+    }
+    return best;
+}
+
+// The host builder's split decision for task t (Builder::build in bvh_build.cpp), from its 12
+// encoded bound words tbw and either its bins (serial scan) or the best split found by a wave.
+__device__ void decide_task(uint32_t t, const Task tk, const uint32_t* tbw, const uint32_t* tbins, const Best* given,
+                            const Params prm, Decision* __restrict__ dec, float* __restrict__ node_box,
+                            uint32_t* __restrict__ node_leaf, double* __restrict__ node_sah,
+                            uint32_t* __restrict__ node_depth, uint32_t* __restrict__ any_median) {
     const uint32_t n = tk.end - tk.beg;
     float lo[3], hi[3], clo[3], chi[3];
     for (int k = 0; k < 3; ++k) {
-        lo[k] = fdec(tb[12 * (size_t)t + k]);
-        hi[k] = fdec(tb[12 * (size_t)t + 3 + k]);
-        clo[k] = fdec(tb[12 * (size_t)t + 6 + k]);
-        chi[k] = fdec(tb[12 * (size_t)t + 9 + k]);
+        lo[k] = fdec(tbw[k]);
+        hi[k] = fdec(tbw[3 + k]);
+        clo[k] = fdec(tbw[6 + k]);
+        chi[k] = fdec(tbw[9 + k]);
         node_box[6 * (size_t)tk.node + k] = lo[k];
         node_box[6 * (size_t)tk.node + 3 + k] = hi[k];
     }
@@ -299,52 +423,10 @@ __global__ void k_split(const Task* __restrict__ tasks, uint32_t ntasks, const u
     double best_cost = DBL_MAX;
     const bool force_median = tk.depth + levels_needed(n, prm.max_leaf) + 1 >= prm.max_depth;
     if (n > 1 && !force_median) {
-        const uint32_t words = 3 * B * 7;
-        const uint32_t* tbins = bins + (size_t)t * words;
-        double rA[kMaxBins];
-        uint32_t rN[kMaxBins];
-        for (int k = 0; k < 3; ++k) {
-            const float ext = chi[k] - clo[k];
-            if (!(ext > 0.0f)) continue;
-            // suffix areas and counts, then the prefix on the way up (the host's lA/rA loops)
-            float alo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ahi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-            uint32_t cnt = 0;
-            for (int b = (int)B - 1; b >= 0; --b) {
-                const uint32_t* q = tbins + (k * B + (uint32_t)b) * 7;
-                if (q[6]) {
-                    for (int a = 0; a < 3; ++a) {
-                        alo[a] = fminf(alo[a], fdec(q[a]));
-                        ahi[a] = fmaxf(ahi[a], fdec(q[3 + a]));
-                    }
-                }
-                cnt += q[6];
-                rA[b] = box_area(alo, ahi);
-                rN[b] = cnt;
-            }
-            for (int a = 0; a < 3; ++a) {
-                alo[a] = FLT_MAX;
-                ahi[a] = -FLT_MAX;
-            }
-            cnt = 0;
-            for (uint32_t b = 0; b + 1 < B; ++b) {
-                const uint32_t* q = tbins + (k * B + b) * 7;
-                if (q[6]) {
-                    for (int a = 0; a < 3; ++a) {
-                        alo[a] = fminf(alo[a], fdec(q[a]));
-                        ahi[a] = fmaxf(ahi[a], fdec(q[3 + a]));
-                    }
-                }
-                cnt += q[6];
-                const double lA = box_area(alo, ahi);
-                if (cnt == 0 || rN[b + 1] == 0) continue;
-                const double c = lA * cnt + rA[b + 1] * rN[b + 1];
-                if (c < best_cost) {
-                    best_cost = c;
-                    best_axis = k;
-                    best_bin = b;
-                }
-            }
-        }
+        const Best bb = given ? *given : serial_best(tbins, B, clo, chi);
+        best_axis = bb.axis;
+        best_bin = bb.bin;
+        best_cost = bb.cost;
     }
     bool leaf = n == 1;
     if (!leaf) {
@@ -379,6 +461,100 @@ __global__ void k_split(const Task* __restrict__ tasks, uint32_t ntasks, const u
         node_sah[tk.node] = parea;
     }
     dec[t] = d;
+}
+
+__global__ void k_split(const Task* __restrict__ tasks, uint32_t ntasks, const uint32_t* __restrict__ tb,
+                        const uint32_t* __restrict__ slot, const uint32_t* __restrict__ bins, Params prm,
+                        Decision* __restrict__ dec,
+                        float* __restrict__ node_box, uint32_t* __restrict__ node_leaf, double* __restrict__ node_sah,
+                        uint32_t* __restrict__ node_depth, uint32_t* __restrict__ any_median) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntasks) return;
+    const Task tk = tasks[t];
+    if (tk.end - tk.beg <= kSmallTask) return;
+    decide_task(t, tk, tb + 12 * (size_t)t, bins + (size_t)slot[t] * 3 * prm.bins * 7, nullptr, prm, dec,
+                node_box, node_leaf, node_sah, node_depth, any_median);
+}
+
+// One wave per small task: bounds (wave reductions), bins (in the wave's LDS), then the split
+// decision by lane 0 from the LDS bins. No atomics outside LDS.
+__global__ void __launch_bounds__(256) k_small_tasks(const Task* __restrict__ tasks, uint32_t ntasks,
+                                                     const uint32_t* __restrict__ ids, const PBox* __restrict__ pb,
+                                                     const float* __restrict__ cen, Params prm,
+                                                     uint32_t* __restrict__ tb, Decision* __restrict__ dec,
+                                                     float* __restrict__ node_box, uint32_t* __restrict__ node_leaf,
+                                                     double* __restrict__ node_sah, uint32_t* __restrict__ node_depth,
+                                                     uint32_t* __restrict__ any_median) {
+    __shared__ uint32_t wl[4][3 * kMaxBins * 7];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (t >= ntasks) return;  // whole waves
+    const Task tk = tasks[t];
+    if (tk.end - tk.beg > kSmallTask) return;
+    uint32_t v[12];
+    for (int k = 0; k < 3; ++k) {
+        v[k] = kEncMax;
+        v[3 + k] = kEncMin;
+        v[6 + k] = kEncMax;
+        v[9 + k] = kEncMin;
+    }
+    for (uint32_t i = tk.beg + lane; i < tk.end; i += 64) {
+        const uint32_t p = ids[i];
+        const PBox b = pb[p];
+        for (int k = 0; k < 3; ++k) {
+            v[k] = min(v[k], fenc(b.lo[k]));
+            v[3 + k] = max(v[3 + k], fenc(b.hi[k]));
+            const uint32_t c = fenc(cen[3 * (size_t)p + k]);
+            v[6 + k] = min(v[6 + k], c);
+            v[9 + k] = max(v[9 + k], c);
+        }
+    }
+    for (int q = 0; q < 12; ++q) {
+        const bool mn = q < 3 || (q >= 6 && q < 9);
+        v[q] = mn ? wave_red<true>(v[q]) : wave_red<false>(v[q]);  // every lane holds the result
+    }
+    if (lane == 0)
+        for (int q = 0; q < 12; ++q) tb[12 * (size_t)t + q] = v[q];
+    const uint32_t B = prm.bins, words = 3 * B * 7;
+    uint32_t* L = wl[w];
+    for (uint32_t j = lane; j < words; j += 64) {
+        const uint32_t r = j % 7;
+        L[j] = r < 3 ? kEncMax : (r < 6 ? kEncMin : 0u);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    float clo[3], scale[3];
+    bool use[3];
+    for (int k = 0; k < 3; ++k) {
+        clo[k] = fdec(v[6 + k]);
+        const float ext = fdec(v[9 + k]) - clo[k];
+        use[k] = ext > 0.0f;
+        scale[k] = use[k] ? (float)B / ext : 0.0f;
+    }
+    for (uint32_t i = tk.beg + lane; i < tk.end; i += 64) {
+        const uint32_t p = ids[i];
+        const PBox b = pb[p];
+        for (int k = 0; k < 3; ++k) {
+            if (!use[k]) continue;
+            const uint32_t bi = bin_of(cen[3 * (size_t)p + k], clo[k], scale[k], B);
+            uint32_t* q = L + (k * B + bi) * 7;
+            for (int a = 0; a < 3; ++a) {
+                atomicMin(q + a, fenc(b.lo[a]));
+                atomicMax(q + 3 + a, fenc(b.hi[a]));
+            }
+            atomicAdd(q + 6, 1u);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    float cl[3], ch[3];
+    for (int k = 0; k < 3; ++k) {
+        cl[k] = fdec(v[6 + k]);
+        ch[k] = fdec(v[9 + k]);
+    }
+    const Best best = wave_best(L, B, cl, ch, lane);
+    if (lane == 0)
+        decide_task(t, tk, v, L, &best, prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
 }
 
 // Left flag of every position of a split (binned) or median task.
@@ -439,6 +615,13 @@ __global__ void k_children(const Task* __restrict__ tasks, uint32_t ntasks, Deci
     node_child[2 * (size_t)tk.node + 1] = l + 1;
     next[2 * r] = Task{tk.beg, mid, l, tk.depth + 1};
     next[2 * r + 1] = Task{mid, tk.end, l + 1, tk.depth + 1};
+}
+
+// Big-task flag, for the bins slots of the position-parallel kernels.
+__global__ void k_big_flags(const Task* __restrict__ tasks, uint32_t ntasks, uint32_t* __restrict__ f) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntasks) return;
+    f[t] = (t < ntasks && tasks[t].end - tasks[t].beg > kSmallTask) ? 1u : 0u;
 }
 
 // Split flag (1 for split and median tasks) for the child numbering.
@@ -568,6 +751,7 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
     uint32_t* ids = m.get<uint32_t>(n);
     uint32_t* ids2 = m.get<uint32_t>(n);
     uint32_t* pos_task = m.get<uint32_t>(n);
+    uint32_t* pos_big = m.get<uint32_t>(n);
     uint32_t* flag = m.get<uint32_t>((size_t)n + 1);
     uint32_t* pre = m.get<uint32_t>((size_t)n + 1);
     uint32_t* scan_tmp = m.get<uint32_t>(scan_temp_words((size_t)n + 1));
@@ -588,9 +772,11 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
     double* node_sah = m.get<double>(max_nodes);
     uint32_t* node_depth = m.get<uint32_t>(max_nodes);
     uint32_t* any_median = m.get<uint32_t>(1);
-    // bins: allocated for the largest level seen so far
-    uint32_t* bins = nullptr;
-    size_t bins_tasks = 0;
+    // bins of the big tasks (more than kSmallTask primitives each, so at most n / kSmallTask)
+    const size_t max_big = (size_t)n / (kSmallTask + 1) + 1;
+    const size_t bwords = max_big * 3 * B * 7;
+    uint32_t* bins = m.get<uint32_t>(bwords);
+    uint32_t* bslot = m.get<uint32_t>((size_t)max_tasks + 1);
 
     hipLaunchKernelGGL(k_prim_boxes, grid_of(n), dim3(256), 0, s, d_pts, stride, d_idx, ntris, d_sph, nspheres, pb,
                        cen, ids);
@@ -600,20 +786,18 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
     Params prm{n, B, opt.max_leaf, opt.max_depth, opt.traversal_cost};
     std::vector<float> hcen;  // host centroids, for a median split (computed on first need)
     while (ntasks) {
-        if (ntasks > bins_tasks) {
-            bins_tasks = std::max<size_t>(ntasks, 2 * bins_tasks);
-            bins_tasks = std::min<size_t>(bins_tasks, max_tasks);
-            bins = m.get<uint32_t>(bins_tasks * 3 * B * 7);
-        }
-        hipLaunchKernelGGL(k_task_of, grid_of(n), dim3(256), 0, s, tasks, ntasks, n, pos_task);
+        hipLaunchKernelGGL(k_task_of, grid_of(n), dim3(256), 0, s, tasks, ntasks, n, pos_task, pos_big);
+        hipLaunchKernelGGL(k_big_flags, grid_of((size_t)ntasks + 1), dim3(256), 0, s, tasks, ntasks, sflag);
+        check(scan_exclusive(sflag, bslot, (size_t)ntasks + 1, stmp, s), "scan");
         hipLaunchKernelGGL(k_clear_bounds, grid_of(12 * (size_t)ntasks), dim3(256), 0, s, tb, ntasks);
-        hipLaunchKernelGGL(k_bounds, grid_of(n), dim3(256), 0, s, ids, pos_task, n, pb, cen, tb);
-        const size_t bw = (size_t)ntasks * 3 * B * 7;
-        hipLaunchKernelGGL(k_clear_bins, grid_of(bw), dim3(256), 0, s, bins, bw);
-        hipLaunchKernelGGL(k_bin, grid_of(n), dim3(256), 0, s, ids, pos_task, n, pb, cen, tb, B, bins);
+        hipLaunchKernelGGL(k_bounds, grid_of(n), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb);
+        hipLaunchKernelGGL(k_clear_bins, grid_of(bwords), dim3(256), 0, s, bins, bwords);
+        hipLaunchKernelGGL(k_bin, grid_of(n), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb, B, bslot, bins);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, any_median, (size_t)1, 0u);
-        hipLaunchKernelGGL(k_split, grid_of(ntasks, 128), dim3(128), 0, s, tasks, ntasks, tb, bins, prm, dec,
+        hipLaunchKernelGGL(k_split, grid_of(ntasks, 128), dim3(128), 0, s, tasks, ntasks, tb, bslot, bins, prm, dec,
                            node_box, node_leaf, node_sah, node_depth, any_median);
+        hipLaunchKernelGGL(k_small_tasks, grid_of(64 * (size_t)ntasks), dim3(256), 0, s, tasks, ntasks, ids, pb, cen,
+                           prm, tb, dec, node_box, node_leaf, node_sah, node_depth, any_median);
         check(hipGetLastError(), "level kernels");
         // median splits (near the depth limit, or no centroid extent): sort the segment on the host
         uint32_t hany = 0;
