@@ -320,6 +320,26 @@ int rgpu_wavefront_stat(rgpu_wavefront* w, uint64_t* total_samples);
 int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, float* paths, void* ext,
                               float* shade, int32_t* vol, uint32_t* items);
 
+/* Intermediate image of a render: per pixel the half-float mean radiance (IEEE binary16 bits,
+ * round to nearest even) and the sample count, widened from the reference's 16 bits to 32
+ * (RGB16IntermediateValueType, kernels/types.cl; accumlation_to_intermediate,
+ * kernels/accumlation.cl:6-25, without the preview mutex). 12 bytes. */
+typedef struct rgpu_rgb16 {
+    uint16_t r, g, b, pad;
+    uint32_t count;
+} rgpu_rgb16;
+/* The lane's intermediate image into dst (device memory, one rgpu_rgb16 per pixel). Single
+ * renders only. */
+int rgpu_wavefront_intermediate(rgpu_wavefront* w, rgpu_stream stream, rgpu_rgb16* dst);
+/* a = count-weighted mean of a and b, counts summed (merge_intermediate,
+ * kernels/accumlation.cl:27-50; the merge of several lanes in
+ * common/wavefront_path_tracing.hpp:962-991). Device buffers of npixels entries. */
+int rgpu_merge_intermediate(rgpu_ctx* ctx, rgpu_stream stream, rgpu_rgb16* a, const rgpu_rgb16* b, uint32_t npixels);
+/* RGBA8 of an intermediate: pow(mean, 1/2.2) * 256 clamped, alpha 255 (tonemap,
+ * kernels/accumlation.cl:68-79). rgba8: device memory, 4 bytes per pixel. */
+int rgpu_tonemap_intermediate(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_rgb16* src, uint32_t npixels,
+                              uint8_t* rgba8);
+
 /* Checkpoint / resume of a single render (no batch, no attached guide): a host snapshot of
  * everything a lane carries from one step to the next (rays, path, extension and shading
  * records, medium lists, RNG states, queues, pixel cursor, accumulation, camera). Resuming it

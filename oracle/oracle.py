@@ -366,3 +366,63 @@ class OracleWavefront:
             lib.orc_wf_destroy(self.h)
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ output path (numpy)
+# Intermediate image: half-float means and a sample count per pixel (RGB16IntermediateValueType,
+# kernels/types.cl), the count widened to 32 bits.
+RGB16_DTYPE = np.dtype([("r", "<u2"), ("g", "<u2"), ("b", "<u2"), ("pad", "<u2"), ("count", "<u4")])
+
+
+def _half_bits(x):
+    return np.asarray(x, np.float32).astype(np.float16).view(np.uint16)
+
+
+def _half_value(bits):
+    return np.asarray(bits, np.uint16).view(np.float16).astype(np.float32)
+
+
+def to_intermediate(rgb_sums, counts):
+    """accumlation_to_intermediate (kernels/accumlation.cl:6-25): div = 1 / count in float, 0 when
+    not finite; each channel sum (as float) times div, rounded to half (nearest even)."""
+    rgb = np.asarray(rgb_sums).astype(np.float32)
+    cnt = np.asarray(counts, np.uint32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        div = np.float32(1.0) / cnt.astype(np.float32)
+    div = np.where(np.isfinite(div), div, np.float32(0.0)).astype(np.float32)
+    out = np.zeros(len(cnt), RGB16_DTYPE)
+    for k, name in enumerate("rgb"):
+        out[name] = _half_bits(rgb[:, k] * div)
+    out["count"] = cnt
+    return out
+
+
+def merge_intermediate(a, b):
+    """merge_intermediate (kernels/accumlation.cl:27-50): (a * sa + b * sb) * (1 / (sa + sb)) in
+    float, rounded to half; zero when both counts are zero."""
+    sa = a["count"].astype(np.uint32)
+    sb = b["count"].astype(np.uint32)
+    sab = (sa + sb).astype(np.uint32)
+    out = np.zeros(len(a), RGB16_DTYPE)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        one = np.float32(1.0) / sab.astype(np.float32)
+    fa, fb = sa.astype(np.float32), sb.astype(np.float32)
+    for name in "rgb":
+        with np.errstate(invalid="ignore"):  # 0 * inf where both counts are zero; replaced below
+            v = (_half_value(a[name]) * fa + _half_value(b[name]) * fb) * one
+        out[name] = np.where(sab == 0, np.uint16(0), _half_bits(v))
+    out["count"] = sab
+    return out
+
+
+def tonemap_intermediate(x):
+    """tonemap (kernels/accumlation.cl:68-79): int(pow(mean, 1/2.2) * 256) clamped to 0..255, alpha
+    255; the power in double, rounded to float, as the device computes it."""
+    out = np.zeros((len(x), 4), np.uint8)
+    e = np.float64(np.float32(1.0) / np.float32(2.2))
+    for k, name in enumerate("rgb"):
+        m = _half_value(x[name]).astype(np.float64)
+        p = np.power(m, e).astype(np.float32) * np.float32(256.0)
+        out[:, k] = np.clip(p.astype(np.int64), 0, 255)
+    out[:, 3] = 255
+    return out

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libraccoongpu.so")
 RGPU_ABI_VERSION = 2
 RGPU_OK = 0
 RGPU_ERR_INVALID_ARG = -1
+RGB16_DTYPE = np.dtype([("r", "<u2"), ("g", "<u2"), ("b", "<u2"), ("pad", "<u2"), ("count", "<u4")])
 TREELET_OFF = 0xFFFFFFFF
 ERRORS = {
     -1: "invalid argument",
@@ -171,6 +172,20 @@ class Context:
         """treelet: stackless top-of-tree records per workgroup in LDS (0 default, TREELET_OFF none)."""
         self.check(lib.rgpu_set_trace_tuning(self.h, waves_per_simd, refill, leaf_batch, treelet),
                    "rgpu_set_trace_tuning")
+
+    def merge_intermediate(self, a, b):
+        """Host intermediates in, their count-weighted merge out (rgpu_merge_intermediate)."""
+        n = len(a)
+        da, db = self.to_device(np.ascontiguousarray(a, RGB16_DTYPE)), self.to_device(np.ascontiguousarray(b, RGB16_DTYPE))
+        self.check(lib.rgpu_merge_intermediate(self.h, None, da.ptr, db.ptr, n), "rgpu_merge_intermediate")
+        return da.download(RGB16_DTYPE, n)
+
+    def tonemap_intermediate(self, x):
+        n = len(x)
+        dx = self.to_device(np.ascontiguousarray(x, RGB16_DTYPE))
+        out = self.alloc(max(n, 1) * 4)
+        self.check(lib.rgpu_tonemap_intermediate(self.h, None, dx.ptr, n, out.ptr), "rgpu_tonemap_intermediate")
+        return out.download(np.uint8, (n, 4))
 
     def copy(self, dst, src, nbytes):
         self.check(lib.rgpu_copy(self.h, None, dst.ptr, src.ptr, nbytes), "rgpu_copy")
@@ -404,10 +419,14 @@ _sig("rgpu_wavefront_read_accumulation", C.c_int, vp, vp, vp)
 _sig("rgpu_wavefront_tonemap", C.c_int, vp, vp)
 _sig("rgpu_wavefront_stat", C.c_int, vp, C.POINTER(C.c_uint64))
 _sig("rgpu_wavefront_read_state", C.c_int, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("rgpu_wavefront_intermediate", C.c_int, vp, vp, vp)
+_sig("rgpu_merge_intermediate", C.c_int, vp, vp, vp, vp, u32)
+_sig("rgpu_tonemap_intermediate", C.c_int, vp, vp, vp, u32, vp)
 _sig("rgpu_wavefront_checkpoint_size", C.c_int, vp, C.POINTER(C.c_uint64))
 _sig("rgpu_wavefront_checkpoint", C.c_int, vp, vp, C.c_uint64)
 _sig("rgpu_wavefront_resume", C.c_int, vp, vp, C.c_uint64)
-EXPORTED += ["rgpu_bvh_build_device"]
+EXPORTED += ["rgpu_bvh_build_device", "rgpu_wavefront_intermediate", "rgpu_merge_intermediate",
+             "rgpu_tonemap_intermediate"]
 EXPORTED += ["rgpu_wavefront_checkpoint_size", "rgpu_wavefront_checkpoint", "rgpu_wavefront_resume"]
 _sig("rgpu_wavefront_graph_capture", C.c_int, vp, u32)
 _sig("rgpu_wavefront_graph_launch", C.c_int, vp, vp, u32)
@@ -656,6 +675,12 @@ class Wavefront:
                                                                 ("rng", "rays", "paths", "ext", "shade", "vol",
                                                                  "items")]), "rgpu_wavefront_read_state")
         return st
+
+    def intermediate(self, dev=None):
+        """The lane's intermediate image (RGB16_DTYPE, one per pixel); into device buffer dev if given."""
+        out = dev if dev is not None else self.ctx.alloc(max(self.npixels, 1) * 12)
+        self.ctx.check(lib.rgpu_wavefront_intermediate(self.h, None, out.ptr), "rgpu_wavefront_intermediate")
+        return out if dev is not None else out.download(RGB16_DTYPE, self.npixels)
 
     def checkpoint(self):
         """Host snapshot of the lane (numpy uint8 array) that resume() continues exactly."""

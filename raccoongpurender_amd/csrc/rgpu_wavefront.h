@@ -185,6 +185,16 @@ hipError_t guide_gradients(const DevGuide& g, uint32_t rows, hipStream_t s);
 hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hipStream_t s);
 hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s);
 hipError_t wf_stat(const DevState& st, uint32_t npixels, bool accum64, unsigned long long* total, hipStream_t s);
+// Intermediate image (kernels/accumlation.cl): half-float means and a 32-bit sample count per
+// pixel, 12 bytes (rgpu_rgb16 in the public header).
+struct RGB16 {
+    uint16_t r, g, b, pad;
+    uint32_t count;
+};
+static_assert(sizeof(RGB16) == 12, "intermediate pixel is 12 bytes");
+hipError_t wf_intermediate(const DevState& st, uint32_t npixels, bool accum64, RGB16* out, hipStream_t s);
+hipError_t merge_intermediate(RGB16* a, const RGB16* b, uint32_t npixels, hipStream_t s);
+hipError_t tonemap_intermediate(const RGB16* in, uint32_t npixels, void* rgba8, hipStream_t s);
 // Flat-batch environment sampling: states (4 words per sample) advance in place; normals (3 floats
 // per sample) select the six-axis tables, null the uniform-weight one; out holds 4 floats per
 // sample (direction, pdf). envmap_pdf evaluates the same densities for given directions.

@@ -1235,6 +1235,77 @@ __global__ void __launch_bounds__(256) k_tonemap(DevState s, uint32_t npixels, u
     out[i] = make_uchar4(c[0], c[1], c[2], 255);
 }
 
+// accumlation_to_intermediate (kernels/accumlation.cl:6-25) without its mutex: per pixel the
+// half-float mean and the sample count, widened to 32 bits.
+__global__ void __launch_bounds__(256) k_to_intermediate(DevState s, uint32_t npixels, uint32_t accum64,
+                                                         RGB16* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npixels) return;
+    float r, g, b;
+    uint32_t cnt;
+    if (accum64) {
+        cnt = s.accum_count[i];
+        r = (float)s.accum64[3 * (size_t)i + 0];
+        g = (float)s.accum64[3 * (size_t)i + 1];
+        b = (float)s.accum64[3 * (size_t)i + 2];
+    } else {
+        const float4 a = reinterpret_cast<const float4*>(s.accum)[i];
+        r = a.x;
+        g = a.y;
+        b = a.z;
+        cnt = __float_as_uint(a.w);
+    }
+    float div = 1.0f / (float)cnt;
+    if (!isfinite(div)) div = 0.0f;
+    RGB16 o;
+    o.r = __half_as_ushort(__float2half_rn(r * div));
+    o.g = __half_as_ushort(__float2half_rn(g * div));
+    o.b = __half_as_ushort(__float2half_rn(b * div));
+    o.pad = 0;
+    o.count = cnt;
+    out[i] = o;
+}
+
+// merge_intermediate (kernels/accumlation.cl:27-50): count-weighted mean of two intermediates
+// into a, counts 32-bit.
+__global__ void __launch_bounds__(256) k_merge_intermediate(RGB16* __restrict__ a, const RGB16* __restrict__ b,
+                                                            uint32_t npixels) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npixels) return;
+    RGB16 x = a[i];
+    const RGB16 y = b[i];
+    const uint32_t sab = x.count + y.count;
+    if (sab == 0) {
+        a[i] = RGB16{0, 0, 0, 0, 0u};
+        return;
+    }
+    const float one_over = 1.0f / (float)sab, sa = (float)x.count, sb = (float)y.count;
+    const uint16_t* xa = &x.r;
+    const uint16_t* ya = &y.r;
+    uint16_t m[3];
+    for (int k = 0; k < 3; ++k) {
+        const float v = (__half2float(__ushort_as_half(xa[k])) * sa + __half2float(__ushort_as_half(ya[k])) * sb) * one_over;
+        m[k] = __half_as_ushort(__float2half_rn(v));
+    }
+    a[i] = RGB16{m[0], m[1], m[2], 0, sab};
+}
+
+// tonemap (kernels/accumlation.cl:68-79) of an intermediate: pow(x, 1/2.2) * 256, clamped.
+__global__ void __launch_bounds__(256) k_tonemap_intermediate(const RGB16* __restrict__ in, uint32_t npixels,
+                                                              uchar4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npixels) return;
+    const RGB16 x = in[i];
+    const float m[3] = {__half2float(__ushort_as_half(x.r)), __half2float(__ushort_as_half(x.g)),
+                        __half2float(__ushort_as_half(x.b))};
+    unsigned char c[3];
+    for (int k = 0; k < 3; ++k) {
+        int v = (int)(fpow(m[k], 1.0f / 2.2f) * 256.0f);
+        c[k] = (unsigned char)min(max(v, 0), 255);
+    }
+    out[i] = make_uchar4(c[0], c[1], c[2], 255);
+}
+
 // stat (kernels/stat.cl): total number of accumulated samples.
 __global__ void __launch_bounds__(256) k_stat(DevState s, uint32_t npixels, uint32_t accum64,
                                               unsigned long long* total) {
@@ -1270,6 +1341,26 @@ hipError_t envmap_pdf(const DevEnvmap& e, const float* dirs, const float* normal
 hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* rgba8, hipStream_t s) {
     dim3 grid((npixels + 255) / 256);
     hipLaunchKernelGGL(k_tonemap, grid, dim3(256), 0, s, st, npixels, accum64 ? 1u : 0u, (uchar4*)rgba8);
+    return hipGetLastError();
+}
+
+hipError_t wf_intermediate(const DevState& st, uint32_t npixels, bool accum64, RGB16* out, hipStream_t s) {
+    if (npixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_intermediate, dim3((npixels + 255) / 256), dim3(256), 0, s, st, npixels, accum64 ? 1u : 0u,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t merge_intermediate(RGB16* a, const RGB16* b, uint32_t npixels, hipStream_t s) {
+    if (npixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_merge_intermediate, dim3((npixels + 255) / 256), dim3(256), 0, s, a, b, npixels);
+    return hipGetLastError();
+}
+
+hipError_t tonemap_intermediate(const RGB16* in, uint32_t npixels, void* rgba8, hipStream_t s) {
+    if (npixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tonemap_intermediate, dim3((npixels + 255) / 256), dim3(256), 0, s, in, npixels,
+                       (uchar4*)rgba8);
     return hipGetLastError();
 }
 

@@ -940,6 +940,39 @@ int rgpu_wavefront_read_state(rgpu_wavefront* w, uint32_t* rng, float* rays, flo
     return RGPU_OK;
 }
 
+// ------------------------------------------------------------------ intermediate images
+int rgpu_wavefront_intermediate(rgpu_wavefront* w, rgpu_stream stream, rgpu_rgb16* dst) {
+    if (!w || !dst) return RGPU_ERR_INVALID_ARG;
+    if (!w->initialized) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_initialize was not called");
+    if (w->is_batch()) return fail(w->ctx, RGPU_ERR_UNSUPPORTED, "a batch has one image per job");
+    if ((uintptr_t)dst & 3) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "dst needs 4-byte alignment");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, wf_intermediate(w->st, w->npixels, w->params.accum64 != 0, reinterpret_cast<RGB16*>(dst),
+                                  stream_of(ctx, stream)));
+    return RGPU_OK;
+}
+
+int rgpu_merge_intermediate(rgpu_ctx* ctx, rgpu_stream stream, rgpu_rgb16* a, const rgpu_rgb16* b, uint32_t npixels) {
+    if (!ctx) return RGPU_ERR_INVALID_ARG;
+    if (npixels && (!a || !b)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null intermediate");
+    if (((uintptr_t)a | (uintptr_t)b) & 3) return fail(ctx, RGPU_ERR_INVALID_ARG, "intermediates need 4-byte alignment");
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, merge_intermediate(reinterpret_cast<RGB16*>(a), reinterpret_cast<const RGB16*>(b), npixels,
+                                     stream_of(ctx, stream)));
+    return RGPU_OK;
+}
+
+int rgpu_tonemap_intermediate(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_rgb16* src, uint32_t npixels,
+                              uint8_t* rgba8) {
+    if (!ctx) return RGPU_ERR_INVALID_ARG;
+    if (npixels && (!src || !rgba8)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null buffer");
+    if (((uintptr_t)src | (uintptr_t)rgba8) & 3) return fail(ctx, RGPU_ERR_INVALID_ARG, "buffers need 4-byte alignment");
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, tonemap_intermediate(reinterpret_cast<const RGB16*>(src), npixels, rgba8, stream_of(ctx, stream)));
+    return RGPU_OK;
+}
+
 // ------------------------------------------------------------------ checkpoint / resume
 namespace {
 struct CheckpointHeader {

@@ -8,7 +8,7 @@ adds them with float atomics in arbitrary order).
 import numpy as np
 import pytest
 
-from raccoongpurender_amd import render_scenes as R
+from raccoongpurender_amd import _lib, render_scenes as R
 from tests import wf_helpers as W
 
 pytestmark = pytest.mark.gpu
@@ -154,3 +154,31 @@ def test_obj_mesh_scene(ctx):
     sd = R.mesh_scene(pts, tri, 40, 30, env_size=(64, 32))
     print("case OBJ mesh scene", flush=True)
     _compare(ctx, sd, 40 * 30 * 2, 10)
+
+
+def test_intermediate_merge_and_tonemap(ctx):
+    # the output path (kernels/accumlation.cl): intermediate, count-weighted merge of two lanes'
+    # images, tonemap; bit-exact against the oracle's restatement
+    import oracle as O
+    sd = R.cornell_box(32, 24, segments=16, rings=8)
+    print("case intermediate / merge / tonemap", flush=True)
+    lanes = []
+    for lane_index in (0, 1):
+        g = W.gpu_lane(ctx, sd, 32 * 24)
+        g.initialize(W.camera(sd["camera"]), lane_index=lane_index, num_lanes=2)
+        g.step(7 + 5 * lane_index)
+        lanes.append(g)
+    ims = []
+    for g in lanes:
+        rgb, cnt = g.accumulation()
+        im = g.intermediate()
+        assert np.array_equal(im, O.to_intermediate(rgb, cnt))
+        assert np.array_equal(ctx.tonemap_intermediate(im), g.tonemap())
+        assert np.abs(O.tonemap_intermediate(im).astype(int) - g.tonemap().astype(int)).max() <= 1
+        ims.append(im)
+    merged = ctx.merge_intermediate(ims[0], ims[1])
+    assert np.array_equal(merged, O.merge_intermediate(ims[0], ims[1]))
+    # pixels with no samples on one or both sides
+    empty = np.zeros(len(ims[0]), _lib.RGB16_DTYPE)
+    assert np.array_equal(ctx.merge_intermediate(ims[0], empty), O.merge_intermediate(ims[0], empty))
+    assert np.array_equal(ctx.merge_intermediate(empty, empty), empty)
