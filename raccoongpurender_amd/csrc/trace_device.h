@@ -142,26 +142,30 @@ struct Cnt {
     uint32_t branch = 0, leaf = 0, parent = 0, prim = 0, node = 0;
 };
 
-// The far children a stackless walk has deferred, newest first, in four registers. The
+// The far children a stackless walk has deferred, newest first, in six registers. The
 // newest deferred child always belongs to the lowest set bit of the bitstack, so a backtrack
-// can jump to it without climbing the parent pointers; once more than four are pending the
+// can jump to it without climbing the parent pointers; once more than six are pending the
 // oldest fall out, and a backtrack that finds the stack empty climbs as the reference does.
 // Only loads are saved: the visit order is the reference's.
 struct ShortStack {
-    uint32_t s0, s1, s2, s3, n;
+    uint32_t s0, s1, s2, s3, s4, s5, n;
     __device__ __forceinline__ void clear() { n = 0; }
     __device__ __forceinline__ void push(uint32_t x) {
+        s5 = s4;
+        s4 = s3;
         s3 = s2;
         s2 = s1;
         s1 = s0;
         s0 = x;
-        n = n < 4 ? n + 1 : 4;
+        n = n < 6 ? n + 1 : 6;
     }
     __device__ __forceinline__ uint32_t pop() {
         const uint32_t x = s0;
         s0 = s1;
         s1 = s2;
         s2 = s3;
+        s3 = s4;
+        s4 = s5;
         --n;
         return x;
     }
