@@ -533,7 +533,8 @@ __device__ __forceinline__ int32_t trace_batched(const TraceScene& sc, F3 ro, F3
         if (__ballot(live) == 0) break;
         const bool pending = walk_pending(st);
         const bool walking = live && !pending;
-        if ((uint32_t)__popcll(__ballot(pending)) >= leaf_batch || __ballot(walking) == 0) {
+        const uint32_t npending = (uint32_t)__popcll(__ballot(pending));
+        if (npending >= leaf_batch || npending >= (uint32_t)__popcll(__ballot(walking))) {
             if (pending) {
                 if constexpr (kVariant == kVariantThreaded)
                     threaded_leaf(st, sc.prims, sc.sphere_begin);

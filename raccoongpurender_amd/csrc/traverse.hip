@@ -108,8 +108,8 @@ __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restri
 // when the wave is idle), so that lanes do not wait for the slowest ray of the wave. No
 // atomics: a wave's cursor is wave-uniform.
 // Leaves are batched: a lane that enters a leaf stops there, and the wave tests primitives only
-// once `leaf_batch` lanes wait at leaves (or none can walk on), so that the primitive tests run
-// with many lanes at once instead of taxing every node visit of the wave. Each ray still
+// once `leaf_batch` lanes wait at leaves or at least as many lanes wait as walk on, so that the
+// primitive tests run with many lanes at once instead of taxing every node visit of the wave. Each ray still
 // visits its nodes and tests its primitives in the order of the one-ray walk.
 // Stackless: the first `top` node records (the top of the tree, numbered breadth-first) are
 // copied to LDS at the start (top * 64 bytes of dynamic shared memory) and read from there.
@@ -158,7 +158,8 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
         const bool pending = walk_pending(st);
         const bool walking = walk_live(st) && !pending;
         const uint32_t npending = (uint32_t)__popcll(__ballot(pending));
-        if (npending >= leaf_batch || __ballot(walking) == 0) {
+        const uint32_t nwalk = (uint32_t)__popcll(__ballot(walking));
+        if (npending >= leaf_batch || npending >= nwalk) {
             if (pending) {
                 if constexpr (kVariant == kVariantThreaded)
                     threaded_leaf(st, sc.prims, sc.sphere_begin);
