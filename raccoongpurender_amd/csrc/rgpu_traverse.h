@@ -50,6 +50,9 @@ struct TraceTuning {
     uint32_t treelet = 0;  // stackless records staged in LDS per workgroup; kTreeletOff: none
 };
 constexpr uint32_t kDefaultLeafBatch = 8;
+// One thread per ray (coherent rays, the wavefront's per-slot trace): leaves wait until as many
+// lanes wait as walk on.
+constexpr uint32_t kPerRayLeafBatch = 64;
 // The scene packer numbers this many top records breadth-first, so that any LDS treelet of up
 // to this size is a prefix of the node array (64 KB of LDS).
 constexpr uint32_t kTreeletMaxNodes = 1024;

@@ -219,8 +219,9 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
     }
     if (block == 0) block = 256;
     dim3 grid((n + block - 1) / block);
-    // one thread per ray: leaves are batched only when asked for (leaf_batch > 1)
-    const uint32_t leaf_batch = tune && tune->leaf_batch ? tune->leaf_batch : 1u;
+    // one thread per ray: leaves batched by default only once as many lanes wait at leaves as
+    // walk on (leaf_batch 64; leaf_batch 1 asks for the plain walk)
+    const uint32_t leaf_batch = tune && tune->leaf_batch ? tune->leaf_batch : kPerRayLeafBatch;
     const int mode = counters ? kWalkCount : (leaf_batch > 1 ? kWalkBatched : kWalkPlain);
     if (sc.nnodes == 0) {
         hipLaunchKernelGGL(k_trace_empty, grid, dim3(block), 0, stream, rays, hits, n);

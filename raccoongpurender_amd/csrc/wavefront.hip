@@ -947,7 +947,7 @@ __global__ void __launch_bounds__(256) k_trace_step(DevState s, WFParams p, J jo
     load_ray_rec(s.ray, slot, &ro, &tmax, &rd, &pixel);
     float t = tmin, u = 0.0f, vv = 0.0f;
     int32_t prim = -1;
-    if (sc.nnodes != 0) prim = trace_batched<kVariant, true>(sc, ro, rd, &t, &u, &vv, kDefaultLeafBatch, top);
+    if (sc.nnodes != 0) prim = trace_batched<kVariant, true>(sc, ro, rd, &t, &u, &vv, kPerRayLeafBatch, top);
     store_extension(s, sc, slot, ro, rd, prim, t, hit_volume, tmin);
 }
 
