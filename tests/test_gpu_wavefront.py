@@ -182,3 +182,8 @@ def test_intermediate_merge_and_tonemap(ctx):
     empty = np.zeros(len(ims[0]), _lib.RGB16_DTYPE)
     assert np.array_equal(ctx.merge_intermediate(ims[0], empty), O.merge_intermediate(ims[0], empty))
     assert np.array_equal(ctx.merge_intermediate(empty, empty), empty)
+    # double-precision accumulation: the sums are rounded to float before the division
+    wide = W.gpu_lane(ctx, sd, 32 * 24, flags=_lib.WF_ACCUM64)
+    wide.step(6)
+    rgb, cnt = wide.accumulation()
+    assert np.array_equal(wide.intermediate(), O.to_intermediate(rgb, cnt))
