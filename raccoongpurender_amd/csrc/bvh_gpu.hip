@@ -340,7 +340,68 @@ __device__ Best serial_best(const uint32_t* tbins, uint32_t B, const float* clo,
 // Wave form (every lane of the wave calls it; bins in LDS): lane b holds bin b of an axis, the
 // prefix and suffix boxes and counts are wave scans (min, max and sums, exact in any order),
 // and the first minimum is a wave reduction over (cost, bin).
+// All three axes at once when 3 W <= 64 lanes (W = B rounded up to a power of two): lane
+// k W + b holds bin b of axis k, the scans run within segments of W lanes, and the lane order
+// is the host's (axis, bin) loop order, so the first minimum is the lowest lane.
+template <uint32_t W>
+__device__ Best wave_best_axes(const uint32_t* L, uint32_t B, const float* clo, const float* chi, uint32_t lane) {
+    const uint32_t k = lane / W, b = lane % W;
+    const bool on = k < 3 && b < B && chi[k < 3 ? k : 0] - clo[k < 3 ? k : 0] > 0.0f;
+    float pl[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ph[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    uint32_t pc = 0;
+    if (on) {
+        const uint32_t* q = L + (k * B + b) * 7;
+        pc = q[6];
+        if (pc)
+            for (int a = 0; a < 3; ++a) {
+                pl[a] = fdec(q[a]);
+                ph[a] = fdec(q[3 + a]);
+            }
+    }
+    float sl[3] = {pl[0], pl[1], pl[2]}, sh[3] = {ph[0], ph[1], ph[2]};
+    uint32_t sc = pc;
+    for (uint32_t off = 1; off < W; off <<= 1) {
+        for (int a = 0; a < 3; ++a) {
+            const float ul = __shfl_up(pl[a], off, W), uh = __shfl_up(ph[a], off, W);
+            const float dl = __shfl_down(sl[a], off, W), dh = __shfl_down(sh[a], off, W);
+            if (b >= off) {
+                pl[a] = fminf(pl[a], ul);
+                ph[a] = fmaxf(ph[a], uh);
+            }
+            if (b + off < W) {
+                sl[a] = fminf(sl[a], dl);
+                sh[a] = fmaxf(sh[a], dh);
+            }
+        }
+        const uint32_t uc = __shfl_up(pc, off, W), dc = __shfl_down(sc, off, W);
+        if (b >= off) pc += uc;
+        if (b + off < W) sc += dc;
+    }
+    float rl[3], rh[3];
+    for (int a = 0; a < 3; ++a) {
+        rl[a] = __shfl_down(sl[a], 1, W);
+        rh[a] = __shfl_down(sh[a], 1, W);
+    }
+    const uint32_t rc = __shfl_down(sc, 1, W);
+    double c = DBL_MAX;
+    if (on && b + 1 < B && pc != 0 && rc != 0) c = box_area(pl, ph) * pc + box_area(rl, rh) * rc;
+    uint32_t idx = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double oc = __shfl_xor(c, off, 64);
+        const uint32_t oi = __shfl_xor(idx, off, 64);
+        if (oc < c || (oc == c && oi < idx)) {
+            c = oc;
+            idx = oi;
+        }
+    }
+    if (!(c < DBL_MAX)) return Best{-1, 0, DBL_MAX};
+    return Best{(int)(idx / W), idx % W, c};
+}
+
 __device__ Best wave_best(const uint32_t* L, uint32_t B, const float* clo, const float* chi, uint32_t lane) {
+    if (B <= 4) return wave_best_axes<4>(L, B, clo, chi, lane);
+    if (B <= 8) return wave_best_axes<8>(L, B, clo, chi, lane);
+    if (B <= 16) return wave_best_axes<16>(L, B, clo, chi, lane);
     Best best{-1, 0, DBL_MAX};
     for (int k = 0; k < 3; ++k) {
         const float ext = chi[k] - clo[k];
