@@ -266,6 +266,17 @@ public:
               nullptr, "rgpu_bvh_build");
         bvh_ = BVHHandle(b);
     }
+    // The same tree built on the GPU of `gpu` (rgpu_bvh_build_device: bins <= 32, finite
+    // coordinates).
+    void buildBVH(GpuContext& gpu, uint32_t max_leaf = 4, uint32_t bins = 16, uint32_t max_depth = 63) {
+        rgpu_bvh_options opt = {RGPU_ABI_VERSION, max_leaf, bins, max_depth};
+        rgpu_bvh* b = nullptr;
+        check(rgpu_bvh_build_device(gpu.get(), points_.data(), 3, (uint32_t)(points_.size() / 3), indices_.data(),
+                                    (uint32_t)(indices_.size() / 3), spheres_.data(),
+                                    (uint32_t)(spheres_.size() / 4), &opt, &b, nullptr),
+              gpu.get(), "rgpu_bvh_build_device");
+        bvh_ = BVHHandle(b);
+    }
     rgpu_bvh_stats bvhStats() const {
         rgpu_bvh_stats st{};
         check(bvh_ ? rgpu_bvh_get_stats(bvh_.get(), &st) : RGPU_ERR_INVALID_ARG, nullptr, "bvhStats (build first)");

@@ -191,6 +191,15 @@ int main(int argc, char** argv) {
                           {std::cos(a) * std::sin(b), std::sin(a), std::cos(a) * std::cos(b)}, 0};
             rays.push_back(r);
         }
+        // the same scene's tree built on the GPU: the host build's node and leaf counts and depth
+        {
+            Scene s2;
+            build(s2);
+            s2.sm.buildBVH(pt.context());
+            const rgpu_bvh_stats d = s2.sm.bvhStats();
+            if (d.nodes != st.nodes || d.leaves != st.leaves || d.depth != st.depth || d.slots != st.slots)
+                throw std::runtime_error("device BVH build differs from the host build");
+        }
         std::vector<rgpu_hit> hs = rt::StacklessBVH(pt.context(), s.sm).intersect(rays);
         std::vector<rgpu_hit> ht = rt::ThreadedBVH(pt.context(), s.sm).intersect(rays);
         dump(dir + "/points.f32", s.points);
