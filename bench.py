@@ -64,9 +64,12 @@ def cpu_baseline(tree, points, indices, rays, budget_s=10.0, cpu_seconds=20.0):
     dt = time.perf_counter() - t0
     n = int(min(len(rays), max(20000, len(probe) / max(dt, 1e-6) * budget_s)))
     sample = rays[:n]
-    reps = max(1, min(8, int(cpu_seconds / threads / max(n / len(probe) * dt, 1e-6) + 0.5)))
     t0 = time.perf_counter()
-    for _ in range(reps):
+    O.intersect_stackless(ref, points, indices, sample, nthreads=threads)
+    first = time.perf_counter() - t0
+    # further passes over the same sample until about cpu_seconds of thread time are spent
+    reps = max(1, min(8, int(cpu_seconds / threads / max(first, 1e-6) + 0.5)))
+    for _ in range(reps - 1):
         O.intersect_stackless(ref, points, indices, sample, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": round(reps * n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
