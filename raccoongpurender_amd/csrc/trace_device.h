@@ -512,8 +512,8 @@ __device__ __forceinline__ bool walk_live(const ThreadedRay& r) { return r.node 
 __device__ __forceinline__ bool walk_pending(const ThreadedRay& r) { return r.leaf != 0; }
 
 // One ray per lane with leaves batched across the wave: a lane that enters a leaf waits until
-// leaf_batch lanes (or every lane still walking) wait too, then the wave tests their primitives
-// together. The lane's own visits and tests are those of trace_stackless / trace_threaded.
+// leaf_batch lanes wait, or at least as many lanes wait as still walk, then the wave tests their
+// primitives together. The lane's own visits and tests are those of trace_stackless / trace_threaded.
 // Lanes whose ray has ended stay in the loop, masked, until the wave is done.
 // kTreelet (stackless): the kernel has staged the first `top` node records in LDS.
 template <int kVariant, bool kTreelet = false>
