@@ -39,7 +39,8 @@ def test_device_build_equals_host_build(ctx, kw):
     _same_tree(_lib.BVH(v, f, device=ctx, **kw), _lib.BVH(v, f, **kw))
 
 
-@pytest.mark.parametrize("case", ["soup", "spheres", "tiny1", "tiny2", "tiny5", "coincident", "depth_limit"])
+@pytest.mark.parametrize("case", ["soup", "spheres", "spheres_only", "tiny1", "tiny2", "tiny5", "coincident",
+                                  "depth_limit"])
 def test_device_build_special_inputs(ctx, case):
     print(f"case device BVH {case}", flush=True)
     sph = None
@@ -51,6 +52,10 @@ def test_device_build_special_inputs(ctx, case):
         v, f = sd["points"], sd["indices"]
         rng = np.random.default_rng(4)
         sph = np.concatenate([rng.random((300, 3)) * 4 - 2, rng.random((300, 1)) * 0.1 + 0.01], axis=1)
+    elif case == "spheres_only":
+        v, f = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)
+        rng = np.random.default_rng(8)
+        sph = np.concatenate([rng.random((3000, 3)) * 10 - 5, rng.random((3000, 1)) * 0.2 + 0.01], axis=1)
     elif case.startswith("tiny"):
         v, f = _soup(int(case[4:]), 5)
     elif case == "coincident":
