@@ -92,3 +92,11 @@ def test_checkpoint_refusals(ctx):
     # a refused snapshot leaves the lane usable once a good one is resumed
     a.resume(blob)
     a.step(1)
+    # a lane with a guide attached carries network state the snapshot does not hold
+    from raccoongpurender_amd import guide as G
+    g = W.gpu_lane(ctx, sd, 1024)
+    G.attach(g, G.Guide(ctx, seed=3))
+    with pytest.raises(_lib.RgpuError):
+        g.checkpoint()
+    with pytest.raises(_lib.RgpuError):
+        g.resume(blob)
