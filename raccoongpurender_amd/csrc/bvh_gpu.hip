@@ -30,8 +30,10 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBins = 32;
 // Tasks above this many primitives are bounded and binned by the position-parallel kernels (bins
 // in device memory at the task's big-task rank); the others by one wave each (k_small_tasks),
-// which also decides their split.
+// which also decides their split, except those of at most kTinyTask primitives, which take one
+// lane each (k_tiny_tasks).
 constexpr uint32_t kSmallTask = 2048;
+constexpr uint32_t kTinyTask = 8;
 
 struct PBox {
     float lo[3], hi[3];
@@ -153,58 +155,89 @@ __device__ __forceinline__ uint32_t wave_red(uint32_t v) {
     return v;
 }
 
+// A workgroup of k_bounds / k_bin covers kSubBlocks runs of 256 positions. A run inside one
+// task is reduced in the workgroup and added to a running partial, which goes to device memory
+// with one atomic per word when the task changes and at the end, so the top levels (few tasks,
+// every workgroup on the same words) take 1/kSubBlocks of the global atomics. A run across task
+// ends, or with positions of small tasks, takes one atomic per primitive and word.
+constexpr uint32_t kSubBlocks = 8;
+
+__device__ __forceinline__ void flush_bounds(const uint32_t* acc, uint32_t t, uint32_t (*part)[12], uint32_t* tb) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int w = 0; w < 12; ++w) {
+        const bool mn = w < 3 || (w >= 6 && w < 9);
+        const uint32_t r = mn ? wave_red<true>(acc[w]) : wave_red<false>(acc[w]);
+        if (lane == 0) part[wave][w] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const uint32_t w = threadIdx.x;
+        const bool mn = w < 3 || (w >= 6 && w < 9);
+        uint32_t r = part[0][w];
+        for (int q = 1; q < 4; ++q) r = mn ? min(r, part[q][w]) : max(r, part[q][w]);
+        if (mn) atomicMin(tb + 12 * (size_t)t + w, r);
+        else atomicMax(tb + 12 * (size_t)t + w, r);
+    }
+    __syncthreads();  // part is written again by the next flush
+}
+
 __global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ pos_task,
                                                 uint32_t n, const PBox* __restrict__ pb, const float* __restrict__ cen,
                                                 uint32_t* __restrict__ tb) {
     __shared__ uint32_t part[4][12];
-    const uint32_t first = blockIdx.x * blockDim.x;
-    const uint32_t i = first + threadIdx.x;
-    const uint32_t last = min(first + blockDim.x, n) - 1;
-    const uint32_t t0 = pos_task[first], t1 = pos_task[last];
-    const bool uniform = t0 != kNone && t0 == t1;  // task segments are contiguous
-    uint32_t v[12];
-    const uint32_t t = i < n ? pos_task[i] : kNone;
-    if (t != kNone) {
-        const uint32_t p = ids[i];
-        const PBox b = pb[p];
-        for (int k = 0; k < 3; ++k) {
-            v[k] = fenc(b.lo[k]);
-            v[3 + k] = fenc(b.hi[k]);
-            const uint32_t c = fenc(cen[3 * (size_t)p + k]);
-            v[6 + k] = c;
-            v[9 + k] = c;
+    uint32_t acc[12];
+    uint32_t acc_t = kNone;  // the task of the running partial (the same in every thread)
+    for (uint32_t j = 0; j < kSubBlocks; ++j) {
+        const uint32_t first = (blockIdx.x * kSubBlocks + j) * blockDim.x;
+        if (first >= n) break;
+        const uint32_t i = first + threadIdx.x;
+        const uint32_t last = min(first + blockDim.x, n) - 1;
+        const uint32_t t0 = pos_task[first], t1 = pos_task[last];
+        const bool uniform = t0 != kNone && t0 == t1;  // task segments are contiguous
+        uint32_t v[12];
+        const uint32_t t = i < n ? pos_task[i] : kNone;
+        if (t != kNone) {
+            const uint32_t p = ids[i];
+            const PBox b = pb[p];
+            for (int k = 0; k < 3; ++k) {
+                v[k] = fenc(b.lo[k]);
+                v[3 + k] = fenc(b.hi[k]);
+                const uint32_t c = fenc(cen[3 * (size_t)p + k]);
+                v[6 + k] = c;
+                v[9 + k] = c;
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                v[k] = kEncMax;
+                v[3 + k] = kEncMin;
+                v[6 + k] = kEncMax;
+                v[9 + k] = kEncMin;
+            }
         }
-    } else {
-        for (int k = 0; k < 3; ++k) {
-            v[k] = kEncMax;
-            v[3 + k] = kEncMin;
-            v[6 + k] = kEncMax;
-            v[9 + k] = kEncMin;
+        if (uniform) {
+            if (t0 != acc_t) {
+                if (acc_t != kNone) flush_bounds(acc, acc_t, part, tb);
+                for (int k = 0; k < 3; ++k) {
+                    acc[k] = kEncMax;
+                    acc[3 + k] = kEncMin;
+                    acc[6 + k] = kEncMax;
+                    acc[9 + k] = kEncMin;
+                }
+                acc_t = t0;
+            }
+            for (int w = 0; w < 12; ++w) {
+                const bool mn = w < 3 || (w >= 6 && w < 9);
+                acc[w] = mn ? min(acc[w], v[w]) : max(acc[w], v[w]);
+            }
+        } else if (t != kNone) {
+            for (int w = 0; w < 12; ++w) {
+                const bool mn = w < 3 || (w >= 6 && w < 9);
+                if (mn) atomicMin(tb + 12 * (size_t)t + w, v[w]);
+                else atomicMax(tb + 12 * (size_t)t + w, v[w]);
+            }
         }
     }
-    if (uniform) {
-        const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int w = 0; w < 12; ++w) {
-            const bool mn = w < 3 || (w >= 6 && w < 9);
-            const uint32_t r = mn ? wave_red<true>(v[w]) : wave_red<false>(v[w]);
-            if (lane == 0) part[wave][w] = r;
-        }
-        __syncthreads();
-        if (threadIdx.x < 12) {
-            const uint32_t w = threadIdx.x;
-            const bool mn = w < 3 || (w >= 6 && w < 9);
-            uint32_t r = part[0][w];
-            for (int q = 1; q < 4; ++q) r = mn ? min(r, part[q][w]) : max(r, part[q][w]);
-            if (mn) atomicMin(tb + 12 * (size_t)t0 + w, r);
-            else atomicMax(tb + 12 * (size_t)t0 + w, r);
-        }
-    } else if (t != kNone) {
-        for (int w = 0; w < 12; ++w) {
-            const bool mn = w < 3 || (w >= 6 && w < 9);
-            if (mn) atomicMin(tb + 12 * (size_t)t + w, v[w]);
-            else atomicMax(tb + 12 * (size_t)t + w, v[w]);
-        }
-    }
+    if (acc_t != kNone) flush_bounds(acc, acc_t, part, tb);
 }
 
 // Bin of centroid coordinate c on an axis whose centroid range starts at clo with
@@ -215,57 +248,74 @@ __device__ __forceinline__ uint32_t bin_of(float c, float clo, float scale, uint
     return (uint32_t)b;
 }
 
+__device__ __forceinline__ void init_bins(uint32_t* lb, uint32_t words) {
+    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) {
+        const uint32_t r = w % 7;
+        lb[w] = r < 3 ? kEncMax : (r < 6 ? kEncMin : 0u);
+    }
+}
+
+__device__ __forceinline__ void flush_bins(const uint32_t* lb, uint32_t* g, uint32_t words) {
+    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) {
+        const uint32_t r = w % 7;
+        const uint32_t v = lb[w];
+        if (r < 3) {
+            if (v != kEncMax) atomicMin(g + w, v);
+        } else if (r < 6) {
+            if (v != kEncMin) atomicMax(g + w, v);
+        } else if (v) {
+            atomicAdd(g + w, v);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_bin(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ pos_task,
                                              uint32_t n, const PBox* __restrict__ pb, const float* __restrict__ cen,
                                              const uint32_t* __restrict__ tb, uint32_t B, const uint32_t* __restrict__ slot,
                                              uint32_t* __restrict__ bins) {
     __shared__ uint32_t lb[3 * kMaxBins * 7];
-    const uint32_t first = blockIdx.x * blockDim.x;
-    const uint32_t i = first + threadIdx.x;
-    const uint32_t last = min(first + blockDim.x, n) - 1;
-    const uint32_t t0 = pos_task[first], t1 = pos_task[last];
-    const bool uniform = t0 != kNone && t0 == t1;
     const uint32_t words = 3 * B * 7;
-    if (uniform) {
-        for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) {
-            const uint32_t r = w % 7;
-            lb[w] = r < 3 ? kEncMax : (r < 6 ? kEncMin : 0u);
-        }
-        __syncthreads();
-    }
-    const uint32_t t = i < n ? pos_task[i] : kNone;
-    if (t != kNone) {
-        const uint32_t p = ids[i];
-        const PBox b = pb[p];
-        uint32_t* dst = uniform ? lb : bins + (size_t)slot[t] * words;
-        for (int k = 0; k < 3; ++k) {
-            const float clo = fdec(tb[12 * (size_t)t + 6 + k]), chi = fdec(tb[12 * (size_t)t + 9 + k]);
-            const float ext = chi - clo;
-            if (!(ext > 0.0f)) continue;
-            const float scale = (float)B / ext;
-            const uint32_t bi = bin_of(cen[3 * (size_t)p + k], clo, scale, B);
-            uint32_t* q = dst + (k * B + bi) * 7;
-            for (int a = 0; a < 3; ++a) {
-                atomicMin(q + a, fenc(b.lo[a]));
-                atomicMax(q + 3 + a, fenc(b.hi[a]));
+    uint32_t lb_t = kNone;  // the task the LDS bins hold (the same in every thread)
+    for (uint32_t j = 0; j < kSubBlocks; ++j) {
+        const uint32_t first = (blockIdx.x * kSubBlocks + j) * blockDim.x;
+        if (first >= n) break;
+        const uint32_t i = first + threadIdx.x;
+        const uint32_t last = min(first + blockDim.x, n) - 1;
+        const uint32_t t0 = pos_task[first], t1 = pos_task[last];
+        const bool uniform = t0 != kNone && t0 == t1;
+        if (uniform && t0 != lb_t) {
+            __syncthreads();  // the LDS atomics of earlier runs have landed
+            if (lb_t != kNone) {
+                flush_bins(lb, bins + (size_t)slot[lb_t] * words, words);
+                __syncthreads();
             }
-            atomicAdd(q + 6, 1u);
+            init_bins(lb, words);
+            __syncthreads();
+            lb_t = t0;
         }
-    }
-    if (uniform) {
-        __syncthreads();
-        uint32_t* g = bins + (size_t)slot[t0] * words;
-        for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) {
-            const uint32_t r = w % 7;
-            const uint32_t v = lb[w];
-            if (r < 3) {
-                if (v != kEncMax) atomicMin(g + w, v);
-            } else if (r < 6) {
-                if (v != kEncMin) atomicMax(g + w, v);
-            } else if (v) {
-                atomicAdd(g + w, v);
+        const uint32_t t = i < n ? pos_task[i] : kNone;
+        if (t != kNone) {
+            const uint32_t p = ids[i];
+            const PBox b = pb[p];
+            uint32_t* dst = uniform ? lb : bins + (size_t)slot[t] * words;
+            for (int k = 0; k < 3; ++k) {
+                const float clo = fdec(tb[12 * (size_t)t + 6 + k]), chi = fdec(tb[12 * (size_t)t + 9 + k]);
+                const float ext = chi - clo;
+                if (!(ext > 0.0f)) continue;
+                const float scale = (float)B / ext;
+                const uint32_t bi = bin_of(cen[3 * (size_t)p + k], clo, scale, B);
+                uint32_t* q = dst + (k * B + bi) * 7;
+                for (int a = 0; a < 3; ++a) {
+                    atomicMin(q + a, fenc(b.lo[a]));
+                    atomicMax(q + 3 + a, fenc(b.hi[a]));
+                }
+                atomicAdd(q + 6, 1u);
             }
         }
+    }
+    if (lb_t != kNone) {
+        __syncthreads();
+        flush_bins(lb, bins + (size_t)slot[lb_t] * words, words);
     }
 }
 
@@ -551,7 +601,7 @@ __global__ void __launch_bounds__(256) k_small_tasks(const Task* __restrict__ ta
     const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (t >= ntasks) return;  // whole waves
     const Task tk = tasks[t];
-    if (tk.end - tk.beg > kSmallTask) return;
+    if (tk.end - tk.beg > kSmallTask || tk.end - tk.beg <= kTinyTask) return;
     uint32_t v[12];
     for (int k = 0; k < 3; ++k) {
         v[k] = kEncMax;
@@ -619,6 +669,110 @@ __global__ void __launch_bounds__(256) k_small_tasks(const Task* __restrict__ ta
 }
 
 // Left flag of every position of a split (binned) or median task.
+// One lane per task of at most kTinyTask primitives. With so few primitives the bin scan comes
+// down to the splits after the occupied bins: an empty bin repeats the cost of the split after
+// the occupied bin below it, and the scan keeps the first minimum in (axis, bin) order, so the
+// candidate for an occupied bin b is the split after b, with the primitives in bins <= b on the
+// left. Boxes are reduced in the encoded form the bins hold, so each cost is the bin scan's to
+// the bit.
+__global__ void __launch_bounds__(256) k_tiny_tasks(const Task* __restrict__ tasks, uint32_t ntasks,
+                                                    const uint32_t* __restrict__ ids, const PBox* __restrict__ pb,
+                                                    const float* __restrict__ cen, Params prm,
+                                                    uint32_t* __restrict__ tb, Decision* __restrict__ dec,
+                                                    float* __restrict__ node_box, uint32_t* __restrict__ node_leaf,
+                                                    double* __restrict__ node_sah, uint32_t* __restrict__ node_depth,
+                                                    uint32_t* __restrict__ any_median) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntasks) return;
+    const Task tk = tasks[t];
+    const uint32_t n = tk.end - tk.beg;
+    if (n > kTinyTask) return;
+    uint32_t lo[kTinyTask][3], hi[kTinyTask][3], v[12];
+    float c[kTinyTask][3];
+    for (int k = 0; k < 3; ++k) {
+        v[k] = kEncMax;
+        v[3 + k] = kEncMin;
+        v[6 + k] = kEncMax;
+        v[9 + k] = kEncMin;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kTinyTask; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            lo[i][k] = kEncMax;
+            hi[i][k] = kEncMin;
+            c[i][k] = 0.0f;
+        }
+        if (i < n) {
+            const uint32_t p = ids[tk.beg + i];
+            const PBox b = pb[p];
+            for (int k = 0; k < 3; ++k) {
+                lo[i][k] = fenc(b.lo[k]);
+                hi[i][k] = fenc(b.hi[k]);
+                c[i][k] = cen[3 * (size_t)p + k];
+                const uint32_t e = fenc(c[i][k]);
+                v[k] = min(v[k], lo[i][k]);
+                v[3 + k] = max(v[3 + k], hi[i][k]);
+                v[6 + k] = min(v[6 + k], e);
+                v[9 + k] = max(v[9 + k], e);
+            }
+        }
+    }
+    for (int q = 0; q < 12; ++q) tb[12 * (size_t)t + q] = v[q];
+    const uint32_t B = prm.bins;
+    Best best{-1, 0, DBL_MAX};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float clo = fdec(v[6 + k]);
+        const float ext = fdec(v[9 + k]) - clo;
+        if (!(ext > 0.0f)) continue;
+        const float scale = (float)B / ext;
+        uint32_t bi[kTinyTask];
+#pragma unroll
+        for (uint32_t i = 0; i < kTinyTask; ++i) bi[i] = i < n ? bin_of(c[i][k], clo, scale, B) : kNone;
+        uint32_t kbin = 0;
+        double kcost = DBL_MAX;
+#pragma unroll
+        for (uint32_t j = 0; j < kTinyTask; ++j) {
+            if (j >= n) continue;
+            const uint32_t s = bi[j];
+            uint32_t L[6] = {kEncMax, kEncMax, kEncMax, kEncMin, kEncMin, kEncMin};
+            uint32_t R[6] = {kEncMax, kEncMax, kEncMax, kEncMin, kEncMin, kEncMin};
+            uint32_t lc = 0, rc = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kTinyTask; ++i) {
+                if (i >= n) continue;
+                const bool left = bi[i] <= s;
+                for (int a = 0; a < 3; ++a) {
+                    if (left) {
+                        L[a] = min(L[a], lo[i][a]);
+                        L[3 + a] = max(L[3 + a], hi[i][a]);
+                    } else {
+                        R[a] = min(R[a], lo[i][a]);
+                        R[3 + a] = max(R[3 + a], hi[i][a]);
+                    }
+                }
+                lc += left ? 1u : 0u;
+                rc += left ? 0u : 1u;
+            }
+            if (rc == 0) continue;  // the highest occupied bin: nothing on the right
+            float ll[3], lh[3], rl[3], rh[3];
+            for (int a = 0; a < 3; ++a) {
+                ll[a] = fdec(L[a]);
+                lh[a] = fdec(L[3 + a]);
+                rl[a] = fdec(R[a]);
+                rh[a] = fdec(R[3 + a]);
+            }
+            const double cost = box_area(ll, lh) * lc + box_area(rl, rh) * rc;
+            if (cost < kcost || (cost == kcost && s < kbin)) {
+                kcost = cost;
+                kbin = s;
+            }
+        }
+        if (kcost < best.cost) best = Best{k, kbin, kcost};
+    }
+    decide_task(t, tk, v, nullptr, &best, prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
+}
+
 __global__ void k_flags(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ pos_task, uint32_t n,
                         const float* __restrict__ cen, const uint32_t* __restrict__ tb, const Decision* __restrict__ dec,
                         uint32_t B, uint32_t* __restrict__ flag) {
@@ -851,14 +1005,17 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
         hipLaunchKernelGGL(k_big_flags, grid_of((size_t)ntasks + 1), dim3(256), 0, s, tasks, ntasks, sflag);
         check(scan_exclusive(sflag, bslot, (size_t)ntasks + 1, stmp, s), "scan");
         hipLaunchKernelGGL(k_clear_bounds, grid_of(12 * (size_t)ntasks), dim3(256), 0, s, tb, ntasks);
-        hipLaunchKernelGGL(k_bounds, grid_of(n), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb);
+        hipLaunchKernelGGL(k_bounds, grid_of(n, 256 * kSubBlocks), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb);
         hipLaunchKernelGGL(k_clear_bins, grid_of(bwords), dim3(256), 0, s, bins, bwords);
-        hipLaunchKernelGGL(k_bin, grid_of(n), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb, B, bslot, bins);
+        hipLaunchKernelGGL(k_bin, grid_of(n, 256 * kSubBlocks), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb, B, bslot,
+                           bins);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, any_median, (size_t)1, 0u);
         hipLaunchKernelGGL(k_split, grid_of(ntasks, 128), dim3(128), 0, s, tasks, ntasks, tb, bslot, bins, prm, dec,
                            node_box, node_leaf, node_sah, node_depth, any_median);
         hipLaunchKernelGGL(k_small_tasks, grid_of(64 * (size_t)ntasks), dim3(256), 0, s, tasks, ntasks, ids, pb, cen,
                            prm, tb, dec, node_box, node_leaf, node_sah, node_depth, any_median);
+        hipLaunchKernelGGL(k_tiny_tasks, grid_of(ntasks), dim3(256), 0, s, tasks, ntasks, ids, pb, cen, prm, tb, dec,
+                           node_box, node_leaf, node_sah, node_depth, any_median);
         check(hipGetLastError(), "level kernels");
         // median splits (near the depth limit, or no centroid extent): sort the segment on the host
         uint32_t hany = 0;
