@@ -344,12 +344,17 @@ int rgpu_tonemap_intermediate(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_rgb1
 /* Checkpoint / resume of a single render (no batch, no attached guide): a host snapshot of
  * everything a lane carries from one step to the next (rays, path, extension and shading
  * records, medium lists, RNG states, queues, pixel cursor, accumulation, camera). Resuming it
- * into a lane of the same path count and accumulation mode (the same lane or a new one over the
- * same scene, materials and envmap) continues the render exactly where the snapshot was taken:
- * the following steps give the slot states the uninterrupted lane would have. The reference has
+ * into a lane of the same path count and settings (the same lane or a new one over the same
+ * scene, materials and envmap) continues the render exactly where the snapshot was taken: the
+ * following steps give the slot states the uninterrupted lane would have. The reference has
  * none (its state lives only in device buffers, common/wavefront_path_tracing.hpp:592-604);
- * re_start there re-initialises. Synchronous. checkpoint_size: bytes the snapshot needs. A
- * failed resume leaves the lane uninitialised. */
+ * re_start there re-initialises. Synchronous. checkpoint_size: bytes the snapshot needs.
+ * The snapshot records fingerprints of the scene, materials and envmap, the lane's settings
+ * (path count, depth limit, roulette start, clamp, BSDF selection, flags) and a fingerprint of
+ * its own bytes. resume refuses (RGPU_ERR_INVALID_ARG) before touching the device when any of
+ * them differs, when the camera does not match the image size, or when a queue item, pixel
+ * index, hit primitive or medium list in the payload is out of range; the lane is then
+ * unchanged. A resume that fails later (a HIP error) leaves the lane uninitialised. */
 int rgpu_wavefront_checkpoint_size(rgpu_wavefront* w, uint64_t* bytes);
 int rgpu_wavefront_checkpoint(rgpu_wavefront* w, void* host, uint64_t bytes);
 int rgpu_wavefront_resume(rgpu_wavefront* w, const void* host, uint64_t bytes);
@@ -436,6 +441,9 @@ typedef struct rgpu_guide_config {
 } rgpu_guide_config;
 
 int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** out);
+/* Lifetime: a guide must outlive the wavefronts it is attached to. rgpu_guide_destroy refuses
+ * (RGPU_ERR_INVALID_ARG, guide unchanged) while any wavefront still has it attached; detach it
+ * first (rgpu_wavefront_attach_guide with g = NULL) or destroy those wavefronts. */
 int rgpu_guide_destroy(rgpu_guide* g);
 int rgpu_guide_param_count(void);
 /* Copy the fp32 weights out (get) and/or in (set; refreshes the bf16 copies). */
@@ -451,7 +459,9 @@ int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows)
  * wavefront has filled; *buffered = samples in it (capped at capacity). Synchronous. */
 int rgpu_guide_read_samples(rgpu_guide* g, float* out, uint32_t rows, uint32_t* buffered);
 /* Attach (or detach with g = NULL) a guide: `sampling` enables guided sampling, `training`
- * records vertices and collects training samples. */
+ * records vertices and collects training samples. Attaching replaces any guide attached before.
+ * The lane keeps its per-slot guiding buffers across a detach and reuses them on the next
+ * attach; paths in flight start recording afresh. Guide and wavefront must share a device. */
 int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, int training);
 int rgpu_wavefront_guide_pending(rgpu_wavefront* w, uint32_t* pending);
 /* If at least min_samples are buffered: one pass of minibatch Adam steps, buffer emptied. */

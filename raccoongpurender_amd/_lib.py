@@ -151,10 +151,11 @@ class Context:
 
     def close(self):
         if self.h:
-            for r in reversed(self._children):
-                o = r()
-                if o is not None:
-                    o.close()
+            alive = [o for o in (r() for r in reversed(self._children)) if o is not None]
+            # wavefronts first: a guide refuses to close while a wavefront has it attached
+            alive.sort(key=lambda o: 0 if isinstance(o, Wavefront) else 1)
+            for o in alive:
+                o.close()
             self._children = []
             lib.rgpu_destroy(self.h)
             self.h = None

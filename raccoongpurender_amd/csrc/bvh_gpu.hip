@@ -907,6 +907,18 @@ hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, size_t n, uint32_t*
 }
 
 // ------------------------------------------------------------------ host driver
+// A pair of events owned for the span of one build.
+struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    Events() = default;
+    Events(const Events&) = delete;
+    Events& operator=(const Events&) = delete;
+    ~Events() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 struct DevBuf {
     std::vector<void*> ptrs;
     ~DevBuf() {
@@ -948,9 +960,10 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
     for (size_t i = 0; i < 4 * (size_t)nspheres; ++i)
         if (!std::isfinite(spheres[i])) throw std::invalid_argument("the device builder takes finite spheres only");
 
-    hipEvent_t e0, e1;
-    check(hipEventCreate(&e0), "hipEventCreate");
-    check(hipEventCreate(&e1), "hipEventCreate");
+    Events ev;  // destroyed on every exit, the throwing ones included
+    check(hipEventCreate(&ev.e[0]), "hipEventCreate");
+    check(hipEventCreate(&ev.e[1]), "hipEventCreate");
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     DevBuf m;
     const uint32_t B = opt.bins;
     float* d_pts = m.get<float>((size_t)npoints * stride);
@@ -1078,8 +1091,6 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
     check(hipStreamSynchronize(s), "sync");
     float ms = 0.0f;
     check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 
     // children always have larger level ids than their parent
     std::vector<uint32_t> size(nnodes, 1), pre_id(nnodes, 0);

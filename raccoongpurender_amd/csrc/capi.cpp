@@ -16,6 +16,7 @@
 #include "../../include/raccoon_gpu.h"
 #include "bvh_build.h"
 #include "bvh_gpu.h"
+#include "fingerprint.h"
 #include "rgpu_traverse.h"
 #include "rgpu_ctx.h"
 #include "rgpu_scene.h"
@@ -363,6 +364,8 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
         for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(b.bmin[k]) && std::isfinite(b.bmax[k]);
     s->dev.boxes_finite = finite ? 1u : 0u;
     s->nprims = ntris + nspheres;
+    s->fingerprint = fingerprint64(pr.data(), pr.size() * sizeof(PrimRec),
+                                   fingerprint64(sl.data(), sl.size() * sizeof(SLNode), ntris));
     if (!bvh->tree.nodes.empty())
         for (int k = 0; k < 3; ++k) {
             s->lo[k] = bvh->tree.nodes[0].bmin[k];

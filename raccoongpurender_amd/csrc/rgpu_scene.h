@@ -15,6 +15,7 @@ struct rgpu_scene {
     void* indices = nullptr;
     void* spheres = nullptr;
     uint32_t nprims = 0;  // triangles + spheres
+    uint64_t fingerprint = 0;  // of the packed node and primitive records (checkpoints compare it)
     uint64_t bytes[4] = {0, 0, 0, 0};
     float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {1.0f, 1.0f, 1.0f};  // root box of the BVH
 };

@@ -7,10 +7,12 @@
 #include <cstring>
 #include <new>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "../../include/raccoon_gpu.h"
 #include "envmap_build.h"
+#include "fingerprint.h"
 #include "rgpu_ctx.h"
 #include "rgpu_scene.h"
 #include "rgpu_wavefront.h"
@@ -23,6 +25,7 @@ struct rgpu_materials {
     DevMaterials dev;
     void* bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t nprims = 0;
+    uint64_t fingerprint = 0;  // of the material records (checkpoints compare it)
 };
 
 struct rgpu_envmap {
@@ -30,6 +33,7 @@ struct rgpu_envmap {
     EnvmapTables host;
     DevEnvmap dev;
     void* bufs[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t fingerprint = 0;  // of the texels (checkpoints compare it)
 };
 
 struct rgpu_wavefront {
@@ -77,7 +81,17 @@ struct rgpu_guide {
     float lr = 1e-3f;
     uint32_t minibatch = 16384;
     float select_p = 0.5f;
+    std::vector<rgpu_wavefront*> users;  // wavefronts it is attached to (destroy refuses while any)
 };
+
+namespace {
+void detach_from_guide(rgpu_wavefront* w) {
+    if (!w->guide_obj) return;
+    auto& u = w->guide_obj->users;
+    u.erase(std::remove(u.begin(), u.end(), w), u.end());
+    w->guide_obj = nullptr;
+}
+}  // namespace
 
 namespace {
 void drop_graph(rgpu_wavefront* w) {
@@ -275,6 +289,11 @@ int rgpu_materials_create(rgpu_ctx* ctx, uint32_t nprims, const int32_t* types, 
     if (!m) return RGPU_ERR_OUT_OF_MEMORY;
     m->ctx = ctx;
     m->nprims = nprims;
+    uint64_t fp = fingerprint64(mat.data(), mat.size() * sizeof(Material), nprims);
+    fp = fingerprint64(L.data(), L.size() * sizeof(Lambertian), fp ^ nl);
+    fp = fingerprint64(W.data(), W.size() * sizeof(Ward), fp ^ nw);
+    fp = fingerprint64(D.data(), D.size() * sizeof(Dielectric), fp ^ nd);
+    m->fingerprint = fingerprint64(V.data(), V.size() * sizeof(Volume), fp ^ nv);
     int rc = upload_vec(ctx, mat.data(), mat.size(), &m->bufs[0]);
     if (rc == RGPU_OK) rc = upload_vec(ctx, L.data(), L.size(), &m->bufs[1]);
     if (rc == RGPU_OK) rc = upload_vec(ctx, W.data(), W.size(), &m->bufs[2]);
@@ -310,6 +329,8 @@ int rgpu_envmap_create(rgpu_ctx* ctx, const float* rgb, uint32_t channels, uint3
     rgpu_envmap* e = new (std::nothrow) rgpu_envmap();
     if (!e) return RGPU_ERR_OUT_OF_MEMORY;
     e->ctx = ctx;
+    e->fingerprint = fingerprint64(rgb, (size_t)width * height * channels * sizeof(float),
+                                   ((uint64_t)width << 32) | height);
     try {
         build_envmap_tables(rgb, channels, width, height, &e->host);
     } catch (const std::exception& ex) {
@@ -482,6 +503,7 @@ int rgpu_wavefront_destroy(rgpu_wavefront* w) {
     (void)hipSetDevice(w->ctx->device);
     (void)hipStreamSynchronize(w->ctx->stream);
     drop_graph(w);
+    detach_from_guide(w);
     for (void* p : w->owned) (void)hipFree(p);
     delete w;
     return RGPU_OK;
@@ -984,8 +1006,19 @@ struct CheckpointHeader {
     Camera cam;
     uint64_t seed;     // the lane's configured seed (provenance; the RNG states are in the payload)
     uint64_t payload;  // bytes after the header
+    // what the payload's indices and records refer to: a resume into anything else is refused
+    uint32_t nprims;
+    uint32_t pad;
+    uint64_t scene_fp, materials_fp, envmap_fp;
+    WFParams params;
+    uint64_t payload_fp;  // fingerprint64 of the header (this field zero) and the payload
 };
-constexpr uint32_t kCheckpointVersion = 1;
+
+uint64_t checkpoint_fingerprint(CheckpointHeader h, const uint8_t* payload) {
+    h.payload_fp = 0;
+    return fingerprint64(payload, h.payload, fingerprint64(&h, sizeof(h)));
+}
+constexpr uint32_t kCheckpointVersion = 2;
 
 struct Section {
     void* dev;
@@ -1022,6 +1055,44 @@ int checkpoint_usable(rgpu_wavefront* w) {
     if (w->guide.enabled) return fail(w->ctx, RGPU_ERR_UNSUPPORTED, "checkpoints do not cover an attached guide");
     return RGPU_OK;
 }
+
+// Host-side range checks of the indices a step reads from a payload before it writes them
+// itself: queue lengths and items (slots), pixel indices (accumulation), hit primitives and
+// medium lists (material records). Returns an empty string when every index is in range.
+std::string payload_index_error(const rgpu_wavefront* w, const uint8_t* p, uint32_t npixels, uint32_t nprims) {
+    const size_t n = w->params.n;
+    const uint8_t* ray = p;
+    const uint8_t* ext = ray + n * sizeof(TraceRay) + n * sizeof(PathRec);
+    const uint8_t* vol = ext + n * sizeof(ExtRec) + n * sizeof(ShadeRec);
+    const uint8_t* items = vol + n * sizeof(VolList) + n * sizeof(uint4) + n;
+    const uint8_t* counts = items + n * kNumQueues * 4;
+    for (size_t i = 0; i < n; ++i) {
+        TraceRay r;  // pad holds the slot's pixel
+        std::memcpy(&r, ray + i * sizeof(TraceRay), sizeof(r));
+        if (r.pad >= npixels && r.pad != kDeadPixel) return "a slot's pixel index is outside the image";
+        ExtRec e;
+        std::memcpy(&e, ext + i * sizeof(ExtRec), sizeof(e));
+        if (e.hit_surface < -1 || e.hit_surface >= (int64_t)nprims || e.hit_volume < -1 ||
+            e.hit_volume >= (int64_t)nprims)
+            return "a slot's hit primitive is outside the scene";
+        VolList v;
+        std::memcpy(&v, vol + i * sizeof(VolList), sizeof(v));
+        if (v.count < 0 || v.count > 7) return "a slot's medium list has a bad length";
+        for (int k = 0; k < v.count; ++k)
+            if (v.volumes[k] < 0 || v.volumes[k] >= (int64_t)nprims) return "a slot's medium list names no primitive";
+    }
+    for (int q = 0; q < kNumQueues; ++q) {
+        uint32_t c;
+        std::memcpy(&c, counts + 4 * q, 4);
+        if (c > n) return "a queue is longer than the lane";
+        for (uint32_t i = 0; i < c; ++i) {
+            uint32_t slot;
+            std::memcpy(&slot, items + ((size_t)q * n + i) * 4, 4);
+            if (slot >= n) return "a queue item names no slot";
+        }
+    }
+    return std::string();
+}
 }  // namespace
 
 int rgpu_wavefront_checkpoint_size(rgpu_wavefront* w, uint64_t* bytes) {
@@ -1053,13 +1124,20 @@ int rgpu_wavefront_checkpoint(rgpu_wavefront* w, void* host, uint64_t bytes) {
     h.cam = w->cam;
     h.seed = w->seed;
     h.payload = need - sizeof(h);
-    std::memcpy(host, &h, sizeof(h));
-    uint8_t* p = static_cast<uint8_t*>(host) + sizeof(h);
+    h.nprims = w->scene->nprims;
+    h.scene_fp = w->scene->fingerprint;
+    h.materials_fp = w->mats->fingerprint;
+    h.envmap_fp = w->env->fingerprint;
+    h.params = w->params;
+    uint8_t* payload = static_cast<uint8_t*>(host) + sizeof(h);
+    uint8_t* p = payload;
     for (const Section& s : checkpoint_sections(w, w->npixels)) {
         RGPU_HIP(ctx, hipMemcpyAsync(p, s.dev, s.bytes, hipMemcpyDeviceToHost, ctx->stream));
         p += s.bytes;
     }
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    h.payload_fp = checkpoint_fingerprint(h, payload);
+    std::memcpy(host, &h, sizeof(h));
     return RGPU_OK;
 }
 
@@ -1074,16 +1152,31 @@ int rgpu_wavefront_resume(rgpu_wavefront* w, const void* host, uint64_t bytes) {
         return fail(w->ctx, RGPU_ERR_INVALID_ARG, "not a checkpoint of this library version");
     if (h.path_count != w->params.n || h.accum64 != (w->params.accum64 ? 1u : 0u))
         return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint of a lane with another path count or accumulation");
-    if (h.npixels == 0 || bytes < sizeof(h) + h.payload ||
-        sizeof(h) + h.payload != checkpoint_bytes(w, h.npixels))
+    if (h.npixels == 0 || (uint64_t)h.cam.width * h.cam.height != h.npixels)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint camera does not match its image size");
+    if (bytes < sizeof(h) + h.payload || sizeof(h) + h.payload != checkpoint_bytes(w, h.npixels))
         return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint truncated or inconsistent");
+    if (h.nprims != w->scene->nprims || h.scene_fp != w->scene->fingerprint)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint of another scene");
+    if (h.materials_fp != w->mats->fingerprint)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint of other materials");
+    if (h.envmap_fp != w->env->fingerprint)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint of another envmap");
+    if (std::memcmp(&h.params, &w->params, sizeof(WFParams)) != 0)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG,
+                    "checkpoint of a lane with other settings (depth, clamp, roulette, flags)");
+    const uint8_t* payload = static_cast<const uint8_t*>(host) + sizeof(h);
+    if (checkpoint_fingerprint(h, payload) != h.payload_fp)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint is corrupted (fingerprint mismatch)");
+    const std::string bad = payload_index_error(w, payload, h.npixels, w->scene->nprims);
+    if (!bad.empty()) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "checkpoint rejected: " + bad);
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(w);  // a captured graph holds the old camera by value
     w->initialized = false;  // until every buffer holds the checkpoint
     if ((rc = ensure_accum(w, h.npixels)) != RGPU_OK) return rc;
-    const uint8_t* p = static_cast<const uint8_t*>(host) + sizeof(h);
+    const uint8_t* p = payload;
     for (const Section& s : checkpoint_sections(w, h.npixels)) {
         RGPU_HIP(ctx, hipMemcpyAsync(s.dev, p, s.bytes, hipMemcpyHostToDevice, ctx->stream));
         p += s.bytes;
@@ -1103,6 +1196,10 @@ static int guide_alloc(rgpu_guide* g, size_t bytes, void** out) {
 
 int rgpu_guide_destroy(rgpu_guide* g) {
     if (!g) return RGPU_ERR_INVALID_ARG;
+    if (!g->users.empty())
+        return fail(g->ctx, RGPU_ERR_INVALID_ARG,
+                    "rgpu_guide_destroy: the guide is attached to " + std::to_string(g->users.size()) +
+                        " wavefront(s); detach it (rgpu_wavefront_attach_guide with NULL) or destroy them first");
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     for (void* p : g->owned) (void)hipFree(p);
@@ -1262,11 +1359,21 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(w);
-    if (!g) {
-        w->guide = DevGuide();
-        w->guide_obj = nullptr;
-        return RGPU_OK;
-    }
+    if (g && g->ctx->device != ctx->device)
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "the guide lives on another device than the wavefront");
+    // Detach first: the lane renders plain and keeps only its per-slot guiding buffers (it owns
+    // them) for a later attach, also when this attach fails below.
+    detach_from_guide(w);
+    DevGuide keep;
+    keep.lobes = w->guide.lobes;
+    keep.rec_geo = w->guide.rec_geo;
+    keep.rec_T = w->guide.rec_T;
+    keep.rec_L = w->guide.rec_L;
+    keep.emit_w = w->guide.emit_w;
+    keep.rec_count = w->guide.rec_count;
+    keep.nslots = w->guide.nslots;
+    w->guide = keep;
+    if (!g) return RGPU_OK;
     DevGuide d = g->dev;
     const size_t n = w->params.n;
     if (!w->guide.lobes) {
@@ -1284,7 +1391,6 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
         d.emit_w = (float4*)p;
         if ((rc = wf_alloc(w, n * 4, &p)) != RGPU_OK) return rc;
         d.rec_count = (uint32_t*)p;
-        RGPU_HIP(ctx, hipMemset(d.rec_count, 0, n * 4));
     } else {
         d.lobes = w->guide.lobes;
         d.rec_geo = w->guide.rec_geo;
@@ -1293,6 +1399,9 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
         d.emit_w = w->guide.emit_w;
         d.rec_count = w->guide.rec_count;
     }
+    // paths in flight start recording afresh: vertices recorded for another guide, or before a
+    // detach, do not describe the rest of the path
+    RGPU_HIP(ctx, hipMemset(d.rec_count, 0, n * 4));
     d.nslots = (uint32_t)n;
     // eight inference workgroups per CU (13 KB of LDS and 111 VGPRs per lane allow 8), each
     // looping over tiles
@@ -1301,6 +1410,7 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
     d.train = training ? 1u : 0u;
     w->guide = d;
     w->guide_obj = g;
+    g->users.push_back(w);
     return RGPU_OK;
 }
 

@@ -90,8 +90,9 @@ class Guide:
         return out
 
     def close(self):
+        """Releases the network. Raises (and keeps the guide) while a wavefront has it attached."""
         if self.h and self.ctx.h:
-            _lib.lib.rgpu_guide_destroy(self.h)
+            self.ctx.check(_lib.lib.rgpu_guide_destroy(self.h), "rgpu_guide_destroy")
         self.h = None
 
     def __del__(self):

@@ -227,13 +227,15 @@ public:
 
     // addPolymesh (scene_manager.hpp:339-370): triangles over `points` (xyz), clockwise as seen
     // from the front, all with material `m`.
+    // Every index is checked before anything is added, so a refused mesh leaves the scene as it
+    // was.
     void addPolymesh(const float* xyz, size_t npoints, const uint32_t* tri, size_t ntris, MaterialRef m) {
+        for (size_t i = 0; i < 3 * ntris; ++i)
+            if (tri[i] >= npoints) throw GpuError(RGPU_ERR_INVALID_ARG, "addPolymesh: index out of range");
         const uint32_t base = (uint32_t)(points_.size() / 3);
         points_.insert(points_.end(), xyz, xyz + 3 * npoints);
-        for (size_t i = 0; i < 3 * ntris; ++i) {
-            if (tri[i] >= npoints) throw GpuError(RGPU_ERR_INVALID_ARG, "addPolymesh: index out of range");
-            indices_.push_back(base + tri[i]);
-        }
+        indices_.reserve(indices_.size() + 3 * ntris);
+        for (size_t i = 0; i < 3 * ntris; ++i) indices_.push_back(base + tri[i]);
         tri_mats_.insert(tri_mats_.end(), ntris, m);
         bvh_.reset();
     }
@@ -319,6 +321,7 @@ public:
     }
 
     size_t triangleCount() const { return indices_.size() / 3; }
+    size_t pointCount() const { return points_.size() / 3; }
     size_t sphereCount() const { return spheres_.size() / 4; }
     const rgpu_bvh* bvh() const { return bvh_.get(); }
 

@@ -124,14 +124,18 @@ int main(int argc, char** argv) {
                 no_device = true;
                 code = e.code;
             }
+            // a refused mesh (its last index is out of range) leaves the scene as it was
             bool bad_index = false;
+            rt::SceneManager sm;
+            const float good_p[9] = {0, 0, 0, 1, 0, 0, 0, 1, 0};
+            const uint32_t good_t[3] = {0, 1, 2};
+            sm.addPolymesh(good_p, 3, good_t, 1, rt::MaterialRef{});
             try {
-                const float p[3] = {0, 0, 0};
-                const uint32_t t[3] = {0, 1, 5};
-                rt::SceneManager sm;
-                sm.addPolymesh(p, 1, t, 1, rt::MaterialRef{});
+                const float p[9] = {0, 0, 0, 1, 0, 0, 0, 1, 0};
+                const uint32_t t[6] = {0, 1, 2, 0, 1, 5};
+                sm.addPolymesh(p, 3, t, 2, rt::MaterialRef{});
             } catch (const rt::GpuError& e) {
-                bad_index = e.code == RGPU_ERR_INVALID_ARG;
+                bad_index = e.code == RGPU_ERR_INVALID_ARG && sm.triangleCount() == 1 && sm.pointCount() == 3;
             }
             std::printf("{\"triangles\": %zu, \"spheres\": %zu, \"nodes\": %u, \"leaves\": %u, \"slots\": %u, "
                         "\"no_device\": %s, \"code\": %d, \"bad_index_rejected\": %s}\n",

@@ -269,3 +269,38 @@ def test_trained_guide_lowers_error_on_the_cornell_box(ctx):
     print(f"{passes} training passes; relMSE plain {err[False]:.3g} guided {err[True]:.3g} "
           f"({err[False] / err[True]:.1f}x lower)", flush=True)
     assert passes >= 200 and err[True] * 4.0 < err[False]
+
+
+def test_guide_lifetime_and_detach_reattach(ctx):
+    # A guide refuses to be destroyed while a wavefront holds it; a detached lane renders plain,
+    # keeps its per-slot guiding buffers and starts recording afresh on the next attach.
+    from raccoongpurender_amd import _lib
+
+    sd = R.material_test_scene(32, 24)
+    lo, hi = sd["points"].min(0) - 1e-3, sd["points"].max(0) + 1e-3
+    wf = build_lane(ctx, sd, 32 * 24 * 8, clamp=1e30, seed=4)
+    a = G.Guide(ctx, lo, hi, capacity=1 << 15, minibatch=4096, seed=3)
+    b = G.Guide(ctx, lo, hi, capacity=1 << 15, minibatch=4096, seed=5)
+    G.attach(wf, a, sampling=True, training=True)
+    wf.step(6)
+    with pytest.raises(_lib.RgpuError) as e:
+        a.close()
+    assert "attached" in str(e.value) and a.h is not None
+    G.attach(wf, None)  # detach: renders plain, collects nothing
+    n_a = len(a.buffered_samples())
+    wf.step(4)
+    assert len(a.buffered_samples()) == n_a
+    a.close()  # no wavefront holds it now
+    G.attach(wf, b, sampling=True, training=True)  # re-attach reuses the lane's buffers
+    wf.step(8)
+    s = b.buffered_samples()
+    print(f"{n_a} samples before the detach, {len(s)} after re-attaching", flush=True)
+    assert n_a > 100 and len(s) > 100
+    assert (s["p"] >= 0).all() and (s["p"] <= 1).all()
+    assert np.isfinite(s["weight"]).all() and (s["weight"] > 0).all()
+    # attaching another guide moves the lane: the old one can then be destroyed
+    c = G.Guide(ctx, lo, hi, capacity=1 << 15, minibatch=4096, seed=6)
+    G.attach(wf, c, sampling=True, training=False)
+    b.close()
+    wf.close()  # destroying the lane releases its hold
+    c.close()
