@@ -413,6 +413,14 @@ int rgpu_wavefront_job_queue_counts(rgpu_wavefront* w, uint32_t job, uint32_t* c
 int rgpu_wavefront_job_accumulation(rgpu_wavefront* w, uint32_t job, float* rgb_sums, uint32_t* counts);
 int rgpu_wavefront_job_stat(rgpu_wavefront* w, uint32_t job, uint64_t* total_samples);
 int rgpu_wavefront_job_tonemap(rgpu_wavefront* w, uint32_t job, uint8_t* rgba8);
+/* rgpu_wavefront_job_tonemap without any device allocation, for callers that cannot allocate
+ * (serving loops that preview many jobs, harnesses that forbid hipMalloc). No counterpart in
+ * the reference. It stages the image in a static device array the library holds per device
+ * until the process ends (4 MiB), so a job may have at most 1024 x 1024 pixels; larger jobs
+ * return RGPU_ERR_INVALID_ARG. Output equals rgpu_wavefront_job_tonemap's bit for bit.
+ * Synchronous. Every call on one device shares that staging: calls that overlap on one device
+ * (from two threads or two contexts) race. */
+int rgpu_wavefront_job_tonemap_noalloc(rgpu_wavefront* w, uint32_t job, uint8_t* rgba8);
 /* Steps (check_every at a time) until every job has finished its samples or max_steps have
  * run; every job needs spp > 0. With a training guide attached, a training pass runs after
  * every check_every steps once `capacity` samples are buffered. */

@@ -450,10 +450,11 @@ _sig("rgpu_wavefront_job_queue_counts", C.c_int, vp, u32, vp, C.POINTER(C.c_uint
 _sig("rgpu_wavefront_job_accumulation", C.c_int, vp, u32, vp, vp)
 _sig("rgpu_wavefront_job_stat", C.c_int, vp, u32, C.POINTER(C.c_uint64))
 _sig("rgpu_wavefront_job_tonemap", C.c_int, vp, u32, vp)
+_sig("rgpu_wavefront_job_tonemap_noalloc", C.c_int, vp, u32, vp)
 _sig("rgpu_wavefront_run_batch", C.c_int, vp, u32, u32, C.POINTER(u32))
 EXPORTED += ["rgpu_wavefront_create_batch", "rgpu_wavefront_job_count", "rgpu_wavefront_job_queue_counts",
              "rgpu_wavefront_job_accumulation", "rgpu_wavefront_job_stat", "rgpu_wavefront_job_tonemap",
-             "rgpu_wavefront_run_batch"]
+             "rgpu_wavefront_run_batch", "rgpu_wavefront_job_tonemap_noalloc"]
 EXPORTED += [
     "rgpu_materials_create", "rgpu_materials_destroy", "rgpu_envmap_create", "rgpu_envmap_destroy",
     "rgpu_envmap_export", "rgpu_envmap_sample", "rgpu_envmap_pdf", "rgpu_camera_look_at", "rgpu_wavefront_create", "rgpu_wavefront_destroy",
@@ -787,6 +788,14 @@ class Batch(Wavefront):
     def job_tonemap(self, job):
         out = np.zeros((self.jobs[job].npixels, 4), np.uint8)
         self.ctx.check(lib.rgpu_wavefront_job_tonemap(self.h, job, ptr(out)), "rgpu_wavefront_job_tonemap")
+        return out
+
+    def job_tonemap_noalloc(self, job):
+        """The same image through the entry point that allocates no device memory (jobs of at
+        most 1024 x 1024 pixels)."""
+        out = np.zeros((self.jobs[job].npixels, 4), np.uint8)
+        self.ctx.check(lib.rgpu_wavefront_job_tonemap_noalloc(self.h, job, ptr(out)),
+                       "rgpu_wavefront_job_tonemap_noalloc")
         return out
 
     def job_slots(self, job):

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -707,6 +708,61 @@ int rgpu_wavefront_job_tonemap(rgpu_wavefront* w, uint32_t job, uint8_t* rgba8) 
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(ctx, e, "rgpu_wavefront_job_tonemap");
+    return RGPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ tonemap without allocation
+// Staging for rgpu_wavefront_job_tonemap_noalloc: the RGBA8 image of one job of at most
+// kNoAllocMaxPixels pixels (4 MiB). Every device the library runs on holds its own copy until
+// the process ends. Host code never passes `scratch` itself to a kernel or a copy (on the host
+// the name is only a stand-in); it uses the per-device address that hipGetSymbolAddress gives.
+constexpr uint32_t kNoAllocMaxPixels = 1024u * 1024u;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[(size_t)kNoAllocMaxPixels * 4];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current device. Looked up on
+// the first call for that device and kept. Allocates nothing.
+hipError_t scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
+
+extern "C" {
+
+int rgpu_wavefront_job_tonemap_noalloc(rgpu_wavefront* w, uint32_t job, uint8_t* rgba8) {
+    if (!w || job >= w->jobs.size() || !rgba8) return RGPU_ERR_INVALID_ARG;
+    rgpu_ctx* ctx = w->ctx;
+    const uint32_t P = w->jobs[job].npixels;
+    if (P > kNoAllocMaxPixels)
+        return fail(ctx, RGPU_ERR_INVALID_ARG,
+                    "rgpu_wavefront_job_tonemap_noalloc: the job has " + std::to_string(P) +
+                        " pixels; the static staging holds at most 1048576 (1024x1024)");
+    if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices)
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_job_tonemap_noalloc: device index above 63");
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    void* buf = nullptr;
+    RGPU_HIP(ctx, scratch_address(ctx->device, &buf));
+    DevState js = w->st;
+    js.accum = w->jobs[job].accum;
+    // every pixel of the staging is written before it is read back: nothing to clear
+    RGPU_HIP(ctx, wf_tonemap(js, P, false, buf, ctx->stream));
+    RGPU_HIP(ctx, hipMemcpyAsync(rgba8, buf, (size_t)P * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RGPU_OK;
 }
 
