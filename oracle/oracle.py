@@ -284,6 +284,12 @@ class OracleEnvmap:
         lib.orc_envmap_sample_each(self.h, p(st), n, p(nrm), p(dirs), p(pdfs))
         return dirs, pdfs, st
 
+    def texel_index(self, dirs):
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(len(dirs), np.int32)
+        lib.orc_envmap_index(self.h, len(dirs), p(dirs), p(out))
+        return out
+
     def pdf_each(self, dirs, normals=None):
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(len(dirs), 3)
@@ -296,6 +302,69 @@ class OracleEnvmap:
             lib.orc_envmap_destroy(self.h)
         except Exception:
             pass
+
+
+_sig("orc_ward_sample", None, vp, u32, vp, vp, vp, vp)
+_sig("orc_ward_eval", None, vp, u32, vp, vp, vp, vp, vp)
+_sig("orc_camera_rays", None, vp, u32, u32, C.c_float, u32, vp, vp, vp, vp)
+_sig("orc_free_flight", None, vp, u32, u32, vp, vp, vp, vp)
+_sig("orc_envmap_index", None, vp, u32, vp, vp)
+
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a if cols is None else a.reshape(-1, cols)
+
+
+def _ward9(ax, ay, refl=(1.0, 1.0, 1.0), edge=None, falloff=1.0):
+    return np.array([ax, ay, *refl, *(refl if edge is None else edge), falloff], np.float32)
+
+
+def ward_sample(ward, wo, n, u):
+    """The lane's Ward half-vector sampler (kernels/ward.cl:40-74 + the anisotropic extension)
+    for rows of wo, normal and (u0, u1)."""
+    wo, n, u = _f32(wo, 3), _f32(n, 3), _f32(u, 2)
+    wi = np.zeros((len(wo), 3), np.float32)
+    lib.orc_ward_sample(p(_f32(ward)), len(wo), p(wo), p(n), p(u), p(wi))
+    return wi
+
+
+def ward_eval(ward, wo, wi, n):
+    """(pdf, f): the lane's Ward pdf (kernels/ward.cl:155-176) and BRDF (ward_stage)."""
+    wo, wi, n = _f32(wo, 3), _f32(wi, 3), _f32(n, 3)
+    pdf = np.zeros(len(wo), np.float32)
+    f = np.zeros((len(wo), 3), np.float32)
+    lib.orc_ward_eval(p(_f32(ward)), len(wo), p(wo), p(wi), p(n), p(pdf), p(f))
+    return pdf, f
+
+
+def camera_rays(camera, pixels, states):
+    """new_path's primary rays (kernels/new_path.cl:44-71) for pixel indices, one state row each
+    (advanced in place): (ro, rd)."""
+    cam = camera.as_array()
+    pixels = np.ascontiguousarray(pixels, np.uint32)
+    assert states.dtype == np.uint32 and states.flags["C_CONTIGUOUS"] and states.shape == (len(pixels), 4)
+    ro = np.zeros((len(pixels), 3), np.float32)
+    rd = np.zeros((len(pixels), 3), np.float32)
+    lib.orc_camera_rays(p(cam), camera.width, camera.height, np.float32(camera.lens_radius), len(pixels), p(pixels),
+                        p(states), p(ro), p(rd))
+    return ro, rd
+
+
+def free_flight(C_by_volume, lists, counts, states):
+    """homogeneous_volume_through (kernels/homogeneous_volume_through.cl:24-40) per row: medium
+    ids (up to 7, indexing C_by_volume) and their count; states advanced in place.
+    Returns (chosen medium or -1, distance)."""
+    Cv = _f32(C_by_volume)
+    n = len(counts)
+    l8 = np.zeros((n, 8), np.int32)
+    l8[:, :7] = np.asarray(lists, np.int32).reshape(n, 7)
+    l8[:, 7] = counts
+    assert states.dtype == np.uint32 and states.flags["C_CONTIGUOUS"] and states.shape == (n, 4)
+    hv = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    lib.orc_free_flight(p(Cv), len(Cv), n, p(l8), p(states), p(hv), p(t))
+    return hv, t
 
 
 class OracleWavefront:

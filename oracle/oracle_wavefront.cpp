@@ -428,6 +428,21 @@ float pdf_ward(const Ward& w, f3 wo, f3 wi, f3 z) {
     float k1 = expf_(ward_exp(w, h, z));
     return k0 * k1;
 }
+// Ward BRDF value f(wo, wi) of ward_stage (kernels/ward.cl:155-176): edge-tinted rho over
+// pi*alpha^2 times the exponential and |wi + wo|^2 / ((wi + wo).n)^4 (Geisler-Moroder form).
+f3 ward_f(const Ward& w, f3 wo, f3 wi, f3 Ng) {
+    f3 lv = wi + wo;
+    f3 h = normalize(lv);
+    float pi = kPiF;
+    float a2 = w.ax == w.ay ? sqr(w.ax) : w.ax * w.ay;
+    float mu = dot(h, wi);
+    float t = powf_(1.0f - mu, 1.0f / w.falloff);
+    f3 rho = F(mix(w.refl[0], w.edge[0], t), mix(w.refl[1], w.edge[1], t), mix(w.refl[2], w.edge[2], t));
+    f3 k0 = rho / (pi * a2);
+    float k1 = expf_(ward_exp(w, h, Ng));
+    float k2 = dot(lv, lv) / sqr(sqr(dot(lv, Ng)));
+    return F(k0.x * k1 * k2, k0.y * k1 * k2, k0.z * k1 * k2);
+}
 
 // ------------------------------------------------------------------ the lane
 struct Cfg {
@@ -472,6 +487,71 @@ struct Lane {
     std::vector<uint32_t> accum_count;
 };
 
+// The primary ray of new_path (kernels/new_path.cl:44-71): the pixel's image-plane point
+// jittered by two draws, the origin on the thin lens by two more (radius sqrt(u0), angle
+// 2 pi u1).
+void camera_ray(const Camera& cam, uint32_t pixel, Rng& st, f3* ro, f3* rd) {
+    uint32_t x = pixel % cam.w, y = pixel / cam.w;
+    float jx = (float)x + uniform(st);
+    float jy = (float)y + uniform(st);
+    f3 on_plane = v3(cam.o) + v3(cam.r) * jx + v3(cam.b) * jy;
+    float u0 = uniform(st);
+    float u1 = uniform(st);
+    float dof = cam.lens;
+    float r = std::sqrt(u0);
+    float theta = (float)((double)u1 * kPiD * 2.0);
+    float lensx = dof * r * cosf_(theta);
+    float lensy = dof * r * sinf_(theta);
+    f3 lens = v3(cam.eye) + v3(cam.up) * lensx + v3(cam.right) * lensy;
+    *ro = lens;
+    *rd = normalize(on_plane - lens);
+}
+
+// homogeneous_volume_through for one slot (kernels/homogeneous_volume_through.cl:24-40): a
+// uniform choice among the media the path is inside (a 62-bit integer draw modulo their count),
+// then a free-flight distance -ln(u) / C of that medium; no medium: no choice, far clip FLT_MAX.
+void free_flight(const std::vector<Material>& mats, const std::vector<Volume>& vols, const InVolumeList& v,
+                 Rng& st, int* hv_out, float* tmin_out) {
+    int hv = -1;
+    if (0 < v.count) {
+        uint64_t u = uniform_integer(st);
+        hv = v.volumes[u % (uint64_t)v.count];
+    }
+    float tmin = FLT_MAX;
+    if (0 <= hv) {
+        const Volume& V = vols[mats[hv].index];
+        float u = uniform(st);
+        tmin = -1.0f / V.C * logf_(u);
+    }
+    *hv_out = hv;
+    *tmin_out = tmin;
+}
+
+// sample_envmap_6axis_stage for one sample (kernels/envmap_sampling.cl:169-217): a table by the
+// squared components of n, a texel by its alias table, a direction in the texel's fragment; the
+// pdf is the n^2-weighted mixture of the three tables at that texel.
+f3 sample_envmap_6axis(const Envmap& env, f3 n, Rng& st, float* pdf_out) {
+    const uint32_t count_env = env.w * env.h;
+    int xt = 0.0f < n.x ? 1 : 2, yt = 0.0f < n.y ? 3 : 4, zt = 0.0f < n.z ? 5 : 6;
+    float u = uniform(st);
+    f3 ap = n * n;
+    int sel = u < ap.x ? xt : (u < ap.x + ap.y ? yt : zt);
+    uint32_t frag = alias_method(st, env.alias[sel].data(), count_env);
+    float pdf = 0.0f;
+    pdf += ap.x * env.pdf[xt][frag];
+    pdf += ap.y * env.pdf[yt][frag];
+    pdf += ap.z * env.pdf[zt][frag];
+    *pdf_out = pdf;
+    return sample_from_fragment(env.frags[frag], st);
+}
+
+// sample_envmap_stage for one sample (kernels/envmap_sampling.cl:219-248): the uniform-weight table.
+f3 sample_envmap_uniform(const Envmap& env, Rng& st, float* pdf_out) {
+    uint32_t frag = alias_method(st, env.alias[0].data(), env.w * env.h);
+    *pdf_out = env.pdf[0][frag];
+    return sample_from_fragment(env.frags[frag], st);
+}
+
 void strategy_selection(Lane& L) {
     for (int k = 0; k < 4; ++k) L.sq[k].clear();
     for (uint32_t item = 0; item < L.cfg.n; ++item) {
@@ -514,7 +594,6 @@ float incident_pdf(const IncidentSample& s) {
 void step(Lane& L) {
     const Cfg& c = L.cfg;
     const uint32_t P = L.cam.w * L.cam.h;
-    const uint32_t count_env = L.env.w * L.env.h;
     // new_path (kernels/new_path.cl:17-83)
     for (uint32_t gid = 0; gid < L.q[0].size(); ++gid) {
         uint32_t pi = L.q[0][gid];
@@ -526,20 +605,7 @@ void step(Lane& L) {
         uint64_t gpi = L.next_pixel + gid;
         uint32_t pixel = (uint32_t)(gpi % (uint64_t)P);
         p.pixel_index = pixel;
-        uint32_t x = pixel % L.cam.w, y = pixel / L.cam.w;
-        float jx = (float)x + uniform(st);
-        float jy = (float)y + uniform(st);
-        f3 on_plane = v3(L.cam.o) + v3(L.cam.r) * jx + v3(L.cam.b) * jy;
-        float u0 = uniform(st);
-        float u1 = uniform(st);
-        float dof = L.cam.lens;
-        float r = std::sqrt(u0);
-        float theta = (float)((double)u1 * kPiD * 2.0);
-        float lensx = dof * r * cosf_(theta);
-        float lensy = dof * r * sinf_(theta);
-        f3 lens = v3(L.cam.eye) + v3(L.cam.up) * lensx + v3(L.cam.right) * lensy;
-        p.ro = lens;
-        p.rd = normalize(on_plane - lens);
+        camera_ray(L.cam, pixel, st, &p.ro, &p.rd);
         L.vol[pi].count = 0;
         L.rng[pi] = st;
         L.shade[pi].Le = F(0, 0, 0);
@@ -560,16 +626,8 @@ void step(Lane& L) {
         f3 wo = -L.path[index].rd;
         f3 n = L.ext[index].Ng;
         if (dot(n, wo) < 0.0f) n = -n;
-        int xt = 0.0f < n.x ? 1 : 2, yt = 0.0f < n.y ? 3 : 4, zt = 0.0f < n.z ? 5 : 6;
-        float u = uniform(st);
-        f3 ap = n * n;
-        int sel = u < ap.x ? xt : (u < ap.x + ap.y ? yt : zt);
-        uint32_t frag = alias_method(st, L.env.alias[sel].data(), count_env);
-        float pdf = 0.0f;
-        pdf += ap.x * L.env.pdf[xt][frag];
-        pdf += ap.y * L.env.pdf[yt][frag];
-        pdf += ap.z * L.env.pdf[zt][frag];
-        f3 wi = sample_from_fragment(L.env.frags[frag], st);
+        float pdf;
+        f3 wi = sample_envmap_6axis(L.env, n, st, &pdf);
         L.isample[index].wi = wi;
         L.isample[index].pdfs[kEnv] = pdf;
         L.rng[index] = st;
@@ -577,9 +635,8 @@ void step(Lane& L) {
     // sample_envmap_stage (envmap_sampling.cl:219-248)
     for (uint32_t index : L.sq[0]) {
         Rng st = L.rng[index];
-        uint32_t frag = alias_method(st, L.env.alias[0].data(), count_env);
-        float pdf = L.env.pdf[0][frag];
-        f3 wi = sample_from_fragment(L.env.frags[frag], st);
+        float pdf;
+        f3 wi = sample_envmap_uniform(L.env, st, &pdf);
         L.isample[index].wi = wi;
         L.isample[index].pdfs[kEnv] = pdf;
         L.rng[index] = st;
@@ -675,17 +732,8 @@ void step(Lane& L) {
         float cosTheta = dot(wi, Ng);
         f3 T = F(0, 0, 0);
         if (0.0f < cosTheta) {
-            f3 lv = wi + wo;
-            f3 h = normalize(lv);
-            float pi = kPiF;
-            float a2 = w.ax == w.ay ? sqr(w.ax) : w.ax * w.ay;
-            float mu = dot(h, wi);
-            float t = powf_(1.0f - mu, 1.0f / w.falloff);
-            f3 rho = F(mix(w.refl[0], w.edge[0], t), mix(w.refl[1], w.edge[1], t), mix(w.refl[2], w.edge[2], t));
-            f3 k0 = rho / (pi * a2);
-            float k1 = expf_(ward_exp(w, h, Ng));
-            float k2 = dot(lv, lv) / sqr(sqr(dot(lv, Ng)));
-            T = F(k0.x * k1 * k2 * cosTheta / pdf, k0.y * k1 * k2 * cosTheta / pdf, k0.z * k1 * k2 * cosTheta / pdf);
+            f3 f = ward_f(w, wo, wi, Ng);
+            T = F(f.x * cosTheta / pdf, f.y * cosTheta / pdf, f.z * cosTheta / pdf);
         }
         L.shade[item].Le = F(0, 0, 0);
         L.shade[item].T = T;
@@ -792,19 +840,10 @@ void step(Lane& L) {
     for (uint32_t item = 0; item < c.n; ++item) {
         Rng st = L.rng[item];
         L.ext[item].hit_surface_material = -1;
-        int hv = -1;
-        InVolumeList v = L.vol[item];
-        if (0 < v.count) {
-            uint64_t u = uniform_integer(st);
-            hv = v.volumes[u % (uint64_t)v.count];
-        }
+        int hv;
+        float tmin;
+        free_flight(L.mats, L.vols, L.vol[item], st, &hv, &tmin);
         L.ext[item].hit_volume_material = hv;
-        float tmin = FLT_MAX;
-        if (0 <= hv) {
-            const Volume& V = L.vols[L.mats[hv].index];
-            float u = uniform(st);
-            tmin = -1.0f / V.C * logf_(u);
-        }
         L.ext[item].tmin = tmin;
         L.rng[item] = st;
     }
@@ -1127,27 +1166,10 @@ void orc_envmap_sample(void* h, uint32_t* state, uint32_t n, const float* normal
     Envmap* e = (Envmap*)h;
     Rng st;
     memcpy(st.s, state, 16);
-    uint32_t count = e->w * e->h;
     for (uint32_t i = 0; i < n; ++i) {
-        f3 wi;
         float pdf;
-        if (normal) {
-            f3 nn = F(normal[0], normal[1], normal[2]);
-            int xt = 0.0f < nn.x ? 1 : 2, yt = 0.0f < nn.y ? 3 : 4, zt = 0.0f < nn.z ? 5 : 6;
-            float u = uniform(st);
-            f3 ap = nn * nn;
-            int sel = u < ap.x ? xt : (u < ap.x + ap.y ? yt : zt);
-            uint32_t frag = alias_method(st, e->alias[sel].data(), count);
-            pdf = 0.0f;
-            pdf += ap.x * e->pdf[xt][frag];
-            pdf += ap.y * e->pdf[yt][frag];
-            pdf += ap.z * e->pdf[zt][frag];
-            wi = sample_from_fragment(e->frags[frag], st);
-        } else {
-            uint32_t frag = alias_method(st, e->alias[0].data(), count);
-            pdf = e->pdf[0][frag];
-            wi = sample_from_fragment(e->frags[frag], st);
-        }
+        f3 wi = normal ? sample_envmap_6axis(*e, F(normal[0], normal[1], normal[2]), st, &pdf)
+                       : sample_envmap_uniform(*e, st, &pdf);
         dirs[3 * i] = wi.x;
         dirs[3 * i + 1] = wi.y;
         dirs[3 * i + 2] = wi.z;
@@ -1175,6 +1197,94 @@ void orc_envmap_pdf(void* h, uint32_t n, const float* dirs, const float* normal,
 void orc_envmap_pdf_each(void* h, uint32_t n, const float* dirs, const float* normals, float* pdfs) {
     for (uint32_t i = 0; i < n; ++i)
         orc_envmap_pdf(h, 1, dirs + 3 * (size_t)i, normals ? normals + 3 * (size_t)i : nullptr, pdfs + i);
+}
+
+// ---------------------------------------------------------------- single-function test hooks
+// The lane's own stage functions over flat batches, so that tests can pin them against closed
+// forms and statistics: Ward (w: 9 floats ax, ay, refl, edge, falloff) sampling, pdf and BRDF.
+void orc_ward_sample(const float* w9, uint32_t n, const float* wo, const float* nrm, const float* u, float* wi) {
+    Ward w{};
+    w.ax = w9[0];
+    w.ay = w9[1];
+    for (int k = 0; k < 3; ++k) {
+        w.refl[k] = w9[2 + k];
+        w.edge[k] = w9[5 + k];
+    }
+    w.falloff = w9[8];
+    for (uint32_t i = 0; i < n; ++i) {
+        f3 r = sample_ward(w, v3(wo + 3 * (size_t)i), v3(nrm + 3 * (size_t)i), u[2 * (size_t)i], u[2 * (size_t)i + 1]);
+        wi[3 * (size_t)i] = r.x;
+        wi[3 * (size_t)i + 1] = r.y;
+        wi[3 * (size_t)i + 2] = r.z;
+    }
+}
+// pdf[i] (solid angle) and f[3i..] (BRDF) of wi given wo about the normal.
+void orc_ward_eval(const float* w9, uint32_t n, const float* wo, const float* wi, const float* nrm, float* pdf,
+                   float* f) {
+    Ward w{};
+    w.ax = w9[0];
+    w.ay = w9[1];
+    for (int k = 0; k < 3; ++k) {
+        w.refl[k] = w9[2 + k];
+        w.edge[k] = w9[5 + k];
+    }
+    w.falloff = w9[8];
+    for (uint32_t i = 0; i < n; ++i) {
+        f3 a = v3(wo + 3 * (size_t)i), b = v3(wi + 3 * (size_t)i), z = v3(nrm + 3 * (size_t)i);
+        if (pdf) pdf[i] = pdf_ward(w, a, b, z);
+        if (f) {
+            f3 r = ward_f(w, a, b, z);
+            f[3 * (size_t)i] = r.x;
+            f[3 * (size_t)i + 1] = r.y;
+            f[3 * (size_t)i + 2] = r.z;
+        }
+    }
+}
+// new_path's primary rays: camera (21 floats eye, forward, up, right, o, r, b), image size and
+// lens radius; one ray per pixel index, each drawing from its own state (4 words, advanced).
+void orc_camera_rays(const float* camera, uint32_t width, uint32_t height, float lens, uint32_t n,
+                     const uint32_t* pixels, uint32_t* states, float* ro, float* rd) {
+    Camera c{};
+    float* dst[7] = {c.eye, c.fwd, c.up, c.right, c.o, c.r, c.b};
+    for (int a = 0; a < 7; ++a)
+        for (int k = 0; k < 3; ++k) dst[a][k] = camera[3 * a + k];
+    c.w = width;
+    c.h = height;
+    c.lens = lens;
+    for (uint32_t i = 0; i < n; ++i) {
+        Rng st;
+        memcpy(st.s, states + 4 * (size_t)i, 16);
+        f3 o, d;
+        camera_ray(c, pixels[i], st, &o, &d);
+        memcpy(states + 4 * (size_t)i, st.s, 16);
+        memcpy(ro + 3 * (size_t)i, &o, 12);
+        memcpy(rd + 3 * (size_t)i, &d, 12);
+    }
+}
+// homogeneous_volume_through per row: the medium list (7 ids + count; ids index extinctions C),
+// a state (advanced); out: the chosen medium (-1: none) and the free-flight distance.
+void orc_free_flight(const float* C, uint32_t nvol, uint32_t n, const int32_t* lists8, uint32_t* states,
+                     int32_t* hv, float* tmin) {
+    std::vector<Material> mats(nvol);
+    std::vector<Volume> vols(nvol);
+    for (uint32_t i = 0; i < nvol; ++i) {
+        mats[i] = Material{kVolume, (int32_t)i};
+        vols[i].C = C[i];
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        InVolumeList v{};
+        for (int k = 0; k < 7; ++k) v.volumes[k] = lists8[8 * (size_t)i + k];
+        v.count = lists8[8 * (size_t)i + 7];
+        Rng st;
+        memcpy(st.s, states + 4 * (size_t)i, 16);
+        free_flight(mats, vols, v, st, hv + i, tmin + i);
+        memcpy(states + 4 * (size_t)i, st.s, 16);
+    }
+}
+// Texel index of directions (envmap_index, kernels/envmap_sampling.cl:100-114).
+void orc_envmap_index(void* h, uint32_t n, const float* dirs, int32_t* out) {
+    Envmap* e = (Envmap*)h;
+    for (uint32_t i = 0; i < n; ++i) out[i] = envmap_index(*e, v3(dirs + 3 * (size_t)i));
 }
 
 }  // extern "C"

@@ -193,6 +193,17 @@ def test_bit_exact_raycast_benchmark_full_size(ctx, c2, variant, rayset):
     g = _lib.trace(ctx, scene, rays, variant)
     o, cnt = _oracle(c2["tree"], v, f, None, rays, variant)
     assert H.compare_hits(g, o) == dict(prim=0, t=0, uv=0)
+    # the instrumented kernel (whose counts price the roofline) counts the oracle's events
+    rd = ctx.to_device(np.ascontiguousarray(rays))
+    hd = ctx.alloc(len(rays) * 16)
+    gcnt = scene.extension_ray_cast_stats(variant, rd, hd, len(rays))
+    gh = hd.download(_lib.HIT_DTYPE, (len(rays),))
+    print(f"counters gpu {gcnt} oracle {cnt}", flush=True)
+    assert gcnt == cnt
+    assert (cnt["branch_visits"] > 0) if variant == "stackless" else (cnt["node_visits"] > 0)
+    assert H.compare_hits(gh, o) == dict(prim=0, t=0, uv=0)
+    rd.free()
+    hd.free()
     # the oracle itself against brute force on a sample
     idx = np.random.default_rng(0).choice(len(rays), 256, replace=False)
     bf = O.intersect_bruteforce(v, f, rays[idx])
