@@ -58,6 +58,7 @@ def cpu_baseline(tree, points, indices, rays, budget_s=10.0, cpu_seconds=20.0):
 
     ref = O.RefStackless(tree)
     threads = O.NTHREADS
+    host = host_cpu_info()
     probe = rays[:20000]
     t0 = time.perf_counter()
     O.intersect_stackless(ref, points, indices, probe, nthreads=threads)
@@ -73,8 +74,29 @@ def cpu_baseline(tree, points, indices, rays, budget_s=10.0, cpu_seconds=20.0):
         O.intersect_stackless(ref, points, indices, sample, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": round(reps * n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "label": "CPU restatement, not reference code",
+            "nproc": host["nproc"], "affinity_cpus": host["affinity"], "cpu_model": host["model"],
             "sample": f"first {n} of the {len(rays)} incoherent rays x {reps} passes, CPU restatement of the "
-                      f"stackless kernel on the reference 96-byte node layout, {threads} threads, {dt:.1f} s"}
+                      f"stackless kernel on the reference 96-byte node layout, {threads} threads "
+                      f"(the host's CPU share for one GPU is 16 of its {host['nproc']} CPUs), {dt:.1f} s"}
+
+
+def host_cpu_info():
+    """nproc, the CPUs this process may run on, and the CPU model name from /proc/cpuinfo."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "affinity": aff, "model": model}
 
 
 def render_bench(steps=32, warmup=16, paths=1 << 22, spp=64, graph_steps=8):
@@ -193,26 +215,8 @@ def batch_bench(total_slots=1 << 22, graph_steps=8, ref_spp=4096):
     from raccoongpurender_amd.guiding_study import rel_mse
 
     t0 = time.perf_counter()
-    variants = RW.batch_scene_variants()
-    specs = RW.batch_jobs()
-    slots = RW.batch_slots(specs, total_slots)
     ctx = _lib.Context(0)
-    dev = []
-    env = None
-    for v in variants:
-        sd = RW.full_render_scene(64, 64, seed=v["seed"], n_large=v["n_large"], n_mid=v["n_mid"])
-        bvh = _lib.BVH(sd["points"], sd["indices"])
-        if env is None:
-            env = _lib.Envmap(ctx, sd["env"])
-        dev.append(dict(bvh=bvh, scene=_lib.Scene(ctx, bvh), mats=_lib.Materials(ctx, sd["materials"]),
-                        cam=sd["camera"]))
-    jobs = []
-    for spec, n in zip(specs, slots):
-        d = dev[spec["variant"]]
-        c = d["cam"]
-        cam = _lib.camera_look_at(c["eye"], c["target"], c["up"], c["fov"], spec["width"], spec["height"],
-                                  c["lens_radius"])
-        jobs.append(_lib.BatchJob(d["scene"], d["mats"], env, cam, n, spp=spec["spp"], seed=spec["seed"]))
+    jobs, specs, variants = RW.device_batch_jobs(ctx, total_slots)
     setup = time.perf_counter() - t0
     budget = sum(j.npixels * j.spp for j in jobs)
     probes = []

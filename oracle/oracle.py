@@ -74,7 +74,12 @@ REF_MT_NODE = np.dtype([
 ])
 assert REF_SL_NODE.itemsize == 96 and REF_MT_NODE.itemsize == 48
 COUNTER_NAMES = ("branch_visits", "leaf_visits", "parent_steps", "prim_tests", "node_visits")
-NTHREADS = min(16, os.cpu_count() or 1)
+# Worker threads of the restatement: the CPUs this process may use, at most 16 (the host's CPU
+# share for one GPU, whatever nproc shows there).
+try:
+    NTHREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+except (AttributeError, OSError):
+    NTHREADS = max(1, min(16, os.cpu_count() or 1))
 
 
 # ------------------------------------------------------------------ RNG
@@ -309,6 +314,26 @@ _sig("orc_ward_eval", None, vp, u32, vp, vp, vp, vp, vp)
 _sig("orc_camera_rays", None, vp, u32, u32, C.c_float, u32, vp, vp, vp, vp)
 _sig("orc_free_flight", None, vp, u32, u32, vp, vp, vp, vp)
 _sig("orc_envmap_index", None, vp, u32, vp, vp)
+_sig("orc_math_f64", None, C.c_int, u32, vp, vp, vp)
+_sig("orc_pow_f32", None, u32, vp, C.c_float, vp)
+MATH_FUNCS = ("exp", "log", "sin", "cos", "atan", "atan2", "pow")
+
+
+def math_f64(name, x, y=None):
+    """The oracle's double-precision exp/log/sin/cos/atan/atan2/pow (oracle_math.h) over arrays."""
+    x = np.ascontiguousarray(x, np.float64)
+    yy = None if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), np.float64)
+    out = np.zeros_like(x)
+    lib.orc_math_f64(MATH_FUNCS.index(name), x.size, p(x), p(yy), p(out))
+    return out
+
+
+def pow_f32(x, y):
+    """(float) pow((double) x, (double) y) with the oracle's pow, elementwise over float32 x."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib.orc_pow_f32(x.size, p(x), np.float32(y), p(out))
+    return out
 
 
 def _f32(a, cols=None):
@@ -486,12 +511,12 @@ def merge_intermediate(a, b):
 
 def tonemap_intermediate(x):
     """tonemap (kernels/accumlation.cl:68-79): int(pow(mean, 1/2.2) * 256) clamped to 0..255, alpha
-    255; the power in double, rounded to float, as the device computes it."""
+    255; the power by the oracle's double pow, rounded to float, as the device computes it."""
     out = np.zeros((len(x), 4), np.uint8)
-    e = np.float64(np.float32(1.0) / np.float32(2.2))
+    e = np.float32(1.0) / np.float32(2.2)
     for k, name in enumerate("rgb"):
-        m = _half_value(x[name]).astype(np.float64)
-        p = np.power(m, e).astype(np.float32) * np.float32(256.0)
+        m = _half_value(x[name]).astype(np.float32)
+        p = pow_f32(m, e) * np.float32(256.0)
         out[:, k] = np.clip(p.astype(np.int64), 0, 255)
     out[:, 3] = 255
     return out

@@ -92,9 +92,20 @@ void lobes(const double* out, Lobe* L) {
     for (int k = 0; k < kLobes; ++k) z += std::exp(out[5 * k + 4] - mx);
     for (int k = 0; k < kLobes; ++k) {
         const double* o = out + 5 * k;
-        double m = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12;
-        for (int a = 0; a < 3; ++a) L[k].mu[a] = o[a] / m;
-        L[k].mnorm = m;
+        // a mean whose largest component is below 1e-15 has no direction (a row whose hidden
+        // units are all zero leaves the outputs at their biases): the lobe takes +z and its mean
+        // no gradient; so does a non-finite one
+        const double mxo = std::fmax(std::fmax(std::fabs(o[0]), std::fabs(o[1])), std::fabs(o[2]));
+        if (mxo > 1e-15 && mxo <= 3.0e38) {
+            const double m = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+            for (int a = 0; a < 3; ++a) L[k].mu[a] = o[a] / m;
+            L[k].mnorm = m;
+        } else {
+            L[k].mu[0] = 0.0;
+            L[k].mu[1] = 0.0;
+            L[k].mu[2] = 1.0;
+            L[k].mnorm = 0.0;
+        }
         L[k].raw_kappa = o[3];
         L[k].kappa = std::exp(std::fmin(std::fmax(o[3], kLogKappaMin), kLogKappaMax));
         L[k].logw = o[4] - mx - std::log(z);
@@ -138,7 +149,7 @@ void head_grad(const double* out, const double* w, double weight, double* dout) 
         bool inside = L[k].raw_kappa > kLogKappaMin && L[k].raw_kappa < kLogKappaMax;
         dout[5 * k + 3] = inside ? -weight * r[k] * dlogf_dk * kap : 0.0;
         for (int a = 0; a < 3; ++a)
-            dout[5 * k + a] = -weight * r[k] * kap * (w[a] - c * L[k].mu[a]) / L[k].mnorm;
+            dout[5 * k + a] = L[k].mnorm > 0.0 ? -weight * r[k] * kap * (w[a] - c * L[k].mu[a]) / L[k].mnorm : 0.0;
     }
 }
 

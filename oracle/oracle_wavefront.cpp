@@ -18,6 +18,8 @@
 #include <cmath>
 #include <vector>
 
+#include "oracle_math.h"
+
 extern "C" int orc_stackless_intersect(const void* nodes, const uint32_t* prim_ids, const float* points,
                                        uint32_t stride, const uint32_t* indices, int32_t sphere_begin,
                                        const float* spheres, uint32_t nrays, const void* rays, void* hits,
@@ -55,13 +57,14 @@ inline f3 normalize(f3 v) {
 }
 inline float fmn(float a, float b) { return std::fmin(a, b); }
 inline float fmx(float a, float b) { return std::fmax(a, b); }
-inline float sinf_(float x) { return (float)std::sin((double)x); }
-inline float cosf_(float x) { return (float)std::cos((double)x); }
-inline float logf_(float x) { return (float)std::log((double)x); }
-inline float expf_(float x) { return (float)std::exp((double)x); }
-inline float atanf_(float x) { return (float)std::atan((double)x); }
-inline float atan2f_(float y, float x) { return (float)std::atan2((double)y, (double)x); }
-inline float powf_(float x, float y) { return (float)std::pow((double)x, (double)y); }
+// The reference's float builtins: the oracle_math.h functions in double, rounded once.
+inline float sinf_(float x) { return (float)orcm::sin((double)x); }
+inline float cosf_(float x) { return (float)orcm::cos((double)x); }
+inline float logf_(float x) { return (float)orcm::log((double)x); }
+inline float expf_(float x) { return (float)orcm::exp((double)x); }
+inline float atanf_(float x) { return (float)orcm::atan((double)x); }
+inline float atan2f_(float y, float x) { return (float)orcm::atan2((double)y, (double)x); }
+inline float powf_(float x, float y) { return (float)orcm::pow((double)x, (double)y); }
 inline float mix(float a, float b, float t) { return a + (b - a) * t; }
 inline float sqr(float x) { return x * x; }
 
@@ -1281,6 +1284,29 @@ void orc_free_flight(const float* C, uint32_t nvol, uint32_t n, const int32_t* l
         memcpy(states + 4 * (size_t)i, st.s, 16);
     }
 }
+// The double-precision functions of oracle_math.h over a batch (fn: 0 exp, 1 log, 2 sin, 3 cos,
+// 4 atan, 5 atan2(x, y), 6 pow(x, y)), for checks against a reference libm.
+void orc_math_f64(int fn, uint32_t n, const double* x, const double* y, double* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const double a = x[i], b = y ? y[i] : 0.0;
+        double r;
+        switch (fn) {
+            case 0: r = orcm::exp(a); break;
+            case 1: r = orcm::log(a); break;
+            case 2: r = orcm::sin(a); break;
+            case 3: r = orcm::cos(a); break;
+            case 4: r = orcm::atan(a); break;
+            case 5: r = orcm::atan2(a, b); break;
+            default: r = orcm::pow(a, b); break;
+        }
+        out[i] = r;
+    }
+}
+// (float)pow((double)x[i], (double)y) as the tonemap evaluates it.
+void orc_pow_f32(uint32_t n, const float* x, float y, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = powf_(x[i], y);
+}
+
 // Texel index of directions (envmap_index, kernels/envmap_sampling.cl:100-114).
 void orc_envmap_index(void* h, uint32_t n, const float* dirs, int32_t* out) {
     Envmap* e = (Envmap*)h;

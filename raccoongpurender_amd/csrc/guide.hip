@@ -107,14 +107,35 @@ __device__ __forceinline__ uint32_t sample_class(float c) {
     return (c >= 1.0f && c <= (float)kGClasses) ? (uint32_t)c : 0u;
 }
 
+// Largest component below which a lobe's raw mean has no direction: every hidden unit of a row
+// can be zero (dead ReLUs), leaving the mean outputs at their biases, which may be zero. Such a
+// lobe (and one with a non-finite mean) takes the mean +z and passes no gradient to its mean
+// outputs. The mean is scaled by its largest component before it is squared, so that large
+// outputs do not overflow.
+constexpr float kMinMean = 1e-15f;
+
+// Unit mean of raw outputs o[0..2] into mu; returns the length of o (0: no direction).
+__device__ __forceinline__ float lobe_mean(const float* o, float* mu) {
+    const float mx = fmaxf(fmaxf(fabsf(o[0]), fabsf(o[1])), fabsf(o[2]));
+    if (!(mx > kMinMean) || !(mx <= 3.0e38f)) {
+        mu[0] = 0.0f;
+        mu[1] = 0.0f;
+        mu[2] = 1.0f;
+        return 0.0f;
+    }
+    const float q0 = o[0] / mx, q1 = o[1] / mx, q2 = o[2] / mx;
+    const float len = sqrtf(q0 * q0 + q1 * q1 + q2 * q2);
+    mu[0] = q0 / len;
+    mu[1] = q1 / len;
+    mu[2] = q2 / len;
+    return len * mx;
+}
+
 // Lobe k of one row from its raw outputs o[0..4] and the row's softmax constants.
 __device__ __forceinline__ void lobe_params(const float* o, float mx, float lz, float lkmax, float* L) {
-    const float m = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
     const float kap = expf(fminf(fmaxf(o[3], kGLogKappaMin), lkmax));
     const float lw = o[4] - mx - lz;
-    L[0] = o[0] / m;
-    L[1] = o[1] / m;
-    L[2] = o[2] / m;
+    lobe_mean(o, L);
     L[3] = kap;
     L[4] = expf(lw);
     L[5] = lw + guide_log_norm(kap);
@@ -164,9 +185,10 @@ __device__ void head_grad_part(const float* out, int part, const float* wi, floa
         const float e = expf(-2.0f * kap);
         const float dlogf = 1.0f / kap - 2.0f * e / (1.0f - e) + cm1[q];
         d[5 * q + 3] = (o[3] > kGLogKappaMin && o[3] < lkmax) ? -weight * r * dlogf * kap : 0.0f;
-        const float m = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) + 1e-12f;
+        float mu[3];
+        const float m = lobe_mean(o, mu);
         const float c = cm1[q] + 1.0f;
-        for (int j = 0; j < 3; ++j) d[5 * q + j] = -weight * r * kap * (wi[j] - c * Lk[j]) / m;
+        for (int j = 0; j < 3; ++j) d[5 * q + j] = m > 0.0f ? -weight * r * kap * (wi[j] - c * Lk[j]) / m : 0.0f;
     }
 }
 
@@ -195,11 +217,16 @@ struct InferLds {
     uint16_t B[kTile * kPH];
 };
 
+// Every thread of the workgroup must call it. Each wave owns its 16 rows of the bf16 tiles, but
+// the fp32 output tile has a wider row pitch: wave 0's output rows cover bytes of A that hold wave
+// 1's layer-3 inputs, and wave 1's layer-2 outputs cover bytes of wave 0's output rows. The
+// barrier keeps every bf16 use of A before any output is written.
 __device__ void infer_forward(InferLds& t, const DevGuide& g, int r0, int lane) {
     float* F = reinterpret_cast<float*>(t.A);
     fwd_layer<kGIn>(t.A, kPX, g.w_bf + kGoW0, g.params + kGoB0, t.B, nullptr, r0, lane, true);
     fwd_layer<kGHid>(t.B, kPH, g.w_bf + kGoW1, g.params + kGoB1, t.A, nullptr, r0, lane, true);
     fwd_layer<kGHid>(t.A, kPH, g.w_bf + kGoW2, g.params + kGoB2, t.B, nullptr, r0, lane, true);
+    __syncthreads();
     fwd_layer<kGHid>(t.B, kPH, g.w_bf + kGoW3, g.params + kGoB3, nullptr, F, r0, lane, false);
 }
 

@@ -26,6 +26,7 @@
 
 #include "guide_device.h"
 #include "rgpu_wavefront.h"
+#include "stage_math.h"
 #include "trace_device.h"
 
 namespace rgpu {
@@ -36,13 +37,15 @@ const double kPiD = 3.14159265358979323846;
 const float kFltMax = 3.402823466e+38f;
 
 // ---------------------------------------------------------------- math
-__device__ __forceinline__ float fsin(float x) { return (float)sin((double)x); }
-__device__ __forceinline__ float fcos(float x) { return (float)cos((double)x); }
-__device__ __forceinline__ float flog(float x) { return (float)log((double)x); }
-__device__ __forceinline__ float fexp(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ float fatan(float x) { return (float)atan((double)x); }
-__device__ __forceinline__ float fatan2(float y, float x) { return (float)atan2((double)y, (double)x); }
-__device__ __forceinline__ float fpow(float x, float y) { return (float)pow((double)x, (double)y); }
+// The reference's float builtins, evaluated in double by the stage_math.h functions (defined
+// results, independent of any libm) and rounded once.
+__device__ __forceinline__ float fsin(float x) { return (float)sm_sin((double)x); }
+__device__ __forceinline__ float fcos(float x) { return (float)sm_cos((double)x); }
+__device__ __forceinline__ float flog(float x) { return (float)sm_log((double)x); }
+__device__ __forceinline__ float fexp(float x) { return (float)sm_exp((double)x); }
+__device__ __forceinline__ float fatan(float x) { return (float)sm_atan((double)x); }
+__device__ __forceinline__ float fatan2(float y, float x) { return (float)sm_atan2((double)y, (double)x); }
+__device__ __forceinline__ float fpow(float x, float y) { return (float)sm_pow((double)x, (double)y); }
 // "x * M_PI * 2.0f" with OpenCL's double M_PI (kernels/lambertian.cl:11, new_path.cl:63)
 __device__ __forceinline__ float two_pi_times(float u) { return (float)((double)u * kPiD * 2.0); }
 

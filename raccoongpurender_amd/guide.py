@@ -33,9 +33,11 @@ _sig("rgpu_guide_read_samples", C.c_int, vp, vp, u32, C.POINTER(u32))
 _sig("rgpu_wavefront_attach_guide", C.c_int, vp, vp, C.c_int, C.c_int)
 _sig("rgpu_wavefront_guide_pending", C.c_int, vp, C.POINTER(u32))
 _sig("rgpu_wavefront_guide_train", C.c_int, vp, u32, C.POINTER(u32))
+_sig("rgpu_wavefront_guide_lobes", C.c_int, vp, u32, u32, vp)
 _lib.EXPORTED += ["rgpu_guide_create", "rgpu_guide_destroy", "rgpu_guide_param_count", "rgpu_guide_params",
                   "rgpu_guide_forward", "rgpu_guide_gradients", "rgpu_guide_train_samples", "rgpu_guide_read_samples",
-                  "rgpu_wavefront_attach_guide", "rgpu_wavefront_guide_pending", "rgpu_wavefront_guide_train"]
+                  "rgpu_wavefront_attach_guide", "rgpu_wavefront_guide_pending", "rgpu_wavefront_guide_train",
+                  "rgpu_wavefront_guide_lobes"]
 PARAM_COUNT = _lib.lib.rgpu_guide_param_count()
 
 
@@ -115,6 +117,15 @@ def pending(wavefront):
     return n.value
 
 
+def slot_lobes(wavefront, first, count):
+    """Lobe records of slots [first, first + count) as the last inference wrote them: an array
+    (count, 8, 6) of (mean xyz, kappa, weight, log(weight C(kappa)))."""
+    out = np.zeros((count, LOBES, 6), np.float32)
+    wavefront.ctx.check(_lib.lib.rgpu_wavefront_guide_lobes(wavefront.h, first, count, ptr(out)),
+                        "rgpu_wavefront_guide_lobes")
+    return out
+
+
 def train_pass(wavefront, min_samples):
     n = C.c_uint32(0)
     wavefront.ctx.check(_lib.lib.rgpu_wavefront_guide_train(wavefront.h, min_samples, C.byref(n)),
@@ -127,7 +138,12 @@ def raw_to_lobes(out):
     out = np.asarray(out, np.float64)
     o = out[:, :40].reshape(-1, LOBES, 5)
     m = o[..., :3]
-    mu = m / (np.linalg.norm(m, axis=-1, keepdims=True) + 1e-12)
+    mx = np.abs(m).max(-1, keepdims=True)
+    # a mean whose largest component is below 1e-15 (or not finite) has no direction: the
+    # kernels give that lobe +z
+    ok = (mx > 1e-15) & (mx <= 3.0e38)
+    n = np.sqrt(np.where(ok, (m * m).sum(-1, keepdims=True), 1.0))
+    mu = np.where(ok, m / n, np.array([0.0, 0.0, 1.0]))
     kappa = np.exp(np.clip(o[..., 3], -3.0, 7.5))
     lg = o[..., 4] - o[..., 4].max(-1, keepdims=True)
     w = np.exp(lg) / np.exp(lg).sum(-1, keepdims=True)

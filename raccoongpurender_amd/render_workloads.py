@@ -110,3 +110,33 @@ def batch_slots(jobs, total=1 << 22, minimum=256):
     slots = (np.ceil(raw / 256) * 256).astype(np.int64)
     cap = np.array([max(minimum, (j["width"] * j["height"] * j["spp"] + 255) // 256 * 256) for j in jobs])
     return [int(x) for x in np.minimum(slots, cap)]
+
+
+def device_batch_jobs(ctx, total_slots=1 << 22, count=256, variants=8, seed=99):
+    """The batched workload on the device: 8 scene variants (BVH, scene, materials) sharing one
+    envmap, and the 256 jobs (camera of the variant at the job's resolution, spp, seed, slots).
+    Returns (jobs, specs, variant descriptions); each job keeps its device objects alive."""
+    from . import _lib
+
+    vdesc = batch_scene_variants(variants, seed=seed)
+    specs = batch_jobs(count, variants, seed=seed)
+    slots = batch_slots(specs, total_slots)
+    dev = []
+    env = None
+    for v in vdesc:
+        sd = full_render_scene(64, 64, seed=v["seed"], n_large=v["n_large"], n_mid=v["n_mid"])
+        bvh = _lib.BVH(sd["points"], sd["indices"])
+        if env is None:
+            env = _lib.Envmap(ctx, sd["env"])
+        dev.append(dict(bvh=bvh, scene=_lib.Scene(ctx, bvh), mats=_lib.Materials(ctx, sd["materials"]),
+                        cam=sd["camera"], env=env))
+    jobs = []
+    for spec, n in zip(specs, slots):
+        d = dev[spec["variant"]]
+        c = d["cam"]
+        cam = _lib.camera_look_at(c["eye"], c["target"], c["up"], c["fov"], spec["width"], spec["height"],
+                                  c["lens_radius"])
+        j = _lib.BatchJob(d["scene"], d["mats"], env, cam, n, spp=spec["spp"], seed=spec["seed"])
+        j.keep = d
+        jobs.append(j)
+    return jobs, specs, vdesc

@@ -1,0 +1,287 @@
+"""The three render workloads at their own size on the GPU.
+
+- Full render (1920 x 1080, depth 8, 2,017,294 triangles of every material, a medium box and a
+  2048 x 1024 sun sky): 2^16 slots over the middle of the image (lane 1 of 2) equal the oracle
+  lane bit for bit for 4 bounces, for the launch shapes the autotuner chooses between; at the
+  workload's 2^22 slots every bounce keeps all slots in exactly one queue, sums stay finite,
+  and a 64-spp budget starts and finishes every pixel exactly 64 times.
+- The guided render of that scene: training samples stay in the scene box with finite
+  positive weights, and at an equal spp budget the guided image mean equals the plain one
+  within its standard error (both unclamped and unbiased).
+- The batched workload (256 jobs over 8 scene variants, 92k-2.0M triangles, seed 99): one job's
+  slots equal a single render of that job bit for bit while the whole batch steps, and the
+  batch runs to completion, plain and guided, with every pixel of every job at its spp.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from raccoongpurender_amd import _lib, guide as G, render_workloads as RW
+from tests import wf_helpers as W
+
+pytestmark = pytest.mark.gpu
+
+NO_CLAMP = 3.0e38  # keep every finite sample so that counts are exact
+
+
+@pytest.fixture(scope="module")
+def c3():
+    sd = RW.full_render_scene()
+    assert len(sd["indices"]) == 2017294
+    return sd
+
+
+def _camera(sd):
+    return W.camera(sd["camera"])
+
+
+def _explain(sg, so, mm, limit=4):
+    """The differing fields and, for the first few differing slots, both sides' records with
+    the float bit patterns of the compared fields."""
+    lines = [f"mismatching slots per field: {mm}"]
+    bad = np.any(sg["rng"] != so["rng"], axis=1)
+    checked = {"rays": ("ro", "rd", "pixel"), "paths": ("T", "L", "logic_i"),
+               "ext": ("hit_surface", "hit_volume", "tmin"), "shade": ("Le", "T")}
+    for grp, names in checked.items():
+        for name in names:
+            x, y = np.asarray(sg[grp][name]), np.asarray(so[grp][name])
+            ne = x.view(np.uint32) != y.view(np.uint32) if x.dtype.kind == "f" else x != y
+            bad |= ne.reshape(len(ne), -1).any(axis=1)
+    hit = sg["ext"]["hit_surface"] >= 0
+    bad |= np.any(sg["ext"]["Ng"].view(np.uint32) != so["ext"]["Ng"].view(np.uint32), axis=1) & hit
+    for i in np.flatnonzero(bad)[:limit]:
+        for side, st in (("gpu", sg), ("orc", so)):
+            lines.append(f"slot {i} {side}: rays {st['rays'][i]} paths {st['paths'][i]} ext {st['ext'][i]} "
+                         f"shade {st['shade'][i]} vol {st['vol'][i]} rng {st['rng'][i]}")
+            lines.append(f"slot {i} {side} bits: T {st['paths'][i]['T'].view(np.uint32)} "
+                         f"L {st['paths'][i]['L'].view(np.uint32)} shade.T {st['shade'][i]['T'].view(np.uint32)} "
+                         f"shade.Le {st['shade'][i]['Le'].view(np.uint32)}")
+    return "\n".join(lines)
+
+
+def test_full_render_matches_oracle_at_2e16_slots(ctx, c3):
+    n = 1 << 16
+    print("case full render 1920x1080, 2^16 slots from the middle row, 4 bounces", flush=True)
+    # one oracle lane per traversal order: exact t-ties (the medium box's floor lies in the
+    # ground plane) go to the primitive tested last, which depends on the traversal order
+    oracles = {}
+    for variant in ("stackless", "threaded"):
+        oracles[variant] = W.oracle_lane(c3, n, max_depth=8, variant=variant)
+        oracles[variant].initialize(1, 2)  # pixel cursor at P/2: geometry, not sky
+    lanes = {}
+    for variant, block in (("stackless", 64), ("stackless", 0), ("threaded", 256)):
+        g = W.gpu_lane(ctx, c3, n, max_depth=8, variant=variant)
+        g.set_launch(variant, block, 2048)
+        g.initialize(_camera(c3), 1, 2)
+        lanes[(variant, block)] = g
+    for s in range(4):
+        ref = {}
+        for variant, o in oracles.items():
+            o.step()
+            ref[variant] = (*o.queue_counts(), o.state())
+        for key, g in lanes.items():
+            co, po, so = ref[key[0]]
+            g.step()
+            cg, pg = g.queue_counts()
+            assert np.array_equal(cg, co) and pg == po, (key, s, cg, co, pg, po)
+            sg = g.state()
+            mm = W.state_mismatches(sg, so)
+            if not all(v == 0 for v in mm.values()):
+                print(f"{key} bounce {s}:\n" + _explain(sg, so, mm), flush=True)
+            assert all(v == 0 for v in mm.values()), (key, s, mm)
+            for k in range(7):
+                assert np.array_equal(sg["items"][k, :cg[k]], so["items"][k, :co[k]]), (key, s, k)
+        print(f"bounce {s}: queues {co.tolist()}", flush=True)
+    assert co[1:].sum() > n // 4, "most paths should still be alive and shading after 4 bounces"
+    for key, g in lanes.items():
+        ro, no = oracles[key[0]].accumulation()
+        rg, ng = g.accumulation()
+        assert np.array_equal(ng, no), key
+        rel = np.abs(rg - ro) / np.maximum(np.abs(ro), 1e-30)
+        assert np.all((rel <= 1e-4) | (np.abs(rg - ro) <= 1e-6)), (key, rel.max())
+        g.close()
+
+
+def test_full_render_at_2e22_slots_keeps_every_slot_queued(ctx, c3):
+    n = 1 << 22
+    print("case full render 1920x1080, 2^22 slots, 24 bounces", flush=True)
+    g = W.gpu_lane(ctx, c3, n, max_depth=8, clamp=NO_CLAMP)
+    g.set_launch("stackless", 64, 2048)
+    g.initialize(_camera(c3))
+    for s in range(24):
+        g.step()
+        counts, cursor = g.queue_counts()
+        assert int(counts.sum()) == n, (s, counts)
+    rgb, cnt = g.accumulation()
+    assert np.isfinite(rgb).all()
+    assert int(cnt.sum()) == g.stat() > 4 * n
+    g.close()
+
+
+def test_full_render_64spp_budget_starts_and_finishes_every_pixel_64_times(ctx, c3):
+    print("case full render 1920x1080, one job of 2^22 slots, 64 spp to completion", flush=True)
+    bvh = _lib.BVH(c3["points"], c3["indices"])
+    scene, mats, env = _lib.Scene(ctx, bvh), _lib.Materials(ctx, c3["materials"]), _lib.Envmap(ctx, c3["env"])
+    job = _lib.BatchJob(scene, mats, env, _camera(c3), 1 << 22, spp=64, seed=0)
+    batch = _lib.Batch(ctx, [job], max_depth=8, clamp=NO_CLAMP)
+    steps = batch.run(max_steps=4000, check_every=16)
+    counts, started = batch.job_queue_counts(0)
+    rgb, cnt = batch.job_accumulation(0)
+    print(f"{steps} bounces, {started} samples started", flush=True)
+    assert int(counts.sum()) == 0 and started == 1920 * 1080 * 64
+    assert cnt.min() == 64 and cnt.max() == 64
+    assert np.isfinite(rgb).all() and rgb.min() >= 0.0
+    assert batch.job_stat(0) == 1920 * 1080 * 64
+    batch.close()
+
+
+def _mean_and_se(rgb, cnt):
+    y = (0.2126 * rgb[:, 0] + 0.7152 * rgb[:, 1] + 0.0722 * rgb[:, 2]).astype(np.float64) / cnt
+    return float(y.mean()), float(y.std(ddof=1) / math.sqrt(len(y)))
+
+
+def test_guided_full_render_is_unbiased_and_trains_on_valid_samples(ctx, c3):
+    print("case guided full render 1920x1080, 2^22 slots, 4 spp plain vs guided", flush=True)
+    lo = c3["points"].min(0) - 1e-3
+    hi = c3["points"].max(0) + 1e-3
+    bvh = _lib.BVH(c3["points"], c3["indices"])
+    scene, mats, env = _lib.Scene(ctx, bvh), _lib.Materials(ctx, c3["materials"]), _lib.Envmap(ctx, c3["env"])
+    # training samples of the guided single render
+    wf = _lib.Wavefront(ctx, scene, mats, env, 1 << 22, max_depth=8, clamp=NO_CLAMP, flags=_lib.WF_CONSISTENT_NORMALS)
+    wf.initialize(_camera(c3))
+    gd = G.Guide(ctx, lo, hi, capacity=1 << 18, minibatch=16384, seed=1)
+    G.attach(wf, gd, sampling=True, training=True)
+    passes = 0
+    for _ in range(6):
+        wf.step(4)
+        # every lobe record an inference wrote (kappa > 0) has a unit mean and weights summing to 1
+        lob = G.slot_lobes(wf, 0, 1 << 20).astype(np.float64)
+        written = lob[:, :, 3] > 0
+        mu_len = np.linalg.norm(lob[:, :, :3], axis=2)
+        bad_mu = written & ~(np.abs(mu_len - 1.0) <= 1e-4)
+        wsum = lob[:, :, 4].sum(1)
+        bad_w = written.any(1) & ~(np.abs(wsum - 1.0) <= 1e-4)
+        if bad_mu.any() or bad_w.any():
+            print(f"lobe records: {written.sum()} written, {bad_mu.sum()} means off unit length, "
+                  f"{bad_w.sum()} slots with weights not summing to 1", flush=True)
+            for sl in np.unique(np.flatnonzero(bad_mu.any(1) | bad_w))[:3]:
+                print(f"  slot {sl}: {lob[sl].tolist()}", flush=True)
+        assert not bad_mu.any() and not bad_w.any()
+        s = gd.buffered_samples()
+        assert len(s) > 0
+        assert np.isfinite(s["p"]).all() and (s["p"] >= 0).all() and (s["p"] <= 1).all()
+        assert np.isfinite(s["weight"]).all() and (s["weight"] > 0).all()
+        nrm = np.linalg.norm(s["wi"].astype(np.float64), axis=1)
+        bad = ~(np.abs(nrm - 1.0) <= 1e-4)
+        if bad.any():
+            worst = np.argsort(-np.abs(np.nan_to_num(nrm, nan=1e9) - 1.0))[:5]
+            print(f"{bad.sum()} of {len(s)} directions off unit length: norms {nrm[worst]}", flush=True)
+            for i in worst:
+                print(f"  row {i}: {s[i]}", flush=True)
+            out = gd.forward(s[worst])
+            mu, kap, wts = G.raw_to_lobes(out)
+            for k, i in enumerate(worst):
+                print(f"  row {i} raw means {out[k, :40].reshape(8, 5)[:, :3].tolist()}", flush=True)
+                print(f"  row {i} kappa {kap[k].tolist()} weights {wts[k].tolist()}", flush=True)
+        assert not bad.any()
+        passes += G.train_pass(wf, 1) > 0
+    assert passes == 6
+    params = gd.params()
+    assert np.isfinite(params).all()
+    G.attach(wf, None)
+    wf.close()
+    # equal spp budgets, plain and guided (the guided run trains as it renders)
+    means = {}
+    for guided in (False, True):
+        job = _lib.BatchJob(scene, mats, env, _camera(c3), 1 << 22, spp=4, seed=7 if guided else 8)
+        b = _lib.Batch(ctx, [job], max_depth=8, clamp=NO_CLAMP, flags=_lib.WF_CONSISTENT_NORMALS)
+        if guided:
+            G.attach(b, gd, sampling=True, training=True)
+        b.run(max_steps=4000, check_every=8)
+        rgb, cnt = b.job_accumulation(0)
+        assert cnt.min() == 4 and cnt.max() == 4 and np.isfinite(rgb).all()
+        means[guided] = _mean_and_se(rgb.astype(np.float64), cnt)
+        if guided:
+            G.attach(b, None)
+        b.close()
+    (mp, sp), (mg, sg) = means[False], means[True]
+    z = abs(mg - mp) / math.sqrt(sp * sp + sg * sg)
+    print(f"image mean plain {mp:.5f} +- {sp:.5f}, guided {mg:.5f} +- {sg:.5f}, z = {z:.2f}", flush=True)
+    assert z < 5.0
+    gd.close()
+
+
+@pytest.fixture(scope="module")
+def c5(ctx):
+    jobs, specs, variants = RW.device_batch_jobs(ctx)
+    assert len(jobs) == 256 and len(variants) == 8
+    return jobs, specs, variants
+
+
+def _job_items(state, totals, first, slots):
+    items = state["items"]
+    out = []
+    for q in range(7):
+        row = items[q, :totals[q]]
+        sel = row[(row >= first) & (row < first + slots)]
+        out.append(sel - first)
+    return out
+
+
+def test_batch_workload_job_equals_its_single_render(ctx, c5):
+    jobs, specs, variants = c5
+    # the job of the largest scene variant with the most slots
+    big = max(range(len(variants)), key=lambda v: variants[v]["triangles"])
+    k = max((i for i, s in enumerate(specs) if s["variant"] == big), key=lambda i: jobs[i].slots)
+    j = jobs[k]
+    print(f"case batch workload: job {k} ({j.camera.width}x{j.camera.height}, {j.slots} slots, "
+          f"{variants[big]['triangles']} tris) against its single render", flush=True)
+    batch = _lib.Batch(ctx, jobs, max_depth=8, clamp=NO_CLAMP)
+    single = _lib.Wavefront(ctx, j.scene, j.materials, j.envmap, j.slots, seed=j.seed, max_depth=8, clamp=NO_CLAMP)
+    single.initialize(j.camera)
+    first = int(batch.first_slot[k])
+    for s in range(5):
+        batch.step()
+        single.step()
+        cj, started = batch.job_queue_counts(k)
+        cs, cursor = single.queue_counts()
+        assert np.array_equal(cj, cs) and started == cursor, (s, cj, cs, started, cursor)
+        sb, ss = batch.state(), single.state()
+        sub = {key: v[first:first + j.slots] for key, v in sb.items() if key != "items"}
+        mm = W.state_mismatches(sub, {key: v for key, v in ss.items() if key != "items"})
+        assert all(v == 0 for v in mm.values()), (s, mm)
+        tot, _ = batch.queue_counts()
+        runs = _job_items(sb, tot, first, j.slots)
+        for q in range(7):
+            assert np.array_equal(runs[q], ss["items"][q, :cs[q]]), (s, q)
+    rb, nb = batch.job_accumulation(k)
+    rs, ns = single.accumulation()
+    assert np.array_equal(nb, ns)
+    assert np.allclose(rb, rs, rtol=1e-4, atol=1e-6)
+    single.close()
+    batch.close()
+
+
+@pytest.mark.parametrize("guided", [False, True])
+def test_batch_workload_runs_to_completion(ctx, c5, guided):
+    jobs, specs, variants = c5
+    print(f"case batch workload: 256 jobs to completion, guided={guided}", flush=True)
+    batch = _lib.Batch(ctx, jobs, max_depth=8, clamp=NO_CLAMP)
+    gd = None
+    if guided:
+        gd = G.Guide(ctx, capacity=1 << 18, minibatch=16384, seed=1)
+        G.attach(batch, gd, sampling=True, training=True)
+    steps = batch.run(max_steps=20000, check_every=16)
+    counts, _ = batch.queue_counts()
+    print(f"{steps} bounces", flush=True)
+    assert int(counts.sum()) == 0
+    for k, (j, spec) in enumerate(zip(jobs, specs)):
+        _, cnt = batch.job_accumulation(k)
+        assert cnt.min() == spec["spp"] and cnt.max() == spec["spp"], (k, spec, cnt.min(), cnt.max())
+        assert batch.job_stat(k) == j.npixels * spec["spp"]
+    if gd is not None:
+        assert np.isfinite(gd.params()).all()
+        G.attach(batch, None)
+        gd.close()
+    batch.close()

@@ -133,3 +133,23 @@ def test_emulated_rounding_is_close_to_exact():
     b = M.forward(params, x, emulate=True)
     assert np.abs(a - b).max() < 0.05 * (1 + np.abs(a).max())
     assert M.bf16(1.0 + 2 ** -9) == 1.0 and M.bf16(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7  # ties to even
+
+
+def test_zero_length_mean_gives_a_unit_lobe():
+    """A row whose hidden units are all zero leaves a lobe's mean outputs at their biases, which
+    can be exactly zero: that lobe must still be a normalised vMF about a unit mean (+z), so that
+    its samples are directions and the mixture integrates to one."""
+    out = np.zeros(M.N_OUT)
+    out[3::5][:8] = np.log(50.0)  # sharp lobes: a zero-length mean would shrink every sample
+    out[4::5][:8] = np.linspace(-1.0, 1.0, 8)
+    out[5:8] = (0.3, -0.2, 0.9)  # lobe 1 keeps a mean of its own
+    rng = np.random.default_rng(3)
+    w = M.sample(out, rng.random((20000, 3)))
+    assert np.allclose(np.linalg.norm(w, axis=1), 1.0, atol=1e-12)
+    d = rng.normal(size=(400000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    q = M.pdf(np.tile(out, (len(d), 1)), d)
+    assert abs(4.0 * np.pi * q.mean() - 1.0) < 0.02
+    # the lobe's mean is +z: its density peaks there
+    q_z, q_mz = M.pdf(np.tile(out, (2, 1)), np.array([[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]]))
+    assert q_z > 100.0 * q_mz
