@@ -97,6 +97,10 @@ int rgpu_download(rgpu_ctx* ctx, rgpu_stream stream, void* dst, const void* src,
 /* Fill `count` 32-bit words with `value` (OpenCLBuffer::fill, raccoon_ocl.hpp:427-434). */
 int rgpu_fill_u32(rgpu_ctx* ctx, rgpu_stream stream, void* dst, uint32_t value, size_t count);
 int rgpu_synchronize(rgpu_ctx* ctx, rgpu_stream stream);
+/* A stream of the context's device at the default priority (the reference's per-lane step and
+ * finalize queues, common/wavefront_path_tracing.hpp:346-348), and its release (after it drains). */
+int rgpu_stream_create(rgpu_ctx* ctx, rgpu_stream* out);
+int rgpu_stream_destroy(rgpu_ctx* ctx, rgpu_stream stream);
 /* Device-to-device copy with the library's own copy kernel (copy_if_locked without the
  * lock, kernels/mutex.cl:19-24); asynchronous on `stream`. */
 int rgpu_copy(rgpu_ctx* ctx, rgpu_stream stream, void* dst, const void* src, size_t bytes);
@@ -332,6 +336,23 @@ typedef struct rgpu_rgb16 {
 /* The lane's intermediate image into dst (device memory, one rgpu_rgb16 per pixel). Single
  * renders only. */
 int rgpu_wavefront_intermediate(rgpu_wavefront* w, rgpu_stream stream, rgpu_rgb16* dst);
+/* Preview handoff between a step thread and a preview thread (the weak device mutex of
+ * kernels/mutex.cl with accumlation_to_intermediate and copy_if_locked, kernels/accumlation.cl:6-25;
+ * WavefrontLane::step and update_intermediate, common/wavefront_path_tracing.hpp:877-898,942-960).
+ * set_publish(every_steps): with every_steps > 0, every that many rgpu_wavefront_step steps (and
+ * once after each replay of a captured graph) end by trying a device-side lock and, if they get
+ * it, refreshing the lane's published intermediate image; 0 switches publishing off. Single
+ * renders only; synchronous.
+ * update_intermediate: tries the same lock on `stream` and, if it gets it, copies the published
+ * image into dst (device memory, one rgpu_rgb16 per pixel); otherwise dst keeps its previous
+ * contents. took (device memory, may be NULL) receives 1 or 0. Asynchronous on `stream`.
+ * Neither side ever waits for the other: a step that finds the lock held skips publishing, a
+ * preview that finds it held keeps its image, so a copy is never torn and pixel counts in dst
+ * never decrease. Intended use: one thread calls rgpu_wavefront_step on one stream while another
+ * calls update_intermediate (then rgpu_merge_intermediate / rgpu_tonemap_intermediate) on a
+ * second stream. initialize, resume, set_publish and destroy must not overlap a preview call. */
+int rgpu_wavefront_set_publish(rgpu_wavefront* w, uint32_t every_steps);
+int rgpu_wavefront_update_intermediate(rgpu_wavefront* w, rgpu_stream stream, rgpu_rgb16* dst, int* took);
 /* a = count-weighted mean of a and b, counts summed (merge_intermediate,
  * kernels/accumlation.cl:27-50; the merge of several lanes in
  * common/wavefront_path_tracing.hpp:962-991). Device buffers of npixels entries. */

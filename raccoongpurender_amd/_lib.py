@@ -191,6 +191,10 @@ class Context:
     def copy(self, dst, src, nbytes):
         self.check(lib.rgpu_copy(self.h, None, dst.ptr, src.ptr, nbytes), "rgpu_copy")
 
+    def stream(self):
+        """A second stream of this device at the default priority (Stream.close releases it)."""
+        return Stream(self)
+
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
 
@@ -199,6 +203,30 @@ class Context:
         b = DeviceBuffer(self, arr.nbytes)
         b.upload(arr)
         return b
+
+
+class Stream:
+    """A HIP stream of a context (rgpu_stream_create), for work that overlaps the context's own."""
+
+    def __init__(self, ctx):
+        h = vp()
+        ctx.check(lib.rgpu_stream_create(ctx.h, C.byref(h)), "rgpu_stream_create")
+        self.ctx, self.h = ctx, h
+        ctx.adopt(self)
+
+    def synchronize(self):
+        self.ctx.check(lib.rgpu_synchronize(self.ctx.h, self.h), "rgpu_synchronize")
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib.rgpu_stream_destroy(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:
@@ -215,10 +243,11 @@ class DeviceBuffer:
         assert arr.nbytes <= self.nbytes
         self.ctx.check(lib.rgpu_upload(self.ctx.h, None, self.ptr, ptr(arr), arr.nbytes), "rgpu_upload")
 
-    def download(self, dtype, shape):
+    def download(self, dtype, shape, stream=None):
         out = np.empty(shape, dtype=dtype)
         assert out.nbytes <= self.nbytes
-        self.ctx.check(lib.rgpu_download(self.ctx.h, None, ptr(out), self.ptr, out.nbytes), "rgpu_download")
+        self.ctx.check(lib.rgpu_download(self.ctx.h, stream.h if stream is not None else None, ptr(out), self.ptr,
+                                         out.nbytes), "rgpu_download")
         return out
 
     def fill_u32(self, value, count=None):
@@ -429,6 +458,12 @@ _sig("rgpu_wavefront_resume", C.c_int, vp, vp, C.c_uint64)
 EXPORTED += ["rgpu_bvh_build_device", "rgpu_wavefront_intermediate", "rgpu_merge_intermediate",
              "rgpu_tonemap_intermediate"]
 EXPORTED += ["rgpu_wavefront_checkpoint_size", "rgpu_wavefront_checkpoint", "rgpu_wavefront_resume"]
+_sig("rgpu_stream_create", C.c_int, vp, C.POINTER(vp))
+_sig("rgpu_stream_destroy", C.c_int, vp, vp)
+_sig("rgpu_wavefront_set_publish", C.c_int, vp, u32)
+_sig("rgpu_wavefront_update_intermediate", C.c_int, vp, vp, vp, vp)
+EXPORTED += ["rgpu_stream_create", "rgpu_stream_destroy", "rgpu_wavefront_set_publish",
+             "rgpu_wavefront_update_intermediate"]
 _sig("rgpu_wavefront_graph_capture", C.c_int, vp, u32)
 _sig("rgpu_wavefront_graph_launch", C.c_int, vp, vp, u32)
 _sig("rgpu_wavefront_set_launch", C.c_int, vp, C.c_int, u32, u32)
@@ -683,6 +718,18 @@ class Wavefront:
         out = dev if dev is not None else self.ctx.alloc(max(self.npixels, 1) * 12)
         self.ctx.check(lib.rgpu_wavefront_intermediate(self.h, None, out.ptr), "rgpu_wavefront_intermediate")
         return out if dev is not None else out.download(RGB16_DTYPE, self.npixels)
+
+    def set_publish(self, every_steps=1):
+        """Publish the intermediate image for previews every `every_steps` steps (0: off)."""
+        self.ctx.check(lib.rgpu_wavefront_set_publish(self.h, every_steps), "rgpu_wavefront_set_publish")
+
+    def update_intermediate(self, dst, stream=None, took=None):
+        """Preview side of the handoff: copy the published image into device buffer dst if the
+        lock is free (dst unchanged otherwise); `took` (a 4-byte device buffer) receives 1 or 0.
+        Asynchronous on `stream` (a Stream, or None for the context's stream)."""
+        self.ctx.check(lib.rgpu_wavefront_update_intermediate(self.h, stream.h if stream is not None else None,
+                                                              dst.ptr, took.ptr if took is not None else None),
+                       "rgpu_wavefront_update_intermediate")
 
     def checkpoint(self):
         """Host snapshot of the lane (numpy uint8 array) that resume() continues exactly."""

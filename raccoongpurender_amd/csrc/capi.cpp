@@ -176,6 +176,24 @@ int rgpu_copy(rgpu_ctx* ctx, rgpu_stream stream, void* dst, const void* src, siz
     return RGPU_OK;
 }
 
+int rgpu_stream_create(rgpu_ctx* ctx, rgpu_stream* out) {
+    if (!ctx || !out) return RGPU_ERR_INVALID_ARG;
+    *out = nullptr;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = nullptr;
+    RGPU_HIP(ctx, hipStreamCreate(&s));  // default priority
+    *out = (rgpu_stream)s;
+    return RGPU_OK;
+}
+
+int rgpu_stream_destroy(rgpu_ctx* ctx, rgpu_stream stream) {
+    if (!ctx || !stream) return RGPU_ERR_INVALID_ARG;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
+    RGPU_HIP(ctx, hipStreamDestroy((hipStream_t)stream));
+    return RGPU_OK;
+}
+
 int rgpu_synchronize(rgpu_ctx* ctx, rgpu_stream stream) {
     if (!ctx) return RGPU_ERR_INVALID_ARG;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));

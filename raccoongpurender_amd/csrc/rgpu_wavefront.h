@@ -194,6 +194,14 @@ struct RGB16 {
 static_assert(sizeof(RGB16) == 12, "intermediate pixel is 12 bytes");
 hipError_t wf_intermediate(const DevState& st, uint32_t npixels, bool accum64, RGB16* out, hipStream_t s);
 hipError_t merge_intermediate(RGB16* a, const RGB16* b, uint32_t npixels, hipStream_t s);
+// Preview handoff (kernels/mutex.cl): words[0] the weak mutex, words[1] the step side's flag,
+// words[2] the preview side's flag. publish refreshes `shared` from the accumulation when the
+// step side takes the mutex; take copies `shared` into dst when the preview side takes it (and
+// writes 1 or 0 to *took, device memory, if given). Neither waits.
+hipError_t handoff_init(int* words, hipStream_t s);
+hipError_t handoff_publish(const DevState& st, uint32_t npixels, bool accum64, RGB16* shared, int* words,
+                           hipStream_t s);
+hipError_t handoff_take(const RGB16* shared, RGB16* dst, uint32_t npixels, int* words, int* took, hipStream_t s);
 hipError_t tonemap_intermediate(const RGB16* in, uint32_t npixels, void* rgba8, hipStream_t s);
 // Flat-batch environment sampling: states (4 words per sample) advance in place; normals (3 floats
 // per sample) select the six-axis tables, null the uniform-weight one; out holds 4 floats per

@@ -1240,10 +1240,12 @@ __global__ void __launch_bounds__(256) k_tonemap(DevState s, uint32_t npixels, u
 
 // accumlation_to_intermediate (kernels/accumlation.cl:6-25) without its mutex: per pixel the
 // half-float mean and the sample count, widened to 32 bits.
+// held (optional): the kernel does nothing unless *held is set (copy_if_locked, kernels/mutex.cl).
 __global__ void __launch_bounds__(256) k_to_intermediate(DevState s, uint32_t npixels, uint32_t accum64,
-                                                         RGB16* __restrict__ out) {
+                                                         RGB16* __restrict__ out, const int* held) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npixels) return;
+    if (held && *held == 0) return;
     float r, g, b;
     uint32_t cnt;
     if (accum64) {
@@ -1350,7 +1352,62 @@ hipError_t wf_tonemap(const DevState& st, uint32_t npixels, bool accum64, void* 
 hipError_t wf_intermediate(const DevState& st, uint32_t npixels, bool accum64, RGB16* out, hipStream_t s) {
     if (npixels == 0) return hipSuccess;
     hipLaunchKernelGGL(k_to_intermediate, dim3((npixels + 255) / 256), dim3(256), 0, s, st, npixels, accum64 ? 1u : 0u,
-                       out);
+                       out, (const int*)nullptr);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- preview handoff
+// The weak device mutex of kernels/mutex.cl: word 1 = free. A side takes it with one
+// compare-and-swap (1 -> 0) and records in its own flag whether it did; it never waits, so the
+// step stream and the preview stream never block each other. Work guarded by the flag runs in
+// later kernels of the same stream (kernel boundaries order the data), and the holder frees the
+// word in a final kernel.
+__global__ void k_try_lock(int* mutex, int* held) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *held = atomicCAS(mutex, 1, 0) == 1 ? 1 : 0;
+}
+__global__ void k_unlock(int* mutex, int* held) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *held != 0) {
+        atomicExch(mutex, 1);
+        *held = 0;
+    }
+}
+__global__ void __launch_bounds__(256) k_copy_rgb16_if(const RGB16* __restrict__ src, RGB16* __restrict__ dst,
+                                                      uint32_t npixels, const int* held) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npixels || *held == 0) return;
+    dst[i] = src[i];
+}
+__global__ void k_copy_flag(const int* src, int* dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
+}
+__global__ void k_fill_i32(int* p, int v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t handoff_init(int* words, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, words, 0, 4u);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, words, 1, 1u);  // word 0: the mutex, free
+    return hipGetLastError();
+}
+
+hipError_t handoff_publish(const DevState& st, uint32_t npixels, bool accum64, RGB16* shared, int* words,
+                           hipStream_t s) {
+    if (npixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_try_lock, dim3(1), dim3(64), 0, s, words, words + 1);
+    hipLaunchKernelGGL(k_to_intermediate, dim3((npixels + 255) / 256), dim3(256), 0, s, st, npixels,
+                       accum64 ? 1u : 0u, shared, (const int*)(words + 1));
+    hipLaunchKernelGGL(k_unlock, dim3(1), dim3(64), 0, s, words, words + 1);
+    return hipGetLastError();
+}
+
+hipError_t handoff_take(const RGB16* shared, RGB16* dst, uint32_t npixels, int* words, int* took, hipStream_t s) {
+    if (npixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_try_lock, dim3(1), dim3(64), 0, s, words, words + 2);
+    hipLaunchKernelGGL(k_copy_rgb16_if, dim3((npixels + 255) / 256), dim3(256), 0, s, shared, dst, npixels,
+                       (const int*)(words + 2));
+    if (took) hipLaunchKernelGGL(k_copy_flag, dim3(1), dim3(64), 0, s, (const int*)(words + 2), took);
+    hipLaunchKernelGGL(k_unlock, dim3(1), dim3(64), 0, s, words, words + 2);
     return hipGetLastError();
 }
 

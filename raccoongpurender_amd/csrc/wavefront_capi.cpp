@@ -72,6 +72,13 @@ struct rgpu_wavefront {
     std::vector<Job> jobs;
     DevJobs djobs;
     bool is_batch() const { return !jobs.empty(); }
+    // preview handoff (rgpu_wavefront_set_publish): the published intermediate image and the
+    // weak-mutex words (mutex, step side's flag, preview side's flag)
+    uint32_t publish_every = 0;
+    uint64_t publish_tick = 0;
+    RGB16* pub_image = nullptr;
+    uint32_t pub_pixels = 0;
+    int* pub_words = nullptr;
 };
 
 struct rgpu_guide {
@@ -466,6 +473,38 @@ int rgpu_camera_look_at(const float* eye, const float* target, const float* up, 
 }
 
 // (Re)allocates the accumulation and tonemap buffers of a single render for P pixels.
+// The published image and mutex of the preview handoff, sized to the lane's image and reset (mutex
+// free, nothing published). Called when publishing is switched on and whenever the lane is
+// (re)initialised; never while a preview call may run.
+static int ensure_publish(rgpu_wavefront* w) {
+    if (!w->publish_every || !w->npixels) return RGPU_OK;
+    rgpu_ctx* ctx = w->ctx;
+    void* ptr;
+    int rc;
+    if (!w->pub_words) {
+        if ((rc = wf_alloc(w, 16, &ptr)) != RGPU_OK) return rc;
+        w->pub_words = (int*)ptr;
+    }
+    if (w->pub_pixels != w->npixels) {
+        for (auto it = w->owned.begin(); it != w->owned.end(); ++it)
+            if (*it == (void*)w->pub_image) {
+                (void)hipFree(*it);
+                w->owned.erase(it);
+                break;
+            }
+        w->pub_image = nullptr;
+        w->pub_pixels = 0;
+        if ((rc = wf_alloc(w, (size_t)w->npixels * sizeof(RGB16), &ptr)) != RGPU_OK) return rc;
+        w->pub_image = (RGB16*)ptr;
+        w->pub_pixels = w->npixels;
+    }
+    RGPU_HIP(ctx, hipMemsetAsync(w->pub_image, 0, (size_t)w->pub_pixels * sizeof(RGB16), ctx->stream));
+    RGPU_HIP(ctx, handoff_init(w->pub_words, ctx->stream));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    w->publish_tick = 0;
+    return RGPU_OK;
+}
+
 static int ensure_accum(rgpu_wavefront* w, uint32_t P) {
     if (P == w->npixels) return RGPU_OK;
     for (void* q : {(void*)w->st.accum, (void*)w->st.accum64, (void*)w->st.accum_count, (void*)w->tonemap_buf}) {
@@ -812,6 +851,7 @@ int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* cam, uint32_
     RGPU_HIP(ctx, wf_initialize(w->st, w->params, seed_offset, next_pixel, P, ctx->stream));
     RGPU_HIP(ctx, wf_clear_accum(w->st, P, w->params.accum64 != 0, ctx->stream));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((arc = ensure_publish(w)) != RGPU_OK) return arc;
     w->initialized = true;
     return RGPU_OK;
 }
@@ -822,7 +862,37 @@ int rgpu_wavefront_step(rgpu_wavefront* w, rgpu_stream stream, uint32_t steps) {
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = stream_of(ctx, stream);
-    for (uint32_t i = 0; i < steps; ++i) RGPU_HIP(ctx, step_once(w, s));
+    for (uint32_t i = 0; i < steps; ++i) {
+        RGPU_HIP(ctx, step_once(w, s));
+        // the end of a step publishes the intermediate if the preview side does not hold it
+        // (wavefront_path_tracing.hpp:877-898)
+        if (w->publish_every && w->pub_image && ++w->publish_tick % w->publish_every == 0)
+            RGPU_HIP(ctx, handoff_publish(w->st, w->npixels, w->params.accum64 != 0, w->pub_image, w->pub_words, s));
+    }
+    return RGPU_OK;
+}
+
+int rgpu_wavefront_set_publish(rgpu_wavefront* w, uint32_t every_steps) {
+    if (!w) return RGPU_ERR_INVALID_ARG;
+    if (w->is_batch()) return fail(w->ctx, RGPU_ERR_UNSUPPORTED, "previews cover single renders, not batches");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(w);  // a captured graph holds the previous publishing choice
+    w->publish_every = every_steps;
+    return ensure_publish(w);
+}
+
+int rgpu_wavefront_update_intermediate(rgpu_wavefront* w, rgpu_stream stream, rgpu_rgb16* dst, int* took) {
+    if (!w || !dst) return RGPU_ERR_INVALID_ARG;
+    if (!w->initialized) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_initialize was not called");
+    if (!w->publish_every || !w->pub_image)
+        return fail(w->ctx, RGPU_ERR_INVALID_ARG, "publishing is off (rgpu_wavefront_set_publish)");
+    if (((uintptr_t)dst | (uintptr_t)took) & 3) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "buffers need 4-byte alignment");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    RGPU_HIP(ctx, handoff_take(w->pub_image, reinterpret_cast<RGB16*>(dst), w->pub_pixels, w->pub_words, took,
+                               stream_of(ctx, stream)));
     return RGPU_OK;
 }
 
@@ -837,6 +907,9 @@ int rgpu_wavefront_graph_capture(rgpu_wavefront* w, uint32_t steps) {
     RGPU_HIP(ctx, hipStreamCreate(&cs));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     for (uint32_t i = 0; i < steps && e == hipSuccess; ++i) e = step_once(w, cs);
+    // with publishing on, a replay publishes once, after its last step
+    if (e == hipSuccess && w->publish_every && w->pub_image)
+        e = handoff_publish(w->st, w->npixels, w->params.accum64 != 0, w->pub_image, w->pub_words, cs);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e == hipSuccess) e = e2;
@@ -1239,6 +1312,7 @@ int rgpu_wavefront_resume(rgpu_wavefront* w, const void* host, uint64_t bytes) {
     }
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
     w->cam = h.cam;
+    if ((rc = ensure_publish(w)) != RGPU_OK) return rc;
     w->initialized = true;
     return RGPU_OK;
 }

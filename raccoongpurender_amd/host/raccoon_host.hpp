@@ -3,8 +3,10 @@
 //   SceneManager            common/scene_manager.hpp:330-650 (geometry, materials, envmap,
 //                           buildBVH, createBuffer/createMaterialBuffer/createEnvmapBuffer)
 //   WavefrontLane           common/wavefront_path_tracing.hpp:339-1153 (setup, initialize, step,
-//                           update_intermediate, re_start, step_count, stat_avg_sample)
-//   WavefrontPathTracing    common/wavefront_path_tracing.hpp:1159-1470 (launch_fixed, re_start,
+//                           update_intermediate, merge_from, finalize_color, re_start,
+//                           step_count, stat_avg_sample)
+//   WavefrontPathTracing    common/wavefront_path_tracing.hpp:1159-1470 (continuous rendering with
+//                           a step thread and an image thread, launch_fixed, re_start,
 //                           create_color_image, onColorRecieved) for one GPU
 //   StacklessBVH/ThreadedBVH  the two traversals of common/stackless_bvh.hpp and
 //                           common/threaded_bvh.hpp as GPU ray casters over one built tree
@@ -15,10 +17,14 @@
 
 #include <stdint.h>
 
+#include <atomic>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <mutex>
+#include <thread>
 #include <sstream>
 #include <memory>
 #include <stdexcept>
@@ -103,6 +109,46 @@ using MaterialBuffer = Handle<rgpu_materials, rgpu_materials_destroy>;
 using EnvmapBuffer = Handle<rgpu_envmap, rgpu_envmap_destroy>;
 using BVHHandle = Handle<rgpu_bvh, rgpu_bvh_destroy>;
 using WavefrontHandle = Handle<rgpu_wavefront, rgpu_wavefront_destroy>;
+
+// A second HIP stream of a context at the default priority (the reference's per-lane queues,
+// wavefront_path_tracing.hpp:346-348).
+class GpuStream {
+public:
+    explicit GpuStream(GpuContext& ctx) : ctx_(ctx) {
+        check(rgpu_stream_create(ctx.get(), &s_), ctx.get(), "rgpu_stream_create");
+    }
+    ~GpuStream() {
+        if (s_) rgpu_stream_destroy(ctx_.get(), s_);
+    }
+    GpuStream(const GpuStream&) = delete;
+    GpuStream& operator=(const GpuStream&) = delete;
+    rgpu_stream get() const { return s_; }
+    void synchronize() const { check(rgpu_synchronize(ctx_.get(), s_), ctx_.get(), "rgpu_synchronize"); }
+
+private:
+    GpuContext& ctx_;
+    rgpu_stream s_ = nullptr;
+};
+
+// Device memory of a context (an OpenCLBuffer).
+class DeviceMemory {
+public:
+    DeviceMemory(GpuContext& ctx, size_t bytes) : ctx_(ctx), bytes_(bytes) {
+        check(rgpu_alloc(ctx.get(), bytes, &p_), ctx.get(), "rgpu_alloc");
+    }
+    ~DeviceMemory() {
+        if (p_) rgpu_free(ctx_.get(), p_);
+    }
+    DeviceMemory(const DeviceMemory&) = delete;
+    DeviceMemory& operator=(const DeviceMemory&) = delete;
+    void* get() const { return p_; }
+    size_t bytes() const { return bytes_; }
+
+private:
+    GpuContext& ctx_;
+    size_t bytes_ = 0;
+    void* p_ = nullptr;
+};
 
 // 8-bit colour of the finalised image (RGBA8ValueType, wavefront_path_tracing.hpp:87-92).
 struct RGBA8ValueType {
@@ -348,11 +394,13 @@ struct LaneOptions {
     int variant = RGPU_VARIANT_STACKLESS;
 };
 
-// WavefrontLane (wavefront_path_tracing.hpp:339-1153) on one GPU.
+// WavefrontLane (wavefront_path_tracing.hpp:339-1153) on one GPU. Steps run on the context's
+// stream; the finalize side (update_intermediate, merge_from, finalize_color) runs on the lane's
+// own finalize stream, so an image thread can work beside a step thread.
 class WavefrontLane {
 public:
     WavefrontLane(GpuContext& lane, int wavefrontPathCount, LaneOptions options = LaneOptions())
-        : lane_(lane), path_count_(wavefrontPathCount), options_(options) {}
+        : lane_(lane), path_count_(wavefrontPathCount), options_(options), finalize_(lane) {}
 
     // setup(camera, sceneManager) (wavefront_path_tracing.hpp:451-530): device buffers of the
     // scene, materials and envmap, and the resident path state.
@@ -371,30 +419,92 @@ public:
     }
     // initialize(lane_index, num_device) (wavefront_path_tracing.hpp:532-570)
     void initialize(int lane_index, int num_device) {
+        std::lock_guard<std::mutex> g(finalize_mu_);
         check(rgpu_wavefront_initialize(wf_.get(), &camera_, (uint32_t)lane_index, (uint32_t)num_device),
               lane_.get(), "rgpu_wavefront_initialize");
         lane_index_ = lane_index;
         num_device_ = num_device;
         step_count_ = 0;
+        size_images();
     }
-    // step() (wavefront_path_tracing.hpp:591-939): one bounce of every resident path.
+    // Previews beside running steps: every step publishes the intermediate image for
+    // update_intermediate through the library's weak device mutex (kernels/mutex.cl).
+    void enable_previews(uint32_t every_steps = 1) {
+        std::lock_guard<std::mutex> g(finalize_mu_);
+        check(rgpu_wavefront_set_publish(wf_.get(), every_steps), lane_.get(), "rgpu_wavefront_set_publish");
+        previews_ = every_steps != 0;
+    }
+    // step() (wavefront_path_tracing.hpp:591-939): one bounce of every resident path. A pending
+    // re_start is applied first, on the stepping thread, as the reference does (592-604).
     void step() {
+        {
+            std::lock_guard<std::mutex> g(restart_mu_);
+            if (restart_pending_) {
+                restart_pending_ = false;
+                camera_ = restart_camera_;
+                initialize(restart_lane_, num_device_);
+            }
+        }
         check(rgpu_wavefront_step(wf_.get(), nullptr, 1), lane_.get(), "rgpu_wavefront_step");
-        ++step_count_;
+        step_count_.fetch_add(1);
     }
-    // update_intermediate() + the finalize pass: mean radiance as RGBA8 (accumlation.cl:6-25,68-79)
-    std::vector<RGBA8ValueType> update_intermediate() {
-        std::vector<RGBA8ValueType> img((size_t)camera_.width * camera_.height);
-        check(rgpu_wavefront_tonemap(wf_.get(), reinterpret_cast<uint8_t*>(img.data())), lane_.get(),
-              "rgpu_wavefront_tonemap");
+    // update_intermediate() (wavefront_path_tracing.hpp:942-960): the lane's intermediate image
+    // (RGB16 means and counts) on the finalize stream. With previews on, it is the image the
+    // last step published, copied if the step side does not hold the lock (otherwise the
+    // previous intermediate stays); without, it is made from the accumulation after the steps
+    // queued so far. Returns whether the intermediate was refreshed.
+    bool update_intermediate() {
+        std::lock_guard<std::mutex> g(finalize_mu_);
+        if (previews_) {
+            check(rgpu_wavefront_update_intermediate(wf_.get(), finalize_.get(), intermediate(),
+                                                     static_cast<int*>(took_->get())),
+                  lane_.get(), "rgpu_wavefront_update_intermediate");
+            int took = 0;
+            check(rgpu_download(lane_.get(), finalize_.get(), &took, took_->get(), sizeof(int)), lane_.get(),
+                  "rgpu_download");
+            return took != 0;
+        }
+        lane_.synchronize();  // the steps queued on the context's stream
+        check(rgpu_wavefront_intermediate(wf_.get(), finalize_.get(), intermediate()), lane_.get(),
+              "rgpu_wavefront_intermediate");
+        finalize_.synchronize();
+        return true;
+    }
+    // merge_from(other) (wavefront_path_tracing.hpp:962-991): this intermediate becomes the
+    // count-weighted mean of both lanes' (on one GPU, device to device).
+    void merge_from(WavefrontLane& other) {
+        std::lock_guard<std::mutex> g(finalize_mu_);
+        other.finalize_.synchronize();
+        check(rgpu_merge_intermediate(lane_.get(), finalize_.get(), intermediate(), other.intermediate(), npixels()),
+              lane_.get(), "rgpu_merge_intermediate");
+    }
+    // finalize_color() (wavefront_path_tracing.hpp:993-1007): tonemap of the intermediate, read back.
+    std::vector<RGBA8ValueType> finalize_color() {
+        std::lock_guard<std::mutex> g(finalize_mu_);
+        std::vector<RGBA8ValueType> img(npixels());
+        check(rgpu_tonemap_intermediate(lane_.get(), finalize_.get(), intermediate(), npixels(),
+                                        static_cast<uint8_t*>(final_color_->get())),
+              lane_.get(), "rgpu_tonemap_intermediate");
+        check(rgpu_download(lane_.get(), finalize_.get(), img.data(), final_color_->get(), img.size() * 4),
+              lane_.get(), "rgpu_download");
         return img;
     }
-    // re_start(lane_index, camera) (wavefront_path_tracing.hpp:1013-1018)
-    void re_start(int lane_index, const rgpu_camera& camera) {
-        camera_ = camera;
-        initialize(lane_index, num_device_);
+    // The intermediate on the host (12 bytes per pixel: half-float means and a 32-bit count).
+    std::vector<rgpu_rgb16> read_intermediate() {
+        std::lock_guard<std::mutex> g(finalize_mu_);
+        std::vector<rgpu_rgb16> out(npixels());
+        check(rgpu_download(lane_.get(), finalize_.get(), out.data(), intermediate(), out.size() * sizeof(rgpu_rgb16)),
+              lane_.get(), "rgpu_download");
+        return out;
     }
-    int step_count() const { return step_count_; }
+    // re_start(lane_index, camera) (wavefront_path_tracing.hpp:1013-1018): applied by the next step.
+    void re_start(int lane_index, const rgpu_camera& camera) {
+        std::lock_guard<std::mutex> g(restart_mu_);
+        restart_pending_ = true;
+        restart_lane_ = lane_index;
+        restart_camera_ = camera;
+    }
+    int step_count() const { return step_count_.load(); }
     // Snapshot of the lane that resume() continues exactly (no counterpart in the reference,
     // whose lane state lives only in device buffers).
     struct Checkpoint {
@@ -405,14 +515,16 @@ public:
     Checkpoint checkpoint() const {
         uint64_t n = 0;
         check(rgpu_wavefront_checkpoint_size(wf_.get(), &n), lane_.get(), "rgpu_wavefront_checkpoint_size");
-        Checkpoint c{camera_, step_count_, std::vector<uint8_t>(n)};
+        Checkpoint c{camera_, step_count_.load(), std::vector<uint8_t>(n)};
         check(rgpu_wavefront_checkpoint(wf_.get(), c.bytes.data(), n), lane_.get(), "rgpu_wavefront_checkpoint");
         return c;
     }
     void resume(const Checkpoint& c) {
+        std::lock_guard<std::mutex> g(finalize_mu_);
         check(rgpu_wavefront_resume(wf_.get(), c.bytes.data(), c.bytes.size()), lane_.get(), "rgpu_wavefront_resume");
         camera_ = c.camera;
         step_count_ = c.step_count;
+        size_images();
     }
     // stat_avg_sample(): samples per pixel so far (kernels/stat.cl)
     float stat_avg_sample() const {
@@ -422,22 +534,47 @@ public:
     }
     const rgpu_camera& camera() const { return camera_; }
     rgpu_wavefront* handle() const { return wf_.get(); }
+    uint32_t npixels() const { return camera_.width * camera_.height; }
 
 private:
+    rgpu_rgb16* intermediate() const { return static_cast<rgpu_rgb16*>(intermediate_->get()); }
+    // device images of the finalize side, sized to the camera
+    void size_images() {
+        const size_t P = npixels();
+        if (!intermediate_ || intermediate_->bytes() != P * sizeof(rgpu_rgb16)) {
+            intermediate_.reset(new DeviceMemory(lane_, P * sizeof(rgpu_rgb16)));
+            final_color_.reset(new DeviceMemory(lane_, P * 4));
+            check(rgpu_fill_u32(lane_.get(), finalize_.get(), intermediate_->get(), 0u, P * 3), lane_.get(),
+                  "rgpu_fill_u32");
+            finalize_.synchronize();
+        }
+        if (!took_) took_.reset(new DeviceMemory(lane_, sizeof(int)));
+    }
+
     GpuContext& lane_;
     int path_count_;
     LaneOptions options_;
     rgpu_camera camera_{};
-    int lane_index_ = 0, num_device_ = 1, step_count_ = 0;
+    int lane_index_ = 0, num_device_ = 1;
+    std::atomic<int> step_count_{0};
     SceneBuffer scene_;
     MaterialBuffer materials_;
     EnvmapBuffer envmap_;
     WavefrontHandle wf_;
+    GpuStream finalize_;
+    std::unique_ptr<DeviceMemory> intermediate_, final_color_, took_;
+    bool previews_ = false;
+    std::mutex finalize_mu_;  // the finalize side against (re)initialisation
+    std::mutex restart_mu_;
+    bool restart_pending_ = false;
+    int restart_lane_ = 0;
+    rgpu_camera restart_camera_{};
 };
 
 // WavefrontPathTracing (wavefront_path_tracing.hpp:1159-1470) on one GPU (RenderMode_SingleGPU).
 class WavefrontPathTracing {
 public:
+    // Fixed launches: the caller steps (launch_fixed) and asks for images (create_color_image).
     WavefrontPathTracing(const SceneManager& sceneManager, const rgpu_camera& camera,
                          std::function<void(RGBA8ValueType*, int, int)> colorRecieved, int wavefrontPathCount = 1 << 22,
                          int device_index = 0, LaneOptions options = LaneOptions())
@@ -446,27 +583,94 @@ public:
         lane_->setup(camera, sceneManager);
         lane_->initialize(0, 1);
     }
+    // Continuous rendering (wavefront_path_tracing.hpp:1286-1313): a worker thread steps the lane
+    // until stop() or destruction, and an image thread, every margin_period_ms once the lane has
+    // taken more than 4 steps, makes an image through the preview handoff (update_intermediate
+    // on its own stream while steps run) and passes it to onColorRecieved.
+    WavefrontPathTracing(const SceneManager& sceneManager, const rgpu_camera& camera, int margin_period_ms,
+                         std::function<void(RGBA8ValueType*, int, int)> colorRecieved, int wavefrontPathCount = 1 << 22,
+                         int device_index = 0, LaneOptions options = LaneOptions())
+        : onColorRecieved(std::move(colorRecieved)), context_(device_index) {
+        lane_.reset(new WavefrontLane(context_, wavefrontPathCount, options));
+        lane_->setup(camera, sceneManager);
+        lane_->initialize(0, 1);
+        lane_->enable_previews(1);
+        continue_ = true;
+        workers_.emplace_back([this]() {
+            guarded([this]() {
+                int queued = 0;
+                while (continue_) {
+                    lane_->step();
+                    // bound the work queued ahead of the GPU (the reference throttles on markers)
+                    if (++queued % 4 == 0) context_.synchronize();
+                }
+                context_.synchronize();
+            });
+        });
+        workers_.emplace_back([this, margin_period_ms]() {
+            guarded([this, margin_period_ms]() {
+                while (continue_) {
+                    if (4 < lane_->step_count()) create_color_image();
+                    std::this_thread::sleep_for(std::chrono::milliseconds(margin_period_ms));
+                }
+            });
+        });
+    }
+    ~WavefrontPathTracing() { stop(); }
+    // Ends continuous rendering (joins the threads); rethrows a worker's error.
+    void stop() {
+        continue_ = false;
+        for (std::thread& t : workers_)
+            if (t.joinable()) t.join();
+        workers_.clear();
+        std::lock_guard<std::mutex> g(error_mu_);
+        if (error_) {
+            std::exception_ptr e = error_;
+            error_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+    bool running() const { return continue_.load(); }
     // launch_fixed(steps) (wavefront_path_tracing.hpp:1443-1458)
     void launch_fixed(int steps) {
         for (int i = 0; i < steps; ++i) lane_->step();
         create_color_image();
     }
-    // re_start(camera) (wavefront_path_tracing.hpp:1460-1464)
+    // re_start(camera) (wavefront_path_tracing.hpp:1460-1464): the next step restarts the lane.
     void re_start(const rgpu_camera& camera) { lane_->re_start(0, camera); }
     // create_color_image() (wavefront_path_tracing.hpp:1345-1408): with one lane there is
     // nothing to merge; the finalised image goes to onColorRecieved.
     void create_color_image() {
-        std::vector<RGBA8ValueType> img = lane_->update_intermediate();
+        lane_->update_intermediate();
+        std::vector<RGBA8ValueType> img = lane_->finalize_color();
+        images_.fetch_add(1);
         if (onColorRecieved) onColorRecieved(img.data(), (int)lane_->camera().width, (int)lane_->camera().height);
     }
+    int images() const { return images_.load(); }
     WavefrontLane& lane() { return *lane_; }
     GpuContext& context() { return context_; }
 
     std::function<void(RGBA8ValueType*, int, int)> onColorRecieved;
 
 private:
+    template <class F>
+    void guarded(F f) {
+        try {
+            f();
+        } catch (...) {
+            std::lock_guard<std::mutex> g(error_mu_);
+            if (!error_) error_ = std::current_exception();
+            continue_ = false;
+        }
+    }
+
     GpuContext context_;
     std::unique_ptr<WavefrontLane> lane_;
+    std::atomic<bool> continue_{false};
+    std::atomic<int> images_{0};
+    std::vector<std::thread> workers_;
+    std::mutex error_mu_;
+    std::exception_ptr error_;
 };
 
 // A built tree cast against on the GPU with one of the two traversals: the stackless walk of

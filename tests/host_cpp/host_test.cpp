@@ -10,6 +10,9 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <chrono>
+#include <mutex>
+#include <thread>
 
 #include "../../raccoongpurender_amd/host/raccoon_host.hpp"
 
@@ -187,6 +190,42 @@ int main(int argc, char** argv) {
             if (lane.stat_avg_sample() != spp_a || lane.step_count() != ck.step_count + 3)
                 throw std::runtime_error("resumed lane diverged from the uninterrupted one");
         }
+        // continuous rendering (wavefront_path_tracing.hpp:1286-1313): a step thread and an image
+        // thread; previews come through the handoff while steps run, their sample totals never
+        // decrease, and stop() ends both threads
+        int live_images = 0, live_steps = 0;
+        bool live_monotonic = true, live_alpha = true;
+        uint64_t live_last_total = 0, live_final_total = 0, live_stat_total = 0;
+        {
+            std::mutex m;
+            rt::WavefrontPathTracing* self = nullptr;
+            rt::WavefrontPathTracing live(
+                s.sm, cam, 2,
+                [&](rt::RGBA8ValueType* px, int width, int height) {
+                    std::lock_guard<std::mutex> g(m);
+                    for (int i = 0; i < width * height; ++i) live_alpha = live_alpha && px[i].a == 255;
+                    if (!self) return;
+                    uint64_t total = 0;
+                    for (const rgpu_rgb16& v : self->lane().read_intermediate()) total += v.count;
+                    live_monotonic = live_monotonic && total >= live_last_total;
+                    live_last_total = total;
+                },
+                64 * 48 * 4, 0, opt);
+            {
+                std::lock_guard<std::mutex> g(m);
+                self = &live;
+            }
+            const auto t0 = std::chrono::steady_clock::now();
+            while (live.images() < 8 && std::chrono::steady_clock::now() - t0 < std::chrono::seconds(30))
+                std::this_thread::sleep_for(std::chrono::milliseconds(5));
+            live.stop();
+            live_images = live.images();
+            live_steps = live.lane().step_count();
+            // with the step thread gone nothing contends: the update takes the published image
+            if (!live.lane().update_intermediate()) throw std::runtime_error("an uncontended preview did not take the lock");
+            for (const rgpu_rgb16& v : live.lane().read_intermediate()) live_final_total += v.count;
+            live_stat_total = (uint64_t)((double)live.lane().stat_avg_sample() * 64 * 48 + 0.5);
+        }
         // ray casts through both BVH walks
         std::vector<rgpu_ray> rays;
         for (int i = 0; i < 1000; ++i) {
@@ -230,8 +269,12 @@ int main(int argc, char** argv) {
              std::vector<int32_t>((const int32_t*)hs.data(), (const int32_t*)hs.data() + 4 * hs.size()));
         dump(dir + "/hits_threaded.i32",
              std::vector<int32_t>((const int32_t*)ht.data(), (const int32_t*)ht.data() + 4 * ht.size()));
-        std::printf("{\"width\": %d, \"height\": %d, \"spp\": %.6f, \"steps\": %d, \"nodes\": %u, \"paths\": %d}\n", w, h,
-                    spp, steps, st.nodes, 64 * 48 * 4);
+        std::printf("{\"width\": %d, \"height\": %d, \"spp\": %.6f, \"steps\": %d, \"nodes\": %u, \"paths\": %d, "
+                    "\"live_images\": %d, \"live_steps\": %d, \"live_monotonic\": %s, \"live_alpha\": %s, "
+                    "\"live_last_total\": %llu, \"live_final_total\": %llu, \"live_stat_total\": %llu}\n",
+                    w, h, spp, steps, st.nodes, 64 * 48 * 4, live_images, live_steps, live_monotonic ? "true" : "false",
+                    live_alpha ? "true" : "false", (unsigned long long)live_last_total,
+                    (unsigned long long)live_final_total, (unsigned long long)live_stat_total);
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "host_test: %s\n", e.what());

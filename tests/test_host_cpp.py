@@ -65,6 +65,11 @@ def test_host_surface_renders_like_the_binding(ctx, tmp_path):
     wf.step(12)
     img = wf.tonemap().reshape(-1).astype(np.int32)
     assert abs(wf.stat() / (w * h) - out["spp"]) < 1e-6 and out["steps"] == 12
+    # continuous rendering: images arrived while the step thread ran, their totals never fell,
+    # and after stop() the preview holds a published snapshot no newer than the lane
+    assert out["live_images"] >= 8 and out["live_steps"] > 4
+    assert out["live_monotonic"] and out["live_alpha"]
+    assert 0 < out["live_last_total"] <= out["live_final_total"] <= out["live_stat_total"]
     for name in ("image.u8", "image_again.u8"):
         host = _load(d, name, np.uint8).astype(np.int32)
         assert host.size == img.size and np.abs(host - img).max() <= 1, name
