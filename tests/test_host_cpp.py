@@ -125,3 +125,22 @@ def test_obj_loader_rejects_bad_records(tmp_path, bad):
     assert out == {"error": -1}
     with pytest.raises(ValueError):
         scenes.load_obj(bad)
+
+
+REF_SCENES = "/root/reference/scenes"
+
+
+@needs_bin
+@pytest.mark.parametrize("name,ntris", [("bunny", 34916), ("dragon", 100000)])
+def test_obj_readers_agree_on_the_reference_meshes(name, ntris):
+    """The reference's own meshes (scenes/bunny.obj, scenes/dragon.obj), read by the C++ reader
+    (rt::load_obj) and the Python one (scenes.load_obj): the same points bit for bit and the same
+    triangles. Runs where the reference's scene files are present."""
+    path = os.path.join(REF_SCENES, name + ".obj")
+    if not os.path.exists(path):
+        pytest.skip("the reference's scene files are not on this machine")
+    out = json.loads(subprocess.run([BIN, "obj", path], check=True, capture_output=True, text=True).stdout)
+    pts, tri = scenes.load_obj(path)
+    assert len(tri) == ntris
+    assert np.array_equal(np.asarray(out["points_bits"], np.uint32), pts.reshape(-1).view(np.uint32))
+    assert np.array_equal(np.asarray(out["indices"], np.uint32), tri.reshape(-1))
