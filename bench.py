@@ -129,6 +129,16 @@ def render_bench(steps=32, warmup=16, paths=1 << 22, spp=64, graph_steps=8):
     print(f"[bench] render: {ms:.3f} ms/step, {(s1 - s0) / sec / 1e6:.2f} Msamples/s, "
           f"{paths * steps / sec / 1e6:.0f} Mrays/s, launch {'/'.join(str(x) for x in launch)}",
           file=sys.stderr, flush=True)
+    # where a bounce's time goes: each stage timed in-band and priced with algorithmic bytes
+    from raccoongpurender_amd import stage_profile
+
+    prof = stage_profile.profile_lane(ctx, wf, wf._keep[1], 16, launch[0], launch[1], launch[2], paths)
+    out["roofline"] = {"bound": "hbm", "peak": stage_profile.HBM_PEAK_GBS, "unit": "GB/s",
+                       "ms_per_bounce": prof["ms_per_bounce"], "stages": prof["stages"],
+                       "traversal": prof["traversal"]}
+    for name, row in prof["stages"].items():
+        print(f"[bench] render stage {name:18s} {row['ms']:7.4f} ms  {row.get('gbs') or 0:7.1f} GB/s", file=sys.stderr,
+              flush=True)
     wf.close()
     # the complete 64-spp image: a one-job batch with a sample budget, bounce loop from a graph
     bvh, scene, mats, env = wf._keep

@@ -309,6 +309,13 @@ int rgpu_wavefront_initialize(rgpu_wavefront* w, const rgpu_camera* camera, uint
 int rgpu_wavefront_step(rgpu_wavefront* w, rgpu_stream stream, uint32_t steps);
 /* Runs `steps` steps between two events on the context stream; mean ms per step. */
 int rgpu_wavefront_time_steps(rgpu_wavefront* w, uint32_t steps, double* ms_per_step);
+/* Runs `steps` steps (<= 4096) with HIP events between their stages on the context stream and
+ * returns the mean ms per step of each of the RGPU_STAGES stages, in the order of
+ * WavefrontLane::step (wavefront_path_tracing.hpp:606-875): new path; guide inference (0 when no
+ * guide samples); lambertian; Ward; mirror and dielectric; medium boundary and inside; extension
+ * ray cast with the medium free flight; logic; queue scan and scatter. Synchronous. */
+#define RGPU_STAGES 9
+int rgpu_wavefront_time_stages(rgpu_wavefront* w, uint32_t steps, double* ms_per_stage);
 /* Queue lengths after the last step (new path, lambertian, specular, dielectric, ward,
  * volume boundary, volume inside) and the pixel cursor next_pixel_index. Synchronous. */
 int rgpu_wavefront_queue_counts(rgpu_wavefront* w, uint32_t* counts7, uint64_t* next_pixel);

@@ -458,6 +458,10 @@ _sig("rgpu_wavefront_resume", C.c_int, vp, vp, C.c_uint64)
 EXPORTED += ["rgpu_bvh_build_device", "rgpu_wavefront_intermediate", "rgpu_merge_intermediate",
              "rgpu_tonemap_intermediate"]
 EXPORTED += ["rgpu_wavefront_checkpoint_size", "rgpu_wavefront_checkpoint", "rgpu_wavefront_resume"]
+_sig("rgpu_wavefront_time_stages", C.c_int, vp, u32, C.POINTER(C.c_double))
+EXPORTED += ["rgpu_wavefront_time_stages"]
+STAGE_NAMES = ("new_path", "guide_infer", "lambertian", "ward", "mirror_dielectric", "medium", "ray_cast", "logic",
+               "queues")
 _sig("rgpu_stream_create", C.c_int, vp, C.POINTER(vp))
 _sig("rgpu_stream_destroy", C.c_int, vp, vp)
 _sig("rgpu_wavefront_set_publish", C.c_int, vp, u32)
@@ -718,6 +722,12 @@ class Wavefront:
         out = dev if dev is not None else self.ctx.alloc(max(self.npixels, 1) * 12)
         self.ctx.check(lib.rgpu_wavefront_intermediate(self.h, None, out.ptr), "rgpu_wavefront_intermediate")
         return out if dev is not None else out.download(RGB16_DTYPE, self.npixels)
+
+    def time_stages(self, steps):
+        """Mean ms per step of each stage of the bounce (STAGE_NAMES), HIP-event timed."""
+        out = (C.c_double * len(STAGE_NAMES))()
+        self.ctx.check(lib.rgpu_wavefront_time_stages(self.h, steps, out), "rgpu_wavefront_time_stages")
+        return dict(zip(STAGE_NAMES, (float(x) for x in out)))
 
     def set_publish(self, every_steps=1):
         """Publish the intermediate image for previews every `every_steps` steps (0: off)."""

@@ -32,10 +32,13 @@ constexpr int kPX = kGIn + 8;              // LDS pitch of the input tile (u16),
 constexpr int kPH = kGHid + 8;             // LDS pitch of hidden tiles
 constexpr int kPF = kGOut + 4;             // LDS pitch of the fp32 output tile
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-    return (uint16_t)(u >> 16);
+// float -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// two floats -> packed bf16 pair, a in the low half
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -56,49 +59,131 @@ __device__ __forceinline__ f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
 }
 
 // Forward layer for this wave's 16 rows: out = act(X W^T + b), X: LDS [rows][K] pitch px.
-template <int K>
-__device__ void fwd_layer(const uint16_t* X, int px, const uint16_t* W, const float* bias, uint16_t* H,
+// The product is formed transposed (A: weights, B: rows), so that each lane ends with four
+// consecutive outputs of one row and writes them with one 8-byte (bf16) or 16-byte (fp32)
+// store.
+// W: rows of K weights at pitch wp (global memory or LDS). NT: 16-column output tiles formed
+// (the last layer's outputs 48..63 feed nothing; inference skips them).
+template <int K, int NT = 4>
+__device__ void fwd_layer(const uint16_t* X, int px, const uint16_t* W, int wp, const float* bias, uint16_t* H,
                           float* F, int r0, int lane, bool relu) {
-    for (int nt = 0; nt < 4; ++nt) {
+    const int row = r0 + (lane & 15);
+    bf16x8 x[K / 32];  // this wave's rows, read once for all output tiles
+#pragma unroll
+    for (int kb = 0; kb < K / 32; ++kb) x[kb] = ld8(X + row * px + kb * 32 + (lane >> 4) * 8);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
         for (int kb = 0; kb < K / 32; ++kb) {
-            bf16x8 a = ld8(X + (r0 + (lane & 15)) * px + kb * 32 + (lane >> 4) * 8);
-            bf16x8 b = ld8(W + (nt * 16 + (lane & 15)) * K + kb * 32 + (lane >> 4) * 8);
-            acc = mma(a, b, acc);
+            const bf16x8 w = ld8(W + (nt * 16 + (lane & 15)) * wp + kb * 32 + (lane >> 4) * 8);
+            acc = mma(w, x[kb], acc);
         }
-        const int col = nt * 16 + (lane & 15);
-        const float bb = bias[col];
-        for (int r = 0; r < 4; ++r) {
-            const int row = r0 + (lane >> 4) * 4 + r;
-            float v = acc[r] + bb;
-            if (F) F[row * kPF + col] = v;
-            else H[row * kPH + col] = f2bf(relu ? fmaxf(v, 0.0f) : v);
+        const int col = nt * 16 + (lane >> 4) * 4;
+        const float4 bb = *reinterpret_cast<const float4*>(bias + col);
+        float v0 = acc[0] + bb.x, v1 = acc[1] + bb.y, v2 = acc[2] + bb.z, v3 = acc[3] + bb.w;
+        if (F) {
+            *reinterpret_cast<float4*>(F + row * kPF + col) = make_float4(v0, v1, v2, v3);
+        } else {
+            if (relu) {
+                v0 = fmaxf(v0, 0.0f);
+                v1 = fmaxf(v1, 0.0f);
+                v2 = fmaxf(v2, 0.0f);
+                v3 = fmaxf(v3, 0.0f);
+            }
+            *reinterpret_cast<uint2*>(H + row * kPH + col) = make_uint2(f2bf2(v0, v1), f2bf2(v2, v3));
         }
     }
 }
 
+// sin(pi x) and cos(pi x) for x in [0, 2]: x = r + q/2 with q = rint(2x) (exact, |r| <= 1/4),
+// Taylor series of sin(pi r) to r^9 and cos(pi r) to r^10 (truncation below 3e-9), then the
+// quadrant q mod 4.
+__device__ __forceinline__ void sincospi_02(float x, float* s, float* c) {
+    const float q = rintf(2.0f * x);
+    const float r = x - 0.5f * q;
+    const float z = r * r;
+    float ps = __builtin_fmaf(z, 0.0821458866f, -0.599264529f);  // pi^9/9!, -pi^7/7!
+    ps = __builtin_fmaf(z, ps, 2.55016404f);                       // pi^5/5!
+    ps = __builtin_fmaf(z, ps, -5.16771278f);                      // -pi^3/3!
+    ps = __builtin_fmaf(z, ps, 3.14159265f);                       // pi
+    const float sr = r * ps;
+    float pc = __builtin_fmaf(z, -0.0258068914f, 0.235330630f);  // -pi^10/10!, pi^8/8!
+    pc = __builtin_fmaf(z, pc, -1.33526277f);                      // -pi^6/6!
+    pc = __builtin_fmaf(z, pc, 4.05871213f);                       // pi^4/4!
+    pc = __builtin_fmaf(z, pc, -4.93480220f);                      // -pi^2/2!
+    const float cr = __builtin_fmaf(z, pc, 1.0f);
+    const int qi = (int)q;
+    const float a = (qi & 1) ? cr : sr;
+    const float b = (qi & 1) ? sr : cr;
+    *s = (qi & 2) ? -a : a;
+    *c = ((qi + 1) & 2) ? -b : b;
+}
+
 // Encoded input features of one row (written as bf16 into X): for band k and axis a,
-// sin and cos of 2^k pi p_a. 2^k p_a is exact in fp32 and so is its reduction modulo 2,
-// which leaves sinpi/cospi a small argument. Then the normal, the outgoing direction and
-// the scene class one-hot (class code c = 1..kGClasses sets feature 77 + c).
+// features 6k + 2a and 6k + 2a + 1 are sin and cos of 2^k pi p_a. 2^k p_a is exact in fp32 and
+// so is its reduction modulo 2, which leaves sincospi_02 an argument in [0, 2). Then the normal,
+// the outgoing direction and the scene class one-hot (class code c = 1..kGClasses sets
+// feature 77 + c). Features 2j and 2j + 1 form pair j; each of the row's four lanes writes nine
+// sin/cos pairs and three plain pairs, one 4-byte store per pair.
 __device__ void encode_row(uint16_t* X, const float* p01, const float* n, const float* wo, uint32_t cls,
                            int part) {
-    // part 0..3: each of four lanes writes 24 of the 96 features
-    for (int f = part * 24; f < part * 24 + 24; ++f) {
-        float v;
-        if (f < 72) {
-            const int k = f / 6, a = (f % 6) / 2;
-            float x = ldexpf(p01[a], k);
-            x = x - 2.0f * floorf(0.5f * x);
-            v = (f & 1) ? cospif(x) : sinpif(x);
-        } else if (f < 75) {
-            v = n[f - 72];
-        } else if (f < 78) {
-            v = wo[f - 75];
-        } else {
-            v = (uint32_t)(f - 77) == cls ? 1.0f : 0.0f;
+    uint32_t* X2 = reinterpret_cast<uint32_t*>(X);
+    // The 36 sin/cos pairs form 12 chains (axis a, bands k0..k0+2, k0 = 0, 3, 6, 9); chain
+    // c = part + 4 m (m = 0..2) belongs to this lane, with axis c mod 3 = (part + m) mod 3 and
+    // k0 = 3 (c / 3). The first band of a chain is evaluated, the next two follow by the double
+    // angle formulas sin 2t = 2 sin t cos t, cos 2t = (cos t - sin t)(cos t + sin t), which add
+    // a few float ulps, far below bf16 resolution.
+    const int rot = part == 3 ? 0 : part;
+    float v0 = rot == 0 ? p01[0] : rot == 1 ? p01[1] : p01[2];
+    float v1 = rot == 0 ? p01[1] : rot == 1 ? p01[2] : p01[0];
+    float v2 = rot == 0 ? p01[2] : rot == 1 ? p01[0] : p01[1];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const int c = part + 4 * m;
+        const int k0 = c - c % 3;  // 3 (c / 3)
+        const int a = (part + m) % 3;
+        float x = ldexpf(v0, k0);
+        x = x - 2.0f * floorf(0.5f * x);
+        float sv, cv;
+        sincospi_02(x, &sv, &cv);
+        int j = k0 * 3 + a;  // pair of band k0, axis a
+        X2[j] = f2bf2(sv, cv);
+#pragma unroll
+        for (int d = 1; d < 3; ++d) {
+            const float s2 = (sv + sv) * cv;
+            cv = (cv - sv) * (cv + sv);
+            sv = s2;
+            j += 3;
+            X2[j] = f2bf2(sv, cv);
         }
-        X[f] = f2bf(v);
+        const float t = v0;  // next chain: next axis
+        v0 = v1;
+        v1 = v2;
+        v2 = t;
+    }
+    // pairs 36..47: features 72..95. Pair 36 + part: (n0, n1), (n2, wo0), (wo1, wo2) or the
+    // class features 78, 79; pairs 40 + part and 44 + part: class features 80 + 2 part ...
+    float lo, hi;
+    if (part == 0) {
+        lo = n[0];
+        hi = n[1];
+    } else if (part == 1) {
+        lo = n[2];
+        hi = wo[0];
+    } else if (part == 2) {
+        lo = wo[1];
+        hi = wo[2];
+    } else {
+        lo = cls == 1u ? 1.0f : 0.0f;
+        hi = cls == 2u ? 1.0f : 0.0f;
+    }
+    X2[36 + part] = f2bf2(lo, hi);
+#pragma unroll
+    for (int i = 10; i < 12; ++i) {
+        const int j = part + 4 * i;
+        const uint32_t c = (uint32_t)(2 * j - 77);  // class code of the pair's first feature
+        X2[j] = f2bf2(c == cls ? 1.0f : 0.0f, c + 1u == cls ? 1.0f : 0.0f);
     }
 }
 
@@ -114,7 +199,11 @@ __device__ __forceinline__ uint32_t sample_class(float c) {
 // outputs do not overflow.
 constexpr float kMinMean = 1e-15f;
 
-// Unit mean of raw outputs o[0..2] into mu; returns the length of o (0: no direction).
+// Unit mean of raw outputs o[0..2] into mu; returns the length of o (0: no direction). The
+// head's reciprocals, square roots, exponentials and logarithms use the hardware's
+// single-instruction forms (v_rcp_f32, v_rsq_f32, v_exp_f32, v_log_f32: about one ulp), since
+// the lobes only need to be consistent between sampling and evaluation, which both read the
+// stored records.
 __device__ __forceinline__ float lobe_mean(const float* o, float* mu) {
     const float mx = fmaxf(fmaxf(fabsf(o[0]), fabsf(o[1])), fabsf(o[2]));
     if (!(mx > kMinMean) || !(mx <= 3.0e38f)) {
@@ -123,22 +212,26 @@ __device__ __forceinline__ float lobe_mean(const float* o, float* mu) {
         mu[2] = 1.0f;
         return 0.0f;
     }
-    const float q0 = o[0] / mx, q1 = o[1] / mx, q2 = o[2] / mx;
-    const float len = sqrtf(q0 * q0 + q1 * q1 + q2 * q2);
-    mu[0] = q0 / len;
-    mu[1] = q1 / len;
-    mu[2] = q2 / len;
-    return len * mx;
+    const float inv = __builtin_amdgcn_rcpf(mx);
+    const float q0 = o[0] * inv, q1 = o[1] * inv, q2 = o[2] * inv;
+    const float len2 = q0 * q0 + q1 * q1 + q2 * q2;  // in [1, 3]
+    const float il = __builtin_amdgcn_rsqf(len2);
+    mu[0] = q0 * il;
+    mu[1] = q1 * il;
+    mu[2] = q2 * il;
+    return (len2 * il) * mx;
 }
 
-// Lobe k of one row from its raw outputs o[0..4] and the row's softmax constants.
-__device__ __forceinline__ void lobe_params(const float* o, float mx, float lz, float lkmax, float* L) {
-    const float kap = expf(fminf(fmaxf(o[3], kGLogKappaMin), lkmax));
-    const float lw = o[4] - mx - lz;
+// Lobe k of one row from its raw outputs o[0..4], the row's log-sum-exp shift (max + log of the
+// sum) and the lobe's softmax weight w. log C(kappa) = log kappa - log 2 pi - log(1 - e^-2kappa),
+// with log kappa the clamped raw output itself.
+__device__ __forceinline__ void lobe_params(const float* o, float shift, float w, float lkmax, float* L) {
+    const float lk = fminf(fmaxf(o[3], kGLogKappaMin), lkmax);
+    const float kap = __expf(lk);
     lobe_mean(o, L);
     L[3] = kap;
-    L[4] = expf(lw);
-    L[5] = lw + guide_log_norm(kap);
+    L[4] = w;
+    L[5] = (o[4] - shift) + ((lk - 1.83787706641f) - __logf(1.0f - __expf(-2.0f * kap)));
 }
 
 // The mixture head runs on four lanes per row (lanes 4r..4r+3 of a workgroup hold row r),
@@ -157,9 +250,11 @@ __device__ void head_lobes_part(const float* out, int part, float lkmax, float* 
     const float* o0 = out + 5 * (2 * part);
     const float* o1 = o0 + 5;
     const float mx = quad_max(fmaxf(o0[4], o1[4]));
-    const float lz = logf(quad_sum(expf(o0[4] - mx) + expf(o1[4] - mx)));
-    lobe_params(o0, mx, lz, lkmax, L);
-    lobe_params(o1, mx, lz, lkmax, L + kGLobeFloats);
+    const float e0 = __expf(o0[4] - mx), e1 = __expf(o1[4] - mx);
+    const float sum = quad_sum(e0 + e1);  // >= 1: the largest term is e^0
+    const float shift = mx + __logf(sum), is = __builtin_amdgcn_rcpf(sum);
+    lobe_params(o0, shift, e0 * is, lkmax, L);
+    lobe_params(o1, shift, e1 * is, lkmax, L + kGLobeFloats);
 }
 
 // d(-weight * log q(wi)) / d(raw outputs) for this lane's two lobes: d[j] is the derivative
@@ -193,7 +288,7 @@ __device__ void head_grad_part(const float* out, int part, const float* wi, floa
 }
 
 // Shared forward pass of a tile: X filled by the caller; leaves H1..H3 (bf16) and F (fp32 outputs).
-struct TileLds {
+struct alignas(16) TileLds {
     uint16_t X[kTile * kPX];
     uint16_t H1[kTile * kPH];
     uint16_t H2[kTile * kPH];
@@ -202,85 +297,118 @@ struct TileLds {
 };
 
 __device__ void tile_forward(TileLds& t, const DevGuide& g, int r0, int lane) {
-    fwd_layer<kGIn>(t.X, kPX, g.w_bf + kGoW0, g.params + kGoB0, t.H1, nullptr, r0, lane, true);
-    fwd_layer<kGHid>(t.H1, kPH, g.w_bf + kGoW1, g.params + kGoB1, t.H2, nullptr, r0, lane, true);
-    fwd_layer<kGHid>(t.H2, kPH, g.w_bf + kGoW2, g.params + kGoB2, t.H3, nullptr, r0, lane, true);
-    fwd_layer<kGHid>(t.H3, kPH, g.w_bf + kGoW3, g.params + kGoB3, nullptr, t.F, r0, lane, false);
+    fwd_layer<kGIn>(t.X, kPX, g.w_bf + kGoW0, kGIn, g.params + kGoB0, t.H1, nullptr, r0, lane, true);
+    fwd_layer<kGHid>(t.H1, kPH, g.w_bf + kGoW1, kGHid, g.params + kGoB1, t.H2, nullptr, r0, lane, true);
+    fwd_layer<kGHid>(t.H2, kPH, g.w_bf + kGoW2, kGHid, g.params + kGoB2, t.H3, nullptr, r0, lane, true);
+    fwd_layer<kGHid>(t.H3, kPH, g.w_bf + kGoW3, kGHid, g.params + kGoB3, nullptr, t.F, r0, lane, false);
 }
 
-// Inference keeps no activations for a backward pass, so two buffers suffice: each wave works
-// on its own 16 rows, layer l reads one buffer and writes the other (A: input, H2 and the fp32
-// output; B: H1 and H3). 13 KB instead of 29 KB per workgroup.
-constexpr int kInferA = (kTile * kPX * 2 > kTile * kPF * 4 ? kTile * kPX * 2 : kTile * kPF * 4) / 2;
-struct InferLds {
-    uint16_t A[kInferA];
-    uint16_t B[kTile * kPH];
+// Inference runs one workgroup of kInferWaves waves per CU. The workgroup copies the bf16
+// weights and the biases into LDS once (rows padded to the activation pitches), so that every
+// layer reads both MFMA operands from LDS; each wave then works through 16-row blocks of the
+// queues on its own: two private activation buffers (A: input, H2 and the fp32 outputs; B: H1
+// and H3) and no workgroup barrier inside the loop.
+constexpr int kInferWaves = 16;
+constexpr int kInferWaveA = (16 * kPX * 2 > 16 * kPF * 4 ? 16 * kPX * 2 : 16 * kPF * 4) / 2;
+struct alignas(16) InferWaveLds {
+    uint16_t A[kInferWaveA];
+    uint16_t B[16 * kPH];
 };
+struct alignas(16) InferLds {
+    uint16_t W0[kGHid * kPX];
+    uint16_t W1[kGHid * kPH];
+    uint16_t W2[kGHid * kPH];
+    uint16_t W3[kGOut * kPH];
+    float bias[4][kGHid];
+    InferWaveLds wave[kInferWaves];
+};
+static_assert(sizeof(InferLds) <= 160 * 1024, "inference LDS exceeds a CU's 160 KB");
 
-// Every thread of the workgroup must call it. Each wave owns its 16 rows of the bf16 tiles, but
-// the fp32 output tile has a wider row pitch: wave 0's output rows cover bytes of A that hold wave
-// 1's layer-3 inputs, and wave 1's layer-2 outputs cover bytes of wave 0's output rows. The
-// barrier keeps every bf16 use of A before any output is written.
-__device__ void infer_forward(InferLds& t, const DevGuide& g, int r0, int lane) {
-    float* F = reinterpret_cast<float*>(t.A);
-    fwd_layer<kGIn>(t.A, kPX, g.w_bf + kGoW0, g.params + kGoB0, t.B, nullptr, r0, lane, true);
-    fwd_layer<kGHid>(t.B, kPH, g.w_bf + kGoW1, g.params + kGoB1, t.A, nullptr, r0, lane, true);
-    fwd_layer<kGHid>(t.A, kPH, g.w_bf + kGoW2, g.params + kGoB2, t.B, nullptr, r0, lane, true);
-    __syncthreads();
-    fwd_layer<kGHid>(t.B, kPH, g.w_bf + kGoW3, g.params + kGoB3, nullptr, F, r0, lane, false);
+// Orders this wave's LDS accesses across lanes: a wave's LDS operations execute in program
+// order, so the fences only stop the compiler from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inference over the lambertian and ward queues: lobe parameters for every guided vertex.
-__global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, DevGuide g, DevJobs jobs) {
-    __shared__ InferLds t;
+// Copies `rows` rows of `cols` bf16 weights (cols a multiple of 8) to LDS at pitch `pitch`.
+__device__ __forceinline__ void stage_weights(const uint16_t* src, int rows, int cols, uint16_t* dst, int pitch) {
+    const int chunks = cols / 8;
+    for (int i = threadIdx.x; i < rows * chunks; i += blockDim.x) {
+        const int r = i / chunks, c = i - r * chunks;
+        *reinterpret_cast<uint4*>(dst + r * pitch + 8 * c) = *reinterpret_cast<const uint4*>(src + r * cols + 8 * c);
+    }
+}
+
+// The inference loop of one wave over its 16-row blocks of the lambertian and ward queues (the
+// workgroup has staged the weights). kJobs: the vertex's guide frame comes from its job's
+// table entry, else from the guide (a single render). Two-stage software pipeline, unrolled
+// twice so that the two vertex register sets alternate without copies: while block b is
+// encoded and run through the network, the vertex records of block b + stride and the queue
+// entries of block b + 2 stride are in flight. Every load is issued unconditionally (positions
+// past the queue end read its last entry; those rows run but are not stored), so that no
+// branch makes the compiler wait for the loads in flight.
+template <bool kJobs>
+__device__ void infer_blocks(InferLds& t, const DevState& s, const WFParams& p, const DevGuide& g,
+                             const DevJobs& jobs) {
     const uint32_t c1 = s.counts[kMatLambertian], c4 = s.counts[kMatWard];
     const uint32_t rows = c1 + c4;
+    const uint32_t nblocks = (rows + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r0 = wave * 16;
-    for (uint32_t base = blockIdx.x * kTile; base < rows; base += gridDim.x * kTile) {
-        // encode: 4 threads per row
+    const int row = lane >> 2, part = lane & 3;  // encode and head: 4 lanes per row
+    uint16_t* A = t.wave[wave].A;
+    uint16_t* B = t.wave[wave].B;
+    float* F = reinterpret_cast<float*>(A);
+    const uint32_t stride = gridDim.x * kInferWaves;
+    uint32_t blk = blockIdx.x * kInferWaves + wave;
+    if (blk >= nblocks) return;
+    auto slot_at = [&](uint32_t b) -> uint32_t {
+        const uint32_t i = min(b * 16 + row, rows - 1);
+        return s.items[i < c1 ? (size_t)kMatLambertian * p.n + i : (size_t)kMatWard * p.n + (i - c1)];
+    };
+    struct Vtx {
+        TraceRay r;
+        ExtRec e;
+        uint32_t job;
+    };
+    auto fetch = [&](uint32_t slot, Vtx& v) {
+        v.r = s.ray[slot];
+        v.e = s.ext[slot];
+        if (kJobs) v.job = jobs.job_of_block[slot / kJobBlock];
+    };
+    auto process = [&](uint32_t b, uint32_t slot, const Vtx& v) {
         {
-            const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
-            const uint32_t i = base + row;
-            float p01[3] = {0, 0, 0}, n[3] = {0, 0, 0}, wo[3] = {0, 0, 0};
-            uint32_t cls = 0;
-            if (i < rows) {
-                const uint32_t slot = i < c1 ? s.items[(size_t)kMatLambertian * p.n + i]
-                                             : s.items[(size_t)kMatWard * p.n + (i - c1)];
-                const TraceRay r = s.ray[slot];
-                const ExtRec e = s.ext[slot];
-                const GuideFrame fr = jobs.njobs ? jobs.jobs[jobs.job_of_block[slot / kJobBlock]].gframe
-                                                 : GuideFrame{{g.lo[0], g.lo[1], g.lo[2]},
-                                                              {g.inv_ext[0], g.inv_ext[1], g.inv_ext[2]}, 0u};
-                cls = fr.cls;
-                for (int a = 0; a < 3; ++a) {
-                    float P = r.ro[a] + r.rd[a] * e.tmin;
-                    p01[a] = (P - fr.lo[a]) * fr.inv_ext[a];
-                    wo[a] = -r.rd[a];
-                }
-                n[0] = e.ngx;
-                n[1] = e.ngy;
-                n[2] = e.ngz;
-                if (n[0] * wo[0] + n[1] * wo[1] + n[2] * wo[2] < 0.0f) {
-                    n[0] = -n[0];
-                    n[1] = -n[1];
-                    n[2] = -n[2];
-                }
+            const GuideFrame fr = kJobs ? jobs.jobs[v.job].gframe
+                                        : GuideFrame{{g.lo[0], g.lo[1], g.lo[2]},
+                                                     {g.inv_ext[0], g.inv_ext[1], g.inv_ext[2]}, 0u};
+            float p01[3], n[3], wo[3];
+            for (int a = 0; a < 3; ++a) {
+                float P = v.r.ro[a] + v.r.rd[a] * v.e.tmin;
+                p01[a] = (P - fr.lo[a]) * fr.inv_ext[a];
+                wo[a] = -v.r.rd[a];
             }
-            encode_row(t.A + row * kPX, p01, n, wo, cls, part);
+            // the normal flipped toward wo
+            const float sg = v.e.ngx * wo[0] + v.e.ngy * wo[1] + v.e.ngz * wo[2] < 0.0f ? -1.0f : 1.0f;
+            n[0] = sg * v.e.ngx;
+            n[1] = sg * v.e.ngy;
+            n[2] = sg * v.e.ngz;
+            encode_row(A + row * kPX, p01, n, wo, fr.cls, part);
         }
-        __syncthreads();
-        infer_forward(t, g, r0, lane);
-        __syncthreads();
+        wave_lds_sync();
+        fwd_layer<kGIn>(A, kPX, t.W0, kPX, t.bias[0], B, nullptr, 0, lane, true);
+        wave_lds_sync();
+        fwd_layer<kGHid>(B, kPH, t.W1, kPH, t.bias[1], A, nullptr, 0, lane, true);
+        wave_lds_sync();
+        fwd_layer<kGHid>(A, kPH, t.W2, kPH, t.bias[2], B, nullptr, 0, lane, true);
+        wave_lds_sync();
+        fwd_layer<kGHid, (5 * kGLobes + 15) / 16>(B, kPH, t.W3, kPH, t.bias[3], nullptr, F, 0, lane, false);
+        wave_lds_sync();
         {
-            const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
-            const uint32_t i = base + row;
             float L[2 * kGLobeFloats];
             // all four lanes of a row take part
-            head_lobes_part(reinterpret_cast<const float*>(t.A) + row * kPF, part, g.log_kappa_max, L);
-            if (i < rows) {
-                const uint32_t slot = i < c1 ? s.items[(size_t)kMatLambertian * p.n + i]
-                                             : s.items[(size_t)kMatWard * p.n + (i - c1)];
+            head_lobes_part(F + row * kPF, part, g.log_kappa_max, L);
+            if (b * 16 + row < rows) {
                 float4* dst =
                     reinterpret_cast<float4*>(g.lobes + (size_t)slot * kGSlotFloats + 2 * kGLobeFloats * part);
                 dst[0] = make_float4(L[0], L[1], L[2], L[3]);
@@ -288,8 +416,43 @@ __global__ void __launch_bounds__(128) k_guide_infer(DevState s, WFParams p, Dev
                 dst[2] = make_float4(L[8], L[9], L[10], L[11]);
             }
         }
-        __syncthreads();
+        wave_lds_sync();  // the next block's encode overwrites the outputs
+    };
+    Vtx va, vb;
+    uint32_t sa = slot_at(blk);
+    fetch(sa, va);
+    uint32_t sb = slot_at(blk + stride);
+    while (true) {
+        fetch(sb, vb);
+        const uint32_t sa2 = slot_at(blk + 2 * stride);
+        process(blk, sa, va);
+        blk += stride;
+        if (blk >= nblocks) break;
+        fetch(sa2, va);
+        const uint32_t sb2 = slot_at(blk + 2 * stride);
+        process(blk, sb, vb);
+        blk += stride;
+        if (blk >= nblocks) break;
+        sa = sa2;
+        sb = sb2;
     }
+}
+
+// Inference over the lambertian and ward queues: lobe parameters for every guided vertex.
+__global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevState s, WFParams p, DevGuide g, DevJobs jobs) {
+    __shared__ InferLds t;
+    stage_weights(g.w_bf + kGoW0, kGHid, kGIn, t.W0, kPX);
+    stage_weights(g.w_bf + kGoW1, kGHid, kGHid, t.W1, kPH);
+    stage_weights(g.w_bf + kGoW2, kGHid, kGHid, t.W2, kPH);
+    stage_weights(g.w_bf + kGoW3, kGOut, kGHid, t.W3, kPH);
+    if (threadIdx.x < 4 * kGHid) {
+        const int l = threadIdx.x / kGHid, c = threadIdx.x % kGHid;
+        const int off = l == 0 ? kGoB0 : l == 1 ? kGoB1 : l == 2 ? kGoB2 : kGoB3;
+        t.bias[l][c] = g.params[off + c];
+    }
+    __syncthreads();
+    if (jobs.njobs) infer_blocks<true>(t, s, p, g, jobs);
+    else infer_blocks<false>(t, s, p, g, jobs);
 }
 
 // Back-propagation through one layer for this wave's 16 rows:
@@ -530,8 +693,8 @@ __global__ void __launch_bounds__(256) k_adam(DevGuide g, uint32_t groups, float
 
 hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, const DevJobs& jobs,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_infer, dim3(g.infer_blocks ? g.infer_blocks : 2048u), dim3(128), 0, s, st, p, g,
-                       jobs);
+    hipLaunchKernelGGL(k_guide_infer, dim3(g.infer_blocks ? g.infer_blocks : 256u), dim3(64 * kInferWaves), 0, s, st,
+                       p, g, jobs);
     return hipGetLastError();
 }
 

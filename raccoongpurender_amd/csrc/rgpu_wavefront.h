@@ -171,9 +171,12 @@ hipError_t wf_initialize_jobs(const DevState& st, const WFParams& p, const DevJo
 // One bounce. persistent_blocks: grid of the queue-driven kernels; trace_block: workgroup
 // size of the extension ray cast (64, 128 or 256; 0 = persistent waves with lane refill). jobs: null for a single render (scene,
 // materials, envmap and camera as given), else the batch's job table (those are ignored).
+// marks (optional): kStageMarks events recorded between the stages of the bounce, in the order
+// of rgpu_wavefront_time_stages.
+constexpr int kStageMarks = 10;
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
                    const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
-                   int trace_block, hipStream_t s, const DevJobs* jobs = nullptr);
+                   int trace_block, hipStream_t s, const DevJobs* jobs = nullptr, hipEvent_t* marks = nullptr);
 // Path-guiding network (guide.hip).
 hipError_t guide_infer(const DevState& st, const WFParams& p, const DevGuide& g, const DevJobs& jobs,
                        hipStream_t s);

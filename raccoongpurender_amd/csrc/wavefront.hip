@@ -1455,25 +1455,38 @@ hipError_t wf_clear_accum(const DevState& st, uint32_t npixels, bool accum64, hi
 namespace {
 template <class J>
 hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const DevGuide& g, int variant,
-                     int persistent_blocks, int trace_block, hipStream_t s) {
+                     int persistent_blocks, int trace_block, hipStream_t s, hipEvent_t* marks) {
     const dim3 qgrid(persistent_blocks > 0 ? persistent_blocks : 1024), blk(256);
     const dim3 full((p.n + 255) / 256);
     const uint32_t tb = (trace_block == 64 || trace_block == 128) ? (uint32_t)trace_block : 256u;
     const dim3 tgrid((p.n + tb - 1) / tb), tblk(tb);
+    // optional stage marks: marks[k] is recorded before stage k (kStageMarks - 1 stages)
+    auto mark = [&](int k) {
+        if (marks) (void)hipEventRecord(marks[k], s);
+    };
+    mark(0);
     hipLaunchKernelGGL(k_new_path<J>, qgrid, blk, 0, s, st, p, jobs, g);
+    mark(1);
     if (g.enabled) {
         hipError_t e = guide_infer(st, p, g, jobs.table(), s);
         if (e != hipSuccess) return e;
+        mark(2);
         hipLaunchKernelGGL((k_shade_lambertian<true, J>), qgrid, blk, 0, s, st, p, jobs, g);
+        mark(3);
         hipLaunchKernelGGL((k_shade_ward<true, J>), qgrid, blk, 0, s, st, p, jobs, g);
     } else {
+        mark(2);
         hipLaunchKernelGGL((k_shade_lambertian<false, J>), qgrid, blk, 0, s, st, p, jobs, g);
+        mark(3);
         hipLaunchKernelGGL((k_shade_ward<false, J>), qgrid, blk, 0, s, st, p, jobs, g);
     }
+    mark(4);
     hipLaunchKernelGGL(k_shade_specular, qgrid, blk, 0, s, st, p);
     hipLaunchKernelGGL(k_shade_dielectric<J>, qgrid, blk, 0, s, st, p, jobs);
+    mark(5);
     hipLaunchKernelGGL(k_shade_volume_surface<J>, qgrid, blk, 0, s, st, p, jobs);
     hipLaunchKernelGGL(k_shade_volume_inside<J>, qgrid, blk, 0, s, st, p, jobs);
+    mark(6);
     // trace_block 0: a single render casts through the persistent ray-cast kernel; a batch (several
     // scenes) keeps one thread per slot (tgrid is 256 wide then)
     if (!J::kBatch && trace_block == 0) {
@@ -1491,19 +1504,22 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
         hipLaunchKernelGGL((k_trace_step<kVariantStackless, J>), tgrid, tblk, (size_t)top * 64u, s, st, p, jobs,
                            top);
     }
+    mark(7);
     hipLaunchKernelGGL(k_logic<J>, full, blk, 0, s, st, p, jobs, g);
+    mark(8);
     hipLaunchKernelGGL(k_scan_queues, dim3(kNumQueues + 1), dim3(1024), 0, s, st,
                        g.train ? g.sample_count : (uint32_t*)nullptr, g.capacity);
     hipLaunchKernelGGL(k_scatter_queues, full, blk, 0, s, st, p, g, jobs.table());
+    mark(9);
     return hipGetLastError();
 }
 }  // namespace
 
 hipError_t wf_step(const DevState& st, const WFParams& p, const TraceScene& sc, const DevMaterials& m,
                    const DevEnvmap& env, const Camera& cam, const DevGuide& g, int variant, int persistent_blocks,
-                   int trace_block, hipStream_t s, const DevJobs* jobs) {
-    if (jobs) return step_impl(st, p, ManyJobs{*jobs}, g, variant, persistent_blocks, trace_block, s);
-    return step_impl(st, p, OneJob{sc, m, env, cam}, g, variant, persistent_blocks, trace_block, s);
+                   int trace_block, hipStream_t s, const DevJobs* jobs, hipEvent_t* marks) {
+    if (jobs) return step_impl(st, p, ManyJobs{*jobs}, g, variant, persistent_blocks, trace_block, s, marks);
+    return step_impl(st, p, OneJob{sc, m, env, cam}, g, variant, persistent_blocks, trace_block, s, marks);
 }
 
 }  // namespace rgpu

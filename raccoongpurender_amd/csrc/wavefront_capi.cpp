@@ -136,12 +136,12 @@ int wf_alloc(rgpu_wavefront* w, size_t bytes, void** out) {
     return RGPU_OK;
 }
 
-hipError_t step_once(rgpu_wavefront* w, hipStream_t s) {
+hipError_t step_once(rgpu_wavefront* w, hipStream_t s, hipEvent_t* marks = nullptr) {
     if (w->is_batch())
         return wf_step(w->st, w->params, TraceScene{}, DevMaterials{}, DevEnvmap{}, Camera{}, w->guide, w->variant,
-                       w->queue_blocks, w->trace_block, s, &w->djobs);
+                       w->queue_blocks, w->trace_block, s, &w->djobs, marks);
     return wf_step(w->st, w->params, w->scene->dev, w->mats->dev, w->env->dev, w->cam, w->guide, w->variant,
-                   w->queue_blocks, w->trace_block, s);
+                   w->queue_blocks, w->trace_block, s, nullptr, marks);
 }
 
 Camera to_camera(const rgpu_camera* cam) {
@@ -1012,6 +1012,30 @@ int rgpu_wavefront_time_steps(rgpu_wavefront* w, uint32_t steps, double* ms_per_
     return RGPU_OK;
 }
 
+int rgpu_wavefront_time_stages(rgpu_wavefront* w, uint32_t steps, double* ms_per_stage) {
+    if (!w || !ms_per_stage || steps == 0 || steps > 4096) return RGPU_ERR_INVALID_ARG;
+    if (!w->initialized) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "rgpu_wavefront_initialize was not called");
+    rgpu_ctx* ctx = w->ctx;
+    RGPU_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<hipEvent_t> ev((size_t)steps * kStageMarks, nullptr);
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < ev.size() && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+    for (uint32_t k = 0; k < steps && e == hipSuccess; ++k) e = step_once(w, ctx->stream, ev.data() + (size_t)k * kStageMarks);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    std::vector<double> sum(kStageMarks - 1, 0.0);
+    for (uint32_t k = 0; k < steps && e == hipSuccess; ++k)
+        for (int j = 0; j + 1 < kStageMarks && e == hipSuccess; ++j) {
+            float ms = 0.0f;
+            e = hipEventElapsedTime(&ms, ev[(size_t)k * kStageMarks + j], ev[(size_t)k * kStageMarks + j + 1]);
+            sum[j] += ms;
+        }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rgpu_wavefront_time_stages");
+    for (int j = 0; j + 1 < kStageMarks; ++j) ms_per_stage[j] = sum[j] / steps;
+    return RGPU_OK;
+}
+
 int rgpu_wavefront_queue_counts(rgpu_wavefront* w, uint32_t* counts7, uint64_t* next_pixel) {
     if (!w || !counts7) return RGPU_ERR_INVALID_ARG;
     rgpu_ctx* ctx = w->ctx;
@@ -1535,7 +1559,7 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
     d.nslots = (uint32_t)n;
     // eight inference workgroups per CU (13 KB of LDS and 111 VGPRs per lane allow 8), each
     // looping over tiles
-    d.infer_blocks = (uint32_t)(ctx->cu_count > 0 ? 8 * ctx->cu_count : 2048);
+    d.infer_blocks = (uint32_t)(ctx->cu_count > 0 ? ctx->cu_count : 256);  // one 16-wave workgroup per CU
     d.enabled = sampling ? 1u : 0u;
     d.train = training ? 1u : 0u;
     w->guide = d;

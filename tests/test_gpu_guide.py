@@ -304,3 +304,49 @@ def test_guide_lifetime_and_detach_reattach(ctx):
     b.close()
     wf.close()  # destroying the lane releases its hold
     c.close()
+
+
+def test_inference_lobes_match_the_forward_pass(ctx):
+    # The lobe records the bounce's inference writes for its lambertian and Ward vertices (weights
+    # staged in LDS, chained sin/cos encoding, one-instruction head) against the tested forward
+    # pass on the same vertices, encoded here from the lane's state, and the head in float64.
+    sd = R.material_test_scene(48, 32, volume=True)
+    lo = (sd["points"].min(0) - 1e-3).astype(np.float32)
+    hi = (sd["points"].max(0) + 1e-3).astype(np.float32)
+    wf = build_lane(ctx, sd, 48 * 32 * 16, clamp=1e30, seed=9)
+    gd = G.Guide(ctx, lo, hi, capacity=1 << 15, minibatch=4096, seed=2)
+    G.attach(wf, gd, sampling=True, training=True)
+    for _ in range(3):  # a trained network, not the initial one
+        wf.step(4)
+        G.train_pass(wf, 1 << 15)
+    wf.step(1)
+    st = wf.state()
+    counts, _ = wf.queue_counts()
+    slots = np.concatenate([st["items"][1, :counts[1]], st["items"][4, :counts[4]]]).astype(np.int64)
+    assert len(slots) > 1000
+    wf.step(1)  # its inference runs over these two queues
+    lob = G.slot_lobes(wf, 0, wf.n)[slots].astype(np.float64)
+    ray, ext = st["rays"][slots], st["ext"][slots]
+    inv = np.float32(1.0) / (hi - lo)
+    s = np.zeros(len(slots), G.SAMPLE_DTYPE)
+    P = ray["ro"] + ray["rd"] * ext["tmin"][:, None]
+    s["p"] = (P - lo) * inv
+    wo = -ray["rd"]
+    ng = ext["Ng"]
+    d = ng[:, 0] * wo[:, 0] + ng[:, 1] * wo[:, 1] + ng[:, 2] * wo[:, 2]
+    s["n"] = np.where((d < 0)[:, None], -ng, ng)
+    s["wo"] = wo
+    out = gd.forward(s).astype(np.float64)
+    mu, kappa, w = G.raw_to_lobes(out)
+    lw = np.log(w)
+    logwc = lw + np.log(kappa) - math.log(2.0 * math.pi) - np.log1p(-np.exp(-2.0 * kappa))
+    err = {"mu": np.abs(lob[:, :, :3] - mu).max(), "kappa": (np.abs(lob[:, :, 3] - kappa) / kappa).max(),
+           "w": (np.abs(lob[:, :, 4] - w) / (w + 1e-7)).max(),
+           "log(w C)": np.abs(lob[:, :, 5] - logwc).max()}
+    print(f"{len(slots)} vertices: " + ", ".join(f"{k} {v:.3g}" for k, v in err.items()), flush=True)
+    # hardware rcp/rsq/exp/log, about one float ulp each; exp's argument is scaled by log2(e) in
+    # float, which adds |x| * 6e-8 relative for weights far below the largest (w ~ e^-30: 2e-6)
+    assert err["mu"] < 2e-6 and err["kappa"] < 2e-6 and err["w"] < 2e-5 and err["log(w C)"] < 2e-5
+    G.attach(wf, None)
+    wf.close()
+    gd.close()

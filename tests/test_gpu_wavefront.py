@@ -62,6 +62,26 @@ def test_all_materials_with_consistent_normals(ctx):
     _compare(ctx, sd, 32 * 24 * 3, 10, consistent=True)
 
 
+def test_stage_timing_runs_the_same_bounce(ctx):
+    # rgpu_wavefront_time_stages brackets each stage of the bounce with HIP events; the bounces
+    # it runs are the plain step's, bit for bit against the oracle, and the stages that had work
+    # report time.
+    sd = R.material_test_scene(48, 32, volume=True)
+    n = 48 * 32 * 4
+    g = W.gpu_lane(ctx, sd, n)
+    o = W.oracle_lane(sd, n)
+    for s in range(4):
+        t = g.time_stages(1)
+        o.step()
+        assert set(t) == set(_lib.STAGE_NAMES)
+        assert all(v >= 0.0 for v in t.values()) and t["ray_cast"] > 0 and t["logic"] > 0, t
+        mm = W.state_mismatches(g.state(), o.state())
+        assert all(v == 0 for v in mm.values()), (s, mm)
+    cg, pg = g.queue_counts()
+    co, po = o.queue_counts()
+    assert np.array_equal(cg, co) and pg == po
+
+
 def test_options_depth_clamp_selection(ctx):
     sd = R.material_test_scene(32, 24)
     _compare(ctx, sd, 32 * 24 * 2, 8, max_depth=3, clamp=50.0, bxdf_p=0.3, rr_start=2, seed=1234)
