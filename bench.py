@@ -365,7 +365,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level (8 = 1,310,720 triangles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--variant", choices=["auto", "stackless", "threaded"], default="auto")
+    ap.add_argument("--variant", choices=["auto", "stackless", "threaded", "threaded_single"], default="auto")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run the library's copy kernel on MiB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -422,7 +422,7 @@ def main():
     hits = ctx.alloc(n * 16)
 
     results = {}
-    variants = ["stackless", "threaded"] if args.variant == "auto" else [args.variant]
+    variants = ["stackless", "threaded", "threaded_single"] if args.variant == "auto" else [args.variant]
     # incoherent rays: the persistent kernel (block 0: waves refill idle lanes from their ray
     # chunk); coherent primary rays: one thread per ray (block 256), which keeps neighbouring
     # pixels in one wave for the whole traversal

@@ -37,7 +37,9 @@ extern "C" {
 
 /* traversal variants */
 #define RGPU_VARIANT_STACKLESS 0 /* parent-pointer BVH2, bitstack (kernels/extension_ray_cast_stackless.cl) */
-#define RGPU_VARIANT_THREADED 1  /* skip-pointer BVH2, 6 direction tables (kernels/extension_ray_cast.cl) */
+#define RGPU_VARIANT_THREADED 1  /* skip-pointer BVH2, a table per ray-direction class (kernels/extension_ray_cast.cl) */
+#define RGPU_VARIANT_THREADED_SINGLE 2 /* skip-pointer BVH2, one table in the build's child order
+                                        * (TBVHNode / intersect_tbvh, common/threaded_bvh.hpp:269-396) */
 
 typedef struct rgpu_ctx rgpu_ctx;
 typedef struct rgpu_bvh rgpu_bvh;
@@ -139,14 +141,16 @@ int rgpu_bvh_export(const rgpu_bvh* bvh, float* bounds, int32_t* left, int32_t* 
 
 /* ---------------------------------------------------------------- scene (device)
  * Replaces SceneManager::createBuffer (common/scene_manager.hpp:579-591) and the
- * SceneBuffer it fills (common/scene_manager.hpp:284-294): packs the stackless and
- * threaded node records and the primitive records and uploads them with the
+ * SceneBuffer it fills (common/scene_manager.hpp:284-294): packs the stackless node
+ * records, the threaded node tables (one per ray-direction class and one for the single-table
+ * walk, each in its own pre-order) and the primitive records, and uploads them with the
  * vertices. Host arrays are copied; the caller keeps ownership. */
 int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, uint32_t stride, uint32_t npoints,
                       const uint32_t* indices, uint32_t ntris, const float* spheres, uint32_t nspheres,
                       rgpu_scene** out);
 int rgpu_scene_destroy(rgpu_scene* scene);
-/* Device bytes held by the scene per table: [stackless nodes, threaded tables, primitive records, vertices]. */
+/* Device bytes held by the scene per table: [stackless nodes, the seven threaded tables,
+ * primitive records, vertices]. */
 int rgpu_scene_bytes(const rgpu_scene* scene, uint64_t* out4);
 
 /* ---------------------------------------------------------------- extension ray cast

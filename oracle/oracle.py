@@ -50,6 +50,7 @@ _sig("orc_stackless_build", C.c_int, u32, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("orc_stackless_intersect", C.c_int, vp, vp, vp, u32, vp, C.c_int32, vp, u32, vp, vp, vp, C.c_int)
 _sig("orc_mtbvh_build", C.c_int, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("orc_mtbvh_intersect", C.c_int, vp, vp, u32, vp, vp, u32, vp, C.c_int32, vp, u32, vp, vp, vp, C.c_int)
+_sig("orc_tbvh_intersect", C.c_int, vp, vp, u32, vp, vp, u32, vp, C.c_int32, vp, u32, vp, vp, vp, C.c_int)
 _sig("orc_bruteforce_intersect", C.c_int, u32, vp, u32, vp, C.c_int32, vp, u32, vp, vp, C.c_int)
 _sig("orc_hit_normals", None, u32, vp, vp, vp, u32, vp, C.c_int32, vp, vp)
 
@@ -146,13 +147,15 @@ class RefStackless:
 
 
 class RefMTBVH:
-    """Reference threaded layout: MTBVHNode + links[6][node][hit, miss] (common/threaded_bvh.hpp:398-525)."""
+    """Reference threaded layout: MTBVHNode + links[order][node][hit, miss]: orders 0..5 the
+    multi-threaded BVH's direction classes (common/threaded_bvh.hpp:398-525), order 6 the
+    single-table threaded BVH (common/threaded_bvh.hpp:269-364)."""
 
     def __init__(self, tree):
         nn = len(tree["left"])
         self.nn = nn
         self.nodes = np.zeros(nn, REF_MT_NODE)
-        self.links = np.zeros((6, nn, 2), np.int32)
+        self.links = np.zeros((7, nn, 2), np.int32)
         self.prim_ids = np.zeros(max(len(tree["prim_ids"]), 1), np.uint32)
         args = [np.ascontiguousarray(tree[k]) for k in ("left", "right", "bounds", "prim_beg", "prim_cnt", "prim_ids")]
         r = lib.orc_mtbvh_build(nn, *[p(a) for a in args], p(self.nodes), p(self.links), p(self.prim_ids))
@@ -185,6 +188,17 @@ def intersect_mtbvh(ref, points, indices, rays, spheres=None, nthreads=NTHREADS)
     cnt = np.zeros(5, np.uint64)
     lib.orc_mtbvh_intersect(p(ref.nodes), p(ref.links), ref.nn, p(ref.prim_ids), p(points), points.shape[1],
                             p(indices), ntris, p(spheres), len(rays), p(rays), p(hits), p(cnt), nthreads)
+    return hits, dict(zip(COUNTER_NAMES, (int(x) for x in cnt)))
+
+
+def intersect_tbvh(ref, points, indices, rays, spheres=None, nthreads=NTHREADS):
+    """The single-table threaded walk (common/threaded_bvh.hpp:366-395) over ref.links[6]."""
+    points, indices, ntris, spheres = _geom(points, indices, spheres)
+    rays = np.ascontiguousarray(rays)
+    hits = np.zeros(len(rays), HIT_DTYPE)
+    cnt = np.zeros(5, np.uint64)
+    lib.orc_tbvh_intersect(p(ref.nodes), p(ref.links), ref.nn, p(ref.prim_ids), p(points), points.shape[1],
+                           p(indices), ntris, p(spheres), len(rays), p(rays), p(hits), p(cnt), nthreads)
     return hits, dict(zip(COUNTER_NAMES, (int(x) for x in cnt)))
 
 
@@ -402,7 +416,7 @@ class OracleWavefront:
         self.keep = []
         d = orc_wf_desc()
         d.n, d.rr_start, d.max_depth, d.quirks = path_count, rr_start, max_depth, 1 if quirks else 0
-        d.variant = 1 if variant == "threaded" else 0
+        d.variant = {"stackless": 0, "threaded": 1, "threaded_single": 2}[variant]
         d.nthreads, d.clamp, d.bxdf_p, d.seed = nthreads, clamp, bxdf_p, seed
         d.points, d.stride, d.indices = p(self.points), self.points.shape[1], p(self.indices)
         d.sphere_begin, d.spheres = self.ntris, p(self.spheres)

@@ -375,24 +375,35 @@ void link_direction(const MutTree& m, int dir, int32_t node, int32_t sibling, in
     }
 }
 
-// common/threaded_bvh.hpp:527-569 (intersect_mtbvh) and kernels/extension_ray_cast.cl:44-94,
-// with the incoming far clip honoured (the GPU stackless kernel reads it from ExtensionResult.tmin).
-Hit intersect_mtbvh(const RefMTBVHNode* nodes, const int32_t* links, uint32_t nn, const uint32_t* prim_ids,
-                    const Scene& sc, const Ray& ray, Counters* c) {
+// common/threaded_bvh.hpp:269-364 (TBVHNode, allocateThreadedBVH, linkThreadedBVH): the single
+// threaded table over the build's own child order, stored as the seventh link table. The hit
+// link is always the array neighbour (-1 past the end); a branch's miss link is its sibling (L
+// or root) or its parent's sibling (R); a leaf's miss link equals its hit link.
+void link_single(const Tree& t, int32_t node, int32_t sibling, int32_t parent_sibling,
+                 const std::vector<uint32_t>& index, std::vector<int32_t>* links, uint32_t nn) {
+    int32_t* hm = links->data() + (size_t)6 * nn * 2;
+    uint32_t me = index[node];
+    uint32_t neighbor = me + 1;
+    hm[2 * me + 0] = neighbor < nn ? (int32_t)neighbor : -1;
+    if (t.left[node] >= 0) {
+        if (sibling >= 0) hm[2 * me + 1] = (int32_t)index[sibling];
+        else hm[2 * me + 1] = parent_sibling >= 0 ? (int32_t)index[parent_sibling] : -1;
+        link_single(t, t.left[node], t.right[node], t.right[node], index, links, nn);
+        link_single(t, t.right[node], -1, parent_sibling, index, links, nn);
+    } else {
+        hm[2 * me + 1] = hm[2 * me + 0];
+    }
+}
+
+// The walk of intersect_mtbvh and intersect_tbvh over one link table hm.
+Hit walk_links(const RefMTBVHNode* nodes, const int32_t* hm, const uint32_t* prim_ids, const Scene& sc,
+               const Ray& ray, Counters* c) {
     V3 ro = {ray.ro[0], ray.ro[1], ray.ro[2]};
     V3 rd = {ray.rd[0], ray.rd[1], ray.rd[2]};
-    float ax = std::fabs(rd.x), ay = std::fabs(rd.y), az = std::fabs(rd.z);
-    float maxYZ = fmx(ay, az);
-    float maxXZ = fmx(ax, az);
-    int dir;
-    if (maxYZ < ax) dir = 0.0f < rd.x ? 0 : 1;
-    else if (maxXZ < ay) dir = 0.0f < rd.y ? 2 : 3;
-    else dir = 0.0f < rd.z ? 4 : 5;
     V3 inv = {1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
     float tmin = ray.tmax;
     float u = 0.0f, v = 0.0f;
     int32_t prim = -1;
-    const int32_t* hm = links + (size_t)dir * nn * 2;
     int32_t node = 0;
     while (0 <= node) {
         c->node_visits++;
@@ -423,6 +434,27 @@ Hit intersect_mtbvh(const RefMTBVHNode* nodes, const int32_t* links, uint32_t nn
     h.u = prim >= 0 ? u : 0.0f;
     h.v = prim >= 0 ? v : 0.0f;
     return h;
+}
+
+// common/threaded_bvh.hpp:527-569 (intersect_mtbvh) and kernels/extension_ray_cast.cl:44-94,
+// with the incoming far clip honoured (the GPU stackless kernel reads it from ExtensionResult.tmin).
+Hit intersect_mtbvh(const RefMTBVHNode* nodes, const int32_t* links, uint32_t nn, const uint32_t* prim_ids,
+                    const Scene& sc, const Ray& ray, Counters* c) {
+    float ax = std::fabs(ray.rd[0]), ay = std::fabs(ray.rd[1]), az = std::fabs(ray.rd[2]);
+    float maxYZ = fmx(ay, az);
+    float maxXZ = fmx(ax, az);
+    int dir;
+    if (maxYZ < ax) dir = 0.0f < ray.rd[0] ? 0 : 1;
+    else if (maxXZ < ay) dir = 0.0f < ray.rd[1] ? 2 : 3;
+    else dir = 0.0f < ray.rd[2] ? 4 : 5;
+    return walk_links(nodes, links + (size_t)dir * nn * 2, prim_ids, sc, ray, c);
+}
+
+// common/threaded_bvh.hpp:366-395 (intersect_tbvh): the same walk over the single table, for
+// every ray, with the incoming far clip honoured.
+Hit intersect_tbvh(const RefMTBVHNode* nodes, const int32_t* links, uint32_t nn, const uint32_t* prim_ids,
+                   const Scene& sc, const Ray& ray, Counters* c) {
+    return walk_links(nodes, links + (size_t)6 * nn * 2, prim_ids, sc, ray, c);
 }
 
 Hit intersect_brute(uint32_t nprims, const Scene& sc, const Ray& ray, Counters* c) {
@@ -596,7 +628,8 @@ int orc_stackless_intersect(const void* nodes, const uint32_t* prim_ids, const f
 }
 
 // ---- reference threaded (MTBVH) layout from a generic tree.
-// out_nodes: nn * 48 bytes (RefMTBVHNode); out_links: 6 * nn * 2 int32; out_prim_ids: leaf slots.
+// out_nodes: nn * 48 bytes (RefMTBVHNode); out_links: 7 * nn * 2 int32 (the six direction
+// classes, then the single threaded table); out_prim_ids: leaf slots.
 int orc_mtbvh_build(uint32_t nn, const int32_t* left, const int32_t* right, const float* bounds,
                     const uint32_t* prim_beg, const uint32_t* prim_cnt, const uint32_t* prim_ids,
                     void* out_nodes, int32_t* out_links, uint32_t* out_prim_ids) {
@@ -636,11 +669,12 @@ int orc_mtbvh_build(uint32_t nn, const int32_t* left, const int32_t* right, cons
     m.t = &t;
     m.L.assign(left, left + nn);
     m.R.assign(right, right + nn);
-    std::vector<int32_t> links((size_t)6 * nn * 2, -1);
+    std::vector<int32_t> links((size_t)7 * nn * 2, -1);
     for (int dir = 0; dir < 6; ++dir) {
         sort_for_direction(&m, 0, dir);
         link_direction(m, dir, 0, -1, -1, index, &links, nn);
     }
+    link_single(t, 0, -1, -1, index, &links, nn);
     memcpy(out_nodes, nodes.data(), nn * sizeof(RefMTBVHNode));
     memcpy(out_links, links.data(), links.size() * sizeof(int32_t));
     memcpy(out_prim_ids, pids.data(), pids.size() * sizeof(uint32_t));
@@ -659,6 +693,24 @@ int orc_mtbvh_intersect(const void* nodes, const int32_t* links, uint32_t nn, co
     parallel_rays(nrays, nthreads, [&](uint32_t b, uint32_t e, int t) {
         Counters c = {};
         for (uint32_t i = b; i < e; ++i) h[i] = intersect_mtbvh(nd, links, nn, prim_ids, sc, r[i], &c);
+        per[t] = c;
+    });
+    sum_counters(per, counters);
+    return 0;
+}
+
+int orc_tbvh_intersect(const void* nodes, const int32_t* links, uint32_t nn, const uint32_t* prim_ids,
+                       const float* points, uint32_t stride, const uint32_t* indices, int32_t sphere_begin,
+                       const float* spheres, uint32_t nrays, const void* rays, void* hits, uint64_t* counters,
+                       int nthreads) {
+    Scene sc{points, stride, indices, sphere_begin, spheres};
+    const RefMTBVHNode* nd = (const RefMTBVHNode*)nodes;
+    const Ray* r = (const Ray*)rays;
+    Hit* h = (Hit*)hits;
+    std::vector<Counters> per(nthreads > 0 ? nthreads : 1, Counters{});
+    parallel_rays(nrays, nthreads, [&](uint32_t b, uint32_t e, int t) {
+        Counters c = {};
+        for (uint32_t i = b; i < e; ++i) h[i] = intersect_tbvh(nd, links, nn, prim_ids, sc, r[i], &c);
         per[t] = c;
     });
     sum_counters(per, counters);

@@ -28,6 +28,10 @@ extern "C" int orc_mtbvh_intersect(const void* nodes, const int32_t* links, uint
                                    const float* points, uint32_t stride, const uint32_t* indices, int32_t sphere_begin,
                                    const float* spheres, uint32_t nrays, const void* rays, void* hits,
                                    uint64_t* counters, int nthreads);
+extern "C" int orc_tbvh_intersect(const void* nodes, const int32_t* links, uint32_t nn, const uint32_t* prim_ids,
+                                  const float* points, uint32_t stride, const uint32_t* indices, int32_t sphere_begin,
+                                  const float* spheres, uint32_t nrays, const void* rays, void* hits,
+                                  uint64_t* counters, int nthreads);
 extern "C" void orc_hit_normals(uint32_t nhits, const void* rays, const void* hits, const float* points,
                                 uint32_t stride, const uint32_t* indices, int32_t sphere_begin, const float* spheres,
                                 float* out_ng);
@@ -861,6 +865,9 @@ void step(Lane& L) {
         if (c.variant == 1)
             orc_mtbvh_intersect(L.mt_nodes, L.mt_links, L.mt_nn, L.mt_prims, L.points, L.stride, L.indices,
                                 L.sphere_begin, L.spheres, c.n, rays.data(), hits.data(), nullptr, (int)c.nthreads);
+        else if (c.variant == 2)
+            orc_tbvh_intersect(L.mt_nodes, L.mt_links, L.mt_nn, L.mt_prims, L.points, L.stride, L.indices,
+                               L.sphere_begin, L.spheres, c.n, rays.data(), hits.data(), nullptr, (int)c.nthreads);
         else
             orc_stackless_intersect(L.sl_nodes, L.sl_prims, L.points, L.stride, L.indices, L.sphere_begin, L.spheres,
                                     c.n, rays.data(), hits.data(), nullptr, (int)c.nthreads);

@@ -27,7 +27,8 @@ ERRORS = {
 }
 VARIANT_STACKLESS = 0
 VARIANT_THREADED = 1
-VARIANTS = {"stackless": VARIANT_STACKLESS, "threaded": VARIANT_THREADED}
+VARIANT_THREADED_SINGLE = 2
+VARIANTS = {"stackless": VARIANT_STACKLESS, "threaded": VARIANT_THREADED, "threaded_single": VARIANT_THREADED_SINGLE}
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -769,7 +770,7 @@ class Wavefront:
         v, b, q = C.c_int(0), C.c_uint32(0), C.c_uint32(0)
         self.ctx.check(lib.rgpu_wavefront_get_launch(self.h, C.byref(v), C.byref(b), C.byref(q)),
                        "rgpu_wavefront_get_launch")
-        return {0: "stackless", 1: "threaded"}[v.value], b.value, q.value
+        return {v: k for k, v in VARIANTS.items()}[v.value], b.value, q.value
 
     def autotune(self, camera, warmup=4, steps=8):
         ms = C.c_double(0)

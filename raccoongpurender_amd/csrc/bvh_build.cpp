@@ -361,11 +361,11 @@ void number_top_breadth_first(std::vector<SLNode>* nodes, uint32_t top) {
 
 void pack_threaded(const BVHTree& t, std::vector<MTNode>* out) {
     const uint32_t nn = (uint32_t)t.nodes.size();
-    out->assign((size_t)6 * nn, MTNode());
+    out->assign((size_t)kMTOrders * nn, MTNode());
     if (nn == 0) return;
     // Mutable child order; the six direction sorts are applied one after another to
     // the same tree, as common/threaded_bvh.hpp:518-525 does, so a child pair whose
-    // centres tie is swapped by every sort.
+    // centres tie is swapped by every sort. The single table keeps the build's order.
     std::vector<int32_t> L(nn), R(nn);
     for (uint32_t i = 0; i < nn; ++i) {
         L[i] = t.nodes[i].left;
@@ -375,17 +375,24 @@ void pack_threaded(const BVHTree& t, std::vector<MTNode>* out) {
     for (int32_t i = (int32_t)nn - 1; i >= 0; --i)
         if (L[i] >= 0) size[i] = 1 + size[L[i]] + size[R[i]];
     std::vector<std::pair<uint32_t, uint32_t>> stack;  // (node, position in this direction's order)
-    for (int dir = 0; dir < 6; ++dir) {
-        const int axis = dir / 2;
-        const bool negative = (dir & 1) != 0;
-        for (uint32_t i = 0; i < nn; ++i) {
-            if (L[i] < 0) continue;
-            const BuildNode& a = t.nodes[L[i]];
-            const BuildNode& b = t.nodes[R[i]];
-            float lc = 0.5f * (a.bmin[axis] + a.bmax[axis]);
-            float rc = 0.5f * (b.bmin[axis] + b.bmax[axis]);
-            if (negative) std::swap(lc, rc);
-            if (!(lc < rc)) std::swap(L[i], R[i]);
+    for (int dir = 0; dir < kMTOrders; ++dir) {
+        if (dir < 6) {
+            const int axis = dir / 2;
+            const bool negative = (dir & 1) != 0;
+            for (uint32_t i = 0; i < nn; ++i) {
+                if (L[i] < 0) continue;
+                const BuildNode& a = t.nodes[L[i]];
+                const BuildNode& b = t.nodes[R[i]];
+                float lc = 0.5f * (a.bmin[axis] + a.bmax[axis]);
+                float rc = 0.5f * (b.bmin[axis] + b.bmax[axis]);
+                if (negative) std::swap(lc, rc);
+                if (!(lc < rc)) std::swap(L[i], R[i]);
+            }
+        } else {
+            for (uint32_t i = 0; i < nn; ++i) {
+                L[i] = t.nodes[i].left;
+                R[i] = t.nodes[i].right;
+            }
         }
         MTNode* tab = out->data() + (size_t)dir * nn;
         stack.clear();

@@ -23,6 +23,10 @@
 
 using namespace rgpu;
 
+static_assert(RGPU_VARIANT_STACKLESS == kVariantStackless && RGPU_VARIANT_THREADED == kVariantThreaded &&
+                  RGPU_VARIANT_THREADED_SINGLE == kVariantSingleThreaded,
+              "public traversal variant ids must equal the kernels'");
+
 struct rgpu_bvh {
     BVHTree tree;
 };
@@ -409,7 +413,7 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
                             const rgpu_ray* rays, rgpu_hit* hits, uint32_t count, uint32_t block) {
     if (!ctx || !scene) return RGPU_ERR_INVALID_ARG;
     if (count && (!rays || !hits)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null ray or hit buffer");
-    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED)
+    if (!valid_variant(variant))
         return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
     if (block != 0 && (block % 64 != 0 || block > 1024))
         return fail(ctx, RGPU_ERR_INVALID_ARG, "block must be a multiple of 64 up to 1024");
@@ -441,7 +445,7 @@ int rgpu_extension_ray_cast_stats(rgpu_ctx* ctx, const rgpu_scene* scene, int va
                                   rgpu_hit* hits, uint32_t count, uint64_t* counters5) {
     if (!ctx || !scene || !counters5) return RGPU_ERR_INVALID_ARG;
     if (count && (!rays || !hits)) return fail(ctx, RGPU_ERR_INVALID_ARG, "null ray or hit buffer");
-    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED)
+    if (!valid_variant(variant))
         return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     unsigned long long* d = nullptr;

@@ -22,4 +22,7 @@ namespace rgpu {
 int fail(rgpu_ctx* ctx, int code, const std::string& msg);
 int hip_fail(rgpu_ctx* ctx, hipError_t e, const char* what);
 hipStream_t stream_of(rgpu_ctx* ctx, void* s);
+// RGPU_VARIANT_STACKLESS (0), _THREADED (1) or _THREADED_SINGLE (2), which equal the kernels'
+// kVariant* ids (checked in capi.cpp)
+inline bool valid_variant(int v) { return v >= 0 && v <= 2; }
 }  // namespace rgpu

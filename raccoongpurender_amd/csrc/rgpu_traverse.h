@@ -22,12 +22,14 @@ struct alignas(16) TraceHit {
     float v;
 };
 
-enum { kVariantStackless = 0, kVariantThreaded = 1 };
+// kVariantThreaded: the multi-threaded BVH (a link table per ray-direction class);
+// kVariantSingleThreaded: the single-table threaded BVH. Both walk the same box records.
+enum { kVariantStackless = 0, kVariantThreaded = 1, kVariantSingleThreaded = 2 };
 
 // Device-resident scene as the kernels see it.
 struct TraceScene {
     const SLNode* sl_nodes = nullptr;
-    const MTNode* mt_nodes = nullptr;  // 6 tables of nnodes records
+    const MTNode* mt_nodes = nullptr;  // kMTOrders tables of nnodes records
     const PrimRec* prims = nullptr;
     const float4* points = nullptr;    // original vertices (for geometric normals)
     const uint32_t* indices = nullptr;  // 3 per triangle

@@ -37,11 +37,13 @@ static_assert(sizeof(SLNode) == 64, "stackless node record must be 64 bytes");
 static const uint32_t kLeafFlag = 0x80000000u;
 static const uint32_t kInvalid = 0xFFFFFFFFu;
 
-// Threaded (skip-pointer) node, one table per ray-direction class (+X,-X,+Y,-Y,+Z,-Z).
-// Each table is the tree re-laid out in the depth-first pre-order of that direction's
-// child ordering (common/threaded_bvh.hpp:494-516), so the hit link of a branch is
-// always the next record and need not be stored; the miss link is the record that
-// follows the node's subtree (common/threaded_bvh.hpp:417-484).
+// Threaded (skip-pointer) node, one table per traversal order: the six ray-direction classes
+// (+X,-X,+Y,-Y,+Z,-Z) of the multi-threaded BVH, then the single-table threaded BVH in the
+// build's own child order (common/threaded_bvh.hpp:269-364). Each table is the tree re-laid out
+// in the depth-first pre-order of that order's child ordering (common/threaded_bvh.hpp:494-516
+// for the directions), so the hit link of a branch is always the next record and need not be
+// stored; the miss link is the record that follows the node's subtree
+// (common/threaded_bvh.hpp:417-484, 309-360).
 //   q0 = { bmin.x, bmin.y, bmin.z, bmax.x }
 //   q1 = { bmax.y, bmax.z, miss, leaf }
 // leaf == 0 for a branch; otherwise kLeafFlag | ((count-1) << 28) | first slot.
@@ -53,6 +55,8 @@ struct alignas(16) MTNode {
     uint32_t leaf;
 };
 static_assert(sizeof(MTNode) == 32, "threaded node record must be 32 bytes");
+constexpr int kMTOrders = 7;  // tables: 6 direction classes, then the single table
+constexpr int kMTSingleOrder = 6;
 
 // Primitive record, one per leaf slot, in leaf order. The triangle edges are the
 // exact float differences v1-v0 and v2-v0 that the Moller-Trumbore test forms

@@ -264,13 +264,21 @@ __device__ __forceinline__ uint32_t direction_class(F3 rd) {
     return 0.0f < rd.z ? 4u : 5u;
 }
 
-// Threaded traversal of one ray (common/threaded_bvh.hpp:527-569).
-template <bool kCount>
+// The table a ray walks: its direction class for the multi-threaded BVH, the single table
+// otherwise (common/threaded_bvh.hpp:527-547, 366-395).
+template <int kVariant>
+__device__ __forceinline__ const MTNode* threaded_table(const MTNode* tables, uint32_t nnodes, F3 rd) {
+    const uint32_t order = kVariant == kVariantSingleThreaded ? (uint32_t)kMTSingleOrder : direction_class(rd);
+    return tables + (size_t)order * nnodes;
+}
+
+// Threaded traversal of one ray (common/threaded_bvh.hpp:527-569, 366-395).
+template <bool kCount, int kVariant>
 __device__ __forceinline__ int32_t trace_threaded(const MTNode* __restrict__ tables, uint32_t nnodes,
                                                   const PrimRec* __restrict__ prims, int32_t sphere_begin,
                                                   bool boxes_finite, F3 ro, F3 rd, float* tmin_io, float* uo,
                                                   float* vo, Cnt* cnt) {
-    const MTNode* tab = tables + (size_t)direction_class(rd) * nnodes;
+    const MTNode* tab = threaded_table<kVariant>(tables, nnodes, rd);
     const F3 inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
     const bool plain = plain_slabs(boxes_finite, ro, inv);
     float tmin = *tmin_io;
@@ -314,9 +322,10 @@ struct ThreadedRay {
     uint32_t leaf;  // primitive word of an entered leaf not yet tested (0: none)
 };
 
+template <int kVariant>
 __device__ __forceinline__ void threaded_begin(ThreadedRay& r, const MTNode* tables, uint32_t nnodes, F3 ro, F3 rd,
                                                float tmax) {
-    r.tab = tables + (size_t)direction_class(rd) * nnodes;
+    r.tab = threaded_table<kVariant>(tables, nnodes, rd);
     r.ro = ro;
     r.rd = rd;
     r.inv = f3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
@@ -521,10 +530,10 @@ __device__ __forceinline__ int32_t trace_batched(const TraceScene& sc, F3 ro, F3
                                                  float* vo, uint32_t leaf_batch, uint32_t top = 0) {
     const SLTreelet tsrc{reinterpret_cast<const float4*>(sc.sl_nodes), top};
     const SLGlobal gsrc{reinterpret_cast<const float4*>(sc.sl_nodes)};
-    using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
+    using State = typename std::conditional<kVariant == kVariantStackless, StacklessRay, ThreadedRay>::type;
     State st;
-    if constexpr (kVariant == kVariantThreaded)
-        threaded_begin(st, sc.mt_nodes, sc.nnodes, ro, rd, *tmin_io);
+    if constexpr (kVariant != kVariantStackless)
+        threaded_begin<kVariant>(st, sc.mt_nodes, sc.nnodes, ro, rd, *tmin_io);
     else
         stackless_begin(st, ro, rd, *tmin_io);
     const bool finite = sc.boxes_finite != 0;
@@ -536,7 +545,7 @@ __device__ __forceinline__ int32_t trace_batched(const TraceScene& sc, F3 ro, F3
         const uint32_t npending = (uint32_t)__popcll(__ballot(pending));
         if (npending >= leaf_batch || npending >= (uint32_t)__popcll(__ballot(walking))) {
             if (pending) {
-                if constexpr (kVariant == kVariantThreaded)
+                if constexpr (kVariant != kVariantStackless)
                     threaded_leaf(st, sc.prims, sc.sphere_begin);
                 else if constexpr (kTreelet)
                     stackless_leaf(st, tsrc, sc.prims, sc.sphere_begin);
@@ -544,7 +553,7 @@ __device__ __forceinline__ int32_t trace_batched(const TraceScene& sc, F3 ro, F3
                     stackless_leaf(st, gsrc, sc.prims, sc.sphere_begin);
             }
         } else if (walking) {
-            if constexpr (kVariant == kVariantThreaded)
+            if constexpr (kVariant != kVariantStackless)
                 threaded_visit(st, finite);
             else if constexpr (kTreelet)
                 stackless_visit(st, tsrc, finite);

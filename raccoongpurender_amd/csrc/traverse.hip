@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(256) k_trace_stackless(const TraceRay* __restr
     if (kMode == kWalkCount) flush_counts(counters, c);
 }
 
-template <int kMode>
+// kVariant: kVariantThreaded (a table per direction class) or kVariantSingleThreaded.
+template <int kMode, int kVariant>
 __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restrict__ rays,
                                                         TraceHit* __restrict__ hits, uint32_t n, TraceScene sc,
                                                         uint32_t leaf_batch, unsigned long long* counters) {
@@ -94,10 +95,11 @@ __global__ void __launch_bounds__(256) k_trace_threaded(const TraceRay* __restri
         float t;
         load_ray(rays, i, &ro, &rd, &t);
         float u, v;
-        int32_t prim = kMode == kWalkBatched ? trace_batched<kVariantThreaded>(sc, ro, rd, &t, &u, &v, leaf_batch)
-                                         : trace_threaded<kMode == kWalkCount>(sc.mt_nodes, sc.nnodes, sc.prims,
-                                                                            sc.sphere_begin, sc.boxes_finite != 0,
-                                                                            ro, rd, &t, &u, &v, &c);
+        int32_t prim = kMode == kWalkBatched
+                           ? trace_batched<kVariant>(sc, ro, rd, &t, &u, &v, leaf_batch)
+                           : trace_threaded<kMode == kWalkCount, kVariant>(sc.mt_nodes, sc.nnodes, sc.prims,
+                                                                           sc.sphere_begin, sc.boxes_finite != 0, ro,
+                                                                           rd, &t, &u, &v, &c);
         store_hit(hits, i, prim, t, u, v);
     }
     if (kMode == kWalkCount) flush_counts(counters, c);
@@ -129,10 +131,10 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
     uint64_t next = (uint64_t)wave * chunk;
     const uint64_t end = next + chunk < (uint64_t)n ? next + chunk : (uint64_t)n;
     uint32_t ray = kInvalid;
-    using State = typename std::conditional<kVariant == kVariantThreaded, ThreadedRay, StacklessRay>::type;
+    using State = typename std::conditional<kVariant == kVariantStackless, StacklessRay, ThreadedRay>::type;
     State st;
     st.node = kInvalid;
-    if constexpr (kVariant == kVariantThreaded) st.leaf = 0;
+    if constexpr (kVariant != kVariantStackless) st.leaf = 0;
     const bool finite = sc.boxes_finite != 0;
     for (;;) {
         const bool idle = !walk_live(st);
@@ -146,8 +148,8 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
                     F3 ro, rd;
                     float tmax;
                     load_ray(rays, ray, &ro, &rd, &tmax);
-                    if constexpr (kVariant == kVariantThreaded)
-                        threaded_begin(st, sc.mt_nodes, sc.nnodes, ro, rd, tmax);
+                    if constexpr (kVariant != kVariantStackless)
+                        threaded_begin<kVariant>(st, sc.mt_nodes, sc.nnodes, ro, rd, tmax);
                     else
                         stackless_begin(st, ro, rd, tmax);
                 }
@@ -161,14 +163,14 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
         const uint32_t nwalk = (uint32_t)__popcll(__ballot(walking));
         if (npending >= leaf_batch || npending >= nwalk) {
             if (pending) {
-                if constexpr (kVariant == kVariantThreaded)
+                if constexpr (kVariant != kVariantStackless)
                     threaded_leaf(st, sc.prims, sc.sphere_begin);
                 else
                     stackless_leaf(st, src, sc.prims, sc.sphere_begin);
                 if (!walk_live(st)) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
             }
         } else if (walking) {
-            if constexpr (kVariant == kVariantThreaded)
+            if constexpr (kVariant != kVariantStackless)
                 threaded_visit(st, finite);
             else
                 stackless_visit(st, src, finite);
@@ -188,6 +190,20 @@ __global__ void __launch_bounds__(256) k_trace_empty(const TraceRay* __restrict_
     store_hit(hits, i, -1, t, 0.0f, 0.0f);
 }
 
+template <int kVariant>
+void launch_threaded(int mode, dim3 grid, uint32_t block, hipStream_t stream, const TraceRay* rays, TraceHit* hits,
+                     uint32_t n, const TraceScene& sc, uint32_t leaf_batch, unsigned long long* counters) {
+    if (mode == kWalkCount)
+        hipLaunchKernelGGL((k_trace_threaded<kWalkCount, kVariant>), grid, dim3(block), 0, stream, rays, hits, n, sc,
+                           leaf_batch, counters);
+    else if (mode == kWalkBatched)
+        hipLaunchKernelGGL((k_trace_threaded<kWalkBatched, kVariant>), grid, dim3(block), 0, stream, rays, hits, n, sc,
+                           leaf_batch, counters);
+    else
+        hipLaunchKernelGGL((k_trace_threaded<kWalkPlain, kVariant>), grid, dim3(block), 0, stream, rays, hits, n, sc,
+                           leaf_batch, counters);
+}
+
 hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays, TraceHit* hits, uint32_t n,
                         uint32_t block, hipStream_t stream, unsigned long long* counters, const TraceTuning* tune) {
     if (n == 0) return hipSuccess;
@@ -197,7 +213,8 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
         const uint32_t wps = tune && tune->waves_per_simd ? tune->waves_per_simd : 8u;
         // measured on the 1.3M-triangle incoherent set: refills at 16 (stackless) or 48 (threaded)
         // idle lanes, primitive tests once 8 lanes wait at leaves
-        const uint32_t refill = tune && tune->refill ? tune->refill : (variant == kVariantThreaded ? 48u : 16u);
+        const uint32_t refill =
+            tune && tune->refill ? tune->refill : (variant != kVariantStackless ? 48u : 16u);
         const uint32_t leaf_batch = tune && tune->leaf_batch ? tune->leaf_batch : kDefaultLeafBatch;
         const uint32_t blocks = cus * wps;  // 4 waves (one per SIMD) per 256-thread block
         const uint32_t waves = blocks * 4u;
@@ -205,6 +222,9 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
         if (variant == kVariantThreaded) {
             hipLaunchKernelGGL(k_trace_persistent<kVariantThreaded>, dim3(blocks), dim3(256), 0, stream, rays, hits,
                                n, sc, chunk, refill, leaf_batch, 0u);
+        } else if (variant == kVariantSingleThreaded) {
+            hipLaunchKernelGGL(k_trace_persistent<kVariantSingleThreaded>, dim3(blocks), dim3(256), 0, stream, rays,
+                               hits, n, sc, chunk, refill, leaf_batch, 0u);
         } else {
             // the top of the tree in LDS: 320 records (20 KB) let the 8 workgroups per CU of the
             // default launch share the CU's 160 KB
@@ -226,15 +246,9 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
     if (sc.nnodes == 0) {
         hipLaunchKernelGGL(k_trace_empty, grid, dim3(block), 0, stream, rays, hits, n);
     } else if (variant == kVariantThreaded) {
-        if (mode == kWalkCount)
-            hipLaunchKernelGGL(k_trace_threaded<kWalkCount>, grid, dim3(block), 0, stream, rays, hits, n, sc, leaf_batch,
-                               counters);
-        else if (mode == kWalkBatched)
-            hipLaunchKernelGGL(k_trace_threaded<kWalkBatched>, grid, dim3(block), 0, stream, rays, hits, n, sc,
-                               leaf_batch, counters);
-        else
-            hipLaunchKernelGGL(k_trace_threaded<kWalkPlain>, grid, dim3(block), 0, stream, rays, hits, n, sc, leaf_batch,
-                               counters);
+        launch_threaded<kVariantThreaded>(mode, grid, block, stream, rays, hits, n, sc, leaf_batch, counters);
+    } else if (variant == kVariantSingleThreaded) {
+        launch_threaded<kVariantSingleThreaded>(mode, grid, block, stream, rays, hits, n, sc, leaf_batch, counters);
     } else {
         if (mode == kWalkCount)
             hipLaunchKernelGGL(k_trace_stackless<kWalkCount>, grid, dim3(block), 0, stream, rays, hits, n, sc,

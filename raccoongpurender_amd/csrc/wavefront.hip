@@ -1498,6 +1498,8 @@ hipError_t step_impl(const DevState& st, const WFParams& p, const J& jobs, const
         }
     } else if (variant == kVariantThreaded) {
         hipLaunchKernelGGL((k_trace_step<kVariantThreaded, J>), tgrid, tblk, 0, s, st, p, jobs, 0u);
+    } else if (variant == kVariantSingleThreaded) {
+        hipLaunchKernelGGL((k_trace_step<kVariantSingleThreaded, J>), tgrid, tblk, 0, s, st, p, jobs, 0u);
     } else {
         // the top of the tree in LDS: 80 records per wave, so that 8 waves per SIMD fit the 160 KB
         const uint32_t top = kDefaultTreelet * tb / 256u;

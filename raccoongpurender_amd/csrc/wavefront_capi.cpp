@@ -164,7 +164,7 @@ Camera to_camera(const rgpu_camera* cam) {
 // Options shared by single renders and batches.
 int parse_config(rgpu_ctx* ctx, const rgpu_wavefront_config* cfg, rgpu_wavefront* w) {
     if (cfg->abi_version != RGPU_ABI_VERSION) return fail(ctx, RGPU_ERR_INVALID_ARG, "abi_version mismatch");
-    if (cfg->variant != RGPU_VARIANT_STACKLESS && cfg->variant != RGPU_VARIANT_THREADED)
+    if (!valid_variant((int)cfg->variant))
         return fail(ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
     WFParams& p = w->params;
     p.rr_start = cfg->rr_start ? cfg->rr_start : 25u;
@@ -935,7 +935,7 @@ int rgpu_wavefront_graph_launch(rgpu_wavefront* w, rgpu_stream stream, uint32_t 
 
 int rgpu_wavefront_set_launch(rgpu_wavefront* w, int variant, uint32_t trace_block, uint32_t queue_blocks) {
     if (!w) return RGPU_ERR_INVALID_ARG;
-    if (variant != RGPU_VARIANT_STACKLESS && variant != RGPU_VARIANT_THREADED)
+    if (!valid_variant(variant))
         return fail(w->ctx, RGPU_ERR_INVALID_ARG, "unknown traversal variant");
     if (trace_block != 0 && trace_block != 64 && trace_block != 128 && trace_block != 256)
         return fail(w->ctx, RGPU_ERR_INVALID_ARG, "trace_block must be 0 (persistent), 64, 128 or 256");
@@ -962,7 +962,7 @@ int rgpu_wavefront_autotune(rgpu_wavefront* w, const rgpu_camera* camera, uint32
                             double* best_ms_per_step) {
     if (!w || (!camera && !w->is_batch()) || steps == 0) return RGPU_ERR_INVALID_ARG;
     const int cu = w->ctx->cu_count > 0 ? w->ctx->cu_count : 256;
-    const int variants[2] = {RGPU_VARIANT_STACKLESS, RGPU_VARIANT_THREADED};
+    const int variants[3] = {RGPU_VARIANT_STACKLESS, RGPU_VARIANT_THREADED, RGPU_VARIANT_THREADED_SINGLE};
     const uint32_t blocks[4] = {0, 64, 128, 256};
     const uint32_t qmul[4] = {2, 4, 8, 16};
     double best = 1e300;

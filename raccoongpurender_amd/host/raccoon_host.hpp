@@ -673,9 +673,10 @@ private:
     std::exception_ptr error_;
 };
 
-// A built tree cast against on the GPU with one of the two traversals: the stackless walk of
-// common/stackless_bvh.hpp (kernels/extension_ray_cast_stackless.cl) or the threaded walk of
-// common/threaded_bvh.hpp (kernels/extension_ray_cast.cl).
+// A built tree cast against on the GPU with one of the traversals: the stackless walk of
+// common/stackless_bvh.hpp (kernels/extension_ray_cast_stackless.cl), the multi-threaded walk of
+// common/threaded_bvh.hpp (kernels/extension_ray_cast.cl) or its single-table form
+// (TBVHNode / intersect_tbvh, common/threaded_bvh.hpp:269-396).
 class RayCaster {
 public:
     RayCaster(GpuContext& ctx, const SceneManager& sceneManager, int variant)
@@ -713,6 +714,10 @@ public:
 class ThreadedBVH : public RayCaster {
 public:
     ThreadedBVH(GpuContext& ctx, const SceneManager& sm) : RayCaster(ctx, sm, RGPU_VARIANT_THREADED) {}
+};
+class SingleThreadedBVH : public RayCaster {
+public:
+    SingleThreadedBVH(GpuContext& ctx, const SceneManager& sm) : RayCaster(ctx, sm, RGPU_VARIANT_THREADED_SINGLE) {}
 };
 
 }  // namespace rt

@@ -165,7 +165,7 @@ def main():
     ap.add_argument("--guided", action="store_true")
     ap.add_argument("--trace-block", type=int, default=64, help="0: persistent ray cast; 64/128/256: one thread per slot")
     ap.add_argument("--queue-blocks", type=int, default=4096)
-    ap.add_argument("--variant", choices=["stackless", "threaded"], default="stackless")
+    ap.add_argument("--variant", choices=["stackless", "threaded", "threaded_single"], default="stackless")
     a = ap.parse_args()
     out = profile(a.steps, a.warmup, a.guided, a.trace_block, a.queue_blocks, a.variant)
     for name, row in out["stages"].items():

@@ -245,6 +245,7 @@ int main(int argc, char** argv) {
         }
         std::vector<rgpu_hit> hs = rt::StacklessBVH(pt.context(), s.sm).intersect(rays);
         std::vector<rgpu_hit> ht = rt::ThreadedBVH(pt.context(), s.sm).intersect(rays);
+        std::vector<rgpu_hit> h1 = rt::SingleThreadedBVH(pt.context(), s.sm).intersect(rays);
         dump(dir + "/points.f32", s.points);
         dump(dir + "/indices.u32", s.indices);
         dump(dir + "/spheres.f32", s.spheres);
@@ -269,6 +270,8 @@ int main(int argc, char** argv) {
              std::vector<int32_t>((const int32_t*)hs.data(), (const int32_t*)hs.data() + 4 * hs.size()));
         dump(dir + "/hits_threaded.i32",
              std::vector<int32_t>((const int32_t*)ht.data(), (const int32_t*)ht.data() + 4 * ht.size()));
+        dump(dir + "/hits_threaded_single.i32",
+             std::vector<int32_t>((const int32_t*)h1.data(), (const int32_t*)h1.data() + 4 * h1.size()));
         std::printf("{\"width\": %d, \"height\": %d, \"spp\": %.6f, \"steps\": %d, \"nodes\": %u, \"paths\": %d, "
                     "\"live_images\": %d, \"live_steps\": %d, \"live_monotonic\": %s, \"live_alpha\": %s, "
                     "\"live_last_total\": %llu, \"live_final_total\": %llu, \"live_stat_total\": %llu}\n",

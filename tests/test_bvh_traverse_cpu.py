@@ -97,7 +97,7 @@ def test_reference_threaded_layout_links():
     t = _lib.BVH(v, f).export()
     ref = O.RefMTBVH(t)
     nn = ref.nn
-    for d in range(6):
+    for d in range(7):
         hit, miss = ref.links[d, :, 0], ref.links[d, :, 1]
         # following hit links from the root with no misses walks every node exactly once
         seen = set()
@@ -112,7 +112,40 @@ def test_reference_threaded_layout_links():
         assert np.array_equal(hit[leaf], miss[leaf])
 
 
-@pytest.mark.parametrize("variant", ["stackless", "threaded"])
+def test_reference_single_threaded_table_is_preorder_with_skips():
+    # The single threaded table (common/threaded_bvh.hpp:269-364): nodes numbered in depth-first
+    # pre-order of the build's child order; every hit link is the next number, every miss link
+    # the first number after the node's subtree (-1 past the end), a leaf's two links equal.
+    v, f = scenes.displaced_icosphere(level=3)
+    t = _lib.BVH(v, f).export()
+    left, right = t["left"], t["right"]
+    nn = len(left)
+    order, size = [], np.ones(nn, np.int64)
+    stack = [0]
+    while stack:
+        n = stack.pop()
+        order.append(n)
+        if left[n] >= 0:
+            stack += [right[n], left[n]]
+    for n in reversed(order):
+        if left[n] >= 0:
+            size[n] = 1 + size[left[n]] + size[right[n]]
+    number = np.empty(nn, np.int64)
+    number[order] = np.arange(nn)
+    ref = O.RefMTBVH(t)
+    hit, miss = ref.links[6, :, 0], ref.links[6, :, 1]
+    idx = np.arange(nn)
+    assert np.array_equal(hit, np.where(idx + 1 < nn, idx + 1, -1))
+    skip = np.empty(nn, np.int64)
+    skip[number] = number + size
+    assert np.array_equal(miss, np.where(skip < nn, skip, -1))
+    leaf = ref.nodes["primitive_indices_end"] > ref.nodes["primitive_indices_beg"]
+    assert np.array_equal(hit[leaf], miss[leaf])
+    # the direction tables order children differently but visit the same node set
+    assert not np.array_equal(ref.links[0], ref.links[6])
+
+
+@pytest.mark.parametrize("variant", ["stackless", "threaded", "threaded_single"])
 def test_oracle_traversal_matches_bruteforce(variant):
     for name, v, f, sph in _meshes():
         t = _lib.BVH(v, f, sph).export()
@@ -131,8 +164,10 @@ def test_oracle_traversal_matches_bruteforce(variant):
         for rays, rel, max_tie, allow_grazing in ((generic, 1e-6, 0.01, False), (degenerate, 2e-5, 0.5, True)):
             if variant == "stackless":
                 hits, cnt = O.intersect_stackless(O.RefStackless(t), v, f, rays, sph)
-            else:
+            elif variant == "threaded":
                 hits, cnt = O.intersect_mtbvh(O.RefMTBVH(t), v, f, rays, sph)
+            else:
+                hits, cnt = O.intersect_tbvh(O.RefMTBVH(t), v, f, rays, sph)
             ref = O.intersect_bruteforce(v, f, rays, sph)
             ties, wrong = H.tie_excepted_mismatch(hits, ref, rel=rel)
             if allow_grazing:

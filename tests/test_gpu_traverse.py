@@ -1,8 +1,8 @@
-"""GPU parity of extension ray casting (both traversal variants) against the oracle.
+"""GPU parity of extension ray casting (every traversal variant) against the oracle.
 
 Bit-exact bar: primitive index, t, u and v of every ray equal the oracle's bits when
 the oracle runs the reference algorithm (kernels/extension_ray_cast_stackless.cl,
-common/threaded_bvh.hpp:527-569) on the reference layouts restated from the same tree.
+common/threaded_bvh.hpp:527-569 and 366-395) on the reference layouts restated from the same tree.
 """
 import numpy as np
 import pytest
@@ -14,10 +14,15 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 
+VARIANTS = ("stackless", "threaded", "threaded_single")
+
+
 def _oracle(tree, v, f, sph, rays, variant):
     if variant == "stackless":
         return O.intersect_stackless(O.RefStackless(tree), v, f, rays, sph)
-    return O.intersect_mtbvh(O.RefMTBVH(tree), v, f, rays, sph)
+    if variant == "threaded":
+        return O.intersect_mtbvh(O.RefMTBVH(tree), v, f, rays, sph)
+    return O.intersect_tbvh(O.RefMTBVH(tree), v, f, rays, sph)
 
 
 def _ray_sets(v):
@@ -44,7 +49,7 @@ def _meshes():
     yield "soup+spheres", v, f, sph
 
 
-@pytest.mark.parametrize("variant", ["stackless", "threaded"])
+@pytest.mark.parametrize("variant", VARIANTS)
 def test_bit_exact_small_scenes(ctx, variant):
     for name, v, f, sph in _meshes():
         bvh = _lib.BVH(v, f, sph)
@@ -65,7 +70,7 @@ def test_variants_agree_and_block_sizes_identical(ctx):
     rays = scenes.incoherent_rays(50000, v.min(0), v.max(0), seed=21)
     base = _lib.trace(ctx, scene, rays, "stackless")
     for block in (64, 128, 256):
-        for variant in ("stackless", "threaded"):
+        for variant in VARIANTS:
             g = _lib.trace(ctx, scene, rays, variant, block=block)
             ties, wrong = H.tie_excepted_mismatch(g, base)
             assert wrong == 0
@@ -80,7 +85,7 @@ def test_persistent_launch_shapes_match_per_ray_kernel(ctx):
     scene = _lib.Scene(ctx, _lib.BVH(v, f))
     for rays in (scenes.incoherent_rays(30011, v.min(0), v.max(0), seed=23), scenes.primary_rays(64, 48),
                  scenes.incoherent_rays(37, v.min(0), v.max(0), seed=24)):
-        for variant in ("stackless", "threaded"):
+        for variant in VARIANTS:
             ref = _lib.trace(ctx, scene, rays, variant, block=256)
             try:
                 for wps, refill, lb, tl in ((1, 1, 1, 1), (4, 16, 64, 7), (8, 64, 8, 1024), (16, 8, 32, 0),
@@ -129,7 +134,7 @@ def test_nan_guarded_slabs_in_mixed_waves(ctx):
         bvh = _lib.BVH(vv, ff)
         scene = _lib.Scene(ctx, bvh)
         tree = bvh.export()
-        for variant in ("stackless", "threaded"):
+        for variant in VARIANTS:
             o, _ = _oracle(tree, vv, ff, None, rays, variant)
             for block in (0, 64, 256):
                 print(f"case {name}/{variant}/block {block}", flush=True)
@@ -144,7 +149,7 @@ def test_empty_and_single_triangle(ctx):
     bvh = _lib.BVH(v, f)
     scene = _lib.Scene(ctx, bvh)
     rays = scenes.incoherent_rays(1000, [-1, -1, -1], [1, 1, 1], seed=1)
-    for variant in ("stackless", "threaded"):
+    for variant in VARIANTS:
         g = _lib.trace(ctx, scene, rays, variant)
         assert np.all(g["prim"] == -1) and np.array_equal(g["t"], rays["tmax"])
     v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
@@ -153,7 +158,7 @@ def test_empty_and_single_triangle(ctx):
     scene = _lib.Scene(ctx, bvh)
     rays = scenes.make_rays(np.array([[0.2, 0.2, 1.0], [2, 2, 1.0], [0.1, 0.1, -1.0]]),
                             np.array([[0, 0, -1.0], [0, 0, -1.0], [0, 0, 1.0]]))
-    for variant in ("stackless", "threaded"):
+    for variant in VARIANTS:
         g = _lib.trace(ctx, scene, rays, variant)
         assert g["prim"].tolist() == [0, -1, 0]
         assert g["t"][0] == 1.0 and g["t"][2] == 1.0
@@ -180,7 +185,7 @@ def c2():
     return d
 
 
-@pytest.mark.parametrize("variant", ["stackless", "threaded"])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("rayset", ["coherent", "incoherent"])
 def test_bit_exact_raycast_benchmark_full_size(ctx, c2, variant, rayset):
     """1,310,720 triangles, 4,194,304 rays per set: GPU == oracle bit for bit."""

@@ -38,7 +38,7 @@ def _compare(ctx, sd, n, steps, variant="stackless", **kw):
     return g, o
 
 
-@pytest.mark.parametrize("variant", ["stackless", "threaded"])
+@pytest.mark.parametrize("variant", ["stackless", "threaded", "threaded_single"])
 def test_cornell_box_small(ctx, variant):
     sd = R.cornell_box(48, 48, segments=16, rings=8)
     print(f"case cornell 48x48 {variant}", flush=True)

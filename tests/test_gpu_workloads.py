@@ -66,11 +66,11 @@ def test_full_render_matches_oracle_at_2e16_slots(ctx, c3):
     # one oracle lane per traversal order: exact t-ties (the medium box's floor lies in the
     # ground plane) go to the primitive tested last, which depends on the traversal order
     oracles = {}
-    for variant in ("stackless", "threaded"):
+    for variant in ("stackless", "threaded", "threaded_single"):
         oracles[variant] = W.oracle_lane(c3, n, max_depth=8, variant=variant)
         oracles[variant].initialize(1, 2)  # pixel cursor at P/2: geometry, not sky
     lanes = {}
-    for variant, block in (("stackless", 64), ("stackless", 0), ("threaded", 256)):
+    for variant, block in (("stackless", 64), ("stackless", 0), ("threaded", 256), ("threaded_single", 64)):
         g = W.gpu_lane(ctx, c3, n, max_depth=8, variant=variant)
         g.set_launch(variant, block, 2048)
         g.initialize(_camera(c3), 1, 2)

@@ -74,7 +74,7 @@ def test_host_surface_renders_like_the_binding(ctx, tmp_path):
         host = _load(d, name, np.uint8).astype(np.int32)
         assert host.size == img.size and np.abs(host - img).max() <= 1, name
     rays = _load(d, "rays.f32", np.float32).view(_lib.RAY_DTYPE)
-    for variant in ("stackless", "threaded"):
+    for variant in ("stackless", "threaded", "threaded_single"):
         g = _lib.trace(ctx, scene, rays, variant)
         host = _load(d, f"hits_{variant}.i32", np.int32).view(_lib.HIT_DTYPE)
         assert np.array_equal(g.view(np.int32), host.view(np.int32)), variant
