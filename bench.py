@@ -330,23 +330,28 @@ def envmap_bench(samples=1 << 24, iters=20, width=2048, height=1024):
     return res
 
 
-def guiding_bench(width=480, height=270, seconds=4.0):
-    """Equal-time relMSE of guided against plain sampling on the full-render scene at a reduced
-    resolution, scored against a 16k-spp plain render (all unclamped)."""
+def guiding_bench(width=1920, height=1080, seconds=4.0):
+    """Equal-time relMSE of guided against plain sampling on the full-render scene (1920x1080,
+    2^22 slots each), scored against a 16k-spp plain render (all unclamped): fixed selection
+    probability 0.5 and the learned per-vertex selection probability."""
     from raccoongpurender_amd import _lib, guiding_study, render_workloads
 
     sd = render_workloads.full_render_scene(width, height)
     ctx = _lib.Context(0)
-    out = guiding_study.equal_time_relmse(ctx, sd, seconds=seconds)
-    out["workload"] = f"{width}x{height}, 2017294 tris, same scene as the render line; relMSE vs 16k-spp plain"
+    print(f"[bench] guiding: {width}x{height}, 16k-spp plain reference (about a minute)", file=sys.stderr, flush=True)
+    out = guiding_study.equal_time_relmse(ctx, sd, seconds=seconds, paths=1 << 22, ref_paths=1 << 22)
+    out["workload"] = (f"{width}x{height}, 2017294 tris, same scene as the render line, {seconds} s each; "
+                       "relMSE vs 16k-spp plain")
+    ls = out["learned_selection"]
     print(f"[bench] guiding: relMSE plain {out['relmse_plain']:.4f} guided {out['relmse_guided']:.4f} "
-          f"(trimmed 1%: {out['relmse_trim1pct_plain']:.4f} / {out['relmse_trim1pct_guided']:.4f})",
-          file=sys.stderr, flush=True)
+          f"learned selection {ls['relmse']:.4f} (trimmed 1%: {out['relmse_trim1pct_plain']:.4f} / "
+          f"{out['relmse_trim1pct_guided']:.4f} / {ls['relmse_trim1pct']:.4f})", file=sys.stderr, flush=True)
     # the same comparison on the Cornell box (small area light, no caustics), with a training
     # warm-up inside the guided budget
     cb = guiding_study.cornell_equal_time(ctx)
     out["cornell"] = cb
     print(f"[bench] guiding, Cornell box: relMSE plain {cb['relmse_plain']:.3g} guided {cb['relmse_guided']:.3g} "
+          f"learned selection {cb['learned_selection']['relmse']:.3g} "
           f"({cb['ratio_plain_over_guided']:.1f}x lower at equal time)", file=sys.stderr, flush=True)
     ctx.close()
     return out

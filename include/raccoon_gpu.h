@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RGPU_ABI_VERSION 2
+#define RGPU_ABI_VERSION 3
 
 /* status codes */
 #define RGPU_OK 0
@@ -478,6 +478,11 @@ typedef struct rgpu_guide_config {
     float weight_cap;      /* training weights F/p above it are clamped to it; 0 = no clamp.
                               Affects what the network learns only: the estimator stays unbiased */
     float log_kappa_max;   /* upper clamp of ln(kappa) of a lobe, in (-3, 12]; 0 = 7.5 */
+    uint32_t learn_select; /* 1: each vertex selects the guide with a probability the network learns
+                              (output 40, clamped to [0.05, 0.95]; Mueller et al. 2019), trained on
+                              the log density of the whole one-sample mixture; select_p is then
+                              unused. 0: the fixed select_p. No reference counterpart (the
+                              reference has no path guiding). */
 } rgpu_guide_config;
 
 int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** out);
@@ -489,9 +494,12 @@ int rgpu_guide_param_count(void);
 /* Copy the fp32 weights out (get) and/or in (set; refreshes the bf16 copies). */
 int rgpu_guide_params(rgpu_guide* g, float* get, const float* set);
 /* Test hooks over host rows of 16 floats (position in [0,1]^3, normal, wo, wi, weight, scene
- * class code (0 = none, 1..8: one-hot input), pad 2):
- * raw head outputs (64 per row), the gradient of the mean loss -weight*log q(wi) (at most
- * minibatch rows), and training (minibatch Adam steps over at most capacity rows). */
+ * class code (0 = none, 1..8: one-hot input), the density of the other strategies at wi (0 = not
+ * known), pad 1):
+ * raw head outputs (64 per row), the gradient of the mean loss -weight*log q(wi) (plus
+ * -weight*log of the one-sample mixture density with learn_select, for rows with a known other
+ * density; at most minibatch rows), and training (minibatch Adam steps over at most capacity
+ * rows). */
 int rgpu_guide_forward(rgpu_guide* g, const float* samples, uint32_t rows, float* out);
 int rgpu_guide_gradients(rgpu_guide* g, const float* samples, uint32_t rows, float* grad);
 int rgpu_guide_train_samples(rgpu_guide* g, const float* samples, uint32_t rows);

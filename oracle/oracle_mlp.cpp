@@ -20,6 +20,9 @@ const int oW0 = 0, oB0 = oW0 + kHid * kIn, oW1 = oB0 + kHid, oB1 = oW1 + kHid * 
           oB2 = oW2 + kHid * kHid, oW3 = oB2 + kHid, oB3 = oW3 + kOut * kHid, kParams = oB3 + kOut;
 const double kPi = 3.14159265358979323846;
 const double kLogKappaMin = -3.0, kLogKappaMax = 7.5;
+// learned selection probability: c = clamp(sigmoid(out[40]), 0.05, 0.95)
+const int kSelOut = 5 * kLobes;
+const double kSelMin = 0.05, kSelMax = 0.95;
 
 double bf16(double x) {
     float f = (float)x;
@@ -200,8 +203,11 @@ void orc_mlp_pdf(uint32_t n, const double* out, const double* w, double* pdf) {
 }
 
 // Mean over the batch of -weight * log q(w), and its gradient with respect to every parameter.
-double orc_mlp_loss_grad(const double* params, uint32_t n, const double* x, const double* w, const double* weight,
-                         double* grad, int emulate) {
+// With other_pdf (may be null), rows whose density of the other strategies p_o is positive add
+// -weight * log(c q(w) + (1 - c) p_o), the log density of the one-sample mixture with the
+// learned selection probability c = clamp(sigmoid(out[40]), 0.05, 0.95).
+double orc_mlp_loss_grad_sel(const double* params, uint32_t n, const double* x, const double* w, const double* weight,
+                             const double* other_pdf, double* grad, int emulate) {
     Net net{params, emulate != 0};
     for (int i = 0; i < kParams; ++i) grad[i] = 0.0;
     double loss = 0.0;
@@ -210,9 +216,22 @@ double orc_mlp_loss_grad(const double* params, uint32_t n, const double* x, cons
         forward(net, x + (size_t)kIn * b, &f);
         Lobe L[kLobes];
         lobes(f.out, L);
-        loss += -weight[b] * log_mixture(L, w + 3 * b, nullptr);
+        const double lq = log_mixture(L, w + 3 * b, nullptr);
+        loss += -weight[b] * lq;
         double d3[kOut], dh[kHid], dz[kHid];
         head_grad(f.out, w + 3 * b, weight[b] / n, d3);
+        const double po = other_pdf ? other_pdf[b] : 0.0;
+        if (po > 0.0) {
+            const double sg = 1.0 / (1.0 + std::exp(-f.out[kSelOut]));
+            const double c = std::fmin(std::fmax(sg, kSelMin), kSelMax);
+            const double q = std::exp(lq);
+            const double pm = c * q + (1.0 - c) * po;
+            loss += -weight[b] * std::log(pm);
+            // d(-w log pm)/d(lobe outputs) = (c q / pm) d(-w log q)/d(lobe outputs)
+            const double scale = 1.0 + c * q / pm;
+            for (int i = 0; i < kSelOut; ++i) d3[i] *= scale;
+            d3[kSelOut] = (sg > kSelMin && sg < kSelMax) ? -weight[b] / n * (q - po) * c * (1.0 - c) / pm : 0.0;
+        }
         for (int i = 0; i < kOut; ++i) d3[i] = net.R(d3[i]);
         backprop_layer(net, oW3, oB3, kHid, kOut, f.h3, d3, grad, dh);
         for (int i = 0; i < kHid; ++i) dz[i] = net.R(f.h3[i] > 0.0 ? dh[i] : 0.0);
@@ -223,6 +242,11 @@ double orc_mlp_loss_grad(const double* params, uint32_t n, const double* x, cons
         backprop_layer(net, oW0, oB0, kIn, kHid, f.x, dz, grad, nullptr);
     }
     return loss / n;
+}
+
+double orc_mlp_loss_grad(const double* params, uint32_t n, const double* x, const double* w, const double* weight,
+                         double* grad, int emulate) {
+    return orc_mlp_loss_grad_sel(params, n, x, w, weight, nullptr, grad, emulate);
 }
 
 // Adam (Kingma & Ba 2015), bias-corrected, step index t >= 1.

@@ -13,11 +13,23 @@ O._sig("orc_mlp_encode", None, u32, vp, vp, vp, vp, vp)
 O._sig("orc_mlp_forward", None, vp, u32, vp, vp, C.c_int)
 O._sig("orc_mlp_pdf", None, u32, vp, vp, vp)
 O._sig("orc_mlp_loss_grad", C.c_double, vp, u32, vp, vp, vp, vp, C.c_int)
+O._sig("orc_mlp_loss_grad_sel", C.c_double, vp, u32, vp, vp, vp, vp, vp, C.c_int)
 O._sig("orc_adam", None, u32, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, u32)
 O._sig("orc_mlp_sample", None, vp, u32, vp, vp)
 O._sig("orc_bf16", C.c_double, C.c_double)
 P = O.p
 PARAM_COUNT = lib.orc_mlp_param_count()
+# parameter offsets of the four layers ([out][in] weights, then biases)
+_IN, _HID, _OUT = 96, 64, 64
+OFFSETS = {"W0": 0}
+OFFSETS["B0"] = OFFSETS["W0"] + _HID * _IN
+OFFSETS["W1"] = OFFSETS["B0"] + _HID
+OFFSETS["B1"] = OFFSETS["W1"] + _HID * _HID
+OFFSETS["W2"] = OFFSETS["B1"] + _HID
+OFFSETS["B2"] = OFFSETS["W2"] + _HID * _HID
+OFFSETS["W3"] = OFFSETS["B2"] + _HID
+OFFSETS["B3"] = OFFSETS["W3"] + _OUT * _HID
+assert OFFSETS["B3"] + _OUT == PARAM_COUNT
 N_IN, N_OUT = 96, 64
 
 
@@ -46,11 +58,17 @@ def pdf(out, w):
     return r
 
 
-def loss_grad(params, x, wi, weight, emulate=False):
+def loss_grad(params, x, wi, weight, emulate=False, other_pdf=None):
+    """Mean loss and gradient; with other_pdf (the other strategies' density at wi per row), rows
+    with other_pdf > 0 add the one-sample mixture term of the learned selection probability."""
     params = np.ascontiguousarray(params, np.float64)
     x, wi, weight = [np.ascontiguousarray(a, np.float64) for a in (x, wi, weight)]
     g = np.zeros(PARAM_COUNT)
-    loss = lib.orc_mlp_loss_grad(P(params), len(x), P(x), P(wi), P(weight), P(g), 1 if emulate else 0)
+    if other_pdf is None:
+        loss = lib.orc_mlp_loss_grad(P(params), len(x), P(x), P(wi), P(weight), P(g), 1 if emulate else 0)
+    else:
+        po = np.ascontiguousarray(other_pdf, np.float64)
+        loss = lib.orc_mlp_loss_grad_sel(P(params), len(x), P(x), P(wi), P(weight), P(po), P(g), 1 if emulate else 0)
     return loss, g
 
 

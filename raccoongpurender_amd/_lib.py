@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libraccoongpu.so")
 
-RGPU_ABI_VERSION = 2
+RGPU_ABI_VERSION = 3
 RGPU_OK = 0
 RGPU_ERR_INVALID_ARG = -1
 RGB16_DTYPE = np.dtype([("r", "<u2"), ("g", "<u2"), ("b", "<u2"), ("pad", "<u2"), ("count", "<u4")])

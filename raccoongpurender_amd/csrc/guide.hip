@@ -257,9 +257,15 @@ __device__ void head_lobes_part(const float* out, int part, float lkmax, float* 
     lobe_params(o1, shift, e1 * is, lkmax, L + kGLobeFloats);
 }
 
-// d(-weight * log q(wi)) / d(raw outputs) for this lane's two lobes: d[j] is the derivative
-// by output 10 part + j (the 24 unused outputs have none).
-__device__ void head_grad_part(const float* out, int part, const float* wi, float weight, float lkmax, float* d) {
+// d(loss) / d(raw outputs) for this lane's two lobes: d[j] is the derivative by output
+// 10 part + j; *dsel (every lane) the derivative by the selection logit, output kGSelOut (the 23
+// other outputs have none). The loss is -weight * log q(wi), and with a learned selection
+// probability c = guide_selection(out[kGSelOut]) and a known density p_o > 0 of the other
+// strategies at wi also -weight * log(c q(wi) + (1 - c) p_o), the log density of the whole
+// one-sample mixture (Mueller et al. 2019): its derivative scales the lobes' gradient by
+// 1 + c q / p_mix and gives the logit -weight (q - p_o) c (1 - c) / p_mix inside the clamp.
+__device__ void head_grad_part(const float* out, int part, const float* wi, float weight, float lkmax, float other_pdf,
+                               bool learn_select, float* d, float* dsel) {
     float L[2 * kGLobeFloats], a[2], cm1[2];
     head_lobes_part(out, part, lkmax, L);
     for (int q = 0; q < 2; ++q) {
@@ -284,6 +290,17 @@ __device__ void head_grad_part(const float* out, int part, const float* wi, floa
         const float m = lobe_mean(o, mu);
         const float c = cm1[q] + 1.0f;
         for (int j = 0; j < 3; ++j) d[5 * q + j] = m > 0.0f ? -weight * r * kap * (wi[j] - c * Lk[j]) / m : 0.0f;
+    }
+    *dsel = 0.0f;
+    if (learn_select && other_pdf > 0.0f) {
+        const float z = out[kGSelOut];
+        const float sg = 1.0f / (1.0f + __expf(-z));
+        const float c = fminf(fmaxf(sg, kGSelMin), kGSelMax);
+        const float qv = __expf(lse);
+        const float pm = c * qv + (1.0f - c) * other_pdf;
+        const float scale = 1.0f + c * qv / pm;
+        for (int j = 0; j < 10; ++j) d[j] *= scale;
+        *dsel = (sg > kGSelMin && sg < kGSelMax) ? -weight * (qv - other_pdf) * c * (1.0f - c) / pm : 0.0f;
     }
 }
 
@@ -414,6 +431,7 @@ __device__ void infer_blocks(InferLds& t, const DevState& s, const WFParams& p, 
                 dst[0] = make_float4(L[0], L[1], L[2], L[3]);
                 dst[1] = make_float4(L[4], L[5], L[6], L[7]);
                 dst[2] = make_float4(L[8], L[9], L[10], L[11]);
+                if (g.learn_select && part == 0) g.sel[slot] = guide_selection(F[row * kPF + kGSelOut]);
             }
         }
         wave_lds_sync();  // the next block's encode overwrites the outputs
@@ -521,21 +539,22 @@ __global__ void __launch_bounds__(128) k_guide_fwdbwd(DevGuide g, uint32_t first
     {
         const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
         const bool live = base + row < rows;
-        float wi[3] = {0.0f, 0.0f, 1.0f}, wgt = 0.0f;  // padding rows contribute nothing
+        float wi[3] = {0.0f, 0.0f, 1.0f}, wgt = 0.0f, po = 0.0f;  // padding rows contribute nothing
         if (live) {
             const GuideSample& sm = g.samples[first + base + row];
             wi[0] = sm.wi[0];
             wi[1] = sm.wi[1];
             wi[2] = sm.wi[2];
             wgt = (g.weight_cap > 0.0f ? fminf(sm.weight, g.weight_cap) : sm.weight) * inv_batch;
+            po = sm.other_pdf;
         }
-        float d[10];
-        head_grad_part(t.F + row * kPF, part, wi, wgt, g.log_kappa_max, d);
+        float d[10], dsel;
+        head_grad_part(t.F + row * kPF, part, wi, wgt, g.log_kappa_max, po, g.learn_select != 0, d, &dsel);
         uint16_t* drow = dA + row * kPH;
 #pragma unroll
         for (int i = 0; i < 10; ++i) drow[10 * part + i] = f2bf(d[i]);
 #pragma unroll
-        for (int i = 0; i < 6; ++i) drow[5 * kGLobes + 6 * part + i] = 0;
+        for (int i = 0; i < 6; ++i) drow[5 * kGLobes + 6 * part + i] = part == 0 && i == 0 ? f2bf(dsel) : (uint16_t)0;
     }
     __syncthreads();
     store_cm(t.X, kPX, kGIn, blk + kAcX * kTile);

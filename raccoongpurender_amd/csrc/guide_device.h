@@ -3,8 +3,9 @@
 // Network: 96 encoded inputs (12-band sin/cos of the position normalised to the scene box,
 // normal, outgoing direction, zero padding) -> 3 ReLU layers of width 64 -> 64 outputs, of
 // which 40 parameterise 8 von Mises-Fisher lobes (unnormalised mean direction, log
-// concentration, mixture logit). Weights are kept in fp32 and mirrored in bfloat16 (row-major
-// [out][in]) for the matrix cores.
+// concentration, mixture logit) and output 40 is the logit of the learned selection
+// probability (used when the guide is configured to learn it). Weights are kept in fp32 and
+// mirrored in bfloat16 (row-major [out][in]) for the matrix cores.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -23,13 +24,24 @@ constexpr float kGLogKappaMin = -3.0f, kGLogKappaMax = 7.5f;
 // the mixture needs one exp per lobe.
 constexpr int kGLobeFloats = 6;
 constexpr int kGSlotFloats = kGLobes * kGLobeFloats;  // 48 floats of lobe parameters per slot
+// Learned selection probability of the guide in the one-sample mixture (Mueller et al. 2019,
+// "Neural Importance Sampling", learned selection probabilities): c = sigmoid(output 40),
+// clamped to [kGSelMin, kGSelMax] so that neither strategy is ever switched off.
+constexpr int kGSelOut = 5 * kGLobes;
+constexpr float kGSelMin = 0.05f, kGSelMax = 0.95f;
+__device__ __forceinline__ float guide_selection(float z) {
+    const float c = 1.0f / (1.0f + __expf(-z));
+    return fminf(fmaxf(c, kGSelMin), kGSelMax);
+}
 
 // One training sample: position (normalised to the scene box), normal, outgoing and
 // incident directions, the weight F/p of the direction (luminance of the incident radiance
-// times BSDF cosine over the sampling pdf) and the scene class code of a batch job
-// (0 for a single render, 1..kGClasses for the job's scene; an exact small float).
+// times BSDF cosine over the sampling pdf), the scene class code of a batch job
+// (0 for a single render, 1..kGClasses for the job's scene; an exact small float) and the
+// density of the other strategies (the BSDF/envmap mixture) at wi as the vertex had it, which
+// the learned selection probability is trained on (0: not known, no selection gradient).
 struct GuideSample {
-    float p[3], n[3], wo[3], wi[3], weight, scene_class, pad[2];
+    float p[3], n[3], wo[3], wi[3], weight, scene_class, other_pdf, pad;
 };
 static_assert(sizeof(GuideSample) == 64, "guide sample is 64 bytes");
 
@@ -61,8 +73,9 @@ struct DevGuide {
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     float* lobes = nullptr;           // per slot kGSlotFloats
+    float* sel = nullptr;             // per slot: learned selection probability (learn_select)
     GuideVertex* rec_geo = nullptr;   // kGRecords * nslots
-    float4* rec_T = nullptr;          // kGRecords * nslots (xyz used)
+    float4* rec_T = nullptr;          // kGRecords * nslots (xyz; w: the other strategies' pdf at wi)
     float4* rec_L = nullptr;          // kGRecords * nslots (xyz used)
     uint32_t* rec_count = nullptr;    // per slot
     float4* emit_w = nullptr;         // per slot: weights of the samples a finished path emits
@@ -73,6 +86,7 @@ struct DevGuide {
     uint32_t capacity = 0;
     float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};  // scene box normalisation
     float select_p = 0.5f;            // one-sample MIS selection probability of the guide
+    uint32_t learn_select = 0;        // per-vertex selection probability from output kGSelOut
     uint32_t max_depth = 0;           // guided vertices per path (0 = all)
     float weight_cap = 0.0f;          // training weights are clamped to it (0 = not clamped)
     float log_kappa_max = kGLogKappaMax;  // upper clamp of a lobe's log concentration

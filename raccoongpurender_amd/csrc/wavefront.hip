@@ -505,8 +505,9 @@ __device__ __forceinline__ void spawn(DevState& s, uint32_t slot, F3 ro, F3 rd, 
     store_ray_rec(s.ray, slot, o, kFltMax, wi, pixel);
 }
 
-// Guided one-sample MIS: the guide is chosen with probability g.select_p, otherwise the
-// reference's BSDF / envmap choice is made with the same uniform rescaled.
+// Guided one-sample MIS: the guide is chosen with probability g.select_p (or the vertex's
+// learned selection probability), otherwise the reference's BSDF / envmap choice is made with
+// the same uniform rescaled.
 enum { kStratBxdf = 0, kStratEnv = 1, kStratGuide = 2 };
 // logic_i counts the path's vertices from the camera (1 at the first hit).
 __device__ __forceinline__ bool guided_vertex(const DevGuide& g, uint32_t logic_i) {
@@ -521,9 +522,10 @@ __device__ __forceinline__ int select_strategy(uint4& st, float bxdf_p, bool gui
     return u <= bxdf_p ? kStratBxdf : kStratEnv;
 }
 
-// Remember a guided vertex until its path ends (throughput before it, radiance after its emission).
+// Remember a guided vertex until its path ends (throughput before it, radiance after its
+// emission, and the density of the other strategies at wi, for the learned selection).
 __device__ __forceinline__ void record_vertex(const DevGuide& g, const GuideFrame& fr, uint32_t slot, F3 hit, F3 n,
-                                              F3 wo, F3 wi, const PathRec& pr, F3 Le) {
+                                              F3 wo, F3 wi, const PathRec& pr, F3 Le, float other_pdf) {
     const uint32_t c = g.rec_count[slot];
     if (c >= (uint32_t)kGRecords) return;
     GuideVertex r;
@@ -535,7 +537,7 @@ __device__ __forceinline__ void record_vertex(const DevGuide& g, const GuideFram
     r.wi[0] = wi.x; r.wi[1] = wi.y; r.wi[2] = wi.z;
     const size_t at = (size_t)c * g.nslots + slot;
     g.rec_geo[at] = r;
-    g.rec_T[at] = make_float4(pr.T[0], pr.T[1], pr.T[2], 0.0f);
+    g.rec_T[at] = make_float4(pr.T[0], pr.T[1], pr.T[2], other_pdf);
     g.rec_L[at] = make_float4(pr.L[0] + pr.T[0] * Le.x, pr.L[1] + pr.T[1] * Le.y, pr.L[2] + pr.T[2] * Le.z, 0.0f);
     g.rec_count[slot] = c + 1;
 }
@@ -561,7 +563,7 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
         const F3 wo = neg3(rd);
         const bool guide_here = kGuided && guided_vertex(g, s.path[slot].logic_i);
-        const float gp = guide_here ? g.select_p : 0.0f;
+        const float gp = guide_here ? (g.learn_select ? g.sel[slot] : g.select_p) : 0.0f;
         const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
         const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
                                      : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
@@ -614,7 +616,7 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         store_shade(s.shade, slot, load3(L.Le), T);
         if (guide_here && g.train)
             record_vertex(g, jobs.guide_frame(jb, g), slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot],
-                          load3(L.Le));
+                          load3(L.Le), p.bxdf_p * pdf_b + (1.0f - p.bxdf_p) * pdf_e);
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
     }
@@ -641,7 +643,7 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jo
         const F3 wo = neg3(rd);
         const F3 Ngf = flip_toward(Ng, wo);
         const bool guide_here = kGuided && guided_vertex(g, s.path[slot].logic_i);
-        const float gp = guide_here ? g.select_p : 0.0f;
+        const float gp = guide_here ? (g.learn_select ? g.sel[slot] : g.select_p) : 0.0f;
         const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
         const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
                                      : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
@@ -689,7 +691,7 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jo
         store_shade(s.shade, slot, F3{0.0f, 0.0f, 0.0f}, T);
         if (guide_here && g.train)
             record_vertex(g, jobs.guide_frame(jb, g), slot, add3(ro, mul3(rd, e.tmin)), Ngf, wo, wi, s.path[slot],
-                          F3{0.0f, 0.0f, 0.0f});
+                          F3{0.0f, 0.0f, 0.0f}, p.bxdf_p * pdf_b + (1.0f - p.bxdf_p) * pdf_e);
         spawn(s, slot, ro, rd, e.tmin, wi, Ngf, true, pixel);
         s.rng[slot] = st;
     }
@@ -1202,7 +1204,7 @@ __global__ void __launch_bounds__(256) k_scatter_queues(DevState s, WFParams p, 
                 o[0] = r[0];
                 o[1] = r[1];
                 o[2] = r[2];
-                o[3] = make_float4(wj[j], cls, 0.0f, 0.0f);
+                o[3] = make_float4(wj[j], cls, g.rec_T[(size_t)j * g.nslots + slot].w, 0.0f);
             }
             ++idx;
         }

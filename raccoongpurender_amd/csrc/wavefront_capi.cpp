@@ -1384,6 +1384,11 @@ int rgpu_guide_create(rgpu_ctx* ctx, const rgpu_guide_config* cfg, rgpu_guide** 
     }
     d.weight_cap = cfg->weight_cap;
     d.log_kappa_max = cfg->log_kappa_max != 0.0f ? cfg->log_kappa_max : kGLogKappaMax;
+    if (cfg->learn_select > 1u) {
+        delete g;
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "learn_select must be 0 or 1");
+    }
+    d.learn_select = cfg->learn_select;
     for (int k = 0; k < 3; ++k) {
         float ext = cfg->scene_hi[k] - cfg->scene_lo[k];
         d.lo[k] = cfg->scene_lo[k];
@@ -1520,6 +1525,7 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
     detach_from_guide(w);
     DevGuide keep;
     keep.lobes = w->guide.lobes;
+    keep.sel = w->guide.sel;
     keep.rec_geo = w->guide.rec_geo;
     keep.rec_T = w->guide.rec_T;
     keep.rec_L = w->guide.rec_L;
@@ -1535,6 +1541,8 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
         int rc;
         if ((rc = wf_alloc(w, n * kGSlotFloats * 4, &p)) != RGPU_OK) return rc;
         d.lobes = (float*)p;
+        if ((rc = wf_alloc(w, n * 4, &p)) != RGPU_OK) return rc;
+        d.sel = (float*)p;
         if ((rc = wf_alloc(w, n * sizeof(GuideVertex) * kGRecords, &p)) != RGPU_OK) return rc;
         d.rec_geo = (GuideVertex*)p;
         if ((rc = wf_alloc(w, n * sizeof(float4) * kGRecords, &p)) != RGPU_OK) return rc;
@@ -1547,6 +1555,7 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
         d.rec_count = (uint32_t*)p;
     } else {
         d.lobes = w->guide.lobes;
+        d.sel = w->guide.sel;
         d.rec_geo = w->guide.rec_geo;
         d.rec_T = w->guide.rec_T;
         d.rec_L = w->guide.rec_L;
@@ -1557,9 +1566,9 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
     // detach, do not describe the rest of the path
     RGPU_HIP(ctx, hipMemset(d.rec_count, 0, n * 4));
     d.nslots = (uint32_t)n;
-    // eight inference workgroups per CU (13 KB of LDS and 111 VGPRs per lane allow 8), each
-    // looping over tiles
-    d.infer_blocks = (uint32_t)(ctx->cu_count > 0 ? ctx->cu_count : 256);  // one 16-wave workgroup per CU
+    // inference: one 16-wave workgroup per CU (the staged weights and the waves' activation
+    // buffers fill the CU's LDS), each wave looping over 16-row blocks
+    d.infer_blocks = (uint32_t)(ctx->cu_count > 0 ? ctx->cu_count : 256);
     d.enabled = sampling ? 1u : 0u;
     d.train = training ? 1u : 0u;
     w->guide = d;

@@ -8,7 +8,8 @@ from ._lib import RGPU_ABI_VERSION, ptr
 
 vp, u32 = C.c_void_p, C.c_uint32
 SAMPLE_DTYPE = np.dtype([("p", np.float32, 3), ("n", np.float32, 3), ("wo", np.float32, 3), ("wi", np.float32, 3),
-                         ("weight", np.float32), ("scene_class", np.float32), ("pad", np.float32, 2)])
+                         ("weight", np.float32), ("scene_class", np.float32), ("other_pdf", np.float32),
+                         ("pad", np.float32)])
 assert SAMPLE_DTYPE.itemsize == 64
 N_OUT = 64
 LOBES = 8
@@ -18,7 +19,7 @@ class rgpu_guide_config(C.Structure):
     _fields_ = [("abi_version", u32), ("select_p", C.c_float), ("capacity", u32), ("minibatch", u32),
                 ("learning_rate", C.c_float), ("seed", u32), ("scene_lo", C.c_float * 3),
                 ("scene_hi", C.c_float * 3), ("max_depth", u32), ("weight_cap", C.c_float),
-                ("log_kappa_max", C.c_float)]
+                ("log_kappa_max", C.c_float), ("learn_select", u32)]
 
 
 _sig = _lib._sig
@@ -43,11 +44,12 @@ PARAM_COUNT = _lib.lib.rgpu_guide_param_count()
 
 class Guide:
     def __init__(self, ctx, scene_lo=(0, 0, 0), scene_hi=(1, 1, 1), select_p=0.5, capacity=1 << 18,
-                 minibatch=16384, lr=1e-3, seed=1, max_depth=0, weight_cap=0.0, log_kappa_max=0.0):
+                 minibatch=16384, lr=1e-3, seed=1, max_depth=0, weight_cap=0.0, log_kappa_max=0.0,
+                 learn_select=False):
         self.ctx = ctx
         cfg = rgpu_guide_config(RGPU_ABI_VERSION, select_p, capacity, minibatch, lr, seed,
                                 (C.c_float * 3)(*scene_lo), (C.c_float * 3)(*scene_hi), max_depth, weight_cap,
-                                log_kappa_max)
+                                log_kappa_max, 1 if learn_select else 0)
         h = vp()
         ctx.check(_lib.lib.rgpu_guide_create(ctx.h, C.byref(cfg), C.byref(h)), "rgpu_guide_create")
         self.h = h

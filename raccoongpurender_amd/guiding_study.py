@@ -44,7 +44,8 @@ def render_reference(ctx, sd, spp, paths, seed=777):
 
 
 def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=1 << 18, select_p=0.5, lr=1e-3,
-                 minibatch=16384, max_depth=0, bxdf_p=0.0, weight_cap=0.0, log_kappa_max=0.0, warmup=0.0):
+                 minibatch=16384, max_depth=0, bxdf_p=0.0, weight_cap=0.0, log_kappa_max=0.0, warmup=0.0,
+                 learn_select=False):
     """`seconds` of rendering, plain or guided. A guided render trains while it renders; with
     `warmup` > 0 the image accumulated during the first `warmup` seconds (drawn while the
     network was still untrained) is discarded and accumulation restarts, training going on.
@@ -55,7 +56,7 @@ def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=
         lo = sd["points"].min(0) - 1e-3
         hi = sd["points"].max(0) + 1e-3
         g = Guide(ctx, lo, hi, select_p=select_p, capacity=capacity, lr=lr, seed=seed, minibatch=minibatch,
-                  max_depth=max_depth, weight_cap=weight_cap, log_kappa_max=log_kappa_max)
+                  max_depth=max_depth, weight_cap=weight_cap, log_kappa_max=log_kappa_max, learn_select=learn_select)
         attach(wf, g, sampling=True, training=True)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -78,9 +79,11 @@ def render_timed(ctx, sd, seconds, paths, guided, seed, train_every=4, capacity=
     return y, el, float(cnt.mean()), passes
 
 
-def equal_time_relmse(ctx, sd, seconds=4.0, ref_spp=16384, paths=1 << 20, seed=5, ref_paths=1 << 21):
+def equal_time_relmse(ctx, sd, seconds=4.0, ref_spp=16384, paths=1 << 20, seed=5, ref_paths=1 << 21,
+                      learned_selection=True):
     """Plain and guided renders of `seconds` each (guided time includes inference and
-    training) scored against a `ref_spp` plain render."""
+    training) scored against a `ref_spp` plain render; with learned_selection also a guided
+    render whose vertices learn their selection probability."""
     ref, t_ref, spp_ref = render_reference(ctx, sd, ref_spp, ref_paths)
     plain, t_p, spp_p, _ = render_timed(ctx, sd, seconds, paths, False, seed)
     guided, t_g, spp_g, passes = render_timed(ctx, sd, seconds, paths, True, seed)
@@ -91,6 +94,15 @@ def equal_time_relmse(ctx, sd, seconds=4.0, ref_spp=16384, paths=1 << 20, seed=5
            "guided_training_passes": passes, "reference_spp": round(spp_ref, 1), "reference_seconds": round(t_ref, 1)}
     out["ratio_plain_over_guided"] = round(out["relmse_plain"] / max(out["relmse_guided"], 1e-30), 3)
     out["ratio_trim1pct"] = round(out["relmse_trim1pct_plain"] / max(out["relmse_trim1pct_guided"], 1e-30), 3)
+    if learned_selection:
+        gl, _, spp_l, passes_l = render_timed(ctx, sd, seconds, paths, True, seed, learn_select=True)
+        out["learned_selection"] = {
+            "relmse": round(rel_mse(gl, ref), 4), "relmse_trim1pct": round(rel_mse_trimmed(gl, ref), 4),
+            "spp": round(spp_l, 1), "training_passes": passes_l}
+        out["learned_selection"]["ratio_plain_over_guided"] = round(
+            out["relmse_plain"] / max(out["learned_selection"]["relmse"], 1e-30), 3)
+        out["learned_selection"]["ratio_trim1pct"] = round(
+            out["relmse_trim1pct_plain"] / max(out["learned_selection"]["relmse_trim1pct"], 1e-30), 3)
     return out
 
 
@@ -107,12 +119,17 @@ def cornell_equal_time(ctx, size=64, seconds=1.0, warmup=0.25, ref_spp=1 << 20, 
     plain, _, spp_p, _ = render_timed(ctx, sd, seconds, paths, False, seed)
     guided, _, spp_g, passes = render_timed(ctx, sd, seconds, paths, True, seed, capacity=1 << 16, minibatch=8192,
                                             warmup=warmup)
+    learned, _, spp_l, _ = render_timed(ctx, sd, seconds, paths, True, seed, capacity=1 << 16, minibatch=8192,
+                                        warmup=warmup, learn_select=True)
     out = {"workload": f"Cornell box {size}x{size}, depth 5, {seconds} s each (guided: {warmup} s of training "
                        f"warm-up discarded, training continued), relMSE vs a {ref_spp}-spp plain render",
            "relmse_plain": float(f"{rel_mse(plain, ref):.4g}"), "relmse_guided": float(f"{rel_mse(guided, ref):.4g}"),
            "plain_spp": round(spp_p, 1), "guided_spp": round(spp_g, 1), "guided_training_passes": passes,
            "reference_spp": round(spp_ref, 1), "reference_seconds": round(t_ref, 1)}
     out["ratio_plain_over_guided"] = round(out["relmse_plain"] / max(out["relmse_guided"], 1e-30), 3)
+    out["learned_selection"] = {"relmse": float(f"{rel_mse(learned, ref):.4g}"), "spp": round(spp_l, 1)}
+    out["learned_selection"]["ratio_plain_over_guided"] = round(
+        out["relmse_plain"] / max(out["learned_selection"]["relmse"], 1e-30), 3)
     return out
 
 
@@ -129,11 +146,15 @@ def main():
     ap.add_argument("--height", type=int, default=270)
     ap.add_argument("--seconds", type=float, default=1.0)
     ap.add_argument("--study", action="store_true", help="equal-time relMSE against a 16k-spp plain render")
+    ap.add_argument("--paths", type=int, default=1 << 20, help="wavefront slots of the timed renders")
+    ap.add_argument("--ref-paths", type=int, default=1 << 21, help="wavefront slots of the reference render")
+    ap.add_argument("--ref-spp", type=int, default=16384)
     args = ap.parse_args()
     sd = full_render_scene(args.width, args.height)
     ctx = _lib.Context(0)
     if args.study:
-        print(json.dumps(equal_time_relmse(ctx, sd, seconds=args.seconds)), flush=True)
+        print(json.dumps(equal_time_relmse(ctx, sd, seconds=args.seconds, ref_spp=args.ref_spp, paths=args.paths,
+                                           ref_paths=args.ref_paths)), flush=True)
     else:
         for guided in (False, True):
             _, el, spp, passes = render_timed(ctx, sd, args.seconds, 1 << 20, guided, 5)

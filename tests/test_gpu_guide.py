@@ -106,6 +106,33 @@ def test_gradients_match_oracle(ctx):
             assert np.linalg.norm(gg[a:b] - gr[a:b]) / np.linalg.norm(gr[a:b]) < 3e-2, (a, b)
 
 
+def test_selection_gradients_match_oracle(ctx):
+    # learn_select: rows with a known density of the other strategies add the one-sample mixture
+    # term; the selection logit (output 40) and, scaled, the lobes get its gradient
+    rng = np.random.default_rng(12)
+    g = G.Guide(ctx, seed=4, learn_select=True)
+    w = g.params().astype(np.float64)
+    s = _samples(rng, 512)
+    s["other_pdf"] = np.where(rng.random(512) < 0.75, rng.uniform(0.02, 3.0, 512), 0.0)
+    gg = g.gradients(s).astype(np.float64)
+    _, gr = M.loss_grad(w, _x(s), s["wi"].astype(np.float64), s["weight"].astype(np.float64), emulate=True,
+                        other_pdf=s["other_pdf"].astype(np.float64))
+    cos = gg @ gr / (np.linalg.norm(gg) * np.linalg.norm(gr))
+    rel = np.linalg.norm(gg - gr) / np.linalg.norm(gr)
+    sel = np.r_[M.OFFSETS["W3"] + 40 * 64 + np.arange(64), M.OFFSETS["B3"] + 40]
+    rs = np.linalg.norm(gg[sel] - gr[sel]) / np.linalg.norm(gr[sel])
+    print(f"gradient cosine {cos:.6f} relative error {rel:.3g}; selection output {rs:.3g}", flush=True)
+    assert np.linalg.norm(gr[sel]) > 1e-6
+    assert cos > 0.9999 and rel < 1e-2 and rs < 3e-2
+    # without learn_select the same rows give the plain gradient (no selection term)
+    g0 = G.Guide(ctx, seed=4)
+    g0.set_params(w.astype(np.float32))
+    _, gr0 = M.loss_grad(w, _x(s), s["wi"].astype(np.float64), s["weight"].astype(np.float64), emulate=True)
+    gg0 = g0.gradients(s).astype(np.float64)
+    assert np.abs(gg0[sel]).max() == 0.0
+    assert np.linalg.norm(gg0 - gr0) / np.linalg.norm(gr0) < 1e-2
+
+
 def test_adam_step_matches_closed_form(ctx):
     rng = np.random.default_rng(3)
     g = G.Guide(ctx, seed=5, minibatch=256, lr=1e-3)
@@ -155,11 +182,11 @@ def test_training_learns_a_vmf_target(ctx):
     assert after > before + 1.5 and after > -0.5
 
 
-def _guided_lane(ctx, sd, n, seed, guided, flags=0, passes=12):
+def _guided_lane(ctx, sd, n, seed, guided, flags=0, passes=12, learn_select=False):
     lo, hi = sd["points"].min(0) - 1e-3, sd["points"].max(0) + 1e-3
     wf = build_lane(ctx, sd, n, clamp=1e30, seed=seed, flags=flags)
     if guided:
-        gd = G.Guide(ctx, lo, hi, capacity=1 << 15, minibatch=4096, seed=seed)
+        gd = G.Guide(ctx, lo, hi, capacity=1 << 15, minibatch=4096, seed=seed, learn_select=learn_select)
         G.attach(wf, gd, sampling=True, training=True)
         wf._guide = gd
     for _ in range(passes):
@@ -187,6 +214,23 @@ def test_lambertian_furnace_is_exact_with_guiding(ctx, guided):
     z = abs(m.mean() - 0.25) / se
     print(f"guided={guided} mean {m.mean():.6f} se {se:.3g} z={z:.2f}", flush=True)
     assert z < 5.0
+
+
+def test_furnace_is_exact_with_a_learned_selection_probability(ctx):
+    # Each vertex mixes the guide with its own learned probability; sampling and the mixture pdf
+    # read the same per-slot value, so the convex lambertian furnace stays exact (0.25).
+    sd = R.furnace_scene("lambertian", 32, 32, value=0.5, fill=True)
+    wf = _guided_lane(ctx, sd, 32 * 32 * 32, seed=15, guided=True, learn_select=True)
+    m = _pixel_means(wf)[:, 0]
+    se = m.std(ddof=1) / math.sqrt(m.size)
+    z = abs(m.mean() - 0.25) / se
+    wf.step(4)  # fill the training buffer again (the last pass consumed it)
+    samples = wf._guide.buffered_samples()
+    print(f"learned selection: mean {m.mean():.6f} se {se:.3g} z={z:.2f}; "
+          f"{(samples['other_pdf'] > 0).mean():.3f} of the samples carry the other strategies' pdf", flush=True)
+    assert z < 5.0
+    # the samples a path emits carry the density of the other strategies at wi
+    assert len(samples) > 100 and (samples["other_pdf"] > 0).mean() > 0.9 and np.isfinite(samples["other_pdf"]).all()
 
 
 def test_ward_furnace_guided_matches_plain(ctx):

@@ -64,6 +64,54 @@ def test_gradient_matches_finite_differences():
     assert np.abs(g).max() > 1e-4
 
 
+def test_selection_gradient_matches_finite_differences():
+    # The learned selection probability c = clamp(sigmoid(out[40]), 0.05, 0.95) adds, for rows
+    # with a known density p_o of the other strategies, -w log(c q + (1 - c) p_o): checked by
+    # central differences over the whole loss, on rows with and without p_o.
+    rng = np.random.default_rng(3)
+    params = _rand_params(2)
+    n = 6
+    x = M.encode(rng.random((n, 3)), _unit(rng, n), _unit(rng, n))
+    wi = _unit(rng, n)
+    wt = rng.uniform(0.5, 2.0, n)
+    po = np.array([0.3, 0.0, 1.7, 0.05, 0.0, 0.9])
+    loss, g = M.loss_grad(params, x, wi, wt, other_pdf=po)
+    loss0, g0 = M.loss_grad(params, x, wi, wt)
+    loss_z, g_z = M.loss_grad(params, x, wi, wt, other_pdf=np.zeros(n))
+    assert loss_z == loss0 and np.array_equal(g_z, g0)  # p_o = 0: no selection term
+    assert loss != loss0  # the mixture term of the four rows with p_o > 0
+    # output 40's weights and bias get a gradient only from the selection term
+    sel = np.r_[M.OFFSETS["W3"] + 40 * 64 + np.arange(64), M.OFFSETS["B3"] + 40]
+    assert np.abs(g0[sel]).max() == 0.0 and np.abs(g[sel]).max() > 1e-4
+    idx = np.concatenate([rng.choice(M.PARAM_COUNT, 200, replace=False), sel[::8]])
+    h = 1e-6
+    ok = 0
+    for i in idx:
+        p1, p2 = params.copy(), params.copy()
+        p1[i] += h
+        p2[i] -= h
+        fd = (M.loss_grad(p1, x, wi, wt, other_pdf=po)[0] - M.loss_grad(p2, x, wi, wt, other_pdf=po)[0]) / (2 * h)
+        ok += abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g[i]))
+    assert ok >= len(idx) - 3  # a parameter on a ReLU kink may disagree
+
+
+def test_selection_gradient_moves_toward_the_better_strategy():
+    # One row, everything but the selection bias fixed: where the guide's density at the sampled
+    # direction beats the other strategies' (q > p_o) the loss falls as c rises, and the other
+    # way round; gradient descent on the bias moves c accordingly.
+    params = np.zeros(M.PARAM_COUNT)
+    b40 = M.OFFSETS["B3"] + 40
+    params[M.OFFSETS["B3"] + 3] = 2.0  # one sharp-ish lobe along +z (bias-only network)
+    params[M.OFFSETS["B3"] + 2] = 1.0
+    params[M.OFFSETS["B3"] + 4] = 5.0
+    x = M.encode(np.array([[0.5, 0.5, 0.5]]), np.array([[0.0, 0.0, 1.0]]), np.array([[0.0, 0.0, 1.0]]))
+    wi = np.array([[0.0, 0.0, 1.0]])
+    q = M.pdf(M.forward(params, x), wi)[0]
+    for po, sign in ((q / 10.0, -1.0), (q * 10.0, 1.0)):
+        _, g = M.loss_grad(params, x, wi, np.ones(1), other_pdf=np.array([po]))
+        assert np.sign(g[b40]) == sign, (po, q, g[b40])
+
+
 def _fib_sphere(n):
     i = np.arange(n) + 0.5
     z = 1 - 2 * i / n
