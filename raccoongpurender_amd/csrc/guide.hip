@@ -474,23 +474,31 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevState s, WF
 }
 
 // Back-propagation through one layer for this wave's 16 rows:
-// dIn = (dZ W) masked by In > 0, written as bf16 into dIn. WT: [in][64] transposed weights.
+// dIn = (dZ W) masked by In > 0, written as bf16 into dIn. WT: [in][64] transposed weights. The
+// product is formed transposed (A: WT rows, B: dZ rows), so that each lane holds four
+// consecutive columns of one row: In is read and dIn written 8 bytes at a time.
 template <int K>
 __device__ void bwd_dx(const uint16_t* dZ, const uint16_t* WT, const uint16_t* In, int pin, uint16_t* dIn, int r0,
                        int lane) {
+    const int row = r0 + (lane & 15);
+    bf16x8 z[2];  // this wave's rows of dZ, read once for all output tiles
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) z[kb] = ld8(dZ + row * kPH + kb * 32 + (lane >> 4) * 8);
+#pragma unroll
     for (int it = 0; it < K / 16; ++it) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
-            bf16x8 a = ld8(dZ + (r0 + (lane & 15)) * kPH + kb * 32 + (lane >> 4) * 8);
-            bf16x8 b = ld8(WT + (it * 16 + (lane & 15)) * kGHid + kb * 32 + (lane >> 4) * 8);
-            acc = mma(a, b, acc);
+            const bf16x8 w = ld8(WT + (it * 16 + (lane & 15)) * kGHid + kb * 32 + (lane >> 4) * 8);
+            acc = mma(w, z[kb], acc);
         }
-        const int col = it * 16 + (lane & 15);
-        for (int r = 0; r < 4; ++r) {
-            const int row = r0 + (lane >> 4) * 4 + r;
-            const float x = bf2f(In[row * pin + col]);
-            dIn[row * kPH + col] = f2bf(x > 0.0f ? acc[r] : 0.0f);
-        }
+        const int col = it * 16 + (lane >> 4) * 4;
+        const uint2 in = *reinterpret_cast<const uint2*>(In + row * pin + col);
+        const float x0 = __uint_as_float(in.x << 16), x1 = __uint_as_float(in.x & 0xFFFF0000u);
+        const float x2 = __uint_as_float(in.y << 16), x3 = __uint_as_float(in.y & 0xFFFF0000u);
+        *reinterpret_cast<uint2*>(dIn + row * kPH + col) =
+            make_uint2(f2bf2(x0 > 0.0f ? acc[0] : 0.0f, x1 > 0.0f ? acc[1] : 0.0f),
+                       f2bf2(x2 > 0.0f ? acc[2] : 0.0f, x3 > 0.0f ? acc[3] : 0.0f));
     }
 }
 
@@ -501,9 +509,15 @@ constexpr int kAcD0 = kAcH3 + kGHid, kAcD1 = kAcD0 + kGHid, kAcD2 = kAcD1 + kGHi
 constexpr int kAcCols = kAcD3 + kGOut;  // 544 columns, 34,816 bytes per tile
 
 __device__ __forceinline__ void store_cm(const uint16_t* lds, int pitch, int cols, uint16_t* dst) {
-    for (int idx = threadIdx.x; idx < cols * kTile; idx += blockDim.x) {
-        const int c = idx / kTile, r = idx % kTile;
-        dst[c * kTile + r] = lds[r * pitch + c];
+    // each thread gathers 8 rows of one column from LDS and stores them as one 16-byte word;
+    // four consecutive threads write a column's 64 bytes
+    for (int idx = threadIdx.x; idx < cols * (kTile / 8); idx += blockDim.x) {
+        const int c = idx / (kTile / 8), r0 = (idx % (kTile / 8)) * 8;
+        const uint16_t* src = lds + r0 * pitch + c;
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[j] = (uint32_t)src[(2 * j) * pitch] | ((uint32_t)src[(2 * j + 1) * pitch] << 16);
+        *reinterpret_cast<uint4*>(dst + c * kTile + r0) = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
 
@@ -591,6 +605,8 @@ __global__ void __launch_bounds__(256) k_guide_dw(DevGuide g, uint32_t ntiles, u
     f32x4 acc[6];
     for (int j = 0; j < 6; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float bsum = 0.0f;  // column (threadIdx & 63) of dZ, rows 8 (threadIdx / 64) .. + 7 of each tile
+    // unrolled so that the loads of several tiles are in flight together
+#pragma unroll 4
     for (uint32_t tt = t0; tt < t1; ++tt) {
         const uint16_t* blk = g.act + (size_t)tt * kAcCols * kTile;
 #pragma unroll
