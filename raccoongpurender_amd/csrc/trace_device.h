@@ -224,10 +224,10 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
         // Backtrack: climb while the top bit is 0, then step to the sibling.
         if (bitstack == 0) break;
         if (stk.n != 0) {
-            while ((bitstack & 1) == 0) {
-                if (kCount) cnt->parent++;
-                bitstack = bitstack >> 1;
-            }
+            // the climb the reference makes, one bit per parent, as one shift
+            const uint32_t z = (uint32_t)__builtin_ctzll(bitstack);
+            if (kCount) cnt->parent += z;
+            bitstack = bitstack >> z;
             node = stk.pop();
             bitstack = bitstack ^ 1;
             continue;
@@ -442,7 +442,8 @@ __device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const Src& 
         return;
     }
     if (r.stk.n != 0) {
-        while ((r.bitstack & 1) == 0) r.bitstack = r.bitstack >> 1;
+        // the climb the reference makes, one bit per parent, as one shift (bitstack != 0)
+        r.bitstack = r.bitstack >> __builtin_ctzll(r.bitstack);
         r.node = r.stk.pop();
         r.bitstack = r.bitstack ^ 1;
         return;
