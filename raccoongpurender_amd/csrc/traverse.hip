@@ -161,19 +161,19 @@ __global__ void __launch_bounds__(256) k_trace_persistent(const TraceRay* __rest
         const bool walking = walk_live(st) && !pending;
         const uint32_t npending = (uint32_t)__popcll(__ballot(pending));
         const uint32_t nwalk = (uint32_t)__popcll(__ballot(walking));
-        if (npending >= leaf_batch || npending >= nwalk) {
-            if (pending) {
+        const bool leaf_phase = npending >= leaf_batch || npending >= nwalk;
+        if (leaf_phase ? pending : walking) {
+            if (leaf_phase) {
                 if constexpr (kVariant != kVariantStackless)
                     threaded_leaf(st, sc.prims, sc.sphere_begin);
                 else
                     stackless_leaf(st, src, sc.prims, sc.sphere_begin);
-                if (!walk_live(st)) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
+            } else {
+                if constexpr (kVariant != kVariantStackless)
+                    threaded_visit(st, finite);
+                else
+                    stackless_visit(st, src, finite);
             }
-        } else if (walking) {
-            if constexpr (kVariant != kVariantStackless)
-                threaded_visit(st, finite);
-            else
-                stackless_visit(st, src, finite);
             if (!walk_live(st)) store_hit(hits, ray, st.prim, st.tmin, st.u, st.v);
         }
     }
