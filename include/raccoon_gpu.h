@@ -512,8 +512,10 @@ int rgpu_guide_read_samples(rgpu_guide* g, float* out, uint32_t rows, uint32_t* 
  * attach; paths in flight start recording afresh. Guide and wavefront must share a device. */
 int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, int training);
 int rgpu_wavefront_guide_pending(rgpu_wavefront* w, uint32_t* pending);
-/* Test hook: the lobe records the last guide inference wrote for slots [first, first + count):
- * 48 floats per slot, 8 lobes of (mean x, y, z, kappa, weight, log(weight C(kappa))). */
+/* Test hook: the lobe records the last guide inference wrote, records [first, first + count) in
+ * the order of the bounce's vertices (the lambertian queue, then the Ward queue; rgpu_wavefront_
+ * state's queue items give their slots): 48 floats each, 8 lobes of (mean x, y, z, kappa,
+ * weight, log(weight C(kappa))). */
 int rgpu_wavefront_guide_lobes(rgpu_wavefront* w, uint32_t first, uint32_t count, float* out);
 /* If at least min_samples are buffered: one pass of minibatch Adam steps, buffer emptied. */
 int rgpu_wavefront_guide_train(rgpu_wavefront* w, uint32_t min_samples, uint32_t* trained);

@@ -425,13 +425,13 @@ __device__ void infer_blocks(InferLds& t, const DevState& s, const WFParams& p, 
             float L[2 * kGLobeFloats];
             // all four lanes of a row take part
             head_lobes_part(F + row * kPF, part, g.log_kappa_max, L);
-            if (b * 16 + row < rows) {
-                float4* dst =
-                    reinterpret_cast<float4*>(g.lobes + (size_t)slot * kGSlotFloats + 2 * kGLobeFloats * part);
+            const uint32_t at = b * 16 + row;  // the vertex's position in the two queues
+            if (at < rows) {
+                float4* dst = reinterpret_cast<float4*>(g.lobes + (size_t)at * kGSlotFloats + 2 * kGLobeFloats * part);
                 dst[0] = make_float4(L[0], L[1], L[2], L[3]);
                 dst[1] = make_float4(L[4], L[5], L[6], L[7]);
                 dst[2] = make_float4(L[8], L[9], L[10], L[11]);
-                if (g.learn_select && part == 0) g.sel[slot] = guide_selection(F[row * kPF + kGSelOut]);
+                if (g.learn_select && part == 0) g.sel[at] = guide_selection(F[row * kPF + kGSelOut]);
             }
         }
         wave_lds_sync();  // the next block's encode overwrites the outputs

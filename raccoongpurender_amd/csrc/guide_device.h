@@ -72,8 +72,10 @@ struct DevGuide {
     float* partials = nullptr;        // kGTrainGroups per-workgroup gradient sums
     float* adam_m = nullptr;
     float* adam_v = nullptr;
-    float* lobes = nullptr;           // per slot kGSlotFloats
-    float* sel = nullptr;             // per slot: learned selection probability (learn_select)
+    // per guided vertex of the bounce, in queue order (the lambertian queue, then the Ward
+    // queue), so that inference writes and the shading kernels read them contiguously
+    float* lobes = nullptr;           // kGSlotFloats each
+    float* sel = nullptr;             // learned selection probability (learn_select)
     GuideVertex* rec_geo = nullptr;   // kGRecords * nslots
     float4* rec_T = nullptr;          // kGRecords * nslots (xyz; w: the other strategies' pdf at wi)
     float4* rec_L = nullptr;          // kGRecords * nslots (xyz used)

@@ -563,12 +563,13 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         const F3 Ng = F3{e.ngx, e.ngy, e.ngz};
         const F3 wo = neg3(rd);
         const bool guide_here = kGuided && guided_vertex(g, s.path[slot].logic_i);
-        const float gp = guide_here ? (g.learn_select ? g.sel[slot] : g.select_p) : 0.0f;
+        // the guide's records of this vertex: at its position in the lambertian queue
+        const float gp = guide_here ? (g.learn_select ? g.sel[i] : g.select_p) : 0.0f;
         const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
         const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
                                      : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
         const bool bxdf = strat == kStratBxdf;
-        const float* lobes = kGuided ? g.lobes + (size_t)slot * kGSlotFloats : nullptr;
+        const float* lobes = kGuided ? g.lobes + (size_t)i * kGSlotFloats : nullptr;
         F3 wi;
         float pdf_e = 0.0f, pdf_b;
         if (strat == kStratEnv) wi = sample_env_6axis(env, flip_toward(Ng, wo), st, &pdf_e);
@@ -643,12 +644,14 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jo
         const F3 wo = neg3(rd);
         const F3 Ngf = flip_toward(Ng, wo);
         const bool guide_here = kGuided && guided_vertex(g, s.path[slot].logic_i);
-        const float gp = guide_here ? (g.learn_select ? g.sel[slot] : g.select_p) : 0.0f;
+        // the guide's records of this vertex: after the lambertian queue's, at its position
+        const size_t at = (size_t)s.counts[kMatLambertian] + i;
+        const float gp = guide_here ? (g.learn_select ? g.sel[at] : g.select_p) : 0.0f;
         const float sel_b = (1.0f - gp) * p.bxdf_p, sel_e = (1.0f - gp) * (1.0f - p.bxdf_p);
         const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
                                      : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
         const bool bxdf = strat == kStratBxdf;
-        const float* lobes = kGuided ? g.lobes + (size_t)slot * kGSlotFloats : nullptr;
+        const float* lobes = kGuided ? g.lobes + at * kGSlotFloats : nullptr;
         F3 wi;
         float pdf_e = 0.0f;
         if (strat == kStratEnv) wi = sample_env_6axis(env, Ngf, st, &pdf_e);

@@ -1577,11 +1577,12 @@ int rgpu_wavefront_attach_guide(rgpu_wavefront* w, rgpu_guide* g, int sampling, 
     return RGPU_OK;
 }
 
-// Test hook: the per-slot lobe records of slots [first, first + count) (48 floats per slot:
-// 8 lobes of mean xyz, kappa, weight, log(weight C(kappa))) as the last inference wrote them.
+// Test hook: lobe records [first, first + count) as the last inference wrote them, in vertex
+// order (the lambertian queue, then the Ward queue); 48 floats each: 8 lobes of mean xyz,
+// kappa, weight, log(weight C(kappa)).
 int rgpu_wavefront_guide_lobes(rgpu_wavefront* w, uint32_t first, uint32_t count, float* out) {
     if (!w || (count && !out) || !w->guide.lobes) return RGPU_ERR_INVALID_ARG;
-    if ((uint64_t)first + count > w->params.n) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "slot range outside the lane");
+    if ((uint64_t)first + count > w->params.n) return fail(w->ctx, RGPU_ERR_INVALID_ARG, "record range outside the lane");
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -119,8 +119,9 @@ def pending(wavefront):
     return n.value
 
 
-def slot_lobes(wavefront, first, count):
-    """Lobe records of slots [first, first + count) as the last inference wrote them: an array
+def vertex_lobes(wavefront, first, count):
+    """Lobe records [first, first + count) as the last inference wrote them, one per vertex of
+    that bounce in queue order (the lambertian queue, then the Ward queue): an array
     (count, 8, 6) of (mean xyz, kappa, weight, log(weight C(kappa)))."""
     out = np.zeros((count, LOBES, 6), np.float32)
     wavefront.ctx.check(_lib.lib.rgpu_wavefront_guide_lobes(wavefront.h, first, count, ptr(out)),

@@ -369,7 +369,7 @@ def test_inference_lobes_match_the_forward_pass(ctx):
     slots = np.concatenate([st["items"][1, :counts[1]], st["items"][4, :counts[4]]]).astype(np.int64)
     assert len(slots) > 1000
     wf.step(1)  # its inference runs over these two queues
-    lob = G.slot_lobes(wf, 0, wf.n)[slots].astype(np.float64)
+    lob = G.vertex_lobes(wf, 0, len(slots)).astype(np.float64)  # in the same queue order
     ray, ext = st["rays"][slots], st["ext"][slots]
     inv = np.float32(1.0) / (hi - lo)
     s = np.zeros(len(slots), G.SAMPLE_DTYPE)

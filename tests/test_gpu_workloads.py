@@ -156,7 +156,7 @@ def test_guided_full_render_is_unbiased_and_trains_on_valid_samples(ctx, c3):
     for _ in range(6):
         wf.step(4)
         # every lobe record an inference wrote (kappa > 0) has a unit mean and weights summing to 1
-        lob = G.slot_lobes(wf, 0, 1 << 20).astype(np.float64)
+        lob = G.vertex_lobes(wf, 0, 1 << 20).astype(np.float64)
         written = lob[:, :, 3] > 0
         mu_len = np.linalg.norm(lob[:, :, :3], axis=2)
         bad_mu = written & ~(np.abs(mu_len - 1.0) <= 1e-4)
@@ -164,9 +164,9 @@ def test_guided_full_render_is_unbiased_and_trains_on_valid_samples(ctx, c3):
         bad_w = written.any(1) & ~(np.abs(wsum - 1.0) <= 1e-4)
         if bad_mu.any() or bad_w.any():
             print(f"lobe records: {written.sum()} written, {bad_mu.sum()} means off unit length, "
-                  f"{bad_w.sum()} slots with weights not summing to 1", flush=True)
+                  f"{bad_w.sum()} records with weights not summing to 1", flush=True)
             for sl in np.unique(np.flatnonzero(bad_mu.any(1) | bad_w))[:3]:
-                print(f"  slot {sl}: {lob[sl].tolist()}", flush=True)
+                print(f"  record {sl}: {lob[sl].tolist()}", flush=True)
         assert not bad_mu.any() and not bad_w.any()
         s = gd.buffered_samples()
         assert len(s) > 0
