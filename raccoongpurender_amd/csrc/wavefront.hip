@@ -883,7 +883,13 @@ __device__ __forceinline__ void volume_through(const DevState& s, const DevMater
     *hit_volume = -1;
     if (0 < v.count) {
         unsigned long long u = xo_uint62(st);
-        *hit_volume = v.volumes[u % (unsigned long long)v.count];
+        // the drawn entry by selects: an indexed read of the register copy would go through
+        // scratch memory
+        const uint32_t k = (uint32_t)(u % (unsigned long long)v.count);
+        int32_t h = v.volumes[0];
+#pragma unroll
+        for (uint32_t j = 1; j < 7; ++j) h = k == j ? v.volumes[j] : h;
+        *hit_volume = h;
     }
     *tmin = kFltMax;
     if (0 <= *hit_volume) {
