@@ -607,22 +607,26 @@ public:
                 context_.synchronize();
             });
         });
-        workers_.emplace_back([this, margin_period_ms]() {
-            guarded([this, margin_period_ms]() {
-                while (continue_) {
-                    if (4 < lane_->step_count()) create_color_image();
-                    std::this_thread::sleep_for(std::chrono::milliseconds(margin_period_ms));
-                }
+        try {
+            workers_.emplace_back([this, margin_period_ms]() {
+                guarded([this, margin_period_ms]() {
+                    while (continue_) {
+                        if (4 < lane_->step_count()) create_color_image();
+                        std::this_thread::sleep_for(std::chrono::milliseconds(margin_period_ms));
+                    }
+                });
             });
-        });
+        } catch (...) {
+            join_workers();  // the step thread must not outlive a constructor that failed
+            throw;
+        }
     }
-    ~WavefrontPathTracing() { stop(); }
+    // Joins the threads; a worker's error that stop() has not rethrown is dropped, since a
+    // destructor must not throw.
+    ~WavefrontPathTracing() { join_workers(); }
     // Ends continuous rendering (joins the threads); rethrows a worker's error.
     void stop() {
-        continue_ = false;
-        for (std::thread& t : workers_)
-            if (t.joinable()) t.join();
-        workers_.clear();
+        join_workers();
         std::lock_guard<std::mutex> g(error_mu_);
         if (error_) {
             std::exception_ptr e = error_;
@@ -653,6 +657,12 @@ public:
     std::function<void(RGBA8ValueType*, int, int)> onColorRecieved;
 
 private:
+    void join_workers() {
+        continue_ = false;
+        for (std::thread& t : workers_)
+            if (t.joinable()) t.join();
+        workers_.clear();
+    }
     template <class F>
     void guarded(F f) {
         try {
