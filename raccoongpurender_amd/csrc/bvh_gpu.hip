@@ -3,8 +3,9 @@
 //
 // Every quantity the split decisions read is order-independent: node and centroid boxes are
 // min/max reductions (atomics on an order-preserving integer encoding of the floats), bin counts
-// are sums, and the SAH arithmetic runs per node in one thread with the host builder's exact
-// expressions (float centroids and bin indices, double areas and costs). Primitives move by a
+// are sums, and the SAH search uses the host builder's exact expressions (float centroids and
+// bin indices, double areas and costs), by one wave per task (prefix and suffix scans, then the
+// first minimum in the host's axis and bin order) or one lane per tiny task. Primitives move by a
 // stable partition (a prefix sum over the level's flags), as the host builder's
 // std::stable_partition does. Nodes are numbered level by level on the device and renumbered
 // into the host builder's depth-first pre-order on the host; the leaves' primitive order is then
@@ -110,11 +111,15 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
 
 // Task bounds: 12 encoded words per task (box lo, box hi, centroid lo, centroid hi); the lows
 // start at kEncMax, the highs at kEncMin.
-__global__ void k_clear_bounds(uint32_t* __restrict__ tb, uint32_t ntasks) {
+// Also lists the big tasks by their slot (bslot: the exclusive scan of the big-task flags), so
+// that k_split runs one wave per big task.
+__global__ void k_clear_bounds(uint32_t* __restrict__ tb, uint32_t ntasks, const Task* __restrict__ tasks,
+                               const uint32_t* __restrict__ bslot, uint32_t* __restrict__ big_task) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 12 * ntasks) return;
     const uint32_t w = i % 12;
     tb[i] = (w < 3 || (w >= 6 && w < 9)) ? kEncMax : kEncMin;
+    if (i < ntasks && tasks[i].end - tasks[i].beg > kSmallTask) big_task[bslot[i]] = i;
 }
 // Bins: per task, 3 axes x B bins x 7 words (box lo, box hi, count).
 __global__ void k_clear_bins(uint32_t* __restrict__ bins, size_t words) {
@@ -158,8 +163,10 @@ __device__ __forceinline__ uint32_t wave_red(uint32_t v) {
 // A workgroup of k_bounds / k_bin covers kSubBlocks runs of 256 positions. A run inside one
 // task is reduced in the workgroup and added to a running partial, which goes to device memory
 // with one atomic per word when the task changes and at the end, so the top levels (few tasks,
-// every workgroup on the same words) take 1/kSubBlocks of the global atomics. A run across task
-// ends, or with positions of small tasks, takes one atomic per primitive and word.
+// every workgroup on the same words) take 1/kSubBlocks of the global atomics. In a run across
+// task ends, or with positions of small tasks, k_bounds reduces each task segment of a wave
+// and takes one atomic per segment and word, and k_bin accumulates the run's tasks in its LDS
+// bins one after the other.
 constexpr uint32_t kSubBlocks = 8;
 
 __device__ __forceinline__ void flush_bounds(const uint32_t* acc, uint32_t t, uint32_t (*part)[12], uint32_t* tb) {
@@ -229,11 +236,34 @@ __global__ void __launch_bounds__(256) k_bounds(const uint32_t* __restrict__ ids
                 const bool mn = w < 3 || (w >= 6 && w < 9);
                 acc[w] = mn ? min(acc[w], v[w]) : max(acc[w], v[w]);
             }
-        } else if (t != kNone) {
+        } else {
+            // A task's positions are contiguous, so a wave of this run holds a few task segments:
+            // each segment is reduced across its lanes (an inclusive scan whose steps combine
+            // only lanes of the same task), and its last lane alone takes the atomics. A wave with
+            // no position of a big task has nothing to add (most runs of the deep levels).
+            if (__ballot(t != kNone) == 0) continue;
+            const uint32_t lane = threadIdx.x & 63;
+            bool same[6];
+            for (int k = 0; k < 6; ++k) {
+                const uint32_t o = __shfl_up(t, 1u << k, 64);
+                same[k] = lane >= (1u << k) && o == t;
+            }
             for (int w = 0; w < 12; ++w) {
                 const bool mn = w < 3 || (w >= 6 && w < 9);
-                if (mn) atomicMin(tb + 12 * (size_t)t + w, v[w]);
-                else atomicMax(tb + 12 * (size_t)t + w, v[w]);
+                uint32_t x = v[w];
+                for (int k = 0; k < 6; ++k) {
+                    const uint32_t o = __shfl_up(x, 1u << k, 64);
+                    if (same[k]) x = mn ? min(x, o) : max(x, o);
+                }
+                v[w] = x;
+            }
+            const uint32_t tn = __shfl_down(t, 1u, 64);
+            if (t != kNone && (lane == 63 || tn != t)) {
+                for (int w = 0; w < 12; ++w) {
+                    const bool mn = w < 3 || (w >= 6 && w < 9);
+                    if (mn) atomicMin(tb + 12 * (size_t)t + w, v[w]);
+                    else atomicMax(tb + 12 * (size_t)t + w, v[w]);
+                }
             }
         }
     }
@@ -269,6 +299,27 @@ __device__ __forceinline__ void flush_bins(const uint32_t* lb, uint32_t* g, uint
     }
 }
 
+// One primitive's centroid bin on each axis of its task: box min/max and count, by atomics
+// on dst (LDS bins or the task's bins in device memory).
+__device__ __forceinline__ void bin_primitive(uint32_t p, uint32_t t, const PBox* __restrict__ pb,
+                                              const float* __restrict__ cen, const uint32_t* __restrict__ tb,
+                                              uint32_t B, uint32_t* dst) {
+    const PBox b = pb[p];
+    for (int k = 0; k < 3; ++k) {
+        const float clo = fdec(tb[12 * (size_t)t + 6 + k]), chi = fdec(tb[12 * (size_t)t + 9 + k]);
+        const float ext = chi - clo;
+        if (!(ext > 0.0f)) continue;
+        const float scale = (float)B / ext;
+        const uint32_t bi = bin_of(cen[3 * (size_t)p + k], clo, scale, B);
+        uint32_t* q = dst + (k * B + bi) * 7;
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(q + a, fenc(b.lo[a]));
+            atomicMax(q + 3 + a, fenc(b.hi[a]));
+        }
+        atomicAdd(q + 6, 1u);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_bin(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ pos_task,
                                              uint32_t n, const PBox* __restrict__ pb, const float* __restrict__ cen,
                                              const uint32_t* __restrict__ tb, uint32_t B, const uint32_t* __restrict__ slot,
@@ -282,36 +333,26 @@ __global__ void __launch_bounds__(256) k_bin(const uint32_t* __restrict__ ids, c
         const uint32_t i = first + threadIdx.x;
         const uint32_t last = min(first + blockDim.x, n) - 1;
         const uint32_t t0 = pos_task[first], t1 = pos_task[last];
-        const bool uniform = t0 != kNone && t0 == t1;
-        if (uniform && t0 != lb_t) {
-            __syncthreads();  // the LDS atomics of earlier runs have landed
-            if (lb_t != kNone) {
-                flush_bins(lb, bins + (size_t)slot[lb_t] * words, words);
-                __syncthreads();
-            }
-            init_bins(lb, words);
-            __syncthreads();
-            lb_t = t0;
-        }
         const uint32_t t = i < n ? pos_task[i] : kNone;
-        if (t != kNone) {
-            const uint32_t p = ids[i];
-            const PBox b = pb[p];
-            uint32_t* dst = uniform ? lb : bins + (size_t)slot[t] * words;
-            for (int k = 0; k < 3; ++k) {
-                const float clo = fdec(tb[12 * (size_t)t + 6 + k]), chi = fdec(tb[12 * (size_t)t + 9 + k]);
-                const float ext = chi - clo;
-                if (!(ext > 0.0f)) continue;
-                const float scale = (float)B / ext;
-                const uint32_t bi = bin_of(cen[3 * (size_t)p + k], clo, scale, B);
-                uint32_t* q = dst + (k * B + bi) * 7;
-                for (int a = 0; a < 3; ++a) {
-                    atomicMin(q + a, fenc(b.lo[a]));
-                    atomicMax(q + 3 + a, fenc(b.hi[a]));
+        // The run's first and last tasks (both kNone-free ones) are accumulated in the LDS bins in
+        // turn. A task of this kernel holds more than kSmallTask >= 256 positions, so no other
+        // task can lie inside a run; were one to, its primitives would go to device memory.
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t tt = side == 0 ? t0 : t1;  // the same in every thread
+            if (tt == kNone || (side == 1 && tt == t0)) continue;
+            if (tt != lb_t) {
+                __syncthreads();  // the LDS atomics of earlier runs have landed
+                if (lb_t != kNone) {
+                    flush_bins(lb, bins + (size_t)slot[lb_t] * words, words);
+                    __syncthreads();
                 }
-                atomicAdd(q + 6, 1u);
+                init_bins(lb, words);
+                __syncthreads();
+                lb_t = tt;
             }
+            if (t == tt) bin_primitive(ids[i], t, pb, cen, tb, B, lb);
         }
+        if (t != kNone && t != t0 && t != t1) bin_primitive(ids[i], t, pb, cen, tb, B, bins + (size_t)slot[t] * words);
     }
     if (lb_t != kNone) {
         __syncthreads();
@@ -341,51 +382,6 @@ struct Best {
     double cost;
 };
 
-// Serial form, one thread (bins in device memory).
-__device__ Best serial_best(const uint32_t* tbins, uint32_t B, const float* clo, const float* chi) {
-    Best best{-1, 0, DBL_MAX};
-    double rA[kMaxBins];
-    uint32_t rN[kMaxBins];
-    for (int k = 0; k < 3; ++k) {
-        const float ext = chi[k] - clo[k];
-        if (!(ext > 0.0f)) continue;
-        // suffix areas and counts, then the prefix on the way up (the host's lA/rA loops)
-        float alo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ahi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-        uint32_t cnt = 0;
-        for (int b = (int)B - 1; b >= 0; --b) {
-            const uint32_t* q = tbins + (k * B + (uint32_t)b) * 7;
-            if (q[6]) {
-                for (int a = 0; a < 3; ++a) {
-                    alo[a] = fminf(alo[a], fdec(q[a]));
-                    ahi[a] = fmaxf(ahi[a], fdec(q[3 + a]));
-                }
-            }
-            cnt += q[6];
-            rA[b] = box_area(alo, ahi);
-            rN[b] = cnt;
-        }
-        for (int a = 0; a < 3; ++a) {
-            alo[a] = FLT_MAX;
-            ahi[a] = -FLT_MAX;
-        }
-        cnt = 0;
-        for (uint32_t b = 0; b + 1 < B; ++b) {
-            const uint32_t* q = tbins + (k * B + b) * 7;
-            if (q[6]) {
-                for (int a = 0; a < 3; ++a) {
-                    alo[a] = fminf(alo[a], fdec(q[a]));
-                    ahi[a] = fmaxf(ahi[a], fdec(q[3 + a]));
-                }
-            }
-            cnt += q[6];
-            const double lA = box_area(alo, ahi);
-            if (cnt == 0 || rN[b + 1] == 0) continue;
-            const double c = lA * cnt + rA[b + 1] * rN[b + 1];
-            if (c < best.cost) best = Best{k, b, c};
-        }
-    }
-    return best;
-}
 
 // Wave form (every lane of the wave calls it; bins in LDS): lane b holds bin b of an axis, the
 // prefix and suffix boxes and counts are wave scans (min, max and sums, exact in any order),
@@ -510,8 +506,8 @@ __device__ Best wave_best(const uint32_t* L, uint32_t B, const float* clo, const
 }
 
 // The host builder's split decision for task t (Builder::build in bvh_build.cpp), from its 12
-// encoded bound words tbw and either its bins (serial scan) or the best split found by a wave.
-__device__ void decide_task(uint32_t t, const Task tk, const uint32_t* tbw, const uint32_t* tbins, const Best* given,
+// encoded bound words tbw and the best binned split a wave found over its bins.
+__device__ void decide_task(uint32_t t, const Task tk, const uint32_t* tbw, const Best& bb,
                             const Params prm, Decision* __restrict__ dec, float* __restrict__ node_box,
                             uint32_t* __restrict__ node_leaf, double* __restrict__ node_sah,
                             uint32_t* __restrict__ node_depth, uint32_t* __restrict__ any_median) {
@@ -528,13 +524,11 @@ __device__ void decide_task(uint32_t t, const Task tk, const uint32_t* tbw, cons
     node_depth[tk.node] = tk.depth;
     const double parea = box_area(lo, hi);
     Decision d{kLeaf, 0, 0, 0};
-    const uint32_t B = prm.bins;
     int best_axis = -1;
     uint32_t best_bin = 0;
     double best_cost = DBL_MAX;
     const bool force_median = tk.depth + levels_needed(n, prm.max_leaf) + 1 >= prm.max_depth;
     if (n > 1 && !force_median) {
-        const Best bb = given ? *given : serial_best(tbins, B, clo, chi);
         best_axis = bb.axis;
         best_bin = bb.bin;
         best_cost = bb.cost;
@@ -574,17 +568,28 @@ __device__ void decide_task(uint32_t t, const Task tk, const uint32_t* tbw, cons
     dec[t] = d;
 }
 
-__global__ void k_split(const Task* __restrict__ tasks, uint32_t ntasks, const uint32_t* __restrict__ tb,
-                        const uint32_t* __restrict__ slot, const uint32_t* __restrict__ bins, Params prm,
-                        Decision* __restrict__ dec,
-                        float* __restrict__ node_box, uint32_t* __restrict__ node_leaf, double* __restrict__ node_sah,
-                        uint32_t* __restrict__ node_depth, uint32_t* __restrict__ any_median) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntasks) return;
+// One wave per big task (slot order): the SAH search over its bins in device memory as in
+// k_small_tasks, then the split decision by lane 0.
+__global__ void __launch_bounds__(256) k_split(const Task* __restrict__ tasks, uint32_t ntasks,
+                                               const uint32_t* __restrict__ tb, const uint32_t* __restrict__ slot,
+                                               const uint32_t* __restrict__ big_task, const uint32_t* __restrict__ bins,
+                                               Params prm, Decision* __restrict__ dec, float* __restrict__ node_box,
+                                               uint32_t* __restrict__ node_leaf, double* __restrict__ node_sah,
+                                               uint32_t* __restrict__ node_depth, uint32_t* __restrict__ any_median) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t sl = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (sl >= slot[ntasks]) return;  // whole waves; slot[ntasks] is the number of big tasks
+    const uint32_t t = big_task[sl];
     const Task tk = tasks[t];
-    if (tk.end - tk.beg <= kSmallTask) return;
-    decide_task(t, tk, tb + 12 * (size_t)t, bins + (size_t)slot[t] * 3 * prm.bins * 7, nullptr, prm, dec,
-                node_box, node_leaf, node_sah, node_depth, any_median);
+    const uint32_t* tbw = tb + 12 * (size_t)t;
+    const uint32_t* L = bins + (size_t)sl * 3 * prm.bins * 7;
+    float cl[3], ch[3];
+    for (int k = 0; k < 3; ++k) {
+        cl[k] = fdec(tbw[6 + k]);
+        ch[k] = fdec(tbw[9 + k]);
+    }
+    const Best best = wave_best(L, prm.bins, cl, ch, lane);
+    if (lane == 0) decide_task(t, tk, tbw, best, prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
 }
 
 // One wave per small task: bounds (wave reductions), bins (in the wave's LDS), then the split
@@ -665,7 +670,7 @@ __global__ void __launch_bounds__(256) k_small_tasks(const Task* __restrict__ ta
     }
     const Best best = wave_best(L, B, cl, ch, lane);
     if (lane == 0)
-        decide_task(t, tk, v, L, &best, prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
+        decide_task(t, tk, v, best, prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
 }
 
 // Left flag of every position of a split (binned) or median task.
@@ -770,7 +775,7 @@ __global__ void __launch_bounds__(256) k_tiny_tasks(const Task* __restrict__ tas
         }
         if (kcost < best.cost) best = Best{k, kbin, kcost};
     }
-    decide_task(t, tk, v, nullptr, &best, prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
+    decide_task(t, tk, v, best, prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
 }
 
 __global__ void k_flags(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ pos_task, uint32_t n,
@@ -1005,6 +1010,7 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
     const size_t bwords = max_big * 3 * B * 7;
     uint32_t* bins = m.get<uint32_t>(bwords);
     uint32_t* bslot = m.get<uint32_t>((size_t)max_tasks + 1);
+    uint32_t* big_task = m.get<uint32_t>(max_big);
 
     hipLaunchKernelGGL(k_prim_boxes, grid_of(n), dim3(256), 0, s, d_pts, stride, d_idx, ntris, d_sph, nspheres, pb,
                        cen, ids);
@@ -1017,14 +1023,15 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
         hipLaunchKernelGGL(k_task_of, grid_of(n), dim3(256), 0, s, tasks, ntasks, n, pos_task, pos_big);
         hipLaunchKernelGGL(k_big_flags, grid_of((size_t)ntasks + 1), dim3(256), 0, s, tasks, ntasks, sflag);
         check(scan_exclusive(sflag, bslot, (size_t)ntasks + 1, stmp, s), "scan");
-        hipLaunchKernelGGL(k_clear_bounds, grid_of(12 * (size_t)ntasks), dim3(256), 0, s, tb, ntasks);
+        hipLaunchKernelGGL(k_clear_bounds, grid_of(12 * (size_t)ntasks), dim3(256), 0, s, tb, ntasks, tasks, bslot,
+                           big_task);
         hipLaunchKernelGGL(k_bounds, grid_of(n, 256 * kSubBlocks), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb);
         hipLaunchKernelGGL(k_clear_bins, grid_of(bwords), dim3(256), 0, s, bins, bwords);
         hipLaunchKernelGGL(k_bin, grid_of(n, 256 * kSubBlocks), dim3(256), 0, s, ids, pos_big, n, pb, cen, tb, B, bslot,
                            bins);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, any_median, (size_t)1, 0u);
-        hipLaunchKernelGGL(k_split, grid_of(ntasks, 128), dim3(128), 0, s, tasks, ntasks, tb, bslot, bins, prm, dec,
-                           node_box, node_leaf, node_sah, node_depth, any_median);
+        hipLaunchKernelGGL(k_split, grid_of(64 * max_big), dim3(256), 0, s, tasks, ntasks, tb, bslot, big_task, bins,
+                           prm, dec, node_box, node_leaf, node_sah, node_depth, any_median);
         hipLaunchKernelGGL(k_small_tasks, grid_of(64 * (size_t)ntasks), dim3(256), 0, s, tasks, ntasks, ids, pb, cen,
                            prm, tb, dec, node_box, node_leaf, node_sah, node_depth, any_median);
         hipLaunchKernelGGL(k_tiny_tasks, grid_of(ntasks), dim3(256), 0, s, tasks, ntasks, ids, pb, cen, prm, tb, dec,
