@@ -1037,42 +1037,49 @@ void build_bvh_device(const float* points, uint32_t stride, uint32_t npoints, co
         hipLaunchKernelGGL(k_tiny_tasks, grid_of(ntasks), dim3(256), 0, s, tasks, ntasks, ids, pb, cen, prm, tb, dec,
                            node_box, node_leaf, node_sah, node_depth, any_median);
         check(hipGetLastError(), "level kernels");
-        // median splits (near the depth limit, or no centroid extent): sort the segment on the host
-        uint32_t hany = 0;
+        // The level's partition (flags, scan, stable scatter into ids2) and the numbering of its
+        // splits are queued at once, and the host learns in one round trip both the split count
+        // and whether a task takes an object median. Median splits (near the depth limit, or no
+        // centroid extent) are rare: the host then sorts those segments of ids by centroid and
+        // the partition runs again; the split count does not depend on it.
+        auto partition = [&]() {
+            hipLaunchKernelGGL(k_flags, grid_of((size_t)n + 1), dim3(256), 0, s, ids, pos_task, n, cen, tb, dec, B,
+                               flag);
+            check(scan_exclusive(flag, pre, (size_t)n + 1, scan_tmp, s), "scan");
+            hipLaunchKernelGGL(k_scatter, grid_of(n), dim3(256), 0, s, ids, pos_task, n, tasks, dec, flag, pre, ids2);
+        };
+        partition();
+        hipLaunchKernelGGL(k_split_flags, grid_of((size_t)ntasks + 1), dim3(256), 0, s, dec, ntasks, sflag);
+        check(scan_exclusive(sflag, spre, (size_t)ntasks + 1, stmp, s), "scan");
+        uint32_t hany = 0, nsplit = 0;
         check(hipMemcpyAsync(&hany, any_median, 4, hipMemcpyDeviceToHost, s), "copy");
+        check(hipMemcpyAsync(&nsplit, spre + ntasks, 4, hipMemcpyDeviceToHost, s), "copy");
         check(hipStreamSynchronize(s), "sync");
-        std::vector<Decision> hdec(hany ? ntasks : 0);
-        std::vector<Task> htask(hany ? ntasks : 0);
         if (hany) {
+            std::vector<Decision> hdec(ntasks);
+            std::vector<Task> htask(ntasks);
             check(hipMemcpyAsync(hdec.data(), dec, ntasks * sizeof(Decision), hipMemcpyDeviceToHost, s), "copy");
             check(hipMemcpyAsync(htask.data(), tasks, ntasks * sizeof(Task), hipMemcpyDeviceToHost, s), "copy");
             check(hipStreamSynchronize(s), "sync");
-        }
-        for (uint32_t t = 0; t < (hany ? ntasks : 0u); ++t) {
-            if (hdec[t].kind != kMedian) continue;
-            if (hcen.empty()) {
-                hcen.resize(3 * (size_t)n);
-                check(hipMemcpy(hcen.data(), cen, 12 * (size_t)n, hipMemcpyDeviceToHost), "copy");
+            for (uint32_t t = 0; t < ntasks; ++t) {
+                if (hdec[t].kind != kMedian) continue;
+                if (hcen.empty()) {
+                    hcen.resize(3 * (size_t)n);
+                    check(hipMemcpy(hcen.data(), cen, 12 * (size_t)n, hipMemcpyDeviceToHost), "copy");
+                }
+                const Task& tk = htask[t];
+                std::vector<uint32_t> seg(tk.end - tk.beg);
+                check(hipMemcpy(seg.data(), ids + tk.beg, 4 * seg.size(), hipMemcpyDeviceToHost), "copy");
+                const uint32_t k = hdec[t].axis;
+                std::sort(seg.begin(), seg.end(), [&](uint32_t x, uint32_t y) {
+                    const float cx = hcen[3 * (size_t)x + k], cy = hcen[3 * (size_t)y + k];
+                    return cx < cy || (cx == cy && x < y);
+                });
+                check(hipMemcpy(ids + tk.beg, seg.data(), 4 * seg.size(), hipMemcpyHostToDevice), "copy");
             }
-            const Task& tk = htask[t];
-            std::vector<uint32_t> seg(tk.end - tk.beg);
-            check(hipMemcpy(seg.data(), ids + tk.beg, 4 * seg.size(), hipMemcpyDeviceToHost), "copy");
-            const uint32_t k = hdec[t].axis;
-            std::sort(seg.begin(), seg.end(), [&](uint32_t x, uint32_t y) {
-                const float cx = hcen[3 * (size_t)x + k], cy = hcen[3 * (size_t)y + k];
-                return cx < cy || (cx == cy && x < y);
-            });
-            check(hipMemcpy(ids + tk.beg, seg.data(), 4 * seg.size(), hipMemcpyHostToDevice), "copy");
+            partition();
         }
-        hipLaunchKernelGGL(k_flags, grid_of((size_t)n + 1), dim3(256), 0, s, ids, pos_task, n, cen, tb, dec, B, flag);
-        check(scan_exclusive(flag, pre, (size_t)n + 1, scan_tmp, s), "scan");
-        hipLaunchKernelGGL(k_scatter, grid_of(n), dim3(256), 0, s, ids, pos_task, n, tasks, dec, flag, pre, ids2);
         std::swap(ids, ids2);
-        hipLaunchKernelGGL(k_split_flags, grid_of((size_t)ntasks + 1), dim3(256), 0, s, dec, ntasks, sflag);
-        check(scan_exclusive(sflag, spre, (size_t)ntasks + 1, stmp, s), "scan");
-        uint32_t nsplit = 0;
-        check(hipMemcpyAsync(&nsplit, spre + ntasks, 4, hipMemcpyDeviceToHost, s), "copy");
-        check(hipStreamSynchronize(s), "sync");
         if (nnodes + 2 * (size_t)nsplit > max_nodes || 2 * (size_t)nsplit > max_tasks)
             throw std::runtime_error("device BVH build: node count out of bounds");
         hipLaunchKernelGGL(k_children, grid_of(ntasks), dim3(256), 0, s, tasks, ntasks, dec, spre, pre, nnodes, next,
