@@ -393,8 +393,8 @@ def main():
         print(json.dumps(envmap_bench()), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the path shards by pixel/sample range with a replicated scene; see the design notes).",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (across GPUs the path "
+              "would shard by pixel and sample range, each GPU holding the whole scene).",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,8 +1,8 @@
 // Binned-SAH BVH2 builder and the packers for the device node layouts.
 //
 // The reference builds its tree with Embree's rtcBuildBVH (quality HIGH, binary,
-// leaves of 1..5 primitives; common/stackless_bvh.hpp:108-130). Embree is not part
-// of this build: this is a top-down binned SAH builder (16 bins per axis, leaves of
+// leaves of 1..5 primitives; common/stackless_bvh.hpp:108-130). This library does not
+// use Embree: this is a top-down binned SAH builder (16 bins per axis, leaves of
 // at most 4 primitives) with a depth cap so that every leaf fits the 64-bit
 // bitstack of the stackless traversal. Traversal results are defined on whatever
 // tree is built, so the tree itself carries no parity obligation; it is checked by
