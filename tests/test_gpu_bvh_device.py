@@ -40,7 +40,7 @@ def test_device_build_equals_host_build(ctx, kw):
 
 
 @pytest.mark.parametrize("case", ["soup", "spheres", "spheres_only", "tiny1", "tiny2", "tiny5", "coincident",
-                                  "depth_limit"])
+                                  "depth_limit", "coincident_big", "depth_limit_big"])
 def test_device_build_special_inputs(ctx, case):
     print(f"case device BVH {case}", flush=True)
     sph = None
@@ -58,10 +58,16 @@ def test_device_build_special_inputs(ctx, case):
         sph = np.concatenate([rng.random((3000, 3)) * 10 - 5, rng.random((3000, 1)) * 0.2 + 0.01], axis=1)
     elif case.startswith("tiny"):
         v, f = _soup(int(case[4:]), 5)
-    elif case == "coincident":
+    elif case.startswith("coincident"):
         # identical triangles: no centroid extent on any axis, object median all the way down
-        v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (200, 1))
-        f = np.arange(600, dtype=np.uint32).reshape(-1, 3)
+        # (big: the top tasks hold more than 2048 and go through the position-parallel kernels)
+        m = 5000 if case.endswith("big") else 200
+        v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (m, 1))
+        f = np.arange(3 * m, dtype=np.uint32).reshape(-1, 3)
+    elif case == "depth_limit_big":
+        # medians forced near the depth limit while tasks are still big
+        v, f = _soup(40000, 7)
+        kw = dict(max_depth=8)
     else:
         v, f = _soup(2000, 6)
         kw = dict(max_depth=6)
