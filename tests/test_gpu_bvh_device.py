@@ -4,7 +4,8 @@ The device build must give the host builder's tree node for node: the same boxes
 bit), children, leaf ranges and primitive order, hence the same depth, leaf count and (up to
 the order of a double sum) SAH cost, and the same hits. Cases cover the binned path at several
 bin counts, leaf sizes and node costs, spheres after triangles, tiny inputs, the object-median
-path (coincident centroids; a depth limit that forces it) and the argument checks.
+path (coincident centroids; a depth limit that forces it), also on tasks large enough for the
+position-parallel kernels, and the argument checks.
 """
 import numpy as np
 import pytest
