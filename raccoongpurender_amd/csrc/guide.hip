@@ -427,10 +427,11 @@ __device__ void infer_blocks(InferLds& t, const DevState& s, const WFParams& p, 
             head_lobes_part(F + row * kPF, part, g.log_kappa_max, L);
             const uint32_t at = b * 16 + row;  // the vertex's position in the two queues
             if (at < rows) {
-                float4* dst = reinterpret_cast<float4*>(g.lobes + (size_t)at * kGSlotFloats + 2 * kGLobeFloats * part);
+                // this lane's 12 floats are planes 3 part .. 3 part + 2 of the vertex
+                float4* dst = reinterpret_cast<float4*>(g.lobes) + (size_t)(3 * part) * g.nslots + at;
                 dst[0] = make_float4(L[0], L[1], L[2], L[3]);
-                dst[1] = make_float4(L[4], L[5], L[6], L[7]);
-                dst[2] = make_float4(L[8], L[9], L[10], L[11]);
+                dst[g.nslots] = make_float4(L[4], L[5], L[6], L[7]);
+                dst[2 * (size_t)g.nslots] = make_float4(L[8], L[9], L[10], L[11]);
                 if (g.learn_select && part == 0) g.sel[at] = guide_selection(F[row * kPF + kGSelOut]);
             }
         }

@@ -73,8 +73,9 @@ struct DevGuide {
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     // per guided vertex of the bounce, in queue order (the lambertian queue, then the Ward
-    // queue), so that inference writes and the shading kernels read them contiguously
-    float* lobes = nullptr;           // kGSlotFloats each
+    // queue), as twelve planes of nslots float4 (plane k: floats 4k..4k+3 of every vertex), so
+    // that inference writes and the shading kernels read them contiguously (guide_load_lobes)
+    float* lobes = nullptr;           // kGSlotFloats per vertex
     float* sel = nullptr;             // learned selection probability (learn_select)
     GuideVertex* rec_geo = nullptr;   // kGRecords * nslots
     float4* rec_T = nullptr;          // kGRecords * nslots (xyz; w: the other strategies' pdf at wi)
@@ -113,6 +114,19 @@ __device__ __forceinline__ float guide_pdf(const float* lobes, float wx, float w
         s += expf(L[5] - 0.5f * L[3] * (dx * dx + dy * dy + dz * dz));
     }
     return s;
+}
+
+// The 48 lobe floats of guided vertex `at` from the twelve float4 planes of DevGuide::lobes.
+__device__ __forceinline__ void guide_load_lobes(const float* lobes, uint32_t nslots, size_t at, float* L) {
+    const float4* q = reinterpret_cast<const float4*>(lobes);
+#pragma unroll
+    for (int k = 0; k < kGSlotFloats / 4; ++k) {
+        const float4 v = q[(size_t)k * nslots + at];
+        L[4 * k] = v.x;
+        L[4 * k + 1] = v.y;
+        L[4 * k + 2] = v.z;
+        L[4 * k + 3] = v.w;
+    }
 }
 
 // Sample a direction: lobe by weight (u0), then Jakob's inversion of the vMF (u1, u2).

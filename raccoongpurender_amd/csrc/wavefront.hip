@@ -569,14 +569,14 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
                                      : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
         const bool bxdf = strat == kStratBxdf;
-        const float* lobes = kGuided ? g.lobes + (size_t)i * kGSlotFloats : nullptr;
         F3 wi;
         float pdf_e = 0.0f, pdf_b;
         if (strat == kStratEnv) wi = sample_env_6axis(env, flip_toward(Ng, wo), st, &pdf_e);
         if (kGuided && strat == kStratGuide) {
             float u0 = xo_uniform(st), u1 = xo_uniform(st), u2 = xo_uniform(st);
-            float w[3];
-            guide_sample(lobes, u0, u1, u2, w);
+            float w[3], L[kGSlotFloats];
+            guide_load_lobes(g.lobes, g.nslots, i, L);
+            guide_sample(L, u0, u1, u2, w);
             wi = F3{w[0], w[1], w[2]};
         }
         // The reference samples and evaluates the cosine lobe and the envmap pdf with the
@@ -606,7 +606,11 @@ __global__ void __launch_bounds__(256) k_shade_lambertian(DevState s, WFParams p
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;
-        if (guide_here) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
+        if (guide_here) {
+            float L[kGSlotFloats];
+            guide_load_lobes(g.lobes, g.nslots, i, L);
+            pdf += gp * guide_pdf(L, wi.x, wi.y, wi.z);
+        }
         float c = dot3(wi, Ngf);
         F3 T = F3{0.0f, 0.0f, 0.0f};
         if (0.0 < (double)c) {
@@ -651,14 +655,14 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jo
         const int strat = guide_here ? select_strategy(st, p.bxdf_p, true, gp)
                                      : (select_bxdf(st, sel_b) ? kStratBxdf : kStratEnv);
         const bool bxdf = strat == kStratBxdf;
-        const float* lobes = kGuided ? g.lobes + at * kGSlotFloats : nullptr;
         F3 wi;
         float pdf_e = 0.0f;
         if (strat == kStratEnv) wi = sample_env_6axis(env, Ngf, st, &pdf_e);
         if (kGuided && strat == kStratGuide) {
             float u0 = xo_uniform(st), u1 = xo_uniform(st), u2 = xo_uniform(st);
-            float d[3];
-            guide_sample(lobes, u0, u1, u2, d);
+            float d[3], L[kGSlotFloats];
+            guide_load_lobes(g.lobes, g.nslots, at, L);
+            guide_sample(L, u0, u1, u2, d);
             wi = F3{d[0], d[1], d[2]};
         }
         if (bxdf) {
@@ -671,7 +675,11 @@ __global__ void __launch_bounds__(256) k_shade_ward(DevState s, WFParams p, J jo
         float pdf = 0.0f;
         pdf += sel_b * pdf_b;
         pdf += sel_e * pdf_e;
-        if (guide_here) pdf += gp * guide_pdf(lobes, wi.x, wi.y, wi.z);
+        if (guide_here) {
+            float L[kGSlotFloats];
+            guide_load_lobes(g.lobes, g.nslots, at, L);
+            pdf += gp * guide_pdf(L, wi.x, wi.y, wi.z);
+        }
         float c = dot3(wi, Ngf);
         F3 T = F3{0.0f, 0.0f, 0.0f};
         if (0.0f < c) {

@@ -1586,9 +1586,17 @@ int rgpu_wavefront_guide_lobes(rgpu_wavefront* w, uint32_t first, uint32_t count
     rgpu_ctx* ctx = w->ctx;
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     RGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (count)
-        RGPU_HIP(ctx, hipMemcpy(out, w->guide.lobes + (size_t)first * kGSlotFloats, (size_t)count * kGSlotFloats * 4,
-                                hipMemcpyDeviceToHost));
+    if (count) {
+        // the device holds twelve planes of float4 (plane k: floats 4k..4k+3 of every record)
+        const size_t planes = kGSlotFloats / 4, n = w->guide.nslots;
+        std::vector<float> stage(planes * 4 * (size_t)count);
+        for (size_t k = 0; k < planes; ++k)
+            RGPU_HIP(ctx, hipMemcpy(stage.data() + k * 4 * count, w->guide.lobes + 4 * (k * n + first), 16 * (size_t)count,
+                                    hipMemcpyDeviceToHost));
+        for (size_t r = 0; r < count; ++r)
+            for (size_t k = 0; k < planes; ++k)
+                for (int q = 0; q < 4; ++q) out[r * kGSlotFloats + 4 * k + q] = stage[(k * count + r) * 4 + q];
+    }
     return RGPU_OK;
 }
 
