@@ -492,9 +492,6 @@ __device__ __forceinline__ void stackless_visit(StacklessRay& r, const Src& src,
     }
     stackless_backtrack(r, src, q3);
 }
-__device__ __forceinline__ void stackless_visit(StacklessRay& r, const SLNode* __restrict__ nodes, bool boxes_finite) {
-    stackless_visit(r, SLGlobal{reinterpret_cast<const float4*>(nodes)}, boxes_finite);
-}
 
 // The pending leaf of a lane: its primitives, then the backtrack.
 template <class Src>
@@ -505,10 +502,6 @@ __device__ __forceinline__ void stackless_leaf(StacklessRay& r, const Src& src, 
     const uint32_t beg = q3.x & ~kLeafFlag, end = q3.y;
     for (uint32_t s = beg; s < end; ++s) test_prim(prims, s, sphere_begin, r.ro, r.rd, &r.tmin, &r.u, &r.v, &r.prim);
     stackless_backtrack(r, src, q3);
-}
-__device__ __forceinline__ void stackless_leaf(StacklessRay& r, const SLNode* __restrict__ nodes,
-                                               const PrimRec* __restrict__ prims, int32_t sphere_begin) {
-    stackless_leaf(r, SLGlobal{reinterpret_cast<const float4*>(nodes)}, prims, sphere_begin);
 }
 
 
