@@ -42,8 +42,8 @@ struct TraceScene {
 // Launch shape of the persistent ray-cast kernel: CUs to fill, resident waves per SIMD (grid =
 // cu_count * waves_per_simd workgroups of 256), the idle-lane count that triggers a refill and
 // the count of lanes waiting at leaves that triggers their primitive tests. Zeros take the
-// defaults (256 CUs, 8 waves; refill at 24 lanes and leaves at 6 for the stackless walk, 48
-// and 8 for the threaded one), measured on 4M incoherent rays over 1.3M triangles.
+// defaults (256 CUs, 8 waves; refill at 24 lanes for the stackless walk, 48 for the threaded
+// ones; leaves at 6), measured on 4M incoherent rays over 1.3M triangles.
 struct TraceTuning {
     uint32_t cu_count = 0;
     uint32_t waves_per_simd = 0;
@@ -51,8 +51,7 @@ struct TraceTuning {
     uint32_t leaf_batch = 0;
     uint32_t treelet = 0;  // stackless records staged in LDS per workgroup; kTreeletOff: none
 };
-constexpr uint32_t kDefaultLeafBatch = 8;           // threaded persistent walk
-constexpr uint32_t kDefaultLeafBatchStackless = 6;  // stackless persistent walk
+constexpr uint32_t kDefaultLeafBatch = 6;
 // One thread per ray (coherent rays, the wavefront's per-slot trace): leaves wait until as many
 // lanes wait as walk on.
 constexpr uint32_t kPerRayLeafBatch = 64;
