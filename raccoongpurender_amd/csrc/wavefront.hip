@@ -1139,35 +1139,42 @@ __global__ void __launch_bounds__(256) k_logic(DevState s, WFParams p, J jobs, D
 // kNumQueues scans the training-sample counts on top of the samples already buffered and
 // advances the buffer cursor (saturating at the buffer's capacity).
 __global__ void __launch_bounds__(1024) k_scan_queues(DevState s, uint32_t* sample_count, uint32_t sample_cap) {
-    __shared__ uint32_t part[1024];
+    // One workgroup per queue (and one for the training samples). Each of its 16 waves owns a
+    // contiguous run of the queue's per-block counts and reads it 64 at a time, coalesced: the
+    // runs' totals first, scanned across the waves in LDS, then the exclusive prefixes written
+    // in place, each 64-count piece scanned across the wave.
+    __shared__ uint32_t wtot[16];
     const int q = blockIdx.x;
     if (q == kNumQueues && !sample_count) return;
     uint32_t* c = s.block_counts + (size_t)q * s.nblocks;
     const uint32_t nb = s.nblocks;
-    const uint32_t per = (nb + 1023) / 1024;
-    const uint32_t beg = threadIdx.x * per;
-    const uint32_t end = min(beg + per, nb);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t run = (nb + 15) / 16;  // counts per wave
+    const uint32_t beg = min(wave * run, nb), end = min(beg + run, nb);
     uint32_t sum = 0;
-    for (uint32_t i = beg; i < end; ++i) sum += c[i];
-    part[threadIdx.x] = sum;
+    for (uint32_t i = beg + lane; i < end; i += 64) sum += c[i];
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if (lane == 0) wtot[wave] = sum;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
     const uint32_t base = q == kNumQueues ? *sample_count : 0u;
-    uint32_t run = base + part[threadIdx.x] - sum;  // exclusive prefix of this thread's chunk
-    for (uint32_t i = beg; i < end; ++i) {
-        uint32_t v = c[i];
-        c[i] = run;
-        run += v;
+    uint32_t carry = base;  // exclusive prefix of this wave's run
+    for (uint32_t w = 0; w < wave; ++w) carry += wtot[w];
+    for (uint32_t i0 = beg; i0 < end; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const uint32_t v = i < end ? c[i] : 0u;
+        uint32_t incl = v;  // inclusive scan of the piece across the wave
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        if (i < end) c[i] = carry + incl - v;
+        carry += __shfl(incl, 63, 64);
     }
-    __syncthreads();
     if (threadIdx.x == 1023) {
-        if (q == kNumQueues) *sample_count = min(base + part[1023], sample_cap);
-        else s.counts[q] = part[1023];
+        uint32_t total = 0;
+        for (int w = 0; w < 16; ++w) total += wtot[w];
+        if (q == kNumQueues) *sample_count = min(base + total, sample_cap);
+        else s.counts[q] = total;
     }
 }
 
