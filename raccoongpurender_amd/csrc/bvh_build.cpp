@@ -359,6 +359,19 @@ void number_top_breadth_first(std::vector<SLNode>* nodes, uint32_t top) {
     sl.swap(out);
 }
 
+void mark_leaf_links(std::vector<SLNode>* nodes) {
+    std::vector<SLNode>& sl = *nodes;
+    auto leaf = [&](uint32_t i) { return (sl[i].a & kLeafFlag) != 0; };
+    for (size_t i = 0; i < sl.size(); ++i) {
+        SLNode& s = sl[i];
+        if (!(s.a & kLeafFlag)) {
+            if (leaf(s.a)) s.a |= kChildLeaf;
+            if (leaf(s.b)) s.b |= kChildLeaf;
+        }
+        if (i != 0 && leaf(s.sibling)) s.sibling |= kChildLeaf;  // the root has no sibling
+    }
+}
+
 void pack_threaded(const BVHTree& t, std::vector<MTNode>* out) {
     const uint32_t nn = (uint32_t)t.nodes.size();
     out->assign((size_t)kMTOrders * nn, MTNode());

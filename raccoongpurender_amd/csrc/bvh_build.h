@@ -47,6 +47,8 @@ void pack_stackless(const BVHTree& t, std::vector<SLNode>* out);
 // breadth-first order (root stays 0); the other records keep their relative order. Child,
 // parent and sibling links follow; the walk and its results do not change.
 void number_top_breadth_first(std::vector<SLNode>* nodes, uint32_t top);
+// Sets kChildLeaf on the child and sibling ids that name leaves (after any renumbering).
+void mark_leaf_links(std::vector<SLNode>* nodes);
 // out: kMTOrders tables of t.nodes.size() records, order-major (+X,-X,+Y,-Y,+Z,-Z, then the
 // single table in the build's child order).
 void pack_threaded(const BVHTree& t, std::vector<MTNode>* out);

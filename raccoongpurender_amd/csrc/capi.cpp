@@ -348,6 +348,8 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     if (nspheres && !spheres) return fail(ctx, RGPU_ERR_INVALID_ARG, "bad sphere array");
     if (bvh->tree.prim_ids.size() != (size_t)ntris + nspheres)
         return fail(ctx, RGPU_ERR_INVALID_ARG, "BVH was built for a different primitive count");
+    if (bvh->tree.nodes.size() >= (size_t)kChildLeaf)
+        return fail(ctx, RGPU_ERR_INVALID_ARG, "BVH has 2^30 nodes or more");
     RGPU_GUARD_BEGIN
     RGPU_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<SLNode> sl;
@@ -355,6 +357,7 @@ int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, u
     std::vector<PrimRec> pr;
     pack_stackless(bvh->tree, &sl);
     number_top_breadth_first(&sl, kTreeletMaxNodes);
+    mark_leaf_links(&sl);
     pack_threaded(bvh->tree, &mt);
     pack_prims(bvh->tree, points, stride, indices, ntris, spheres, nspheres, &pr);
     std::vector<float> p4((size_t)npoints * 4, 0.0f);

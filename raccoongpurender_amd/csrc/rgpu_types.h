@@ -21,6 +21,8 @@ namespace rgpu {
 //
 // branch: a = left child id, b = right child id (bit 31 clear in a)
 // leaf:   a = kLeafFlag | first primitive slot, b = one past the last slot
+// A child id (a, b of a branch) and a sibling id carry kChildLeaf when that node is a leaf, so
+// a walk that enters it goes straight to its primitives; node ids stay below 2^30.
 // A branch visit reads q0..q3 (64 B); a leaf visit and every backtrack step read q3 only.
 struct alignas(16) SLNode {
     float lmin[3];
@@ -35,6 +37,7 @@ struct alignas(16) SLNode {
 static_assert(sizeof(SLNode) == 64, "stackless node record must be 64 bytes");
 
 static const uint32_t kLeafFlag = 0x80000000u;
+static const uint32_t kChildLeaf = 0x40000000u;
 static const uint32_t kInvalid = 0xFFFFFFFFu;
 
 // Threaded (skip-pointer) node, one table per traversal order: the six ray-direction classes

@@ -205,11 +205,11 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
                 bitstack = bitstack << 1;
                 if (L && R) {
                     bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
-                    node = nearL ? q3.x : q3.y;
+                    node = (nearL ? q3.x : q3.y) & ~kChildLeaf;
                     stk.push(nearL ? q3.y : q3.x);
                     bitstack = bitstack | 1;
                 } else {
-                    node = L ? q3.x : q3.y;
+                    node = (L ? q3.x : q3.y) & ~kChildLeaf;
                 }
                 continue;
             }
@@ -228,7 +228,7 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
             const uint32_t z = (uint32_t)__builtin_ctzll(bitstack);
             if (kCount) cnt->parent += z;
             bitstack = bitstack >> z;
-            node = stk.pop();
+            node = stk.pop() & ~kChildLeaf;
             bitstack = bitstack ^ 1;
             continue;
         }
@@ -245,7 +245,7 @@ __device__ __forceinline__ int32_t trace_stackless(const SLNode* __restrict__ no
             if (bitstack != 0) q = q3p[4 * node + 3];
         }
         if (done) break;
-        node = q.w;
+        node = q.w & ~kChildLeaf;
         bitstack = bitstack ^ 1;
     }
     *tmin_io = tmin;
@@ -383,8 +383,13 @@ __device__ __forceinline__ void stackless_begin(StacklessRay& r, F3 ro, F3 rd, f
 }
 
 // Pending-leaf mark of a resumable walk: the lane has reached a leaf and waits for its wave to
-// run the leaves together (node ids stay below 2^31; kInvalid is never marked).
+// run the leaves together (node ids stay below 2^30; kInvalid is never marked).
 static const uint32_t kPendingLeaf = 0x80000000u;
+static_assert(kPendingLeaf == kChildLeaf << 1, "a leaf link becomes a pending leaf by one shift");
+
+// The walk state for a child or sibling link: the node id, marked pending when the link names a
+// leaf (its record need not be visited to learn that).
+__device__ __forceinline__ uint32_t enter_link(uint32_t id) { return (id & ~kChildLeaf) | ((id & kChildLeaf) << 1); }
 
 // Where the resumable stackless walk reads node records from. SLGlobal: the node array in
 // device memory. SLTreelet: the first `top` records (the top levels of the tree, numbered
@@ -444,7 +449,7 @@ __device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const Src& 
     if (r.stk.n != 0) {
         // the climb the reference makes, one bit per parent, as one shift (bitstack != 0)
         r.bitstack = r.bitstack >> __builtin_ctzll(r.bitstack);
-        r.node = r.stk.pop();
+        r.node = enter_link(r.stk.pop());
         r.bitstack = r.bitstack ^ 1;
         return;
     }
@@ -458,7 +463,7 @@ __device__ __forceinline__ void stackless_backtrack(StacklessRay& r, const Src& 
         r.bitstack = r.bitstack >> 1;
         if (r.bitstack != 0) q = src.links(r.node);
     }
-    r.node = q.w;
+    r.node = enter_link(q.w);
     r.bitstack = r.bitstack ^ 1;
 }
 
@@ -482,11 +487,11 @@ __device__ __forceinline__ void stackless_visit(StacklessRay& r, const Src& src,
         r.bitstack = r.bitstack << 1;
         if (L && R) {
             bool nearL = (Lh0 == Rh0) ? (Lh1 < Rh1) : (Lh0 < Rh0);
-            r.node = nearL ? q3.x : q3.y;
+            r.node = enter_link(nearL ? q3.x : q3.y);
             r.stk.push(nearL ? q3.y : q3.x);
             r.bitstack = r.bitstack | 1;
         } else {
-            r.node = L ? q3.x : q3.y;
+            r.node = enter_link(L ? q3.x : q3.y);
         }
         return;
     }

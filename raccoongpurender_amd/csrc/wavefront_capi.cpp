@@ -1611,8 +1611,8 @@ int rgpu_wavefront_guide_pending(rgpu_wavefront* w, uint32_t* pending) {
     return RGPU_OK;
 }
 
-// One training pass if at least `min_samples` samples are buffered: minibatch Adam steps over
-// the buffer, then the buffer is emptied. *trained = samples used (0 if none).
+// Copies the first `rows` buffered training samples to the host; *buffered = samples buffered
+// (capped at the capacity).
 int rgpu_guide_read_samples(rgpu_guide* g, float* out, uint32_t rows, uint32_t* buffered) {
     if (!g || (rows && !out)) return RGPU_ERR_INVALID_ARG;
     rgpu_ctx* ctx = g->ctx;
@@ -1628,6 +1628,8 @@ int rgpu_guide_read_samples(rgpu_guide* g, float* out, uint32_t rows, uint32_t* 
     return RGPU_OK;
 }
 
+// One training pass if at least `min_samples` samples are buffered: minibatch Adam steps over
+// the buffer, then the buffer is emptied. *trained = samples used (0 if none).
 int rgpu_wavefront_guide_train(rgpu_wavefront* w, uint32_t min_samples, uint32_t* trained) {
     if (!w || !w->guide_obj) return RGPU_ERR_INVALID_ARG;
     rgpu_guide* g = w->guide_obj;

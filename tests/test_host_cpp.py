@@ -144,3 +144,19 @@ def test_obj_readers_agree_on_the_reference_meshes(name, ntris):
     assert len(tri) == ntris
     assert np.array_equal(np.asarray(out["points_bits"], np.uint32), pts.reshape(-1).view(np.uint32))
     assert np.array_equal(np.asarray(out["indices"], np.uint32), tri.reshape(-1))
+
+
+PACK_BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "host_cpp", "pack_test")
+
+
+@pytest.mark.skipif(not os.path.exists(PACK_BIN), reason="pack_test not built (run __graft_entry__.build())")
+def test_stackless_leaf_links():
+    """The packer marks exactly the child and sibling links that name leaves, and the ids under
+    the marks are the unmarked packing's (tests/host_cpp/pack_test.cpp checks every record of five
+    trees: leaf limits 1, 4 and 8, with and without spheres, and a single-leaf tree)."""
+    res = subprocess.run([PACK_BIN], capture_output=True, text=True, timeout=120)
+    trees = [json.loads(line) for line in res.stdout.splitlines()]
+    assert res.returncode == 0 and len(trees) == 5 and all(t["ok"] for t in trees)
+    for t in trees:
+        # every leaf but a leaf root is some branch's child, so it is marked exactly once
+        assert t["marks"] == (t["leaves"] if t["nodes"] > 1 else 0)
