@@ -167,8 +167,8 @@ int rgpu_extension_ray_cast(rgpu_ctx* ctx, rgpu_stream stream, const rgpu_scene*
  * SIMD (grid = CUs * waves_per_simd workgroups of 256; 1..16), the number of idle lanes of a
  * wave that makes it hand out its next rays (1..64), and the number of lanes waiting at leaves
  * that makes it test their primitives (1..64). 0 keeps the default (8 waves; refill at 24 idle
- * lanes for stackless, 48 for threaded; leaves at 6 waiting lanes, or once as many lanes wait
- * as walk on). leaf_batch also applies to the one-thread-per-ray kernels (block != 0), whose
+ * lanes for stackless, 48 for threaded; leaves at 8 waiting lanes for stackless, 6 for threaded,
+ * or once as many lanes wait as walk on). leaf_batch also applies to the one-thread-per-ray kernels (block != 0), whose
  * default is 64 (leaves only once as many lanes wait as walk on) and where 1 selects the plain
  * one-ray walk. treelet: the number of top-of-tree stackless node records each
  * workgroup of the persistent kernel copies to LDS and reads from there (1..1024, 64 bytes

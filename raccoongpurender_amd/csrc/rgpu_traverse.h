@@ -43,7 +43,8 @@ struct TraceScene {
 // cu_count * waves_per_simd workgroups of 256), the idle-lane count that triggers a refill and
 // the count of lanes waiting at leaves that triggers their primitive tests. Zeros take the
 // defaults (256 CUs, 8 waves; refill at 24 lanes for the stackless walk, 48 for the threaded
-// ones; leaves at 6), measured on 4M incoherent rays over 1.3M triangles.
+// ones; leaves at 8 waiting lanes for the stackless walk, 6 for the threaded ones), measured on
+// 4M incoherent rays over 1.3M triangles.
 struct TraceTuning {
     uint32_t cu_count = 0;
     uint32_t waves_per_simd = 0;
@@ -52,6 +53,7 @@ struct TraceTuning {
     uint32_t treelet = 0;  // stackless records staged in LDS per workgroup; kTreeletOff: none
 };
 constexpr uint32_t kDefaultLeafBatch = 6;
+constexpr uint32_t kDefaultLeafBatchStackless = 8;
 // One thread per ray (coherent rays, the wavefront's per-slot trace): leaves wait until as many
 // lanes wait as walk on.
 constexpr uint32_t kPerRayLeafBatch = 64;

@@ -212,10 +212,12 @@ hipError_t launch_trace(const TraceScene& sc, int variant, const TraceRay* rays,
         const uint32_t cus = tune && tune->cu_count ? tune->cu_count : 256u;
         const uint32_t wps = tune && tune->waves_per_simd ? tune->waves_per_simd : 8u;
         // measured on the 1.3M-triangle incoherent set: refills at 24 (stackless) or 48 (threaded)
-        // idle lanes, primitive tests once 6 lanes wait at leaves
+        // idle lanes, primitive tests once 8 (stackless) or 6 (threaded) lanes wait at leaves
         const uint32_t refill =
             tune && tune->refill ? tune->refill : (variant != kVariantStackless ? 48u : 24u);
-        const uint32_t leaf_batch = tune && tune->leaf_batch ? tune->leaf_batch : kDefaultLeafBatch;
+        const uint32_t leaf_batch = tune && tune->leaf_batch ? tune->leaf_batch
+                                    : variant == kVariantStackless ? kDefaultLeafBatchStackless
+                                                                   : kDefaultLeafBatch;
         const uint32_t blocks = cus * wps;  // 4 waves (one per SIMD) per 256-thread block
         const uint32_t waves = blocks * 4u;
         const uint32_t chunk = (n + waves - 1) / waves;
