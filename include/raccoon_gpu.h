@@ -144,7 +144,9 @@ int rgpu_bvh_export(const rgpu_bvh* bvh, float* bounds, int32_t* left, int32_t* 
  * SceneBuffer it fills (common/scene_manager.hpp:284-294): packs the stackless node
  * records, the threaded node tables (one per ray-direction class and one for the single-table
  * walk, each in its own pre-order) and the primitive records, and uploads them with the
- * vertices. Host arrays are copied; the caller keeps ownership. */
+ * vertices. Host arrays are copied; the caller keeps ownership. A BVH of 2^30 nodes or more
+ * (over 2^29 primitives) is refused with RGPU_ERR_INVALID_ARG: the walk's node ids carry two
+ * flag bits. */
 int rgpu_scene_create(rgpu_ctx* ctx, const rgpu_bvh* bvh, const float* points, uint32_t stride, uint32_t npoints,
                       const uint32_t* indices, uint32_t ntris, const float* spheres, uint32_t nspheres,
                       rgpu_scene** out);
